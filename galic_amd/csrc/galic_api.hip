@@ -1,0 +1,847 @@
+// galic_api.hip -- host side of the C ABI (include/galic_hip.h): context
+// management, uploads, and the driver sequences of the reference optimiser
+// (main.c:93-102, 151-246, 808-874) expressed as launches on one HIP stream.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/galic_hip.h"
+#include "galic_ctx.h"
+#include "galic_kernels.h"
+
+struct galic_ctx {
+  galic::Ctx c;
+  // host copies of the targets and of the geometry, used to build the cuts
+  std::vector<double> targets[GALIC_NTYPES][6];
+  std::vector<double> distance, cellsize;
+  std::vector<double> weights;  // [type][W_COUNT][Nstack]
+  std::vector<int> cutB_of;     // [type][Ngrid]
+  bool initial_done = false;
+  bool fp32_initial_done = false;
+};
+
+namespace {
+
+thread_local std::string g_err;
+galic_ctx *g_bound = nullptr;
+
+int set_err(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if(e_ != hipSuccess)                                                                            \
+      return set_err(e_ == hipErrorOutOfMemory ? GALIC_ERR_NOMEM : GALIC_ERR_HIP, "%s: %s (%s:%d)", \
+                     #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
+  } while(0)
+
+#define CHK(expr)          \
+  do {                     \
+    int r_ = (expr);       \
+    if(r_ != GALIC_OK)     \
+      return r_;           \
+  } while(0)
+
+template <typename T>
+int dalloc(T **ptr, size_t count) {
+  *ptr = nullptr;
+  if(count == 0) return GALIC_OK;
+  HIPCHK(hipMalloc((void **)ptr, count * sizeof(T)));
+  HIPCHK(hipMemset(*ptr, 0, count * sizeof(T)));
+  return GALIC_OK;
+}
+
+template <typename T>
+void dfree(T *&ptr) {
+  if(ptr) (void)hipFree((void *)ptr);
+  ptr = nullptr;
+}
+
+inline int nblocks(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// Newton solve for the spacing factor (grid.c:353-374 and the FG/EG copies).
+double solve_fac(int nbin, double rin, double rmax, double *rmin) {
+  int iter = 0;
+  double dfac, fac = 1.02, y = rmax / rin;
+  do {
+    double f = log((pow(fac, nbin) - 1.0) / (y * (fac - 1.0)));
+    double df = nbin / (fac - 1.0 / pow(fac, nbin - 1)) - 1.0 / (fac - 1.0);
+    dfac = -f / df;
+    fac += dfac;
+    iter++;
+    if(iter > 1000) return -1;
+  } while(fabs(dfac) > 1.0e-8 * fac);
+  *rmin = rin / (fac - 1.0);
+  return fac;
+}
+
+void smooth_stack_host(double *data, int maxlevel) {
+  for(int l = maxlevel - 1; l >= 0; l--) {
+    int n = 1 << l;
+    for(int i = 0; i < n; i++)
+      for(int j = 0; j < n; j++)
+        data[GALIC_STACKOFFSET(l, i, j)] =
+            data[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * j)] + data[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * j)] +
+            data[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * j + 1)] + data[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * j + 1)];
+  }
+}
+
+// The set of nodes where calc_stack_difference's recursion stops
+// (grid.c:1090: l >= maxlevel or ref1 < thresh).  Positions never move, so
+// the reference's cuts are the same at every call.
+void mark_cut(const double *ref, double thresh, int l, int i, int j, int L, std::vector<char> &cut) {
+  int o = GALIC_STACKOFFSET(l, i, j);
+  if(l >= L || ref[o] < thresh) {
+    cut[o] = 1;
+    return;
+  }
+  for(int ii = 2 * i; ii <= 2 * i + 1; ii++)
+    for(int jj = 2 * j; jj <= 2 * j + 1; jj++) mark_cut(ref, thresh, l + 1, ii, jj, L, cut);
+}
+
+bool type_present(const galic_params &p, int t) { return p.NType[t] > 0; }
+
+int alloc_orbit_scratch(galic::Ctx &c, int n) {
+  if(n <= c.max_orbits) return GALIC_OK;
+  dfree(c.o_pidx); dfree(c.o_type); dfree(c.o_x); dfree(c.o_y); dfree(c.o_z);
+  dfree(c.o_vx); dfree(c.o_vy); dfree(c.o_vz); dfree(c.o_mass); dfree(c.o_tint);
+  dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
+  dfree(c.o_steps); dfree(c.o_bbox);
+  CHK(dalloc(&c.o_pidx, n)); CHK(dalloc(&c.o_type, n));
+  CHK(dalloc(&c.o_x, n)); CHK(dalloc(&c.o_y, n)); CHK(dalloc(&c.o_z, n));
+  CHK(dalloc(&c.o_vx, n)); CHK(dalloc(&c.o_vy, n)); CHK(dalloc(&c.o_vz, n));
+  CHK(dalloc(&c.o_mass, n)); CHK(dalloc(&c.o_tint, n));
+  CHK(dalloc(&c.o_tall, n)); CHK(dalloc(&c.o_e0, n)); CHK(dalloc(&c.o_e1, n));
+  CHK(dalloc(&c.o_norbit, n)); CHK(dalloc(&c.o_status, n));
+  CHK(dalloc(&c.o_steps, n)); CHK(dalloc(&c.o_bbox, n));
+  c.max_orbits = n;
+  return GALIC_OK;
+}
+
+int alloc_trial_scratch(galic::Ctx &c, int n) {
+  if(n <= c.max_trials) return GALIC_OK;
+  dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
+  CHK(dalloc(&c.t_idx, n));
+  CHK(dalloc(&c.t_v, 3 * (size_t)n));
+  CHK(dalloc(&c.t_accept, n));
+  CHK(dalloc(&c.t_dF, n));
+  CHK(dalloc(&c.t_best, n));
+  c.max_trials = n;
+  return GALIC_OK;
+}
+
+// Slots hold one private response (5 grids) per orbit of a trial chunk.
+int alloc_slots(galic::Ctx &c, int norbits, size_t bytes_per_orbit) {
+  size_t need = (size_t)norbits * bytes_per_orbit;
+  if(need <= c.slot_bytes) return GALIC_OK;
+  if(c.slots) (void)hipFree(c.slots);
+  c.slots = nullptr;
+  c.slot_bytes = 0;
+  HIPCHK(hipMalloc(&c.slots, need));
+  HIPCHK(hipMemset(c.slots, 0, need));
+  c.slot_bytes = need;
+  return GALIC_OK;
+}
+
+galic::OrbitIn orbit_in(galic::Ctx &c) {
+  galic::OrbitIn O;
+  O.pidx = c.o_pidx;
+  O.ox = c.o_x; O.oy = c.o_y; O.oz = c.o_z;
+  O.ovx = c.o_vx; O.ovy = c.o_vy; O.ovz = c.o_vz;
+  O.omass = c.o_mass; O.otint = c.o_tint; O.otype = c.o_type;
+  return O;
+}
+
+galic::OrbitArgs orbit_args(galic::Ctx &c, int n) {
+  galic::OrbitArgs A;
+  memset(&A, 0, sizeof(A));
+  A.n = n;
+  A.counter = c.counter;
+  A.ox = c.o_x; A.oy = c.o_y; A.oz = c.o_z;
+  A.ovx = c.o_vx; A.ovy = c.o_vy; A.ovz = c.o_vz;
+  A.omass = c.o_mass; A.otint = c.o_tint; A.otype = c.o_type;
+  A.tall = c.o_tall; A.e0 = c.o_e0; A.e1 = c.o_e1;
+  A.norbit = c.o_norbit; A.status = c.o_status; A.steps = c.o_steps; A.bbox = c.o_bbox;
+  A.slot_stride = (size_t)galic::kFields * c.Ngrid;
+  return A;
+}
+
+// Persistent grid: enough workgroups to fill every CU several times over, never
+// more than the orbits need.
+int orbit_grid(int n, int bs) {
+  int dev;
+  (void)hipGetDevice(&dev);
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int want = cus * 8;
+  int need = nblocks(n, bs);
+  return std::max(1, std::min(want, need));
+}
+
+int launch_orbit64(galic::Ctx &c, galic::OrbitArgs A) {
+  if(A.n <= 0) return GALIC_OK;
+  HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  const int bs = 256;
+  hipLaunchKernelGGL(galic::k_orbit64, dim3(orbit_grid(A.n, bs)), dim3(bs), 0, c.stream, c.M, A);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev1, c.stream));
+  HIPCHK(hipEventSynchronize(c.ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  c.last_orbit_ms = ms;
+  c.acc_orbit_ms += ms;
+  c.acc_orbit_launches++;
+  c.acc_orbits += A.n;
+  // orbit status: any step-cap or argument failure is an error (orbit_response.c:449-456)
+  std::vector<int> st(A.n);
+  std::vector<long long> steps(A.n);
+  HIPCHK(hipMemcpy(st.data(), A.status, sizeof(int) * A.n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(steps.data(), A.steps, sizeof(long long) * A.n, hipMemcpyDeviceToHost));
+  long long tot = 0;
+  for(int o = 0; o < A.n; o++) {
+    tot += steps[o];
+    if(st[o] == GALIC_ERR_STEPCAP)
+      return set_err(GALIC_ERR_STEPCAP, "orbit %d exceeded the step cap of %lld steps", o, (long long)c.M.max_steps);
+    if(st[o] != 0) return set_err(GALIC_ERR_ARG, "orbit %d has a non-positive integration time", o);
+  }
+  c.acc_steps += tot;
+  return GALIC_OK;
+}
+
+galic::FitArgs fit_args(galic_ctx *ctx) {
+  galic::Ctx &c = ctx->c;
+  galic::FitArgs F;
+  F.L = c.L;
+  for(int t = 0; t < GALIC_NTYPES; t++) {
+    F.present[t] = (t > 0 && type_present(c.p, t)) ? 1 : 0;
+    F.tovs[t] = c.p.TypeOfVelocityStructure[t];
+  }
+  F.stacks = c.stacks;
+  F.weights = c.weights;
+  F.locA = c.locA;
+  F.locI = c.locI;
+  F.locM = c.locM;
+  F.fit = c.fit;
+  F.lowdisp = c.p.LowerDispLimit;
+  return F;
+}
+
+// Cut-A weights (density cut, fixed by DGs_MassTarget) and, once EGs_MassResponse
+// is known, the cut-B flags and the fine cell -> cut-B node map.
+int build_weights(galic_ctx *ctx, bool with_cutB) {
+  galic::Ctx &c = ctx->c;
+  const int Ns = c.Nstack, L = c.L, nb = c.Nbin;
+  ctx->weights.assign((size_t)GALIC_NTYPES * galic::W_COUNT * Ns, 0.0);
+  ctx->cutB_of.assign((size_t)GALIC_NTYPES * c.Ngrid, 0);
+  std::vector<double> emr(Ns);
+  for(int t = 1; t < GALIC_NTYPES; t++) {
+    if(!type_present(c.p, t) || !c.targets_set[t]) continue;
+    double *W = &ctx->weights[(size_t)t * galic::W_COUNT * Ns];
+    const std::vector<double> &Tm = ctx->targets[t][0];
+    double thr_d = c.p.MinParticlesPerBinForDensityMeasurement * c.p.MType[t] / c.p.NType[t];
+    std::vector<char> cutA(Ns, 0);
+    mark_cut(Tm.data(), thr_d, 0, 0, 0, L, cutA);
+    const double rlim = c.p.R200 / 20.0;
+    for(int o = 0; o < Ns; o++) {
+      if(!cutA[o]) continue;
+      double d = ctx->distance[o];
+      W[galic::W_A * Ns + o] = (d < rlim) ? d : rlim;  // min() of grid.c:1078-1084
+      // S-term weights Tm/E_x for E_x > 0 (grid.c:1128-1133), fields r, t, q, p
+      const int e_of[4] = {2, 3, 5, 4};  // targets[] order: Tm, MT, Er, Et, Ep, Eq
+      for(int k = 0; k < 4; k++) {
+        double e = ctx->targets[t][e_of[k]][o];
+        W[(galic::W_SR + k) * Ns + o] = (e > 0) ? Tm[o] / e : 0.0;
+      }
+    }
+    if(with_cutB) {
+      HIPCHK(hipMemcpy(emr.data(), c.stacks + ((size_t)t * galic::ST_COUNT + galic::ST_EMR) * Ns,
+                       sizeof(double) * Ns, hipMemcpyDeviceToHost));
+      double thr_s = c.p.MinParticlesPerBinForDispersionMeasurement * c.p.MType[t] / c.p.NType[t];
+      std::vector<char> cutB(Ns, 0);
+      mark_cut(emr.data(), thr_s, 0, 0, 0, L, cutB);
+      for(int o = 0; o < Ns; o++) W[galic::W_B * Ns + o] = cutB[o] ? 1.0 : 0.0;
+      int *cb = &ctx->cutB_of[(size_t)t * c.Ngrid];
+      for(int iz = 0; iz < nb; iz++)
+        for(int iR = 0; iR < nb; iR++) {
+          int node = -1;
+          for(int l = 0; l <= L && node < 0; l++) {
+            int o = GALIC_STACKOFFSET(l, iz >> (L - l), iR >> (L - l));
+            if(cutB[o]) node = o;
+          }
+          cb[iz * nb + iR] = node;
+        }
+    }
+  }
+  HIPCHK(hipMemcpy(c.weights, ctx->weights.data(), sizeof(double) * ctx->weights.size(), hipMemcpyHostToDevice));
+  if(with_cutB)
+    HIPCHK(hipMemcpy(c.cutB_of, ctx->cutB_of.data(), sizeof(int) * ctx->cutB_of.size(), hipMemcpyHostToDevice));
+  return GALIC_OK;
+}
+
+int global_fit(galic_ctx *ctx) {
+  galic::Ctx &c = ctx->c;
+  galic::FitArgs F = fit_args(ctx);
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  hipLaunchKernelGGL(galic::k_fit_stacks, dim3(30), dim3(256), 0, c.stream, F);
+  HIPCHK(hipGetLastError());
+  if(!c.cuts_ready) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    CHK(build_weights(ctx, true));
+    c.cuts_ready = true;
+  }
+  hipLaunchKernelGGL(galic::k_fit_sums, dim3(3), dim3(256), 5 * 256 * sizeof(double), c.stream, F);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev1, c.stream));
+  HIPCHK(hipEventSynchronize(c.ev1));
+  HIPCHK(hipEventElapsedTime(&c.last_fit_ms, c.ev0, c.ev1));
+  return GALIC_OK;
+}
+
+int commit(galic_ctx *ctx) {
+  galic::Ctx &c = ctx->c;
+  size_t na = (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid, ni = (size_t)GALIC_NTYPES * 4 * c.Ngrid;
+  hipLaunchKernelGGL(galic::k_commit, dim3(nblocks(na, 256)), dim3(256), 0, c.stream, c.locA, c.locA_d, na);
+  hipLaunchKernelGGL(galic::k_commit, dim3(nblocks(ni, 256)), dim3(256), 0, c.stream, c.locI, c.locI_d, ni);
+  HIPCHK(hipGetLastError());
+  return global_fit(ctx);
+}
+
+}  // namespace
+
+namespace galic {
+galic_ctx *bound_ctx() { return g_bound; }
+Ctx &ctx_state(galic_ctx *ctx) { return ctx->c; }
+}  // namespace galic
+
+// =========================================================================
+extern "C" {
+
+const char *galic_last_error(void) { return g_err.c_str(); }
+int galic_version(void) { return 10100; }
+
+int galic_grid_create(galic_params *p, double outer_frac, double inner_frac) {
+  if(!p || !(outer_frac > 0 && outer_frac < 1) || !(inner_frac > 0 && inner_frac < 1))
+    return set_err(GALIC_ERR_ARG, "enclosed-mass fractions must lie in (0,1)");
+  double fac = outer_frac;
+  p->Rmax = p->Halo_A * (fac + sqrt(fac)) / (1 - fac);
+  fac = inner_frac;
+  double rin = p->Halo_A * (fac + sqrt(fac)) / (1 - fac);
+  p->DG_Fac = solve_fac(1 << p->DG_MaxLevel, rin, p->Rmax, &p->DG_Rmin);
+  p->FG_Fac = solve_fac(p->FG_Nbin, rin, p->Rmax, &p->FG_Rmin);
+  p->EG_Fac = solve_fac(1 << p->EG_MaxLevel, rin, p->Rmax, &p->EG_Rmin);
+  if(p->DG_Fac < 0 || p->FG_Fac < 0 || p->EG_Fac < 0)
+    return set_err(GALIC_ERR_ARG, "Newton iteration for the grid spacing did not converge");
+  return GALIC_OK;
+}
+
+int galic_density_geometry(const galic_params *p, double *cellsize, double *cellvol, double *distance) {
+  if(!p || p->DG_MaxLevel < 1 || p->DG_MaxLevel > 12) return set_err(GALIC_ERR_ARG, "bad DG_MaxLevel");
+  const int L = p->DG_MaxLevel, nb = 1 << L, Ns = GALIC_NSTACK(L);
+  if(distance) {
+    std::vector<double> logR(Ns, 0.0), logZ(Ns, 0.0);
+    for(int k = 0; k < nb; k++)
+      for(int j = 0; j < nb; j++) {
+        double z = p->DG_Rmin * 0.5 * (pow(p->DG_Fac, k) + pow(p->DG_Fac, k + 1) - 2.0);
+        double R = p->DG_Rmin * 0.5 * (pow(p->DG_Fac, j) + pow(p->DG_Fac, j + 1) - 2.0);
+        logR[GALIC_STACKOFFSET(L, k, j)] = log(R);
+        logZ[GALIC_STACKOFFSET(L, k, j)] = log(z);
+      }
+    smooth_stack_host(logR.data(), L);
+    smooth_stack_host(logZ.data(), L);
+    for(int l = L - 1; l >= 0; l--) {
+      int n = 1 << l, f = 1 << (L - l);
+      for(int i = 0; i < n; i++)
+        for(int j = 0; j < n; j++) {
+          logR[GALIC_STACKOFFSET(l, i, j)] /= (f * f);
+          logZ[GALIC_STACKOFFSET(l, i, j)] /= (f * f);
+        }
+    }
+    for(int l = L; l >= 0; l--) {
+      int n = 1 << l;
+      for(int i = 0; i < n; i++)
+        for(int j = 0; j < n; j++)
+          distance[GALIC_STACKOFFSET(l, i, j)] =
+              sqrt(pow(exp(logR[GALIC_STACKOFFSET(l, i, j)]), 2) + pow(exp(logZ[GALIC_STACKOFFSET(l, i, j)]), 2));
+    }
+  }
+  for(int k = 0; k < nb; k++)
+    for(int j = 0; j < nb; j++) {
+      int i = k * nb + j;
+      double r1 = p->DG_Rmin * (pow(p->DG_Fac, j) - 1.0);
+      double r2 = p->DG_Rmin * (pow(p->DG_Fac, j + 1) - 1.0);
+      double z1 = p->DG_Rmin * (pow(p->DG_Fac, k) - 1.0);
+      double z2 = p->DG_Rmin * (pow(p->DG_Fac, k + 1) - 1.0);
+      double vol = M_PI * (r2 * r2 - r1 * r1) * (z2 - z1);
+      vol *= 2;
+      if(cellsize) cellsize[i] = (r2 - r1 > z2 - z1) ? r2 - r1 : z2 - z1;
+      if(cellvol) cellvol[i] = vol;
+    }
+  return GALIC_OK;
+}
+
+int galic_create(const galic_params *p, int device, galic_ctx **out) {
+  if(!p || !out) return set_err(GALIC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if(p->DG_MaxLevel != p->EG_MaxLevel)
+    return set_err(GALIC_ERR_ARG, "DG_MaxLevel (%d) must equal EG_MaxLevel (%d)", p->DG_MaxLevel, p->EG_MaxLevel);
+  if(p->DG_MaxLevel < 2 || p->DG_MaxLevel > 7)
+    return set_err(GALIC_ERR_ARG, "DG_MaxLevel %d outside the supported range 2..7", p->DG_MaxLevel);
+  if(p->FG_Nbin < 2) return set_err(GALIC_ERR_ARG, "FG_Nbin must be at least 2");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if(device < 0 || device >= ndev) return set_err(GALIC_ERR_ARG, "device %d not present (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+
+  galic_ctx *ctx = new galic_ctx();
+  galic::Ctx &c = ctx->c;
+  c.p = *p;
+  c.device = device;
+  c.L = p->DG_MaxLevel;
+  c.Nbin = 1 << c.L;
+  c.Ngrid = c.Nbin * c.Nbin;
+  c.Nstack = GALIC_NSTACK(c.L);
+  c.FG_Ngrid = p->FG_Nbin * p->FG_Nbin;
+  HIPCHK(hipStreamCreate(&c.stream));
+  HIPCHK(hipEventCreate(&c.ev0));
+  HIPCHK(hipEventCreate(&c.ev1));
+
+  CHK(dalloc(&c.FG_Pot, c.FG_Ngrid));
+  CHK(dalloc(&c.FG_DPotDR, c.FG_Ngrid));
+  CHK(dalloc(&c.FG_DPotDz, c.FG_Ngrid));
+  CHK(dalloc(&c.FG_Vstream, (size_t)GALIC_NTYPES * c.FG_Ngrid));
+  CHK(dalloc(&c.FG_Acc_f, c.FG_Ngrid));
+  CHK(dalloc(&c.FG_Vstream_f, (size_t)GALIC_NTYPES * c.FG_Ngrid));
+  CHK(dalloc(&c.DG_CellSize, c.Ngrid));
+  CHK(dalloc(&c.DG_CellSize_f, c.Ngrid));
+  CHK(dalloc(&c.stacks, (size_t)GALIC_NTYPES * galic::ST_COUNT * c.Nstack));
+  CHK(dalloc(&c.weights, (size_t)GALIC_NTYPES * galic::W_COUNT * c.Nstack));
+  CHK(dalloc(&c.cutB_of, (size_t)GALIC_NTYPES * c.Ngrid));
+  CHK(dalloc(&c.locA, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
+  CHK(dalloc(&c.locA_d, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
+  CHK(dalloc(&c.locI, (size_t)GALIC_NTYPES * 4 * c.Ngrid));
+  CHK(dalloc(&c.locI_d, (size_t)GALIC_NTYPES * 4 * c.Ngrid));
+  CHK(dalloc(&c.locM, (size_t)GALIC_NTYPES * c.Ngrid));
+  CHK(dalloc(&c.locA_q, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
+  CHK(dalloc(&c.locA_dq, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
+  CHK(dalloc(&c.locI_q, (size_t)GALIC_NTYPES * 4 * c.Ngrid));
+  CHK(dalloc(&c.locI_dq, (size_t)GALIC_NTYPES * 4 * c.Ngrid));
+  CHK(dalloc(&c.fit, (size_t)GALIC_NTYPES * 16));
+  CHK(dalloc(&c.counter, 1));
+
+  // geometry
+  ctx->distance.assign(c.Nstack, 0.0);
+  ctx->cellsize.assign(c.Ngrid, 0.0);
+  CHK(galic_density_geometry(p, ctx->cellsize.data(), nullptr, ctx->distance.data()));
+  HIPCHK(hipMemcpy(c.DG_CellSize, ctx->cellsize.data(), sizeof(double) * c.Ngrid, hipMemcpyHostToDevice));
+  {
+    std::vector<float> f(c.Ngrid);
+    for(int i = 0; i < c.Ngrid; i++) f[i] = (float)ctx->cellsize[i];
+    HIPCHK(hipMemcpy(c.DG_CellSize_f, f.data(), sizeof(float) * c.Ngrid, hipMemcpyHostToDevice));
+  }
+  for(int t = 1; t < GALIC_NTYPES; t++)
+    for(int k = 0; k < 6; k++) ctx->targets[t][k].assign(c.Nstack, 0.0);
+
+  // device model
+  galic::DevModel &M = c.M;
+  memset(&M, 0, sizeof(M));
+  M.G = p->G; M.V200 = p->V200; M.R200 = p->R200; M.M200 = p->M200;
+  M.Halo_Mass = p->Halo_Mass; M.Halo_A = p->Halo_A;
+  M.Bulge_Mass = p->Bulge_Mass; M.Bulge_A = p->Bulge_A;
+  M.halo_offgrid_fac = (p->Halo_Mass > 0) ? p->M200 / p->Halo_Mass : 0.0;
+  M.use_table = (p->Disk_Mass > 0 || (p->Halo_Mass > 0 && p->SampleForceNhalo > 0) ||
+                 (p->Bulge_Mass > 0 && p->SampleForceNbulge > 0));
+  M.add_halo = (p->Halo_Mass > 0 && p->SampleForceNhalo == 0);
+  M.add_bulge = (p->Bulge_Mass > 0 && p->SampleForceNbulge == 0);
+  for(int t = 0; t < GALIC_NTYPES; t++) M.tovs[t] = p->TypeOfVelocityStructure[t];
+  M.fo = p->TimeStepFactorOrbit;
+  M.fc = p->TimeStepFactorCellCross;
+  M.max_steps = p->MaxStepsPerOrbit > 0 ? p->MaxStepsPerOrbit : 100000000LL;
+  M.FG_Nbin = p->FG_Nbin; M.FG_Rmin = p->FG_Rmin; M.FG_logFac = log(p->FG_Fac);
+  M.DG_Nbin = c.Nbin; M.DG_L = c.L; M.DG_Rmin = p->DG_Rmin; M.DG_logFac = log(p->DG_Fac);
+  M.EG_Nbin = 1 << p->EG_MaxLevel; M.EG_L = p->EG_MaxLevel; M.EG_Rmin = p->EG_Rmin; M.EG_logFac = log(p->EG_Fac);
+  M.fFG_invRmin = (float)(1.0 / p->FG_Rmin); M.fFG_invlogFac = (float)(1.0 / log(p->FG_Fac));
+  M.fDG_invRmin = (float)(1.0 / p->DG_Rmin); M.fDG_invlogFac = (float)(1.0 / log(p->DG_Fac));
+  M.FG_Pot = c.FG_Pot; M.FG_DPotDR = c.FG_DPotDR; M.FG_DPotDz = c.FG_DPotDz;
+  for(int t = 0; t < GALIC_NTYPES; t++) {
+    M.FG_Vstream[t] = c.FG_Vstream + (size_t)t * c.FG_Ngrid;
+    M.FG_Vstream_f[t] = c.FG_Vstream_f + (size_t)t * c.FG_Ngrid;
+  }
+  M.DG_CellSize = c.DG_CellSize;
+  M.DG_CellSize_f = c.DG_CellSize_f;
+  M.FG_Acc_f = c.FG_Acc_f;
+  // fixed-point scales of the FP32 path: 2^-40 in units of mass, of mass*V200^2 for moments
+  c.fp32_scale[0] = ldexp(1.0, 40);
+  for(int f = 1; f < 5; f++) c.fp32_scale[f] = ldexp(1.0, 40) / (p->V200 * p->V200);
+  c.fp32_iscale = ldexp(1.0, 40) / (p->V200 * p->V200);
+  *out = ctx;
+  return GALIC_OK;
+}
+
+int galic_destroy(galic_ctx *ctx) {
+  if(!ctx) return GALIC_OK;
+  galic::Ctx &c = ctx->c;
+  (void)hipSetDevice(c.device);
+  (void)hipStreamSynchronize(c.stream);
+  dfree(c.FG_Pot); dfree(c.FG_DPotDR); dfree(c.FG_DPotDz); dfree(c.FG_Vstream);
+  dfree(c.FG_Acc_f); dfree(c.FG_Vstream_f); dfree(c.DG_CellSize); dfree(c.DG_CellSize_f);
+  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of);
+  dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
+  dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
+  dfree(c.fit); dfree(c.counter);
+  galic::Particles &P = c.P;
+  dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
+  dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
+  dfree(P.type); dfree(P.orbits);
+  dfree(c.o_pidx); dfree(c.o_type); dfree(c.o_x); dfree(c.o_y); dfree(c.o_z);
+  dfree(c.o_vx); dfree(c.o_vy); dfree(c.o_vz); dfree(c.o_mass); dfree(c.o_tint);
+  dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
+  dfree(c.o_steps); dfree(c.o_bbox);
+  dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
+  if(c.slots) (void)hipFree(c.slots);
+  if(c.ev0) (void)hipEventDestroy(c.ev0);
+  if(c.ev1) (void)hipEventDestroy(c.ev1);
+  if(c.stream) (void)hipStreamDestroy(c.stream);
+  if(g_bound == ctx) g_bound = nullptr;
+  delete ctx;
+  return GALIC_OK;
+}
+
+int galic_set_force_tables(galic_ctx *ctx, const double *pot, const double *dpotdr, const double *dpotdz) {
+  if(!ctx || !pot || !dpotdr || !dpotdz) return set_err(GALIC_ERR_ARG, "null argument");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  size_t n = c.FG_Ngrid;
+  HIPCHK(hipMemcpy(c.FG_Pot, pot, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.FG_DPotDR, dpotdr, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.FG_DPotDz, dpotdz, n * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<float2> f(n);
+  for(size_t i = 0; i < n; i++) f[i] = make_float2((float)dpotdr[i], (float)dpotdz[i]);
+  HIPCHK(hipMemcpy(c.FG_Acc_f, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  return GALIC_OK;
+}
+
+int galic_set_vstream(galic_ctx *ctx, int type, const double *vstream) {
+  if(!ctx || type < 1 || type > 3) return set_err(GALIC_ERR_ARG, "bad type %d", type);
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  size_t n = c.FG_Ngrid;
+  std::vector<double> v(n, 0.0);
+  if(vstream) memcpy(v.data(), vstream, n * sizeof(double));
+  HIPCHK(hipMemcpy(c.FG_Vstream + (size_t)type * n, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<float> f(n);
+  for(size_t i = 0; i < n; i++) f[i] = (float)v[i];
+  HIPCHK(hipMemcpy(c.FG_Vstream_f + (size_t)type * n, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  return GALIC_OK;
+}
+
+int galic_set_targets(galic_ctx *ctx, int type, const double *dgs_mass_target, const double *egs_mass_target,
+                      const double *egs_r, const double *egs_t, const double *egs_p, const double *egs_q) {
+  if(!ctx || type < 1 || type > 3) return set_err(GALIC_ERR_ARG, "bad type %d", type);
+  if(!dgs_mass_target || !egs_mass_target || !egs_r || !egs_t || !egs_p || !egs_q)
+    return set_err(GALIC_ERR_ARG, "null target stack");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  const double *src[6] = {dgs_mass_target, egs_mass_target, egs_r, egs_t, egs_p, egs_q};
+  const int dst[6] = {galic::ST_MT, galic::ST_EMT, galic::ST_ER, galic::ST_ET, galic::ST_EP, galic::ST_EQ};
+  for(int k = 0; k < 6; k++) {
+    ctx->targets[type][k].assign(src[k], src[k] + c.Nstack);
+    HIPCHK(hipMemcpy(c.stacks + ((size_t)type * galic::ST_COUNT + dst[k]) * c.Nstack, src[k],
+                     sizeof(double) * c.Nstack, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(c.stacks + ((size_t)type * galic::ST_COUNT + galic::ST_DIST) * c.Nstack, ctx->distance.data(),
+                   sizeof(double) * c.Nstack, hipMemcpyHostToDevice));
+  c.targets_set[type] = true;
+  c.cuts_ready = false;
+  return GALIC_OK;
+}
+
+int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *vel, const double *mass,
+                        const double *tint, const int *type, const double *vesc) {
+  if(!ctx || n <= 0 || !pos || !vel || !mass || !tint || !type) return set_err(GALIC_ERR_ARG, "bad particle input");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  for(int i = 0; i < n; i++) {
+    if(type[i] < 1 || type[i] > 3) return set_err(GALIC_ERR_ARG, "particle %d: unknown type %d", i, type[i]);
+    if(!type_present(c.p, type[i]))
+      return set_err(GALIC_ERR_ARG, "particle %d has type %d but NType[%d] is 0", i, type[i], type[i]);
+  }
+  galic::Particles &P = c.P;
+  dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
+  dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
+  dfree(P.type); dfree(P.orbits);
+  P.n = n;
+  CHK(dalloc(&P.x, n)); CHK(dalloc(&P.y, n)); CHK(dalloc(&P.z, n));
+  CHK(dalloc(&P.vx, n)); CHK(dalloc(&P.vy, n)); CHK(dalloc(&P.vz, n));
+  CHK(dalloc(&P.mass, n)); CHK(dalloc(&P.tint, n)); CHK(dalloc(&P.vesc, n));
+  CHK(dalloc(&P.vr2, n)); CHK(dalloc(&P.vt2, n)); CHK(dalloc(&P.vp2, n)); CHK(dalloc(&P.vq2, n));
+  CHK(dalloc(&P.type, n)); CHK(dalloc(&P.orbits, n));
+  std::vector<double> tmp(n);
+  double *dst3[3] = {P.x, P.y, P.z};
+  for(int k = 0; k < 3; k++) {
+    for(int i = 0; i < n; i++) tmp[i] = pos[3 * i + k];
+    HIPCHK(hipMemcpy(dst3[k], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  double *dstv[3] = {P.vx, P.vy, P.vz};
+  for(int k = 0; k < 3; k++) {
+    for(int i = 0; i < n; i++) tmp[i] = vel[3 * i + k];
+    HIPCHK(hipMemcpy(dstv[k], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(P.mass, mass, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P.tint, tint, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P.type, type, sizeof(int) * n, hipMemcpyHostToDevice));
+  if(vesc) HIPCHK(hipMemcpy(P.vesc, vesc, sizeof(double) * n, hipMemcpyHostToDevice));
+  ctx->initial_done = false;
+  ctx->fp32_initial_done = false;
+  return GALIC_OK;
+}
+
+int galic_get_particles(galic_ctx *ctx, double *vel, int *orbits, double *vr2, double *vt2, double *vp2,
+                        double *vq2) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null context");
+  galic::Ctx &c = ctx->c;
+  galic::Particles &P = c.P;
+  int n = P.n;
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if(vel) {
+    std::vector<double> tmp(n);
+    double *src[3] = {P.vx, P.vy, P.vz};
+    for(int k = 0; k < 3; k++) {
+      HIPCHK(hipMemcpy(tmp.data(), src[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+      for(int i = 0; i < n; i++) vel[3 * i + k] = tmp[i];
+    }
+  }
+  if(orbits) HIPCHK(hipMemcpy(orbits, P.orbits, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if(vr2) HIPCHK(hipMemcpy(vr2, P.vr2, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if(vt2) HIPCHK(hipMemcpy(vt2, P.vt2, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if(vp2) HIPCHK(hipMemcpy(vp2, P.vp2, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if(vq2) HIPCHK(hipMemcpy(vq2, P.vq2, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return GALIC_OK;
+}
+
+int galic_calc_all_response_fields(galic_ctx *ctx) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null context");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  const int n = c.P.n;
+  if(n <= 0) return set_err(GALIC_ERR_STATE, "no particles set");
+  for(int t = 1; t < GALIC_NTYPES; t++)
+    if(type_present(c.p, t) && !c.targets_set[t]) return set_err(GALIC_ERR_STATE, "targets of type %d not set", t);
+  CHK(alloc_orbit_scratch(c, n));
+  HIPCHK(hipMemsetAsync(c.locA, 0, sizeof(double) * GALIC_NTYPES * galic::kFields * c.Ngrid, c.stream));
+  HIPCHK(hipMemsetAsync(c.locA_d, 0, sizeof(double) * GALIC_NTYPES * galic::kFields * c.Ngrid, c.stream));
+  HIPCHK(hipMemsetAsync(c.locI, 0, sizeof(double) * GALIC_NTYPES * 4 * c.Ngrid, c.stream));
+  HIPCHK(hipMemsetAsync(c.locI_d, 0, sizeof(double) * GALIC_NTYPES * 4 * c.Ngrid, c.stream));
+  HIPCHK(hipMemsetAsync(c.locM, 0, sizeof(double) * GALIC_NTYPES * c.Ngrid, c.stream));
+  HIPCHK(hipMemsetAsync(c.fit, 0, sizeof(double) * GALIC_NTYPES * 16, c.stream));
+  c.cuts_ready = false;
+  CHK(build_weights(ctx, false));
+  hipLaunchKernelGGL(galic::k_prepare_all, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.P, orbit_in(c));
+  HIPCHK(hipGetLastError());
+  galic::OrbitArgs A = orbit_args(c, n);
+  A.loc = c.locA;
+  CHK(launch_orbit64(c, A));
+  hipLaunchKernelGGL(galic::k_store_orbits, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.o_norbit,
+                     c.P.orbits);
+  hipLaunchKernelGGL(galic::k_inst_init, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, c.M, c.P, c.locI, c.locM,
+                     c.Ngrid);
+  HIPCHK(hipGetLastError());
+  CHK(global_fit(ctx));
+  ctx->initial_done = true;
+  return GALIC_OK;
+}
+
+int galic_rep(galic_ctx *ctx, int ntrial, const int *idx, const double *vtry, int *accept, double *dF) {
+  if(!ctx || ntrial < 0 || (ntrial > 0 && (!idx || !vtry))) return set_err(GALIC_ERR_ARG, "bad rep arguments");
+  galic::Ctx &c = ctx->c;
+  if(!ctx->initial_done) return set_err(GALIC_ERR_STATE, "galic_calc_all_response_fields must run first");
+  HIPCHK(hipSetDevice(c.device));
+  for(int i = 0; i < ntrial; i++)
+    if(idx[i] < 0 || idx[i] >= c.P.n) return set_err(GALIC_ERR_ARG, "trial %d: particle index %d out of range", i, idx[i]);
+  const size_t slot_bytes_per_orbit = sizeof(double) * galic::kFields * c.Ngrid;
+  // trials per chunk: 2 slots each, about 2 GiB of slots at most
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)ntrial, (2ull << 30) / (2 * slot_bytes_per_orbit)));
+  chunk = std::max(1, std::min(chunk, 8192));
+  CHK(alloc_trial_scratch(c, std::max(ntrial, 1)));
+  CHK(alloc_orbit_scratch(c, 2 * chunk));
+  CHK(alloc_slots(c, 2 * chunk, slot_bytes_per_orbit));
+  if(ntrial > 0) {
+    std::vector<double> tv(3 * (size_t)ntrial);
+    for(int i = 0; i < ntrial; i++)
+      for(int k = 0; k < 3; k++) tv[(size_t)k * ntrial + i] = vtry[3 * i + k];
+    HIPCHK(hipMemcpy(c.t_idx, idx, sizeof(int) * ntrial, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.t_v, tv.data(), sizeof(double) * 3 * ntrial, hipMemcpyHostToDevice));
+  }
+  // init_updates (main.c:844-874): deltas are zero after every commit
+  float eval_ms = 0;
+  for(int base = 0; base < ntrial; base += chunk) {
+    int nt = std::min(chunk, ntrial - base);
+    const double *tvx = c.t_v + base, *tvy = c.t_v + ntrial + base, *tvz = c.t_v + 2 * (size_t)ntrial + base;
+    hipLaunchKernelGGL(galic::k_prepare_trials, dim3(nblocks(2 * nt, 256)), dim3(256), 0, c.stream, nt,
+                       c.t_idx + base, tvx, tvy, tvz, c.P, orbit_in(c));
+    HIPCHK(hipGetLastError());
+    galic::OrbitArgs A = orbit_args(c, 2 * nt);
+    A.slots = (double *)c.slots;
+    A.slot_base = 0;
+    CHK(launch_orbit64(c, A));
+    galic::EvalArgs E;
+    memset(&E, 0, sizeof(E));
+    E.P = c.P;
+    E.tidx = c.t_idx + base;
+    E.tvx = tvx; E.tvy = tvy; E.tvz = tvz;
+    E.stacks = c.stacks;
+    E.weights = c.weights;
+    E.fit = c.fit;
+    E.cutB_of = c.cutB_of;
+    E.slots = (const double *)c.slots;
+    E.slot_stride = A.slot_stride;
+    E.slot_base = 0;
+    E.bbox = c.o_bbox;
+    E.norbit = c.o_norbit;
+    E.locA_d = c.locA_d;
+    E.locI_d = c.locI_d;
+    E.accept = c.t_accept + base;
+    E.dF = c.t_dF + base;
+    for(int t = 0; t < GALIC_NTYPES; t++) {
+      E.MType[t] = c.p.MType[t];
+      E.fac[t][0] = c.p.fac_value_rs[t];
+      E.fac[t][1] = c.p.fac_value_ts[t];
+      E.fac[t][2] = c.p.fac_value_qs[t];
+      E.fac[t][3] = c.p.fac_value_ps[t];
+    }
+    E.lowdisp = c.p.LowerDispLimit;
+    size_t shmem = sizeof(double) * ((c.Nstack - c.Ngrid) + 8 * 256);
+    HIPCHK(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(galic::k_eval_apply64, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, c.stream));
+    HIPCHK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    eval_ms += ms;
+  }
+  c.last_eval_ms = eval_ms;
+  CHK(commit(ctx));
+  if(ntrial > 0) {
+    if(accept) HIPCHK(hipMemcpy(accept, c.t_accept, sizeof(int) * ntrial, hipMemcpyDeviceToHost));
+    if(dF) HIPCHK(hipMemcpy(dF, c.t_dF, sizeof(double) * ntrial, hipMemcpyDeviceToHost));
+  }
+  return GALIC_OK;
+}
+
+int galic_get_stack(galic_ctx *ctx, int type, int which, double *out) {
+  if(!ctx || type < 1 || type > 3 || which < 0 || which > 10 || !out) return set_err(GALIC_ERR_ARG, "bad arguments");
+  galic::Ctx &c = ctx->c;
+  const int map[11] = {galic::ST_RM,  galic::ST_EMR, galic::ST_IR,  galic::ST_IT,  galic::ST_IP, galic::ST_IQ,
+                       galic::ST_RSR, galic::ST_RST, galic::ST_RSQ, galic::ST_RSP, galic::ST_MT};
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  HIPCHK(hipMemcpy(out, c.stacks + ((size_t)type * galic::ST_COUNT + map[which]) * c.Nstack,
+                   sizeof(double) * c.Nstack, hipMemcpyDeviceToHost));
+  return GALIC_OK;
+}
+
+int galic_get_fit(galic_ctx *ctx, int type, double *out8) {
+  if(!ctx || type < 1 || type > 3 || !out8) return set_err(GALIC_ERR_ARG, "bad arguments");
+  galic::Ctx &c = ctx->c;
+  double f[16];
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  HIPCHK(hipMemcpy(f, c.fit + (size_t)type * 16, sizeof(f), hipMemcpyDeviceToHost));
+  memcpy(out8, f, 8 * sizeof(double));
+  return GALIC_OK;
+}
+
+int galic_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const double *vel, const double *mass,
+                          const double *tint, const int *type, double *fields, double *t, int *orbits,
+                          long long *steps, double *e0, double *e1) {
+  if(!ctx || norbit < 0 || (norbit > 0 && (!pos || !vel || !mass || !tint || !type)))
+    return set_err(GALIC_ERR_ARG, "bad orbit arguments");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  for(int o = 0; o < norbit; o++)
+    if(type[o] < 1 || type[o] > 3) return set_err(GALIC_ERR_ARG, "orbit %d: unknown type %d", o, type[o]);
+  const size_t per = sizeof(double) * galic::kFields * c.Ngrid;
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(norbit, 1), (1ull << 30) / per));
+  CHK(alloc_orbit_scratch(c, chunk));
+  if(fields) CHK(alloc_slots(c, chunk, per));
+  std::vector<double> tmp(chunk);
+  for(int base = 0; base < norbit; base += chunk) {
+    int n = std::min(chunk, norbit - base);
+    double *d3[3] = {c.o_x, c.o_y, c.o_z};
+    double *v3[3] = {c.o_vx, c.o_vy, c.o_vz};
+    for(int k = 0; k < 3; k++) {
+      for(int o = 0; o < n; o++) tmp[o] = pos[3 * (size_t)(base + o) + k];
+      HIPCHK(hipMemcpy(d3[k], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      for(int o = 0; o < n; o++) tmp[o] = vel[3 * (size_t)(base + o) + k];
+      HIPCHK(hipMemcpy(v3[k], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(c.o_mass, mass + base, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.o_tint, tint + base, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.o_type, type + base, sizeof(int) * n, hipMemcpyHostToDevice));
+    galic::OrbitArgs A = orbit_args(c, n);
+    if(fields) {
+      HIPCHK(hipMemsetAsync(c.slots, 0, per * n, c.stream));
+      A.slots = (double *)c.slots;
+    } else {
+      // integrate without binning into memory the caller wants: bin into a scratch Loc
+      A.loc = c.locA_d;
+    }
+    CHK(launch_orbit64(c, A));
+    if(!fields) HIPCHK(hipMemsetAsync(c.locA_d, 0, sizeof(double) * GALIC_NTYPES * galic::kFields * c.Ngrid, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if(fields) HIPCHK(hipMemcpy(fields + (size_t)base * galic::kFields * c.Ngrid, c.slots, per * n, hipMemcpyDeviceToHost));
+    if(fields) HIPCHK(hipMemset(c.slots, 0, per * n));
+    if(t) HIPCHK(hipMemcpy(t + base, c.o_tall, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if(orbits) HIPCHK(hipMemcpy(orbits + base, c.o_norbit, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if(steps) HIPCHK(hipMemcpy(steps + base, c.o_steps, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    if(e0) HIPCHK(hipMemcpy(e0 + base, c.o_e0, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if(e1) HIPCHK(hipMemcpy(e1 + base, c.o_e1, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  return GALIC_OK;
+}
+
+int galic_timings(galic_ctx *ctx, double *out7) {
+  if(!ctx || !out7) return set_err(GALIC_ERR_ARG, "null argument");
+  galic::Ctx &c = ctx->c;
+  out7[0] = c.last_orbit_ms;
+  out7[1] = c.last_eval_ms;
+  out7[2] = c.last_fit_ms;
+  out7[3] = c.acc_orbit_ms;
+  out7[4] = (double)c.acc_orbit_launches;
+  out7[5] = (double)c.acc_orbits;
+  out7[6] = (double)c.acc_steps;
+  return GALIC_OK;
+}
+
+int galic_reset_timings(galic_ctx *ctx) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null argument");
+  galic::Ctx &c = ctx->c;
+  c.acc_orbit_ms = 0;
+  c.acc_orbit_launches = 0;
+  c.acc_orbits = 0;
+  c.acc_steps = 0;
+  return GALIC_OK;
+}
+
+int galic_bind(galic_ctx *ctx) {
+  g_bound = ctx;
+  return GALIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// galic_ctx.h -- device-resident state behind the C ABI.
+//
+// The reference keeps this state in globals (allvars.h:308-407, 844-868) and in
+// per-rank copies that MPI_Allreduce sums (main.c:259-285).  On one GPU every
+// grid lives in HBM once.  Layouts:
+//   particles      SoA arrays of length N (pos/vel split per component)
+//   Loc grids      [type][field][cell], field = m, r, t, q, p (orbit-averaged)
+//                  and [type][field][cell], field = r, t, p, q (instantaneous)
+//   stacks         [type][which][Nstack], quadtree levels as STACKOFFSET
+//   orbit slots    [orbit][field][cell], one private response per orbit
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "galic_device.h"
+
+namespace galic {
+
+// stack indices in Ctx::stacks
+enum StackId {
+  ST_MT = 0,   // DGs_MassTarget
+  ST_EMT,      // EGs_MassTarget
+  ST_ER,       // EGs_EgyTarget_r
+  ST_ET,       // EGs_EgyTarget_t
+  ST_EP,       // EGs_EgyTarget_p
+  ST_EQ,       // EGs_EgyTarget_q
+  ST_RM,       // DGs_MassResponse
+  ST_RSR,      // EGs_EgyResponseRS
+  ST_RST,      // EGs_EgyResponseTS
+  ST_RSQ,      // EGs_EgyResponseQS
+  ST_RSP,      // EGs_EgyResponsePS
+  ST_EMR,      // EGs_MassResponse
+  ST_IR,       // EGs_EgyResponse_r
+  ST_IT,       // EGs_EgyResponse_t
+  ST_IP,       // EGs_EgyResponse_p
+  ST_IQ,       // EGs_EgyResponse_q
+  ST_DIST,     // DGs_Distance (type-independent, stored per type for simplicity)
+  ST_COUNT
+};
+
+// per-node weights and residuals used by the incremental merit
+enum WeightId {
+  W_A = 0,  // 1/min(dist,R200/20) on cut-A nodes (density cut), else 0
+  W_SR,     // (Tm/Er)/min(dist,R200/20) on cut-A nodes with Er>0, else 0
+  W_ST,
+  W_SQ,
+  W_SP,
+  W_B,      // 1 on cut-B nodes (cut on EGs_MassResponse), else 0
+  W_COUNT
+};
+
+// orbit-averaged fields (slot order = reference argument order m, r, t, q, p)
+enum { F_M = 0, F_R, F_T, F_Q, F_P };
+// instantaneous fields (add_to_energy_grid order r, t, p, q)
+enum { I_R = 0, I_T, I_P, I_Q };
+
+struct Particles {
+  int n = 0;
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *mass = nullptr, *tint = nullptr, *vesc = nullptr;
+  double *vr2 = nullptr, *vt2 = nullptr, *vp2 = nullptr, *vq2 = nullptr;
+  int *type = nullptr, *orbits = nullptr;
+};
+
+struct Ctx {
+  galic_params p;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevModel M;
+  int FG_Ngrid = 0, Ngrid = 0, Nstack = 0, L = 0, Nbin = 0;
+  bool targets_set[GALIC_NTYPES] = {false, false, false, false};
+  bool cuts_ready = false;
+
+  // tables
+  double *FG_Pot = nullptr, *FG_DPotDR = nullptr, *FG_DPotDz = nullptr;
+  double *FG_Vstream = nullptr;  // [type][FG_Ngrid]
+  double *DG_CellSize = nullptr;
+  float2 *FG_Acc_f = nullptr;
+  float *FG_Vstream_f = nullptr;
+  float *DG_CellSize_f = nullptr;
+
+  // grids
+  double *stacks = nullptr;     // [type][ST_COUNT][Nstack]
+  double *weights = nullptr;    // [type][W_COUNT][Nstack]
+  double *locA = nullptr;       // orbit-averaged Loc [type][5][Ngrid]
+  double *locA_d = nullptr;     // deltas
+  double *locI = nullptr;       // instantaneous Loc [type][4][Ngrid]
+  double *locI_d = nullptr;
+  double *locM = nullptr;       // EG_MassLoc [type][Ngrid]
+  long long *locA_q = nullptr;  // fixed-point forms used by the FP32 path
+  long long *locA_dq = nullptr;
+  long long *locI_q = nullptr;
+  long long *locI_dq = nullptr;
+  double *fit = nullptr;        // [type][16] S, Sdisp_r,t,p,q, Srelfac, Srelfac_count, Sall, raw sums..., firstrun
+  Particles P;
+
+  // orbit slots (FP64 path: double; FP32 path: float)
+  void *slots = nullptr;
+  size_t slot_bytes = 0;
+  int max_slot_orbits = 0;
+
+  // per-orbit scratch
+  int max_orbits = 0;
+  int *o_pidx = nullptr, *o_type = nullptr;
+  double *o_x = nullptr, *o_y = nullptr, *o_z = nullptr;
+  double *o_vx = nullptr, *o_vy = nullptr, *o_vz = nullptr;
+  double *o_mass = nullptr, *o_tint = nullptr;
+  double *o_tall = nullptr, *o_e0 = nullptr, *o_e1 = nullptr;
+  int *o_norbit = nullptr, *o_status = nullptr;
+  long long *o_steps = nullptr;
+  int4 *o_bbox = nullptr;
+  unsigned int *counter = nullptr;
+
+  // per-trial scratch
+  int max_trials = 0;
+  int *t_idx = nullptr;
+  double *t_v = nullptr;  // [3][max_trials]
+  int *cutB_of = nullptr; // [type][Ngrid]
+  int *t_accept = nullptr;
+  double *t_dF = nullptr;
+  int *t_best = nullptr;
+
+  // timing of the last launches (ms), measured with HIP events on ctx stream
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_orbit_ms = 0, last_eval_ms = 0, last_fit_ms = 0;
+  double acc_orbit_ms = 0;
+  long long acc_orbit_launches = 0;
+  long long acc_orbits = 0, acc_steps = 0;
+  double fp32_scale[5];  // fixed-point scales per field (orbit-averaged)
+  double fp32_iscale;    // fixed-point scale of the instantaneous fields
+};
+
+}  // namespace galic

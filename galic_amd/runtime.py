@@ -1,0 +1,152 @@
+"""Host driver over the C ABI: one model resident on one GPU.
+
+Mirrors the reference optimiser's control flow (main.c:83-110): an initial
+response pass, then per iteration a number of reps, each a batch of
+independent trials followed by a commit and a global fit.  All arithmetic runs
+in libgalic_hip.so on the GPU; this module only moves arrays across the ABI.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, dptr, iptr, lptr
+
+
+class GalicGPU:
+    """A model on one GPU (the reference's globals, device resident)."""
+
+    def __init__(self, w, device=0):
+        self.w = w
+        self.lib = _abi.lib()
+        self.h = ctypes.c_void_p()
+        check(self.lib.galic_create(ctypes.byref(w.params), device, ctypes.byref(self.h)))
+        self.L = w.params.DG_MaxLevel
+        self.ngrid = 1 << (2 * self.L)
+        self.nstack = ((1 << (2 + 2 * self.L)) - 1) // 3
+        check(self.lib.galic_set_force_tables(self.h, dptr(w.pot), dptr(w.dpotdr), dptr(w.dpotdz)))
+        for t in (1, 2, 3):
+            check(self.lib.galic_set_vstream(self.h, t, dptr(np.ascontiguousarray(w.vstream[t]))))
+        for t, st in w.targets.items():
+            st = [np.ascontiguousarray(s, dtype=np.float64) for s in st]
+            check(self.lib.galic_set_targets(self.h, t, *[dptr(s) for s in st]))
+        self.set_particles(w.pos, w.vel)
+
+    def set_particles(self, pos, vel):
+        w = self.w
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        check(self.lib.galic_set_particles(self.h, len(w.mass), dptr(pos), dptr(vel), dptr(w.mass), dptr(w.tint),
+                                           iptr(w.ptype), dptr(w.vesc)))
+
+    def close(self):
+        if self.h:
+            self.lib.galic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- FP64 path --
+    def calc_all_response_fields(self):
+        check(self.lib.galic_calc_all_response_fields(self.h))
+
+    def rep(self, idx, vtry):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        vtry = np.ascontiguousarray(vtry, dtype=np.float64)
+        acc = np.zeros(len(idx), dtype=np.int32)
+        dF = np.zeros(len(idx))
+        check(self.lib.galic_rep(self.h, len(idx), iptr(idx), dptr(vtry), iptr(acc), dptr(dF)))
+        return acc, dF
+
+    # ------------------------------------------------------------- FP32 path --
+    def fp32_calc_all_response_fields(self):
+        check(self.lib.galic_fp32_calc_all_response_fields(self.h))
+
+    def fp32_rep(self, idx, k=1, vtry=None, sigma_frac=0.1, seed=1):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        acc = np.zeros(len(idx), dtype=np.int32)
+        vt = None if vtry is None else np.ascontiguousarray(vtry, dtype=np.float64)
+        check(self.lib.galic_fp32_rep(self.h, len(idx), iptr(idx), k, dptr(vt), sigma_frac, seed, iptr(acc)))
+        return acc
+
+    # ---------------------------------------------------------------- reads --
+    def stack(self, t, which):
+        out = np.zeros(self.nstack)
+        check(self.lib.galic_get_stack(self.h, t, which, dptr(out)))
+        return out
+
+    def fit(self, t):
+        out = np.zeros(8)
+        check(self.lib.galic_get_fit(self.h, t, dptr(out)))
+        return out
+
+    def particles(self):
+        n = len(self.w.mass)
+        vel = np.zeros((n, 3))
+        orb = np.zeros(n, dtype=np.int32)
+        v = [np.zeros(n) for _ in range(4)]
+        check(self.lib.galic_get_particles(self.h, dptr(vel), iptr(orb), *[dptr(x) for x in v]))
+        return vel, orb, np.stack(v, -1)
+
+    def orbit_responses(self, pos, vel, mass, tint, ptype, fields=True):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        n = len(pos)
+        mass = np.ascontiguousarray(np.broadcast_to(mass, (n,)), dtype=np.float64)
+        tint = np.ascontiguousarray(np.broadcast_to(tint, (n,)), dtype=np.float64)
+        ptype = np.ascontiguousarray(np.broadcast_to(ptype, (n,)), dtype=np.int32)
+        f = np.zeros((n, 5, self.ngrid)) if fields else None
+        t = np.zeros(n)
+        orb = np.zeros(n, dtype=np.int32)
+        steps = np.zeros(n, dtype=np.int64)
+        e0 = np.zeros(n)
+        e1 = np.zeros(n)
+        check(self.lib.galic_orbit_responses(self.h, n, dptr(pos), dptr(vel), dptr(mass), dptr(tint), iptr(ptype),
+                                             dptr(f), dptr(t), iptr(orb), lptr(steps), dptr(e0), dptr(e1)))
+        return dict(fields=f, t=t, orbits=orb, steps=steps, e0=e0, e1=e1)
+
+    def field_at(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        acc = np.zeros_like(pos)
+        pot = np.zeros(len(pos))
+        check(self.lib.galic_field_at(self.h, len(pos), dptr(pos), dptr(acc), dptr(pot)))
+        return acc, pot
+
+    def timings(self):
+        out = np.zeros(7)
+        check(self.lib.galic_timings(self.h, dptr(out)))
+        keys = ["last_orbit_ms", "last_eval_ms", "last_fit_ms", "orbit_ms", "orbit_launches", "orbits", "steps"]
+        return dict(zip(keys, out.tolist()))
+
+    def reset_timings(self):
+        check(self.lib.galic_reset_timings(self.h))
+
+    def bind(self):
+        check(self.lib.galic_bind(self.h))
+
+
+def tree_forces(params, spos, smass, tpos, precision=0, device=0):
+    spos = np.ascontiguousarray(spos, dtype=np.float64).reshape(-1, 3)
+    smass = np.ascontiguousarray(smass, dtype=np.float64)
+    tpos = np.ascontiguousarray(tpos, dtype=np.float64).reshape(-1, 3)
+    acc = np.zeros_like(tpos)
+    pot = np.zeros(len(tpos))
+    nint = np.zeros(1, dtype=np.int64)
+    check(_abi.lib().galic_tree_forces(ctypes.byref(params), device, len(spos), dptr(spos), dptr(smass), len(tpos),
+                                       dptr(tpos), dptr(acc), dptr(pot), precision, lptr(nint)))
+    return acc, pot, int(nint[0])
+
+
+def tree_fill(params, spos, smass, precision=0, device=0):
+    spos = np.ascontiguousarray(spos, dtype=np.float64).reshape(-1, 3)
+    smass = np.ascontiguousarray(smass, dtype=np.float64)
+    n2 = params.FG_Nbin * params.FG_Nbin
+    pot, dr, dz = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+    nint = np.zeros(1, dtype=np.int64)
+    check(_abi.lib().galic_tree_fill(ctypes.byref(params), device, len(spos), dptr(spos), dptr(smass), dptr(pot),
+                                     dptr(dr), dptr(dz), precision, lptr(nint)))
+    return pot, dr, dz, int(nint[0])

@@ -1,0 +1,169 @@
+/*
+ * galic_hip.h -- C ABI of the MI355X (gfx950) implementation of GALIC's
+ * equilibrium hot path.  Plain pointers and sizes only; every entry point that
+ * can fail returns 0 on success or a negative GALIC_ERR_* code, with a message
+ * available from galic_last_error().
+ *
+ * Layouts (all row-major, as in the reference):
+ *   force tables   FG_Nbin*FG_Nbin doubles, index iz*FG_Nbin + iR   (grid.c:78-80,140-146)
+ *   response grids 2^L * 2^L doubles,      index iz*2^L + iR        (orbit_response.c:283-286)
+ *   stacks         (4^(L+1)-1)/3 doubles,  node (l,i,j) at GALIC_STACKOFFSET(l,i,j) (allvars.h:160)
+ *   positions and velocities of n particles: 3*n doubles, particle-major (x0,y0,z0,x1,...)
+ *   orbit fields   per orbit 5 grids in the order m, r, t, q, p of
+ *                  produce_orbit_response_field_mod's arguments (orbit_response.c:155-157)
+ * Component types are numbered as in the reference: 1 halo, 2 disk, 3 bulge.
+ */
+#ifndef GALIC_HIP_H
+#define GALIC_HIP_H
+
+#include "galic_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALIC_OK 0
+#define GALIC_ERR_ARG (-1)     /* bad argument or size */
+#define GALIC_ERR_HIP (-2)     /* HIP runtime error (message has the HIP error string) */
+#define GALIC_ERR_STEPCAP (-3) /* an orbit exceeded MaxStepsPerOrbit; the reference exits (orbit_response.c:449-456) */
+#define GALIC_ERR_STATE (-4)   /* call out of order (e.g. a rep before the initial response pass) */
+#define GALIC_ERR_NOMEM (-5)   /* device allocation failed */
+
+typedef struct galic_ctx galic_ctx;
+
+const char *galic_last_error(void);
+int galic_version(void);
+
+/* ---- set-up geometry (host) ---- */
+
+/* Replaces the geometry part of forcedensitygrid_create (grid.c:329-480): fills
+ * p->Rmax and the DG/FG/EG Rmin and Fac from Halo_A, DG_MaxLevel, FG_Nbin,
+ * EG_MaxLevel and the two enclosed-mass fractions. */
+int galic_grid_create(galic_params *p, double outer_frac, double inner_frac);
+
+/* DG_CellSize and DG_CellVol (grid.c:830-850; 4^L each) and the DGs_Distance
+ * stack (grid.c:385-416; Nstack).  Any output pointer may be NULL. */
+int galic_density_geometry(const galic_params *p, double *cellsize, double *cellvol, double *distance);
+
+/* ---- context: one model resident on one GPU ---- */
+
+/* Allocates the grids for p on HIP device `device`.  p->DG_MaxLevel must equal
+ * p->EG_MaxLevel: the reference bins the energy fields with density-grid
+ * indices (orbit_response.c:283-319) and is only consistent when they match. */
+int galic_create(const galic_params *p, int device, galic_ctx **out);
+int galic_destroy(galic_ctx *ctx);
+
+/* Force and potential tables, FG_DPotDR/FG_DPotDz/FG_Pot (grid.c:204-216). */
+int galic_set_force_tables(galic_ctx *ctx, const double *pot, const double *dpotdr, const double *dpotdz);
+/* Streaming velocity table FG_Vstream[type] (disp_fields.c:118-132); NULL means zero. */
+int galic_set_vstream(galic_ctx *ctx, int type, const double *vstream);
+/* Target stacks of one component: DGs_MassTarget, EGs_MassTarget and
+ * EGs_EgyTarget_{r,t,p,q} (grid.c:830-877, 941-1076). */
+int galic_set_targets(galic_ctx *ctx, int type, const double *dgs_mass_target, const double *egs_mass_target,
+                      const double *egs_r, const double *egs_t, const double *egs_p, const double *egs_q);
+/* Particle state P[] (allvars.h:844-868): positions, velocities, mass, Tint,
+ * type and Vesc.  Copied to device SoA arrays. */
+int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *vel, const double *mass,
+                        const double *tint, const int *type, const double *vesc);
+/* Reads back P[].Vel, P[].Orbits and P[].vr2..vq2; any pointer may be NULL. */
+int galic_get_particles(galic_ctx *ctx, double *vel, int *orbits, double *vr2, double *vt2, double *vp2,
+                        double *vq2);
+
+/* ---- optimiser, plain FP64 path (the drop-in) ---- */
+
+/* calc_all_response_fields (main.c:151-246): one orbit per particle, Loc grids,
+ * instantaneous grids and the first calc_global_fit. */
+int galic_calc_all_response_fields(galic_ctx *ctx);
+
+/* One rep of the optimisation loop (main.c:96-100): init_updates, optimize()
+ * for each particle idx[i] with trial velocity vtry[3*i..3*i+2] in place of the
+ * GSL draw of main.c:407-435, then commit_updates.  accept[i] = 1 if taken;
+ * dF[i] (optional) = F_try - F_base. */
+int galic_rep(galic_ctx *ctx, int ntrial, const int *idx, const double *vtry, int *accept, double *dF);
+
+/* Stacks: which = 0 DGs_MassResponse, 1 EGs_MassResponse, 2..5 EGs_EgyResponse_{r,t,p,q},
+ * 6..9 EGs_EgyResponse{R,T,Q,P}S, 10 DGs_MassTarget.  out has Nstack doubles. */
+int galic_get_stack(galic_ctx *ctx, int type, int which, double *out);
+
+/* calc_global_fit results of one type: out[0..7] = S, Sdisp_r, Sdisp_t, Sdisp_p,
+ * Sdisp_q, Srelfac, Srelfac_count, Sall (main.c:294-359, 1270). */
+int galic_get_fit(galic_ctx *ctx, int type, double *out8);
+
+/* Orbit responses of many independent orbits (produce_orbit_response_field_mod,
+ * orbit_response.c:155-491, once per orbit): orbit o starts at pos[3*o..] with
+ * velocity vel[3*o..], mass[o], timespan tint[o] and component type[o].
+ * fields (optional) receives 5*4^L doubles per orbit (m, r, t, q, p), scaled by
+ * mass/t as the reference returns them.  t, orbits, steps, e0, e1 are optional
+ * per-orbit outputs (integration time, pericentre passages, leapfrog steps,
+ * start and end energy). */
+int galic_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const double *vel, const double *mass,
+                          const double *tint, const int *type, double *fields, double *t, int *orbits,
+                          long long *steps, double *e0, double *e1);
+
+/* Lookups at n points (pos: 3*n): acceleration (3*n) and potential (n) of
+ * forcegrid_get_acceleration / forcegrid_get_potential (grid.c:59-197). */
+int galic_field_at(galic_ctx *ctx, int n, const double *pos, double *acc, double *pot);
+
+/* ---- FP32 batched path ---- */
+
+/* Initial response pass of the FP32 path: orbits in FP32, grids as 64-bit
+ * fixed point (bit-identical run to run), merit sums in FP64. */
+int galic_fp32_calc_all_response_fields(galic_ctx *ctx);
+
+/* One FP32 rep over ntrial particles idx[] with k trial velocities each
+ * (vtry: 3*k*ntrial, particle-major, trial-minor).  The best trial of a
+ * particle is taken when it lowers the merit.  accept[i] = chosen trial + 1,
+ * or 0.  vtry == NULL draws the trials on the device: Gaussian perturbations of
+ * sigma_frac times the local dispersion targets, counter-based RNG keyed by seed. */
+int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int k, const double *vtry, double sigma_frac,
+                   unsigned long long seed, int *accept);
+
+/* ---- Barnes-Hut tree fill of the force tables ---- */
+
+/* Accelerations and potentials (times G) at ntarget points from nsrc sources
+ * (forcetree.c:41-722, forcetree_walk.c:12-222, gravtree.c:455-463).
+ * precision 0 = FP64 walk, 1 = FP32 walk with FP64 moments.  ninteractions
+ * (optional) receives the interaction count. */
+int galic_tree_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                      int ntarget, const double *tpos, double *acc, double *pot, int precision,
+                      long long *ninteractions);
+
+/* forcedensitygrid_calculate's table fill (grid.c:680-750): targets at
+ * (R_j, z_k) in FG_SECTIONS azimuthal sections, projected on R and averaged.
+ * Writes FG_Nbin^2 doubles each of pot, dpotdr, dpotdz. */
+int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                    double *pot, double *dpotdr, double *dpotdz, int precision, long long *ninteractions);
+
+/* ---- measurement ---- */
+
+/* Device time of the last launches, measured with HIP events on the context's
+ * stream: out[0] orbit kernel ms (last launch), out[1] merit kernel ms,
+ * out[2] fit ms, out[3] accumulated orbit-kernel ms, out[4] orbit-kernel
+ * launches, out[5] orbits integrated, out[6] leapfrog steps integrated. */
+int galic_timings(galic_ctx *ctx, double *out7);
+int galic_reset_timings(galic_ctx *ctx);
+
+/* ---- drop-in entry points with the reference signatures ----
+ * They act on the context made current by galic_bind(), as the reference's
+ * versions act on its globals. */
+int galic_bind(galic_ctx *ctx);
+
+/* orbit_response.c:155-491 (proto.h:145) */
+double produce_orbit_response_field_mod(double *pos, double *vel, int id, double *mfield, double *egyfield_r,
+                                        double *egyfield_t, double *egyfield_q, double *egyfield_p, double mass,
+                                        double timespan, int *orbitstaken, int type);
+/* orbit_response.c:33-151 (proto.h:143): the mass field only */
+double produce_orbit_response_field(double *pos, double *vel, int id, double *mfield, double mass, double timespan,
+                                    int *orbitstaken);
+/* grid.c:59-197 (proto.h:100-103) */
+double forcegrid_get_potential(double *pos);
+double forcegrid_get_escape_speed(double *pos);
+void forcegrid_get_acceleration(double *pos, double *acc);
+/* grid.c:1247-1265 (proto.h:120) */
+void smooth_stack(double *data, int maxlevel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GALIC_HIP_H */

@@ -1,0 +1,144 @@
+"""Parity of the plain FP64 GPU path with the CPU oracle (same inputs).
+
+Tolerance: 1e-10 relative.  Grids are compared cell by cell against
+1e-10 * (|oracle| + 1e-6 * max|oracle|): the floor covers cells that are
+differences of much larger contributions (a delta of near-equal orbits), where
+a single rounding of the larger terms exceeds 1e-10 of the small result.
+Integer outputs (orbit counts, step counts, accept flags) must be equal.
+"""
+import numpy as np
+import pytest
+
+from galic_amd import model
+from galic_amd.runtime import GalicGPU
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def close(a, b, rtol=RTOL, floor=1e-6):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    scale = np.abs(b) + floor * (np.abs(b).max() if b.size else 0.0)
+    bad = np.abs(a - b) > rtol * scale
+    return not bad.any(), (np.abs(a - b) / np.where(scale > 0, scale, 1)).max() if b.size else 0.0
+
+
+def workload(name, n, **kw):
+    kw.setdefault("max_steps", 2000000)
+    cfg = model.config(name, **kw)
+    return model.make_workload(cfg, n=n, target_samples=max(100000, 20 * n))
+
+
+@pytest.fixture(scope="module")
+def halo():
+    return workload("halo-1e5", 3000)
+
+
+@pytest.fixture(scope="module")
+def halodisk():
+    return workload("halo-disk-1e6", 3000)
+
+
+@pytest.mark.parametrize("wname", ["halo", "halodisk"])
+def test_orbit_responses_match_oracle(wname, request, oracle_mod):
+    w = request.getfixturevalue(wname)
+    print(f"[gpu] orbit responses {wname} n=64", flush=True)
+    o = oracle_mod.Oracle(w, set_particles=False)
+    g = GalicGPU(w)
+    sel = np.arange(0, w.n, max(1, w.n // 64))[:64]
+    r = g.orbit_responses(w.pos[sel], w.vel[sel], w.mass[sel], w.tint[sel], w.ptype[sel])
+    nbad = 0
+    for k, i in enumerate(sel):
+        ro = o.orbit(w.pos[i], w.vel[i], w.mass[i], w.tint[i], w.ptype[i])
+        assert r["steps"][k] == ro["steps"], (i, r["steps"][k], ro["steps"])
+        assert r["orbits"][k] == ro["orbits"]
+        assert r["t"][k] == pytest.approx(ro["t"], rel=1e-14)
+        ok, worst = close(r["fields"][k], ro["fields"])
+        if not ok:
+            nbad += 1
+            print("orbit", i, "worst rel", worst)
+        assert r["e1"][k] == pytest.approx(ro["e1"], rel=1e-10)
+    assert nbad == 0
+    g.close()
+
+
+@pytest.mark.parametrize("wname", ["halo", "halodisk"])
+def test_field_lookup_matches_oracle(wname, request, oracle_mod):
+    w = request.getfixturevalue(wname)
+    o = oracle_mod.Oracle(w, set_particles=False)
+    g = GalicGPU(w)
+    acc, pot = g.field_at(w.pos[:500])
+    for i in range(0, 500, 7):
+        np.testing.assert_allclose(acc[i], o.accel(w.pos[i]), rtol=1e-13, atol=1e-13 * np.abs(acc[i]).max())
+        assert pot[i] == pytest.approx(o.potential(w.pos[i]), rel=1e-13)
+    g.close()
+
+
+@pytest.mark.parametrize("wname", ["halo", "halodisk"])
+def test_optimiser_history_matches_oracle(wname, request, oracle_mod):
+    """calc_all_response_fields, then reps of trials: grids, accept flags and
+    the Sall history agree with the oracle."""
+    w = request.getfixturevalue(wname)
+    print(f"[gpu] optimiser history {wname}", flush=True)
+    o = oracle_mod.Oracle(w)
+    g = GalicGPU(w)
+    o.calc_all_response_fields(8)
+    g.calc_all_response_fields()
+    types = w.types
+    for t in types:
+        for which in range(10):
+            ok, worst = close(g.stack(t, which), o.stack(t, which))
+            assert ok, (t, which, worst)
+        np.testing.assert_allclose(g.fit(t), o.fit(t), rtol=RTOL)
+    perm = model.permutation(w, 1)
+    ntr = w.n // 4
+    for rep in range(3):
+        idx = perm[rep * ntr:(rep + 1) * ntr]
+        v = model.trial_velocities(w, idx, 1, rep)
+        acc_o, dF_o = o.rep(idx, v, 8)
+        acc_g, dF_g = g.rep(idx, v)
+        flips = np.nonzero(acc_o != acc_g)[0]
+        assert len(flips) == 0, (rep, flips[:5], dF_o[flips[:5]], dF_g[flips[:5]])
+        scale = np.abs(dF_o).max()
+        np.testing.assert_allclose(dF_g, dF_o, rtol=1e-8, atol=1e-9 * scale)
+        for t in types:
+            for which in range(10):
+                ok, worst = close(g.stack(t, which), o.stack(t, which))
+                assert ok, (rep, t, which, worst)
+            np.testing.assert_allclose(g.fit(t), o.fit(t), rtol=RTOL)
+    vel_g, orb_g, comp_g = g.particles()
+    vel_o, orb_o, comp_o = o.particles()
+    np.testing.assert_array_equal(vel_g, vel_o)
+    np.testing.assert_array_equal(orb_g, orb_o)
+    np.testing.assert_allclose(comp_g, comp_o, rtol=1e-12)
+    g.close()
+
+
+def test_dropin_orbit_signature(halo, oracle_mod):
+    """produce_orbit_response_field_mod with the reference signature (orbit_response.c:155)."""
+    import ctypes
+    w = halo
+    g = GalicGPU(w)
+    g.bind()
+    lib = g.lib
+    ng = g.ngrid
+    bufs = [np.zeros(ng) for _ in range(5)]
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn = lib.produce_orbit_response_field_mod
+    fn.restype = ctypes.c_double
+    fn.argtypes = [dp, dp, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                   ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    norb = ctypes.c_int(0)
+    pos = np.ascontiguousarray(w.pos[5])
+    vel = np.ascontiguousarray(w.vel[5])
+    t = fn(pos.ctypes.data_as(dp), vel.ctypes.data_as(dp), 5, *[b.ctypes.data_as(dp) for b in bufs],
+           w.mass[5], w.tint[5], ctypes.byref(norb), 1)
+    o = oracle_mod.Oracle(w, set_particles=False)
+    ro = o.orbit(w.pos[5], w.vel[5], w.mass[5], w.tint[5], 1)
+    assert t == pytest.approx(ro["t"], rel=1e-14)
+    assert norb.value == ro["orbits"]
+    ok, worst = close(np.stack(bufs), ro["fields"])
+    assert ok, worst
+    g.close()
