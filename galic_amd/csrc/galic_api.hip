@@ -13,6 +13,7 @@
 
 #include "../../include/galic_hip.h"
 #include "galic_ctx.h"
+#include "galic_fast.h"
 #include "galic_kernels.h"
 
 struct galic_ctx {
@@ -321,9 +322,104 @@ int commit(galic_ctx *ctx) {
   return global_fit(ctx);
 }
 
+
+galic::FastModel fast_model(galic::Ctx &c) {
+  galic::FastModel F;
+  memset(&F, 0, sizeof(F));
+  const galic_params &p = c.p;
+  F.use_table = c.M.use_table;
+  F.add_halo = c.M.add_halo;
+  F.add_bulge = c.M.add_bulge;
+  for(int t = 0; t < GALIC_NTYPES; t++) F.tovs[t] = p.TypeOfVelocityStructure[t];
+  F.fg_nbin = p.FG_Nbin;
+  F.dg_nbin = c.Nbin;
+  F.fg_invRmin = (float)(1.0 / p.FG_Rmin);
+  F.fg_invlog2Fac = (float)(1.0 / log2(p.FG_Fac));
+  F.dg_invRmin = (float)(1.0 / p.DG_Rmin);
+  F.dg_invlog2Fac = (float)(1.0 / log2(p.DG_Fac));
+  F.halo_GM = (float)(p.G * p.Halo_Mass);
+  F.halo_a = (float)p.Halo_A;
+  F.halo_eps = (float)(1.0e-6 * p.Halo_A);
+  F.offgrid_fac = (float)c.M.halo_offgrid_fac;
+  F.bulge_GM = (float)(p.G * p.Bulge_Mass);
+  F.bulge_a = (float)p.Bulge_A;
+  F.bulge_eps = (float)(1.0e-6 * p.Bulge_A);
+  F.fo_v200 = (float)(p.TimeStepFactorOrbit * p.V200);
+  F.fc = (float)p.TimeStepFactorCellCross;
+  F.max_steps = c.M.max_steps;
+  F.acc = c.FG_Acc_f;
+  for(int t = 0; t < GALIC_NTYPES; t++) F.vstream[t] = c.FG_Vstream_f + (size_t)t * c.FG_Ngrid;
+  F.cellsize = c.DG_CellSize_f;
+  for(int f = 0; f < 5; f++) F.qscale[f] = c.fp32_scale[f];
+  return F;
+}
+
+galic::OrbitArgs32 orbit_args32(galic::Ctx &c, int n) {
+  galic::OrbitArgs32 A;
+  memset(&A, 0, sizeof(A));
+  A.n = n;
+  A.counter = c.counter;
+  A.ox = c.o_x; A.oy = c.o_y; A.oz = c.o_z;
+  A.ovx = c.o_vx; A.ovy = c.o_vy; A.ovz = c.o_vz;
+  A.omass = c.o_mass; A.otint = c.o_tint; A.otype = c.o_type;
+  A.tall = c.o_tall; A.norbit = c.o_norbit; A.status = c.o_status; A.steps = c.o_steps; A.bbox = c.o_bbox;
+  A.slot_stride = (size_t)galic::kFields * c.Ngrid;
+  return A;
+}
+
+int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
+  if(A.n <= 0) return GALIC_OK;
+  HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  const int bs = 256;
+  hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs)), dim3(bs), 0, c.stream, fast_model(c), A);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev1, c.stream));
+  HIPCHK(hipEventSynchronize(c.ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  c.last_orbit_ms = ms;
+  c.acc_orbit_ms += ms;
+  c.acc_orbit_launches++;
+  c.acc_orbits += A.n;
+  std::vector<int> st(A.n);
+  std::vector<long long> steps(A.n);
+  HIPCHK(hipMemcpy(st.data(), A.status, sizeof(int) * A.n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(steps.data(), A.steps, sizeof(long long) * A.n, hipMemcpyDeviceToHost));
+  long long tot = 0;
+  for(int o = 0; o < A.n; o++) {
+    tot += steps[o];
+    if(st[o] == GALIC_ERR_STEPCAP)
+      return set_err(GALIC_ERR_STEPCAP, "orbit %d exceeded the step cap of %lld steps", o, (long long)c.M.max_steps);
+    if(st[o] != 0) return set_err(GALIC_ERR_ARG, "orbit %d has a non-positive or non-finite timestep", o);
+  }
+  c.acc_steps += tot;
+  return GALIC_OK;
+}
+
+// Loc += delta in fixed point, and the FP64 copies the global fit reads.
+int commit_q(galic::Ctx &c) {
+  size_t na = (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid, ni = (size_t)GALIC_NTYPES * 4 * c.Ngrid;
+  hipLaunchKernelGGL(galic::k_commit_q, dim3(nblocks(na, 256)), dim3(256), 0, c.stream, c.locA_q, c.locA_dq, c.locA,
+                     (size_t)c.Ngrid, galic::kFields, (const double *)c.qinv, na);
+  hipLaunchKernelGGL(galic::k_commit_q, dim3(nblocks(ni, 256)), dim3(256), 0, c.stream, c.locI_q, c.locI_dq, c.locI,
+                     (size_t)c.Ngrid, 4, (const double *)(c.qinv + 5), ni);
+  HIPCHK(hipGetLastError());
+  return GALIC_OK;
+}
+
+size_t slot_budget(galic::Ctx &c) {
+  size_t fr = 0, tot = 0;
+  if(hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
+  size_t cap = (size_t)24 << 30;
+  size_t have = c.slot_bytes + fr / 2;
+  return std::min(cap, have);
+}
+
 }  // namespace
 
 namespace galic {
+int set_error(int code, const char *msg) { return set_err(code, "%s", msg); }
 galic_ctx *bound_ctx() { return g_bound; }
 Ctx &ctx_state(galic_ctx *ctx) { return ctx->c; }
 }  // namespace galic
@@ -440,6 +536,9 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.locA_dq, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
   CHK(dalloc(&c.locI_q, (size_t)GALIC_NTYPES * 4 * c.Ngrid));
   CHK(dalloc(&c.locI_dq, (size_t)GALIC_NTYPES * 4 * c.Ngrid));
+  CHK(dalloc(&c.locM_q, (size_t)GALIC_NTYPES * c.Ngrid));
+  CHK(dalloc(&c.qinv, 9));
+  CHK(dalloc(&c.tovs_dev, GALIC_NTYPES));
   CHK(dalloc(&c.fit, (size_t)GALIC_NTYPES * 16));
   CHK(dalloc(&c.counter, 1));
 
@@ -488,6 +587,15 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   c.fp32_scale[0] = ldexp(1.0, 40);
   for(int f = 1; f < 5; f++) c.fp32_scale[f] = ldexp(1.0, 40) / (p->V200 * p->V200);
   c.fp32_iscale = ldexp(1.0, 40) / (p->V200 * p->V200);
+  {
+    double inv[9];
+    for(int f = 0; f < 5; f++) inv[f] = 1.0 / c.fp32_scale[f];
+    for(int f = 0; f < 4; f++) inv[5 + f] = 1.0 / c.fp32_iscale;
+    HIPCHK(hipMemcpy(c.qinv, inv, sizeof(inv), hipMemcpyHostToDevice));
+    int tv[GALIC_NTYPES];
+    for(int t = 0; t < GALIC_NTYPES; t++) tv[t] = p->TypeOfVelocityStructure[t];
+    HIPCHK(hipMemcpy(c.tovs_dev, tv, sizeof(tv), hipMemcpyHostToDevice));
+  }
   *out = ctx;
   return GALIC_OK;
 }
@@ -502,11 +610,11 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
-  dfree(c.fit); dfree(c.counter);
+  dfree(c.fit); dfree(c.counter); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk);
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
-  dfree(P.type); dfree(P.orbits);
+  dfree(P.type); dfree(P.orbits); dfree(P.disp);
   dfree(c.o_pidx); dfree(c.o_type); dfree(c.o_x); dfree(c.o_y); dfree(c.o_z);
   dfree(c.o_vx); dfree(c.o_vy); dfree(c.o_vz); dfree(c.o_mass); dfree(c.o_tint);
   dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
@@ -583,7 +691,7 @@ int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
-  dfree(P.type); dfree(P.orbits);
+  dfree(P.type); dfree(P.orbits); dfree(P.disp);
   P.n = n;
   CHK(dalloc(&P.x, n)); CHK(dalloc(&P.y, n)); CHK(dalloc(&P.z, n));
   CHK(dalloc(&P.vx, n)); CHK(dalloc(&P.vy, n)); CHK(dalloc(&P.vz, n));
@@ -674,9 +782,9 @@ int galic_rep(galic_ctx *ctx, int ntrial, const int *idx, const double *vtry, in
   for(int i = 0; i < ntrial; i++)
     if(idx[i] < 0 || idx[i] >= c.P.n) return set_err(GALIC_ERR_ARG, "trial %d: particle index %d out of range", i, idx[i]);
   const size_t slot_bytes_per_orbit = sizeof(double) * galic::kFields * c.Ngrid;
-  // trials per chunk: 2 slots each, about 2 GiB of slots at most
-  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)ntrial, (2ull << 30) / (2 * slot_bytes_per_orbit)));
-  chunk = std::max(1, std::min(chunk, 8192));
+  // trials per chunk: 2 slots each, within the slot memory budget
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ntrial, 1),
+                                                        slot_budget(c) / (2 * slot_bytes_per_orbit)));
   CHK(alloc_trial_scratch(c, std::max(ntrial, 1)));
   CHK(alloc_orbit_scratch(c, 2 * chunk));
   CHK(alloc_slots(c, 2 * chunk, slot_bytes_per_orbit));
@@ -813,6 +921,152 @@ int galic_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const d
     if(e0) HIPCHK(hipMemcpy(e0 + base, c.o_e0, sizeof(double) * n, hipMemcpyDeviceToHost));
     if(e1) HIPCHK(hipMemcpy(e1 + base, c.o_e1, sizeof(double) * n, hipMemcpyDeviceToHost));
   }
+  return GALIC_OK;
+}
+
+int galic_set_dispersions(galic_ctx *ctx, int n, const double *disp4) {
+  if(!ctx || !disp4) return set_err(GALIC_ERR_ARG, "null argument");
+  galic::Ctx &c = ctx->c;
+  if(n != c.P.n) return set_err(GALIC_ERR_ARG, "dispersions for %d particles, model has %d", n, c.P.n);
+  HIPCHK(hipSetDevice(c.device));
+  dfree(c.P.disp);
+  CHK(dalloc(&c.P.disp, 4 * (size_t)n));
+  HIPCHK(hipMemcpy(c.P.disp, disp4, sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+  return GALIC_OK;
+}
+
+int galic_fp32_calc_all_response_fields(galic_ctx *ctx) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null context");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  const int n = c.P.n;
+  if(n <= 0) return set_err(GALIC_ERR_STATE, "no particles set");
+  for(int t = 1; t < GALIC_NTYPES; t++)
+    if(type_present(c.p, t) && !c.targets_set[t]) return set_err(GALIC_ERR_STATE, "targets of type %d not set", t);
+  CHK(alloc_orbit_scratch(c, n));
+  const size_t na = (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid, ni = (size_t)GALIC_NTYPES * 4 * c.Ngrid,
+               nm = (size_t)GALIC_NTYPES * c.Ngrid;
+  HIPCHK(hipMemsetAsync(c.locA_q, 0, sizeof(long long) * na, c.stream));
+  HIPCHK(hipMemsetAsync(c.locA_dq, 0, sizeof(long long) * na, c.stream));
+  HIPCHK(hipMemsetAsync(c.locI_q, 0, sizeof(long long) * ni, c.stream));
+  HIPCHK(hipMemsetAsync(c.locI_dq, 0, sizeof(long long) * ni, c.stream));
+  HIPCHK(hipMemsetAsync(c.locM_q, 0, sizeof(long long) * nm, c.stream));
+  HIPCHK(hipMemsetAsync(c.fit, 0, sizeof(double) * GALIC_NTYPES * 16, c.stream));
+  c.cuts_ready = false;
+  CHK(build_weights(ctx, false));
+  hipLaunchKernelGGL(galic::k_prepare_all, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.P, orbit_in(c));
+  HIPCHK(hipGetLastError());
+  galic::OrbitArgs32 A = orbit_args32(c, n);
+  A.locq = c.locA_q;
+  CHK(launch_orbit32(c, A));
+  hipLaunchKernelGGL(galic::k_store_orbits, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.o_norbit,
+                     c.P.orbits);
+  hipLaunchKernelGGL(galic::k_inst_init_q, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, c.M, c.P, c.locI_q,
+                     c.locM_q, c.Ngrid, c.fp32_iscale, c.fp32_scale[0]);
+  HIPCHK(hipGetLastError());
+  CHK(commit_q(c));
+  // EG_MassLoc in FP64 from its fixed-point sum (the zero delta leaves it unchanged)
+  hipLaunchKernelGGL(galic::k_commit_q, dim3(nblocks(nm, 256)), dim3(256), 0, c.stream, c.locM_q, c.locA_dq, c.locM,
+                     (size_t)c.Ngrid, 1, (const double *)c.qinv, nm);
+  HIPCHK(hipGetLastError());
+  CHK(global_fit(ctx));
+  ctx->fp32_initial_done = true;
+  ctx->initial_done = false;
+  return GALIC_OK;
+}
+
+int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const double *vtry, double sigma_frac,
+                   unsigned long long seed, int *accept) {
+  if(!ctx || ntrial < 0 || (ntrial > 0 && !idx) || K < 1 || K > 64)
+    return set_err(GALIC_ERR_ARG, "bad FP32 rep arguments (K must be 1..64)");
+  galic::Ctx &c = ctx->c;
+  if(!ctx->fp32_initial_done) return set_err(GALIC_ERR_STATE, "galic_fp32_calc_all_response_fields must run first");
+  if(!vtry && !c.P.disp) return set_err(GALIC_ERR_STATE, "device trial draws need galic_set_dispersions");
+  HIPCHK(hipSetDevice(c.device));
+  for(int i = 0; i < ntrial; i++)
+    if(idx[i] < 0 || idx[i] >= c.P.n) return set_err(GALIC_ERR_ARG, "trial %d: particle index %d out of range", i, idx[i]);
+  CHK(alloc_trial_scratch(c, std::max(ntrial, 1)));
+  size_t nk = (size_t)std::max(ntrial, 1) * K;
+  if(nk > c.max_ktrials) {
+    dfree(c.t_vk);
+    CHK(dalloc(&c.t_vk, 3 * nk));
+    c.max_ktrials = nk;
+  }
+  double *vtx = c.t_vk, *vty = c.t_vk + nk, *vtz = c.t_vk + 2 * nk;
+  if(ntrial > 0) {
+    HIPCHK(hipMemcpy(c.t_idx, idx, sizeof(int) * ntrial, hipMemcpyHostToDevice));
+    if(vtry) {
+      std::vector<double> tv(3 * nk);
+      for(size_t q = 0; q < (size_t)ntrial * K; q++)
+        for(int k = 0; k < 3; k++) tv[k * nk + q] = vtry[3 * q + k];
+      HIPCHK(hipMemcpy(c.t_vk, tv.data(), sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
+    } else {
+      hipLaunchKernelGGL(galic::k_draw_trials, dim3(nblocks((size_t)ntrial * K, 256)), dim3(256), 0, c.stream, ntrial,
+                         K, c.t_idx, c.P, (const double *)c.P.disp, (const double *)nullptr, sigma_frac,
+                         c.p.MaxVelInUnitsVesc, (uint32_t)seed, (uint32_t)(3000 + (seed >> 32)),
+                         (const int *)c.tovs_dev, vtx, vty, vtz);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ntrial, 1), slot_budget(c) / ((K + 1) * per)));
+  CHK(alloc_orbit_scratch(c, chunk * (K + 1)));
+  CHK(alloc_slots(c, chunk * (K + 1), per));
+  float eval_ms = 0;
+  for(int base = 0; base < ntrial; base += chunk) {
+    int nt = std::min(chunk, ntrial - base);
+    hipLaunchKernelGGL(galic::k_prepare_ktrials, dim3(nblocks((size_t)nt * (K + 1), 256)), dim3(256), 0, c.stream, nt,
+                       K, c.t_idx + base, vtx + (size_t)base * K, vty + (size_t)base * K, vtz + (size_t)base * K, c.P,
+                       orbit_in(c));
+    HIPCHK(hipGetLastError());
+    galic::OrbitArgs32 A = orbit_args32(c, nt * (K + 1));
+    A.slots = (float *)c.slots;
+    CHK(launch_orbit32(c, A));
+    galic::EvalArgs32 E;
+    memset(&E, 0, sizeof(E));
+    E.P = c.P;
+    E.K = K;
+    E.tidx = c.t_idx + base;
+    E.tvx = vtx + (size_t)base * K;
+    E.tvy = vty + (size_t)base * K;
+    E.tvz = vtz + (size_t)base * K;
+    E.stacks = c.stacks;
+    E.weights = c.weights;
+    E.fit = c.fit;
+    E.cutB_of = c.cutB_of;
+    E.slots = (const float *)c.slots;
+    E.slot_stride = A.slot_stride;
+    E.slot_base = 0;
+    E.bbox = c.o_bbox;
+    E.norbit = c.o_norbit;
+    E.locA_dq = c.locA_dq;
+    E.locI_dq = c.locI_dq;
+    E.accept = c.t_accept + base;
+    E.dF = c.t_dF + base;
+    for(int t = 0; t < GALIC_NTYPES; t++) {
+      E.MType[t] = c.p.MType[t];
+      E.fac[t][0] = c.p.fac_value_rs[t];
+      E.fac[t][1] = c.p.fac_value_ts[t];
+      E.fac[t][2] = c.p.fac_value_qs[t];
+      E.fac[t][3] = c.p.fac_value_ps[t];
+    }
+    E.lowdisp = c.p.LowerDispLimit;
+    for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
+    E.iscale = c.fp32_iscale;
+    size_t shmem = sizeof(double) * ((c.Nstack - c.Ngrid) + 8 * 256);
+    HIPCHK(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(galic::k_eval_apply32, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, c.stream));
+    HIPCHK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    eval_ms += ms;
+  }
+  c.last_eval_ms = eval_ms;
+  CHK(commit_q(c));
+  CHK(global_fit(ctx));
+  if(ntrial > 0 && accept) HIPCHK(hipMemcpy(accept, c.t_accept, sizeof(int) * ntrial, hipMemcpyDeviceToHost));
   return GALIC_OK;
 }
 

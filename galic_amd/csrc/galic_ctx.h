@@ -63,6 +63,7 @@ struct Particles {
   double *mass = nullptr, *tint = nullptr, *vesc = nullptr;
   double *vr2 = nullptr, *vt2 = nullptr, *vp2 = nullptr, *vq2 = nullptr;
   int *type = nullptr, *orbits = nullptr;
+  double *disp = nullptr;  // target vr2, vt2, vp2, vq2 per particle (set_particles.c:158-161)
 };
 
 struct Ctx {
@@ -94,6 +95,11 @@ struct Ctx {
   long long *locA_dq = nullptr;
   long long *locI_q = nullptr;
   long long *locI_dq = nullptr;
+  long long *locM_q = nullptr;
+  double *qinv = nullptr;       // 1/scale of the 5 orbit-averaged then 4 instantaneous fields
+  int *tovs_dev = nullptr;      // velocity structure per type, on the device
+  double *t_vk = nullptr;       // K trial velocities per particle, SoA [3][max_ktrials]
+  size_t max_ktrials = 0;
   double *fit = nullptr;        // [type][16] S, Sdisp_r,t,p,q, Srelfac, Srelfac_count, Sall, raw sums..., firstrun
   Particles P;
 

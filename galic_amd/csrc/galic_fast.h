@@ -1,0 +1,75 @@
+// galic_fast.h -- argument blocks of the FP32 batched path (galic_fast.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "galic_ctx.h"
+#include "galic_kernels.h"
+
+namespace galic {
+
+// FP32 forms of the model constants (reciprocals and log2 are fine here).
+struct FastModel {
+  int use_table, add_halo, add_bulge;
+  int tovs[GALIC_NTYPES];
+  int fg_nbin, dg_nbin;
+  float fg_invRmin, fg_invlog2Fac, dg_invRmin, dg_invlog2Fac;
+  float halo_GM, halo_a, halo_eps, offgrid_fac;
+  float bulge_GM, bulge_a, bulge_eps;
+  float fo_v200, fc;
+  long long max_steps;
+  const float2 *acc;  // (DPotDR, DPotDz) per node
+  const float *vstream[GALIC_NTYPES];
+  const float *cellsize;
+  double qscale[kFields];  // fixed-point scale per orbit-averaged field
+};
+
+struct OrbitArgs32 {
+  int n;
+  unsigned int *counter;
+  const double *ox, *oy, *oz, *ovx, *ovy, *ovz, *omass, *otint;
+  const int *otype;
+  float *slots;        // private FP32 response per orbit, or null
+  size_t slot_stride;  // floats per slot (5*Ngrid)
+  int slot_base;
+  long long *locq;     // fixed-point Loc grids [type][5][Ngrid] for direct atomics, or null
+  double *tall;
+  int *norbit, *status;
+  long long *steps;
+  int4 *bbox;
+};
+
+struct EvalArgs32 {
+  Particles P;
+  int K;
+  const int *tidx;
+  const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
+  const double *stacks, *weights, *fit;
+  const int *cutB_of;
+  const float *slots;
+  size_t slot_stride;
+  size_t slot_base;
+  const int4 *bbox;
+  const int *norbit;
+  long long *locA_dq, *locI_dq;
+  int *accept;
+  double *dF;
+  double MType[GALIC_NTYPES];
+  double fac[GALIC_NTYPES][4];
+  double lowdisp;
+  double qscale[kFields];
+  double iscale;
+};
+
+__global__ void k_orbit32(FastModel F, OrbitArgs32 A);
+__global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, const double *disp,
+                              const double *vstr_unused, double sigma_frac, double maxvel, uint32_t seed,
+                              uint32_t stream, const int *tovs_of_type, double *vtx, double *vty, double *vtz);
+__global__ void k_prepare_ktrials(int ntrial, int K, const int *tidx, const double *vtx, const double *vty,
+                                  const double *vtz, Particles P, OrbitIn O);
+__global__ void k_eval_apply32(DevModel M, EvalArgs32 A);
+__global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
+                           const double *inv_scale, size_t n);
+__global__ void k_inst_init_q(DevModel M, Particles P, long long *locIq, long long *locMq, int ngrid, double iscale,
+                              double mscale);
+
+}  // namespace galic

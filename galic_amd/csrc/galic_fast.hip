@@ -1,0 +1,687 @@
+// galic_fast.hip -- the batched FP32 path on gfx950.
+//
+// Same algorithm as the FP64 path (orbit_response.c:155-491, main.c:365-527,
+// 808-841), re-shaped for throughput:
+//   * leapfrog and table interpolation in FP32 with the hardware transcendentals
+//     (v_log_f32, v_rsq_f32, v_rcp_f32); time and energy bookkeeping in FP64;
+//   * the two force tables interleaved as float2 so a bilinear tap is one load;
+//   * algebraic forms of the basis vectors: cos(atan2(y,x)) = x/R, and the
+//     cross products of orbit_response.c:253-271 written out;
+//   * K trial velocities per particle evaluated in one launch, the best taken;
+//   * grids accumulated as 64-bit fixed point with integer atomics, so every
+//     grid is bit-identical run to run whatever the order of the atomics;
+//   * merit sums in FP64 on the same fixed cuts as the FP64 path.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "galic_ctx.h"
+#include "galic_fast.h"
+#include "galic_kernels.h"
+
+namespace galic {
+
+// ------------------------------------------------------------ FP32 lookups --
+
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+struct Fg32 {
+  float R, invR, jd, id;
+};
+
+__device__ __forceinline__ Fg32 fg_index32(const FastModel &F, float x, float y, float z) {
+  Fg32 g;
+  float R2 = x * x + y * y;
+  g.invR = fast_rsq(R2);
+  g.R = R2 * g.invR;
+  if(!(R2 > 0)) {
+    g.R = 0;
+    g.invR = 0;
+  }
+  g.jd = fast_log2(g.R * F.fg_invRmin + 1.0f) * F.fg_invlog2Fac;
+  g.id = fast_log2(fabsf(z) * F.fg_invRmin + 1.0f) * F.fg_invlog2Fac;
+  return g;
+}
+
+__device__ __forceinline__ void accel32(const FastModel &F, float x, float y, float z, const Fg32 &g, float &ax,
+                                        float &ay, float &az) {
+  ax = ay = az = 0.f;
+  if(F.use_table) {
+    const int nb = F.fg_nbin;
+    int jb = (int)g.jd, ib = (int)g.id;
+    if(ib < nb - 1 && jb < nb - 1) {
+      float jf = g.jd - jb, fi = g.id - ib;
+      const float2 *T = F.acc + ib * nb + jb;
+      float2 t00 = T[0], t01 = T[1], t10 = T[nb], t11 = T[nb + 1];
+      float w00 = (1 - fi) * (1 - jf), w01 = (1 - fi) * jf, w10 = fi * (1 - jf), w11 = fi * jf;
+      float accR = t00.x * w00 + t01.x * w01 + t10.x * w10 + t11.x * w11;
+      float accz = t00.y * w00 + t01.y * w01 + t10.y * w10 + t11.y * w11;
+      if(z < 0) accz = -accz;
+      float c = 1.f, s = 0.f;
+      if(g.R > 0) {
+        c = x * g.invR;
+        s = y * g.invR;
+      }
+      ax = accR * c;
+      ay = accR * s;
+      az = accz;
+    } else {
+      float r = sqrtf(x * x + y * y + z * z);
+      float f = F.halo_GM * fast_rcp((r + F.halo_eps) * (r + F.halo_a) * (r + F.halo_a)) * F.offgrid_fac;
+      ax = -f * x;
+      ay = -f * y;
+      az = -f * z;
+    }
+  }
+  if(F.add_halo || F.add_bulge) {
+    float r = sqrtf(x * x + y * y + z * z);
+    float f = 0.f;
+    if(F.add_halo) f += F.halo_GM * fast_rcp((r + F.halo_eps) * (r + F.halo_a) * (r + F.halo_a));
+    if(F.add_bulge) f += F.bulge_GM * fast_rcp((r + F.bulge_eps) * (r + F.bulge_a) * (r + F.bulge_a));
+    ax -= f * x;
+    ay -= f * y;
+    az -= f * z;
+  }
+}
+
+__device__ __forceinline__ float vstream32(const FastModel &F, const Fg32 &g, int type) {
+  const int nb = F.fg_nbin;
+  float br = g.jd, bz = g.id;
+  int iR = (int)br, iz = (int)bz;
+  float fR = br - iR, fz = bz - iz;
+  if(iR >= nb - 1) {
+    iR = nb - 2;
+    fR = 1.f;
+  }
+  if(iz >= nb - 1) {
+    iz = nb - 2;
+    fz = 1.f;
+  }
+  const float *V = F.vstream[type] + iz * nb + iR;
+  return V[0] * (1 - fR) * (1 - fz) + V[nb] * (1 - fR) * fz + V[1] * fR * (1 - fz) + V[nb + 1] * fR * fz;
+}
+
+__device__ __forceinline__ void cell32(const FastModel &F, float R, float z, int &iR, int &iz, float &fR,
+                                       float &fz) {
+  const int nb = F.dg_nbin;
+  float br = fast_log2(R * F.dg_invRmin + 1.0f) * F.dg_invlog2Fac - 0.5f;
+  float bz = fast_log2(fabsf(z) * F.dg_invRmin + 1.0f) * F.dg_invlog2Fac - 0.5f;
+  br = fmaxf(br, 0.f);
+  bz = fmaxf(bz, 0.f);
+  iR = (int)br;
+  iz = (int)bz;
+  fR = br - iR;
+  fz = bz - iz;
+  if(iR >= nb - 1) {
+    iR = nb - 2;
+    fR = 1.f;
+  }
+  if(iz >= nb - 1) {
+    iz = nb - 2;
+    fz = 1.f;
+  }
+}
+
+// vr2, vt2, vq2, vp2 of orbit_response.c:222-273 in closed form:
+// spherical basis Q = (-y, x, 0), T = (-zx, -zy, R^2), |Q| = R, |T| = R r.
+__device__ __forceinline__ void decompose32(const FastModel &F, int tovs, int type, float x, float y, float z,
+                                            float vx, float vy, float vz, const Fg32 &g, float &vr2, float &vt2,
+                                            float &vq2, float &vp2) {
+  float R2 = x * x + y * y;
+  float vq = (vy * x - vx * y) * g.invR;
+  vq2 = vq * vq;
+  float vstr = vstream32(F, g, type);
+  vp2 = (vq - vstr) * (vq - vstr);
+  if(tovs == 2) {
+    float vdx = vx * x + vy * y;
+    vr2 = vdx * vdx * g.invR * g.invR;
+    vt2 = vz * vz;
+  } else {
+    float r2 = R2 + z * z;
+    float ir2 = fast_rcp(r2);
+    float vdx = vx * x + vy * y + vz * z;
+    vr2 = vdx * vdx * ir2;
+    float vdt = R2 * vz - z * (vx * x + vy * y);
+    vt2 = vdt * vdt * ir2 * g.invR * g.invR;
+  }
+}
+
+// ------------------------------------------------------------- FP32 orbits --
+
+struct Run32 {
+  int quad;
+  float w[kFields][4];
+};
+
+__device__ __forceinline__ void run32_reset(Run32 &r, int quad) {
+  r.quad = quad;
+#pragma unroll
+  for(int f = 0; f < kFields; f++)
+#pragma unroll
+    for(int c = 0; c < 4; c++) r.w[f][c] = 0.f;
+}
+
+__device__ __forceinline__ void atomic_q(long long *p, double v) {
+  atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v));
+}
+
+__device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
+                                            const FastModel &F) {
+  if(r.quad < 0) return;
+  const int c[4] = {r.quad, r.quad + 1, r.quad + nb, r.quad + nb + 1};
+  if(slot) {
+#pragma unroll
+    for(int f = 0; f < kFields; f++) {
+      float *s = slot + f * ngrid;
+#pragma unroll
+      for(int k = 0; k < 4; k++) s[c[k]] += fac * r.w[f][k];
+    }
+  } else {
+#pragma unroll
+    for(int f = 0; f < kFields; f++) {
+      long long *s = locq + (size_t)f * ngrid;
+      double sc = F.qscale[f] * (double)fac;
+#pragma unroll
+      for(int k = 0; k < 4; k++) atomic_q(&s[c[k]], sc * (double)r.w[f][k]);
+    }
+  }
+}
+
+__device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
+                                      const FastModel &F, int iR, int iz, float fR, float fz, float dt, float vr2,
+                                      float vt2, float vq2, float vp2, int4 &bb) {
+  float h = 0.5f * dt;
+  float m[4] = {h * (1 - fR) * (1 - fz), h * fR * (1 - fz), h * (1 - fR) * fz, h * fR * fz};
+  int quad = iz * nb + iR;
+  if(quad != r.quad) {
+    run32_flush(r, nb, ngrid, fac, slot, locq, F);
+    run32_reset(r, quad);
+    bb.x = min(bb.x, iR);
+    bb.y = max(bb.y, iR + 1);
+    bb.z = min(bb.z, iz);
+    bb.w = max(bb.w, iz + 1);
+  }
+#pragma unroll
+  for(int k = 0; k < 4; k++) {
+    r.w[F_M][k] += m[k];
+    r.w[F_R][k] += m[k] * vr2;
+    r.w[F_T][k] += m[k] * vt2;
+    r.w[F_Q][k] += m[k] * vq2;
+    r.w[F_P][k] += m[k] * vp2;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) {
+  const int nb = F.dg_nbin, ngrid = nb * nb;
+  while(true) {
+    int o = (int)atomicAdd(A.counter, 1u);
+    if(o >= A.n) break;
+    const int type = A.otype[o];
+    const int tovs = F.tovs[type];
+    float x = (float)A.ox[o], y = (float)A.oy[o], z = (float)A.oz[o];
+    float vx = (float)A.ovx[o], vy = (float)A.ovy[o], vz = (float)A.ovz[o];
+    const double timespan = A.otint[o];
+    const float fac = (float)(A.omass[o] / timespan);
+    float *slot = A.slots ? A.slots + (size_t)(o - A.slot_base) * A.slot_stride : nullptr;
+    long long *locq = A.locq ? A.locq + (size_t)type * kFields * ngrid : nullptr;
+
+    Fg32 g = fg_index32(F, x, y, z);
+    float ax, ay, az;
+    accel32(F, x, y, z, g, ax, ay, az);
+    int iR, iz;
+    float fR, fz;
+    cell32(F, g.R, z, iR, iz, fR, fz);
+    int icell = iz * nb + iR;
+    float vr2, vt2, vq2, vp2;
+    decompose32(F, tovs, type, x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
+    Run32 run;
+    run.quad = -1;
+    int4 bb = make_int4(nb, -1, nb, -1);
+    double tall = 0;
+    float radsign_prev = 0.f;
+    int norbit = 0, flag = 0, status = 0;
+    long long steps = 0;
+    if(!(timespan > 0)) status = GALIC_ERR_ARG;
+    while(!flag && status == 0) {
+      float inv_v = fast_rsq(vx * vx + vy * vy + vz * vz);
+      float inv_a = fast_rsq(ax * ax + ay * ay + az * az);
+      float dt = fminf(F.fo_v200 * inv_a, F.fc * F.cellsize[icell] * inv_v);
+      if(timespan < tall + (double)dt) {
+        dt = (float)(timespan - tall);
+        flag = 1;
+      }
+      if(!(dt >= 0.f && dt < 3.0e38f)) {
+        status = GALIC_ERR_ARG;
+        break;
+      }
+      bin32(run, nb, ngrid, fac, slot, locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+      float hdt = 0.5f * dt;
+      vx += hdt * ax;
+      vy += hdt * ay;
+      vz += hdt * az;
+      x += dt * vx;
+      y += dt * vy;
+      z += dt * vz;
+      g = fg_index32(F, x, y, z);
+      accel32(F, x, y, z, g, ax, ay, az);
+      vx += hdt * ax;
+      vy += hdt * ay;
+      vz += hdt * az;
+      cell32(F, g.R, z, iR, iz, fR, fz);
+      icell = iz * nb + iR;
+      decompose32(F, tovs, type, x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
+      bin32(run, nb, ngrid, fac, slot, locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+      tall += (double)dt;
+      float radsign = vx * x + vy * y + vz * z;
+      if(radsign > 0 && radsign_prev < 0) norbit++;
+      radsign_prev = radsign;
+      steps++;
+      if(steps > F.max_steps) status = GALIC_ERR_STEPCAP;
+    }
+    run32_flush(run, nb, ngrid, fac, slot, locq, F);
+    A.tall[o] = tall;
+    A.norbit[o] = norbit;
+    A.steps[o] = steps;
+    A.bbox[o] = bb;
+    A.status[o] = status;
+  }
+}
+
+// ------------------------------------------------------------ trial draws --
+
+__device__ __forceinline__ void philox_round(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                             uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+  uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+  uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+  c0 = n0;
+  c1 = lo1;
+  c2 = n2;
+  c3 = lo0;
+}
+
+// Philox4x32-10 with the counter layout of galic_amd/rng.py: (id lo, id hi, attempt, block).
+__device__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for(int r = 0; r < 10; r++) {
+    philox_round(c[0], c[1], c[2], c[3], k0, k1);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) / 9007199254740992.0;
+}
+
+// Four standard normals for (id, attempt, block), as rng.gaussians
+__device__ void gauss4(uint64_t id, uint32_t attempt, uint32_t block, uint32_t seed, uint32_t stream, double g[4]) {
+  double u[4];
+  for(int half = 0; half < 2; half++) {
+    uint32_t c[4] = {(uint32_t)id, (uint32_t)(id >> 32), attempt, 2 * block + half};
+    philox4x32(c, seed, stream);
+    u[2 * half] = u01(c[0], c[1]);
+    u[2 * half + 1] = u01(c[2], c[3]);
+  }
+  double r1 = sqrt(-2.0 * log(u[0])), r2 = sqrt(-2.0 * log(u[2]));
+  g[0] = r1 * cos(2 * M_PI * u[1]);
+  g[1] = r1 * sin(2 * M_PI * u[1]);
+  g[2] = r2 * cos(2 * M_PI * u[3]);
+  g[3] = r2 * sin(2 * M_PI * u[3]);
+}
+
+// K perturbations of the current velocity per particle: Gaussian in the local
+// basis of optimize() (main.c:383-405) with sigma_frac times the target
+// dispersions, redrawn while |v| >= MaxVelInUnitsVesc * Vesc (main.c:415-435);
+// after 64 attempts the trial is the current velocity (a no-op trial).
+__global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, const double *disp,
+                              const double *vstr_unused, double sigma_frac, double maxvel, uint32_t seed,
+                              uint32_t stream, const int *tovs_of_type, double *vtx, double *vty, double *vtz) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if(q >= ntrial * K) return;
+  int i = q / K, k = q % K;
+  int p = tidx[i];
+  double x = P.x[p], y = P.y[p], z = P.z[p];
+  double vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
+  int tovs = tovs_of_type[P.type[p]];
+  double phi = atan2(y, x);
+  double sp = sin(phi), cp = cos(phi);
+  double e1[3], e2[3], e3[3];
+  if(tovs == 2) {
+    e1[0] = cp; e1[1] = sp; e1[2] = 0;
+    e2[0] = -sp; e2[1] = cp; e2[2] = 0;
+    e3[0] = 0; e3[1] = 0; e3[2] = 1;
+  } else {
+    double theta = acos(z / sqrt(x * x + y * y + z * z));
+    double st = sin(theta), ct = cos(theta);
+    e1[0] = st * cp; e1[1] = st * sp; e1[2] = ct;
+    e2[0] = -ct * cp; e2[1] = -ct * sp; e2[2] = st;
+    e3[0] = -sp; e3[1] = cp; e3[2] = 0;
+  }
+  double s1 = sigma_frac * sqrt(disp[4 * (size_t)p + 0]);
+  double s2 = sigma_frac * sqrt(disp[4 * (size_t)p + 1]);
+  double s3 = sigma_frac * sqrt(disp[4 * (size_t)p + 2]);
+  double lim = maxvel * P.vesc[p];
+  double ox = vx, oy = vy, oz = vz;
+  for(uint32_t attempt = 0; attempt < 64; attempt++) {
+    double g[4];
+    gauss4((uint64_t)p, attempt, (uint32_t)k, seed, stream, g);
+    double a = g[0] * s1, b = g[1] * s2, c = g[2] * s3;
+    double tx = vx + a * e1[0] + b * e2[0] + c * e3[0];
+    double ty = vy + a * e1[1] + b * e2[1] + c * e3[1];
+    double tz = vz + a * e1[2] + b * e2[2] + c * e3[2];
+    if(sqrt(tx * tx + ty * ty + tz * tz) < lim) {
+      ox = tx;
+      oy = ty;
+      oz = tz;
+      break;
+    }
+  }
+  vtx[q] = ox;
+  vty[q] = oy;
+  vtz[q] = oz;
+}
+
+// Orbits of a K-trial rep: orbit i*(K+1) is particle tidx[i] at its current
+// velocity, orbits i*(K+1)+1+k its trials.
+__global__ void k_prepare_ktrials(int ntrial, int K, const int *tidx, const double *vtx, const double *vty,
+                                  const double *vtz, Particles P, OrbitIn O) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if(o >= ntrial * (K + 1)) return;
+  int i = o / (K + 1), k = o % (K + 1);
+  int p = tidx[i];
+  O.pidx[o] = p;
+  O.ox[o] = P.x[p];
+  O.oy[o] = P.y[p];
+  O.oz[o] = P.z[p];
+  O.omass[o] = P.mass[p];
+  O.otint[o] = P.tint[p];
+  O.otype[o] = P.type[p];
+  if(k == 0) {
+    O.ovx[o] = P.vx[p];
+    O.ovy[o] = P.vy[p];
+    O.ovz[o] = P.vz[p];
+  } else {
+    size_t q = (size_t)i * K + (k - 1);
+    O.ovx[o] = vtx[q];
+    O.ovy[o] = vty[q];
+    O.ovz[o] = vtz[q];
+  }
+}
+
+// -------------------------------------------------------------- FP32 merit --
+
+template <int NV>
+__device__ void block_sum_f(double (&v)[NV], double *lds) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for(int k = 0; k < NV; k++) lds[k * nt + tid] = v[k];
+  __syncthreads();
+  for(int s = nt / 2; s > 0; s >>= 1) {
+    if(tid < s)
+      for(int k = 0; k < NV; k++) lds[k * nt + tid] += lds[k * nt + tid + s];
+    __syncthreads();
+  }
+  for(int k = 0; k < NV; k++) v[k] = lds[k * nt];
+  __syncthreads();
+}
+
+// Change of the instantaneous merit terms, as inst_delta of the FP64 path.
+__device__ double inst_delta_q(const DevModel &M, const double *S, const int *cutB_of, int t, double x, double y,
+                               double z, double mass, const double (&dnew)[4], const double (&dold)[4],
+                               double lowdisp, double (&cw)[4], int (&cc)[4]) {
+  const int L = M.EG_L, nb = M.EG_Nbin, Nstack = GALIC_NSTACK(L);
+  int iR, iz;
+  double fR, fz;
+  cell64(x, y, z, M.EG_Rmin, M.EG_logFac, nb, iR, iz, fR, fz);
+  cw[0] = (1 - fR) * (1 - fz);
+  cw[1] = (fR) * (1 - fz);
+  cw[2] = (1 - fR) * (fz);
+  cw[3] = (fR) * (fz);
+  cc[0] = iz * nb + iR;
+  cc[1] = iz * nb + iR + 1;
+  cc[2] = (iz + 1) * nb + iR;
+  cc[3] = (iz + 1) * nb + iR + 1;
+  const double *MR = S + ST_EMR * Nstack, *MT = S + ST_EMT * Nstack;
+  const int stI[4] = {ST_IR, ST_IT, ST_IP, ST_IQ};
+  const int stE[4] = {ST_ER, ST_ET, ST_EP, ST_EQ};
+  int node[4];
+  for(int k = 0; k < 4; k++) node[k] = cutB_of[cc[k]];
+  double total = 0;
+  const int tovs = M.tovs[t];
+  for(int f = 0; f < 4; f++) {
+    if(f == I_Q && (tovs == 0 || t == 2)) continue;
+    double val = (dnew[f] - dold[f]) * mass;
+    const double *I = S + stI[f] * Nstack, *E = S + stE[f] * Nstack;
+    for(int k = 0; k < 4; k++) {
+      int nd = node[k];
+      bool first = true;
+      for(int j = 0; j < k; j++)
+        if(node[j] == nd) first = false;
+      if(!first) continue;
+      double delta = 0;
+      for(int j = k; j < 4; j++)
+        if(node[j] == nd) delta += cw[j] * val;
+      double mr = MR[nd], mt = MT[nd];
+      if(mr > 0 && mt > 0) {
+        double tgt = E[nd] / mt;
+        double den = tgt > lowdisp ? tgt : lowdisp;
+        total += (fabs((I[nd] + delta) / mr - tgt) - fabs(I[nd] / mr - tgt)) / den;
+      }
+    }
+  }
+  return total;
+}
+
+// One workgroup per particle: the change of the merit for each of its K trials
+// (FP32 slots, FP64 sums on the fixed cuts), the best trial taken when it lowers
+// the merit, deltas added as fixed point, and all K+1 slots cleared.
+__global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) {
+  extern __shared__ double lds[];
+  const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x, K = A.K;
+  const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
+  const int ncoarse = Nstack - ngrid;
+  double *cs = lds;
+  double *red = lds + ncoarse;
+  __shared__ double s_dF[64];
+  __shared__ int s_best;
+
+  const int p = A.tidx[i];
+  const int t = A.P.type[p];
+  const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
+  const double *W = A.weights + (size_t)t * W_COUNT * Nstack;
+  const size_t base = (size_t)i * (K + 1) - A.slot_base;
+  const float *so = A.slots + base * A.slot_stride;
+  const int4 bo = A.bbox[(size_t)i * (K + 1)];
+  const int offL = GALIC_STACKOFFSET(L, 0, 0);
+  const int stR[kFields] = {ST_RM, ST_RSR, ST_RST, ST_RSQ, ST_RSP};
+  const int stT[kFields] = {ST_MT, ST_ER, ST_ET, ST_EQ, ST_EP};
+  const double *den = W + W_A * Nstack;
+  const double x = A.P.x[p], y = A.P.y[p], z = A.P.z[p], mass = A.P.mass[p];
+  const double dold[4] = {A.P.vr2[p], A.P.vt2[p], A.P.vp2[p], A.P.vq2[p]};
+  const double MT = A.MType[t];
+
+  for(int k = 0; k < K; k++) {
+    const float *sn = so + (size_t)(1 + k) * A.slot_stride;
+    const int4 bn = A.bbox[(size_t)i * (K + 1) + 1 + k];
+    int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
+    const int npr = (r1 < r0) ? 0 : (r1 >> 1) - (r0 >> 1) + 1;
+    const int npz = (z1 < z0) ? 0 : (z1 >> 1) - (z0 >> 1) + 1;
+    const int pr0 = r0 >> 1, pz0 = z0 >> 1;
+    double phi[kFields];
+    for(int f = 0; f < kFields; f++) {
+      phi[f] = 0;
+      const double *R = S + stR[f] * Nstack, *T = S + stT[f] * Nstack;
+      const double *ratio = (f == F_M) ? nullptr : W + (W_SR + f - 1) * Nstack;
+      for(int q = tid; q < ncoarse; q += nt) cs[q] = 0;
+      __syncthreads();
+      const float *fo = so + (size_t)f * ngrid, *fn = sn + (size_t)f * ngrid;
+      for(int q = tid; q < npr * npz; q += nt) {
+        int pz = pz0 + q / npr, pr = pr0 + q % npr;
+        int cz[4] = {2 * pz, 2 * pz + 1, 2 * pz, 2 * pz + 1};
+        int cr[4] = {2 * pr, 2 * pr, 2 * pr + 1, 2 * pr + 1};
+        double d[4];
+        for(int c = 0; c < 4; c++) {
+          int cell = cz[c] * nb + cr[c];
+          d[c] = (double)fn[cell] - (double)fo[cell];
+          int node = offL + cell;
+          if(d[c] != 0 && den[node] != 0) {
+            double wgt = ratio ? ratio[node] : 1.0;
+            if(wgt != 0) {
+              double b = R[node] - T[node];
+              phi[f] += (fabs(b + d[c]) - fabs(b)) * wgt / den[node];
+            }
+          }
+        }
+        cs[GALIC_STACKOFFSET(L - 1, pz, pr)] = d[0] + d[1] + d[2] + d[3];
+      }
+      __syncthreads();
+      for(int l = L - 2; l >= 0; l--) {
+        int n = 1 << l;
+        for(int q = tid; q < n * n; q += nt) {
+          int ii = q / n, jj = q % n;
+          cs[GALIC_STACKOFFSET(l, ii, jj)] =
+              cs[GALIC_STACKOFFSET(l + 1, 2 * ii, 2 * jj)] + cs[GALIC_STACKOFFSET(l + 1, 2 * ii + 1, 2 * jj)] +
+              cs[GALIC_STACKOFFSET(l + 1, 2 * ii, 2 * jj + 1)] + cs[GALIC_STACKOFFSET(l + 1, 2 * ii + 1, 2 * jj + 1)];
+        }
+        __syncthreads();
+      }
+      for(int node = tid; node < ncoarse; node += nt) {
+        double dc = cs[node];
+        if(dc != 0 && den[node] != 0) {
+          double wgt = ratio ? ratio[node] : 1.0;
+          if(wgt != 0) {
+            double b = R[node] - T[node];
+            phi[f] += (fabs(b + dc) - fabs(b)) * wgt / den[node];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    block_sum_f<kFields>(phi, red);
+    if(tid == 0) {
+      const size_t q = (size_t)i * K + k;
+      double dnew[4];
+      disp_components64(M, t, x, y, z, A.tvx[q], A.tvy[q], A.tvz[q], dnew[0], dnew[1], dnew[2], dnew[3]);
+      double cw[4];
+      int cc[4];
+      double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
+      const double *fit = A.fit + (size_t)t * 16;
+      s_dF[k] = phi[F_M] / MT +
+                (A.fac[t][0] * phi[F_R] + A.fac[t][1] * phi[F_T] + A.fac[t][2] * phi[F_Q] + A.fac[t][3] * phi[F_P]) /
+                    MT / 4 +
+                fit[5] / fit[6] * vinst;
+    }
+    __syncthreads();
+  }
+  if(tid == 0) {
+    int best = -1;
+    double bv = 0;
+    for(int k = 0; k < K; k++)
+      if(s_dF[k] < bv) {
+        bv = s_dF[k];
+        best = k;
+      }
+    s_best = best;
+    A.accept[i] = best + 1;
+    A.dF[i] = (best >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
+    const int o0 = i * (K + 1);
+    A.P.orbits[p] = (best >= 0) ? A.norbit[o0 + 1 + best] : A.norbit[o0];
+    if(best >= 0) {
+      const size_t q = (size_t)i * K + best;
+      const double vt[3] = {A.tvx[q], A.tvy[q], A.tvz[q]};
+      double dnew[4];
+      disp_components64(M, t, x, y, z, vt[0], vt[1], vt[2], dnew[0], dnew[1], dnew[2], dnew[3]);
+      int iR, iz;
+      double fR, fz;
+      cell64(x, y, z, M.EG_Rmin, M.EG_logFac, M.EG_Nbin, iR, iz, fR, fz);
+      double cw[4] = {(1 - fR) * (1 - fz), fR * (1 - fz), (1 - fR) * fz, fR * fz};
+      int cc[4] = {iz * nb + iR, iz * nb + iR + 1, (iz + 1) * nb + iR, (iz + 1) * nb + iR + 1};
+      long long *lid = A.locI_dq + (size_t)t * 4 * ngrid;
+      for(int f = 0; f < 4; f++) {
+        double val = (dnew[f] - dold[f]) * mass;
+        for(int c = 0; c < 4; c++) atomic_q(&lid[(size_t)f * ngrid + cc[c]], A.iscale * cw[c] * val);
+      }
+      A.P.vx[p] = vt[0];
+      A.P.vy[p] = vt[1];
+      A.P.vz[p] = vt[2];
+      A.P.vr2[p] = dnew[0];
+      A.P.vt2[p] = dnew[1];
+      A.P.vp2[p] = dnew[2];
+      A.P.vq2[p] = dnew[3];
+    }
+  }
+  __syncthreads();
+  const int best = s_best;
+  long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
+  if(best >= 0) {
+    const float *sn = so + (size_t)(1 + best) * A.slot_stride;
+    const int4 bn = A.bbox[(size_t)i * (K + 1) + 1 + best];
+    int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
+    int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
+    for(int q = tid; q < nr * nzz; q += nt) {
+      int cell = (z0 + q / nr) * nb + (r0 + q % nr);
+      for(int f = 0; f < kFields; f++) {
+        size_t o = (size_t)f * ngrid + cell;
+        float d0 = sn[o], d1 = so[o];
+        if(d0 != d1) atomic_q(&lad[o], A.qscale[f] * ((double)d0 - (double)d1));
+      }
+    }
+    __syncthreads();
+  }
+  // clear the K+1 slots inside their boxes
+  for(int k = 0; k <= K; k++) {
+    float *s = (float *)(so + (size_t)k * A.slot_stride);
+    const int4 b = A.bbox[(size_t)i * (K + 1) + k];
+    int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
+    for(int q = tid; q < nr * nzz; q += nt) {
+      int cell = (b.z + q / nr) * nb + (b.x + q % nr);
+      for(int f = 0; f < kFields; f++) s[(size_t)f * ngrid + cell] = 0.f;
+    }
+  }
+}
+
+// ---------------------------------------------------- fixed point commit --
+
+// Loc += delta in fixed point; the FP64 grid the global fit reads is Loc/scale.
+__global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
+                           const double *inv_scale, size_t n) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= n) return;
+  int f = (int)((k / n_per_field) % nfield);
+  long long v = locq[k] + dq[k];
+  locq[k] = v;
+  dq[k] = 0;
+  loc[k] = (double)v * inv_scale[f];
+}
+
+// Instantaneous moments of every particle in fixed point (main.c:226-229).
+__global__ void k_inst_init_q(DevModel M, Particles P, long long *locIq, long long *locMq, int ngrid, double iscale,
+                              double mscale) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if(n >= P.n) return;
+  const int t = P.type[n];
+  double x = P.x[n], y = P.y[n], z = P.z[n];
+  double vr2, vt2, vp2, vq2;
+  disp_components64(M, t, x, y, z, P.vx[n], P.vy[n], P.vz[n], vr2, vt2, vp2, vq2);
+  P.vr2[n] = vr2;
+  P.vt2[n] = vt2;
+  P.vp2[n] = vp2;
+  P.vq2[n] = vq2;
+  int iR, iz;
+  double fR, fz;
+  cell64(x, y, z, M.EG_Rmin, M.EG_logFac, M.EG_Nbin, iR, iz, fR, fz);
+  const int nb = M.EG_Nbin;
+  const double mass = P.mass[n];
+  double w[4] = {(1 - fR) * (1 - fz), (fR) * (1 - fz), (1 - fR) * (fz), (fR) * (fz)};
+  int c[4] = {iz * nb + iR, iz * nb + iR + 1, (iz + 1) * nb + iR, (iz + 1) * nb + iR + 1};
+  double val[4] = {vr2 * mass, vt2 * mass, vp2 * mass, vq2 * mass};
+  long long *li = locIq + (size_t)t * 4 * ngrid;
+  for(int k = 0; k < 4; k++) {
+    atomic_q(&locMq[(size_t)t * ngrid + c[k]], mscale * w[k] * mass);
+    for(int f = 0; f < 4; f++) atomic_q(&li[(size_t)f * ngrid + c[k]], iscale * w[k] * val[f]);
+  }
+}
+
+}  // namespace galic

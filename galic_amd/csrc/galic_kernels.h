@@ -14,6 +14,9 @@
 
 namespace galic {
 
+// Records the message returned by galic_last_error(); returns code.
+int set_error(int code, const char *msg);
+
 struct OrbitArgs {
   int n;               // number of orbits in this launch
   unsigned int *counter;

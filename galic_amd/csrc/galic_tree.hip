@@ -1,0 +1,760 @@
+// galic_tree.hip -- Barnes-Hut fill of the force and potential tables on gfx950.
+//
+// The reference builds an oct-tree by inserting particles one at a time
+// (forcetree.c:369-495), sums monopole moments recursively
+// (forcetree.c:591-722), and walks it once per target with a geometric opening
+// criterion and Gadget-2 spline softening (forcetree_walk.c:12-222).  The tree
+// that insertion produces is unique: a node exists for every cube of the
+// subdivision that holds at least two particles, down to the depth where they
+// separate.  Here the same tree is built in parallel:
+//   1. integer positions exactly as forcetree.c:199-201 (52 mantissa bits of
+//      (x - corner)/len + 1), Morton keys from their top 21 bits per axis with
+//      the child numbering x + 2y + 4z of forcetree.c:216-217;
+//   2. radix sort of the keys;
+//   3. level by level, every node splits its sorted particle range into the 8
+//      child ranges by binary search; ranges of >= 2 particles become nodes;
+//   4. moments bottom-up in FP64, in child order 0..7 as the reference;
+//   5. nextnode/sibling threading top-down, giving the reference's depth-first
+//      walk order;
+//   6. one target per lane walks the tree without a stack.
+// Particles that share all 21 key levels (closer than len/2^21) stay together
+// in a leaf bucket; opening it interacts with them one by one.  The reference
+// would keep splitting; this changes results only for targets within that
+// distance of such a pair.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/galic_hip.h"
+#include "galic_kernels.h"
+
+namespace galic {
+namespace tree {
+
+constexpr int kMaxDepth = 21;
+
+struct Tree {
+  int nsrc = 0;
+  int cap = 0;
+  // sorted sources
+  double *x = nullptr, *y = nullptr, *z = nullptr, *m = nullptr;
+  unsigned long long *key = nullptr;
+  int *nextp = nullptr;  // next walk entry after each particle
+  // nodes
+  int *first = nullptr, *count = nullptr, *depth = nullptr;
+  int *child = nullptr;  // [node][8]: -1, particle (< nsrc) or nsrc + node
+  double *mass = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  float *len = nullptr;
+  int *sibling = nullptr, *nextnode = nullptr, *bucket = nullptr;
+  int *counter = nullptr;
+  std::vector<int> level_lo;  // nodes of depth d are [level_lo[d], level_lo[d+1])
+};
+
+__device__ __forceinline__ unsigned long long spread3(unsigned long long v) {
+  v &= 0x1fffffULL;
+  v = (v | (v << 32)) & 0x1f00000000ffffULL;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+  v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+  v = (v | (v << 2)) & 0x1249249249249249ULL;
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long mantissa(double d) {
+  return (unsigned long long)__double_as_longlong(d) & 0xFFFFFFFFFFFFFull;
+}
+
+// forcetree.c:197-210 integer positions; Morton key from bits 51..31 of each
+__global__ void k_keys(int n, const double *px, const double *py, const double *pz, double cx, double cy, double cz,
+                       double inv_len, unsigned long long *key, int *idx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  unsigned long long xb = mantissa(((px[i] - cx) * inv_len) + 1.0) >> 31;
+  unsigned long long yb = mantissa(((py[i] - cy) * inv_len) + 1.0) >> 31;
+  unsigned long long zb = mantissa(((pz[i] - cz) * inv_len) + 1.0) >> 31;
+  key[i] = spread3(xb) | (spread3(yb) << 1) | (spread3(zb) << 2);
+  idx[i] = i;
+}
+
+__global__ void k_gather(int n, const int *idx, const double *px, const double *py, const double *pz,
+                         const double *pm, double *x, double *y, double *z, double *m) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  int j = idx[i];
+  x[i] = px[j];
+  y[i] = py[j];
+  z[i] = pz[j];
+  m[i] = pm[j];
+}
+
+__device__ __forceinline__ int digit_at(unsigned long long key, int d) {  // child digit below depth d
+  return (int)((key >> (3 * (kMaxDepth - 1 - d))) & 7ull);
+}
+
+// Split every node of depth d into its 8 child ranges (binary search on the
+// sorted keys); ranges of >= 2 particles become nodes of depth d+1.
+__global__ void k_split(int lo, int hi, int d, int nsrc, int cap, const unsigned long long *key, int *first,
+                        int *count, int *depth, int *child, int *bucket, int *counter) {
+  int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if(nd >= hi) return;
+  int f = first[nd], c = count[nd];
+  int *ch = child + (size_t)nd * 8;
+  for(int k = 0; k < 8; k++) ch[k] = -1;
+  if(c >= 2 && d >= kMaxDepth) {  // particles closer than the key resolution: a leaf bucket
+    bucket[nd] = 1;
+    return;
+  }
+  bucket[nd] = 0;
+  int b[9];
+  b[0] = f;
+  b[8] = f + c;
+  for(int dg = 1; dg < 8; dg++) {
+    int l = b[dg - 1], r = f + c;  // first index in [l, r) with digit >= dg
+    while(l < r) {
+      int mid = (l + r) >> 1;
+      if(digit_at(key[mid], d) < dg)
+        l = mid + 1;
+      else
+        r = mid;
+    }
+    b[dg] = l;
+  }
+  for(int dg = 0; dg < 8; dg++) {
+    int cnt = b[dg + 1] - b[dg];
+    if(cnt == 1) {
+      ch[dg] = b[dg];
+    } else if(cnt >= 2) {
+      int id = atomicAdd(counter, 1);
+      if(id >= cap) {
+        ch[dg] = -2;  // overflow, reported by the host
+        continue;
+      }
+      first[id] = b[dg];
+      count[id] = cnt;
+      depth[id] = d + 1;
+      ch[dg] = nsrc + id;
+    }
+  }
+}
+
+// Monopole moments of the nodes of one depth (forcetree.c:619-697), children in order 0..7.
+__global__ void k_moments(int lo, int hi, int nsrc, const int *child, const int *bucket, const int *first,
+                          const int *count, const double *x, const double *y, const double *z, const double *m,
+                          double *mass, double *sx, double *sy, double *sz) {
+  int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if(nd >= hi) return;
+  double ms = 0, s0 = 0, s1 = 0, s2 = 0;
+  if(bucket[nd]) {
+    for(int p = first[nd]; p < first[nd] + count[nd]; p++) {
+      ms += m[p];
+      s0 += m[p] * x[p];
+      s1 += m[p] * y[p];
+      s2 += m[p] * z[p];
+    }
+  } else {
+    const int *ch = child + (size_t)nd * 8;
+    for(int k = 0; k < 8; k++) {
+      int c = ch[k];
+      if(c < 0) continue;
+      if(c < nsrc) {
+        ms += m[c];
+        s0 += m[c] * x[c];
+        s1 += m[c] * y[c];
+        s2 += m[c] * z[c];
+      } else {
+        int q = c - nsrc;
+        ms += mass[q];
+        s0 += mass[q] * sx[q];
+        s1 += mass[q] * sy[q];
+        s2 += mass[q] * sz[q];
+      }
+    }
+  }
+  if(ms) {
+    s0 /= ms;
+    s1 /= ms;
+    s2 /= ms;
+  }
+  mass[nd] = ms;
+  sx[nd] = s0;
+  sy[nd] = s1;
+  sz[nd] = s2;
+}
+
+// nextnode = first child; the sibling of a child is the next child, the last
+// child inherits its parent's sibling (forcetree.c:624-638).
+__global__ void k_thread(int lo, int hi, int nsrc, const int *child, const int *bucket, int *sibling, int *nextnode,
+                         int *nextp) {
+  int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if(nd >= hi) return;
+  int sib = sibling[nd];
+  if(bucket[nd]) {
+    nextnode[nd] = sib;  // opened buckets are walked inline
+    return;
+  }
+  const int *ch = child + (size_t)nd * 8;
+  int list[8], n = 0;
+  for(int k = 0; k < 8; k++)
+    if(ch[k] >= 0) list[n++] = ch[k];
+  nextnode[nd] = n ? list[0] : sib;
+  for(int k = 0; k < n; k++) {
+    int nx = (k + 1 < n) ? list[k + 1] : sib;
+    int c = list[k];
+    if(c < nsrc)
+      nextp[c] = nx;
+    else
+      sibling[c - nsrc] = nx;
+  }
+}
+
+__global__ void k_len(int lo, int hi, double dlen, float *len) {
+  int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if(nd < hi) len[nd] = (float)dlen;
+}
+
+struct WalkArgs {
+  int nsrc, ntarget;
+  const double *tx, *ty, *tz;
+  const double *x, *y, *z, *m;
+  const int *nextp, *first, *count, *bucket, *sibling, *nextnode;
+  const double *mass, *sx, *sy, *sz;
+  const float *len;
+  double theta, h, G;
+  double *ax, *ay, *az, *pot;
+  unsigned long long *nint;
+};
+
+// Gadget-2 cubic spline monopole (forcetree_walk.c:158-191)
+__device__ __forceinline__ void interact64(double mass, double r2, double h, double &fac, double &wp) {
+  double r = sqrt(r2);
+  if(r >= h) {
+    fac = mass / (r2 * r);
+    wp = -mass / r;
+  } else {
+    double h_inv = 1.0 / h;
+    double h3_inv = h_inv * h_inv * h_inv;
+    double u = r * h_inv;
+    if(u < 0.5) {
+      fac = mass * h3_inv * (10.666666666667 + u * u * (32.0 * u - 38.4));
+      wp = mass * h_inv * (-2.8 + u * u * (5.333333333333 + u * u * (6.4 * u - 9.6)));
+    } else {
+      fac = mass * h3_inv *
+            (21.333333333333 - 48.0 * u + 38.4 * u * u - 10.666666666667 * u * u * u - 0.066666666667 / (u * u * u));
+      wp = mass * h_inv *
+           (-3.2 + 0.066666666667 / u + u * u * (10.666666666667 + u * (-16.0 + u * (9.6 - 2.133333333333 * u))));
+    }
+  }
+}
+
+__device__ __forceinline__ void interact32(float mass, float r2, float h, float &fac, float &wp) {
+  float rinv = (r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;
+  float r = r2 * rinv;  // 0 for coincident points, which then take the u = 0 branch
+  if(r >= h) {
+    fac = mass * rinv * rinv * rinv;
+    wp = -mass * rinv;
+  } else {
+    float h_inv = 1.0f / h;
+    float h3_inv = h_inv * h_inv * h_inv;
+    float u = r * h_inv;
+    if(u < 0.5f) {
+      fac = mass * h3_inv * (10.666666666667f + u * u * (32.0f * u - 38.4f));
+      wp = mass * h_inv * (-2.8f + u * u * (5.333333333333f + u * u * (6.4f * u - 9.6f)));
+    } else {
+      float ui = 1.0f / u;
+      fac = mass * h3_inv *
+            (21.333333333333f - 48.0f * u + 38.4f * u * u - 10.666666666667f * u * u * u - 0.066666666667f * ui * ui * ui);
+      wp = mass * h_inv *
+           (-3.2f + 0.066666666667f * ui + u * u * (10.666666666667f + u * (-16.0f + u * (9.6f - 2.133333333333f * u))));
+    }
+  }
+}
+
+// forcetree_walk.c:58-192 for one target per lane (mode 0, one task)
+__global__ void __launch_bounds__(256) k_walk64(WalkArgs A) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= A.ntarget) return;
+  const double px = A.tx[t], py = A.ty[t], pz = A.tz[t];
+  double acc_x = 0, acc_y = 0, acc_z = 0, potv = 0, fac, wp;
+  unsigned long long nint = 0;
+  int no = A.nsrc;  // root
+  while(no >= 0) {
+    double dx, dy, dz, r2, mass;
+    if(no < A.nsrc) {
+      dx = A.x[no] - px;
+      dy = A.y[no] - py;
+      dz = A.z[no] - pz;
+      r2 = dx * dx + dy * dy + dz * dz;
+      mass = A.m[no];
+      no = A.nextp[no];
+    } else {
+      int q = no - A.nsrc;
+      mass = A.mass[q];
+      dx = A.sx[q] - px;
+      dy = A.sy[q] - py;
+      dz = A.sz[q] - pz;
+      r2 = dx * dx + dy * dy + dz * dz;
+      float l = A.len[q];
+      float l2 = l * l;
+      if(l2 > r2 * A.theta * A.theta) {
+        if(A.bucket[q]) {
+          for(int p = A.first[q]; p < A.first[q] + A.count[q]; p++) {
+            double ex = A.x[p] - px, ey = A.y[p] - py, ez = A.z[p] - pz;
+            double e2 = ex * ex + ey * ey + ez * ez;
+            if(A.m[p]) {
+              interact64(A.m[p], e2, A.h, fac, wp);
+              acc_x += ex * fac;
+              acc_y += ey * fac;
+              acc_z += ez * fac;
+              potv += wp;
+              nint++;
+            }
+          }
+          no = A.sibling[q];
+        } else {
+          no = A.nextnode[q];
+        }
+        continue;
+      }
+      no = A.sibling[q];
+    }
+    if(mass) {
+      interact64(mass, r2, A.h, fac, wp);
+      acc_x += dx * fac;
+      acc_y += dy * fac;
+      acc_z += dz * fac;
+      potv += wp;
+      nint++;
+    }
+  }
+  A.ax[t] = acc_x * A.G;
+  A.ay[t] = acc_y * A.G;
+  A.az[t] = acc_z * A.G;
+  A.pot[t] = potv * A.G;
+  atomicAdd(A.nint, nint);
+}
+
+// The same walk with FP32 interactions: separations are formed in FP64 (node
+// centres of mass and particles stay FP64) and rounded once, the kernel and the
+// sums run in FP32 per target, totals in FP64.
+__global__ void __launch_bounds__(256) k_walk32(WalkArgs A) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= A.ntarget) return;
+  const double px = A.tx[t], py = A.ty[t], pz = A.tz[t];
+  float acc_x = 0, acc_y = 0, acc_z = 0, potv = 0, fac, wp;
+  const float h = (float)A.h, th2 = (float)(A.theta * A.theta);
+  unsigned int nint = 0;
+  int no = A.nsrc;
+  while(no >= 0) {
+    float dx, dy, dz, r2, mass;
+    if(no < A.nsrc) {
+      dx = (float)(A.x[no] - px);
+      dy = (float)(A.y[no] - py);
+      dz = (float)(A.z[no] - pz);
+      r2 = dx * dx + dy * dy + dz * dz;
+      mass = (float)A.m[no];
+      no = A.nextp[no];
+    } else {
+      int q = no - A.nsrc;
+      mass = (float)A.mass[q];
+      dx = (float)(A.sx[q] - px);
+      dy = (float)(A.sy[q] - py);
+      dz = (float)(A.sz[q] - pz);
+      r2 = dx * dx + dy * dy + dz * dz;
+      float l = A.len[q];
+      if(l * l > r2 * th2) {
+        if(A.bucket[q]) {
+          for(int p = A.first[q]; p < A.first[q] + A.count[q]; p++) {
+            float ex = (float)(A.x[p] - px), ey = (float)(A.y[p] - py), ez = (float)(A.z[p] - pz);
+            float e2 = ex * ex + ey * ey + ez * ez;
+            if(A.m[p] != 0) {
+              interact32((float)A.m[p], e2, h, fac, wp);
+              acc_x += ex * fac;
+              acc_y += ey * fac;
+              acc_z += ez * fac;
+              potv += wp;
+              nint++;
+            }
+          }
+          no = A.sibling[q];
+        } else {
+          no = A.nextnode[q];
+        }
+        continue;
+      }
+      no = A.sibling[q];
+    }
+    if(mass != 0) {
+      interact32(mass, r2, h, fac, wp);
+      acc_x += dx * fac;
+      acc_y += dy * fac;
+      acc_z += dz * fac;
+      potv += wp;
+      nint++;
+    }
+  }
+  A.ax[t] = (double)acc_x * A.G;
+  A.ay[t] = (double)acc_y * A.G;
+  A.az[t] = (double)acc_z * A.G;
+  A.pot[t] = (double)potv * A.G;
+  atomicAdd(A.nint, (unsigned long long)nint);
+}
+
+// Table projection of grid.c:719-750: radial and vertical components summed
+// over the FG_SECTIONS azimuthal sections in order, then averaged.
+__global__ void k_project(int nb, const double *ax, const double *ay, const double *az, const double *pt,
+                          const double *cs, const double *sn, double *dpotdr, double *dpotdz, double *pot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= nb * nb) return;
+  double r = 0, zz = 0, p = 0;
+  for(int s = 0; s < GALIC_FG_SECTIONS; s++) {
+    size_t n = (size_t)s * nb * nb + i;
+    r += ax[n] * cs[s] + ay[n] * sn[s];
+    zz += az[n];
+    p += pt[n];
+  }
+  dpotdr[i] = r / GALIC_FG_SECTIONS;
+  dpotdz[i] = zz / GALIC_FG_SECTIONS;
+  pot[i] = p / GALIC_FG_SECTIONS;
+}
+
+}  // namespace tree
+}  // namespace galic
+
+namespace {
+
+using namespace galic::tree;
+
+int terr(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  return galic::set_error(code, buf);
+}
+
+#define TCHK(expr)                                                                                 \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if(e_ != hipSuccess) {                                                                         \
+      rc = terr(e_ == hipErrorOutOfMemory ? GALIC_ERR_NOMEM : GALIC_ERR_HIP, "%s: %s", #expr,     \
+                hipGetErrorString(e_));                                                            \
+      goto done;                                                                                   \
+    }                                                                                              \
+  } while(0)
+
+inline int nb256(size_t n) { return (int)((n + 255) / 256); }
+
+struct DevBufs {
+  std::vector<void *> ptrs;
+  template <typename T>
+  hipError_t alloc(T **p, size_t n) {
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+    if(e == hipSuccess) ptrs.push_back((void *)*p);
+    return e;
+  }
+  ~DevBufs() {
+    for(void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+// Build the tree of nsrc sources (device SoA inputs) and walk ntarget targets.
+int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const double *dpx, const double *dpy,
+                    const double *dpz, const double *dpm, int ntarget, const double *dtx, const double *dty,
+                    const double *dtz, double corner[3], double dlen, double *dax, double *day, double *daz,
+                    double *dpot, int precision, long long *ninteractions, float *walk_ms) {
+  int rc = GALIC_OK;
+  DevBufs B;
+  Tree T;
+  unsigned long long *keys_in = nullptr, *nint = nullptr;
+  int *idx_in = nullptr, *idx = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  T.nsrc = nsrc;
+  TCHK(B.alloc(&keys_in, nsrc));
+  TCHK(B.alloc(&T.key, nsrc));
+  TCHK(B.alloc(&idx_in, nsrc));
+  TCHK(B.alloc(&idx, nsrc));
+  TCHK(B.alloc(&T.x, nsrc));
+  TCHK(B.alloc(&T.y, nsrc));
+  TCHK(B.alloc(&T.z, nsrc));
+  TCHK(B.alloc(&T.m, nsrc));
+  TCHK(B.alloc(&T.nextp, nsrc));
+  TCHK(B.alloc(&nint, 1));
+  TCHK(hipMemsetAsync(nint, 0, sizeof(unsigned long long), st));
+  if(nsrc > 0) {
+    hipLaunchKernelGGL(k_keys, dim3(nb256(nsrc)), dim3(256), 0, st, nsrc, dpx, dpy, dpz, corner[0], corner[1],
+                       corner[2], 1.0 / dlen, keys_in, idx_in);
+    TCHK(hipGetLastError());
+    TCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, T.key, idx_in, idx, nsrc, 0, 63, st));
+    TCHK(B.alloc((char **)&tmp, tmp_bytes));
+    TCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, T.key, idx_in, idx, nsrc, 0, 63, st));
+    hipLaunchKernelGGL(k_gather, dim3(nb256(nsrc)), dim3(256), 0, st, nsrc, (const int *)idx, dpx, dpy, dpz, dpm, T.x,
+                       T.y, T.z, T.m);
+    TCHK(hipGetLastError());
+  }
+  {
+    // nodes: the unique insertion tree has fewer than 2 nsrc nodes unless
+    // particles pair up at great depth; grow and rebuild if needed
+    // (the reference raises TreeAllocFactor and retries, forcetree.c:48-70)
+    T.cap = 2 * nsrc + 64;
+    for(int attempt = 0; attempt < 6; attempt++) {
+      TCHK(B.alloc(&T.first, T.cap));
+      TCHK(B.alloc(&T.count, T.cap));
+      TCHK(B.alloc(&T.depth, T.cap));
+      TCHK(B.alloc(&T.child, (size_t)T.cap * 8));
+      TCHK(B.alloc(&T.bucket, T.cap));
+      TCHK(B.alloc(&T.counter, 1));
+      int root[3] = {0, nsrc, 0}, one = 1;
+      TCHK(hipMemcpyAsync(T.first, &root[0], sizeof(int), hipMemcpyHostToDevice, st));
+      TCHK(hipMemcpyAsync(T.count, &root[1], sizeof(int), hipMemcpyHostToDevice, st));
+      TCHK(hipMemcpyAsync(T.depth, &root[2], sizeof(int), hipMemcpyHostToDevice, st));
+      TCHK(hipMemcpyAsync(T.counter, &one, sizeof(int), hipMemcpyHostToDevice, st));
+      T.level_lo.assign(1, 0);
+      int lo = 0, hi = 1, d = 0;
+      bool overflow = false;
+      while(lo < hi) {
+        T.level_lo.push_back(hi);
+        hipLaunchKernelGGL(k_split, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, d, nsrc, T.cap,
+                           (const unsigned long long *)T.key, T.first, T.count, T.depth, T.child, T.bucket, T.counter);
+        TCHK(hipGetLastError());
+        int nhi = 0;
+        TCHK(hipMemcpyAsync(&nhi, T.counter, sizeof(int), hipMemcpyDeviceToHost, st));
+        TCHK(hipStreamSynchronize(st));
+        if(nhi > T.cap) {
+          overflow = true;
+          break;
+        }
+        lo = hi;
+        hi = nhi;
+        d++;
+      }
+      if(!overflow) break;
+      T.cap *= 2;
+      if(attempt == 5) {
+        rc = terr(GALIC_ERR_NOMEM, "tree node pool overflow");
+        goto done;
+      }
+    }
+  }
+  {
+    const int nnodes = T.level_lo.back();
+    const int ndepth = (int)T.level_lo.size() - 1;
+    TCHK(B.alloc(&T.mass, nnodes));
+    TCHK(B.alloc(&T.sx, nnodes));
+    TCHK(B.alloc(&T.sy, nnodes));
+    TCHK(B.alloc(&T.sz, nnodes));
+    TCHK(B.alloc(&T.len, nnodes));
+    TCHK(B.alloc(&T.sibling, nnodes));
+    TCHK(B.alloc(&T.nextnode, nnodes));
+    for(int d = ndepth - 1; d >= 0; d--) {
+      int lo = T.level_lo[d], hi = T.level_lo[d + 1];
+      if(hi <= lo) continue;
+      hipLaunchKernelGGL(k_moments, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, (const int *)T.child,
+                         (const int *)T.bucket, (const int *)T.first, (const int *)T.count, (const double *)T.x,
+                         (const double *)T.y, (const double *)T.z, (const double *)T.m, T.mass, T.sx, T.sy, T.sz);
+      // side length DomainLen / 2^d, held in float as NODE.len (allvars.h:1095)
+      hipLaunchKernelGGL(k_len, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, ldexp(dlen, -d), T.len);
+      TCHK(hipGetLastError());
+    }
+    int minus1 = -1;
+    TCHK(hipMemcpyAsync(T.sibling, &minus1, sizeof(int), hipMemcpyHostToDevice, st));
+    for(int d = 0; d < ndepth; d++) {
+      int lo = T.level_lo[d], hi = T.level_lo[d + 1];
+      if(hi <= lo) continue;
+      hipLaunchKernelGGL(k_thread, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, (const int *)T.child,
+                         (const int *)T.bucket, T.sibling, T.nextnode, T.nextp);
+      TCHK(hipGetLastError());
+    }
+    WalkArgs W;
+    W.nsrc = nsrc;
+    W.ntarget = ntarget;
+    W.tx = dtx;
+    W.ty = dty;
+    W.tz = dtz;
+    W.x = T.x;
+    W.y = T.y;
+    W.z = T.z;
+    W.m = T.m;
+    W.nextp = T.nextp;
+    W.first = T.first;
+    W.count = T.count;
+    W.bucket = T.bucket;
+    W.sibling = T.sibling;
+    W.nextnode = T.nextnode;
+    W.mass = T.mass;
+    W.sx = T.sx;
+    W.sy = T.sy;
+    W.sz = T.sz;
+    W.len = T.len;
+    W.theta = p->ErrTolTheta;
+    W.h = p->ForceSoftening;
+    W.G = p->G;
+    W.ax = dax;
+    W.ay = day;
+    W.az = daz;
+    W.pot = dpot;
+    W.nint = nint;
+    TCHK(hipEventCreate(&e0));
+    TCHK(hipEventCreate(&e1));
+    TCHK(hipEventRecord(e0, st));
+    if(ntarget > 0) {
+      if(precision == 0)
+        hipLaunchKernelGGL(k_walk64, dim3(nb256(ntarget)), dim3(256), 0, st, W);
+      else
+        hipLaunchKernelGGL(k_walk32, dim3(nb256(ntarget)), dim3(256), 0, st, W);
+      TCHK(hipGetLastError());
+    }
+    TCHK(hipEventRecord(e1, st));
+    TCHK(hipEventSynchronize(e1));
+    if(walk_ms) TCHK(hipEventElapsedTime(walk_ms, e0, e1));
+    unsigned long long ni = 0;
+    TCHK(hipMemcpy(&ni, nint, sizeof(ni), hipMemcpyDeviceToHost));
+    if(ninteractions) *ninteractions = (long long)ni;
+  }
+done:
+  if(e0) (void)hipEventDestroy(e0);
+  if(e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// domain_box.c:20-65 over sources and targets
+void domain_extent(int nsrc, const double *spos, int ntarget, const double *tpos, double corner[3], double *len) {
+  double xmin[3] = {1e30, 1e30, 1e30}, xmax[3] = {-1e30, -1e30, -1e30}, l = 0;
+  for(int i = 0; i < nsrc + ntarget; i++) {
+    const double *x = i < nsrc ? &spos[3 * (size_t)i] : &tpos[3 * (size_t)(i - nsrc)];
+    for(int j = 0; j < 3; j++) {
+      if(xmin[j] > x[j]) xmin[j] = x[j];
+      if(xmax[j] < x[j]) xmax[j] = x[j];
+    }
+  }
+  for(int j = 0; j < 3; j++)
+    if(xmax[j] - xmin[j] > l) l = xmax[j] - xmin[j];
+  l *= 1.00001;
+  for(int j = 0; j < 3; j++) corner[j] = 0.5 * (xmin[j] + xmax[j]) - 0.5 * l;
+  *len = l;
+}
+
+int upload_soa(hipStream_t st, int n, const double *aos, double *dx, double *dy, double *dz) {
+  std::vector<double> t(std::max(n, 1));
+  double *d[3] = {dx, dy, dz};
+  for(int k = 0; k < 3; k++) {
+    for(int i = 0; i < n; i++) t[i] = aos[3 * (size_t)i + k];
+    if(n > 0 && hipMemcpyAsync(d[k], t.data(), sizeof(double) * n, hipMemcpyHostToDevice, st) != hipSuccess)
+      return GALIC_ERR_HIP;
+    if(hipStreamSynchronize(st) != hipSuccess) return GALIC_ERR_HIP;
+  }
+  return GALIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int galic_tree_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                      int ntarget, const double *tpos, double *acc, double *pot, int precision,
+                      long long *ninteractions) {
+  if(!p || nsrc < 0 || ntarget < 0 || (nsrc > 0 && (!spos || !smass)) || (ntarget > 0 && (!tpos || !acc || !pot)))
+    return terr(GALIC_ERR_ARG, "bad tree arguments");
+  if(precision != 0 && precision != 1) return terr(GALIC_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+  int rc = GALIC_OK;
+  hipStream_t st = nullptr;
+  DevBufs B;
+  double *px, *py, *pz, *pm, *tx, *ty, *tz, *ax, *ay, *az, *pt;
+  double corner[3], len;
+  float ms = 0;
+  std::vector<double> h;
+  TCHK(hipSetDevice(device));
+  TCHK(hipStreamCreate(&st));
+  domain_extent(nsrc, spos, ntarget, tpos, corner, &len);
+  TCHK(B.alloc(&px, nsrc)); TCHK(B.alloc(&py, nsrc)); TCHK(B.alloc(&pz, nsrc)); TCHK(B.alloc(&pm, nsrc));
+  TCHK(B.alloc(&tx, ntarget)); TCHK(B.alloc(&ty, ntarget)); TCHK(B.alloc(&tz, ntarget));
+  TCHK(B.alloc(&ax, ntarget)); TCHK(B.alloc(&ay, ntarget)); TCHK(B.alloc(&az, ntarget)); TCHK(B.alloc(&pt, ntarget));
+  if((rc = upload_soa(st, nsrc, spos, px, py, pz)) != GALIC_OK) goto done;
+  if((rc = upload_soa(st, ntarget, tpos, tx, ty, tz)) != GALIC_OK) goto done;
+  if(nsrc > 0) TCHK(hipMemcpy(pm, smass, sizeof(double) * nsrc, hipMemcpyHostToDevice));
+  rc = tree_forces_dev(p, st, nsrc, px, py, pz, pm, ntarget, tx, ty, tz, corner, len, ax, ay, az, pt, precision,
+                       ninteractions, &ms);
+  if(rc != GALIC_OK) goto done;
+  h.resize(std::max(ntarget, 1));
+  {
+    double *src[3] = {ax, ay, az};
+    for(int k = 0; k < 3; k++) {
+      if(ntarget > 0) TCHK(hipMemcpy(h.data(), src[k], sizeof(double) * ntarget, hipMemcpyDeviceToHost));
+      for(int i = 0; i < ntarget; i++) acc[3 * (size_t)i + k] = h[i];
+    }
+    if(ntarget > 0) TCHK(hipMemcpy(pot, pt, sizeof(double) * ntarget, hipMemcpyDeviceToHost));
+  }
+done:
+  if(st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
+int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                    double *pot, double *dpotdr, double *dpotdz, int precision, long long *ninteractions) {
+  if(!p || nsrc < 0 || (nsrc > 0 && (!spos || !smass)) || !pot || !dpotdr || !dpotdz || p->FG_Nbin < 2)
+    return terr(GALIC_ERR_ARG, "bad table-fill arguments");
+  if(precision != 0 && precision != 1) return terr(GALIC_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+  const int nb = p->FG_Nbin;
+  const size_t ns = (size_t)GALIC_FG_SECTIONS * nb * nb;
+  if(ns > 0x7fffffff) return terr(GALIC_ERR_ARG, "FG_Nbin too large");
+  // targets of grid.c:680-704: (R_j cos phi_s, R_j sin phi_s, z_k)
+  std::vector<double> tpos(3 * ns), cs(GALIC_FG_SECTIONS), sn(GALIC_FG_SECTIONS);
+  for(int s = 0; s < GALIC_FG_SECTIONS; s++) {
+    double phi = 2 * M_PI / GALIC_FG_SECTIONS * s;
+    cs[s] = cos(phi);
+    sn[s] = sin(phi);
+  }
+  size_t n = 0;
+  for(int s = 0; s < GALIC_FG_SECTIONS; s++)
+    for(int k = 0; k < nb; k++)
+      for(int j = 0; j < nb; j++, n++) {
+        double r1 = p->FG_Rmin * (pow(p->FG_Fac, j) - 1.0);
+        double z1 = p->FG_Rmin * (pow(p->FG_Fac, k) - 1.0);
+        tpos[3 * n] = r1 * cs[s];
+        tpos[3 * n + 1] = r1 * sn[s];
+        tpos[3 * n + 2] = z1;
+      }
+  int rc = GALIC_OK;
+  hipStream_t st = nullptr;
+  DevBufs B;
+  double *px, *py, *pz, *pm, *tx, *ty, *tz, *ax, *ay, *az, *pt, *dcs, *dsn, *r0, *r1, *r2;
+  double corner[3], len;
+  float ms = 0;
+  TCHK(hipSetDevice(device));
+  TCHK(hipStreamCreate(&st));
+  domain_extent(nsrc, spos, (int)ns, tpos.data(), corner, &len);
+  TCHK(B.alloc(&px, nsrc)); TCHK(B.alloc(&py, nsrc)); TCHK(B.alloc(&pz, nsrc)); TCHK(B.alloc(&pm, nsrc));
+  TCHK(B.alloc(&tx, ns)); TCHK(B.alloc(&ty, ns)); TCHK(B.alloc(&tz, ns));
+  TCHK(B.alloc(&ax, ns)); TCHK(B.alloc(&ay, ns)); TCHK(B.alloc(&az, ns)); TCHK(B.alloc(&pt, ns));
+  TCHK(B.alloc(&dcs, GALIC_FG_SECTIONS)); TCHK(B.alloc(&dsn, GALIC_FG_SECTIONS));
+  TCHK(B.alloc(&r0, (size_t)nb * nb)); TCHK(B.alloc(&r1, (size_t)nb * nb)); TCHK(B.alloc(&r2, (size_t)nb * nb));
+  if((rc = upload_soa(st, nsrc, spos, px, py, pz)) != GALIC_OK) goto done;
+  if((rc = upload_soa(st, (int)ns, tpos.data(), tx, ty, tz)) != GALIC_OK) goto done;
+  if(nsrc > 0) TCHK(hipMemcpy(pm, smass, sizeof(double) * nsrc, hipMemcpyHostToDevice));
+  TCHK(hipMemcpy(dcs, cs.data(), sizeof(double) * GALIC_FG_SECTIONS, hipMemcpyHostToDevice));
+  TCHK(hipMemcpy(dsn, sn.data(), sizeof(double) * GALIC_FG_SECTIONS, hipMemcpyHostToDevice));
+  rc = tree_forces_dev(p, st, nsrc, px, py, pz, pm, (int)ns, tx, ty, tz, corner, len, ax, ay, az, pt, precision,
+                       ninteractions, &ms);
+  if(rc != GALIC_OK) goto done;
+  hipLaunchKernelGGL(k_project, dim3(nb256((size_t)nb * nb)), dim3(256), 0, st, nb, (const double *)ax,
+                     (const double *)ay, (const double *)az, (const double *)pt, (const double *)dcs,
+                     (const double *)dsn, r1, r2, r0);
+  TCHK(hipGetLastError());
+  TCHK(hipStreamSynchronize(st));
+  TCHK(hipMemcpy(dpotdr, r1, sizeof(double) * nb * nb, hipMemcpyDeviceToHost));
+  TCHK(hipMemcpy(dpotdz, r2, sizeof(double) * nb * nb, hipMemcpyDeviceToHost));
+  TCHK(hipMemcpy(pot, r0, sizeof(double) * nb * nb, hipMemcpyDeviceToHost));
+done:
+  if(st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
+}  // extern "C"

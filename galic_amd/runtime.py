@@ -31,6 +31,11 @@ class GalicGPU:
             st = [np.ascontiguousarray(s, dtype=np.float64) for s in st]
             check(self.lib.galic_set_targets(self.h, t, *[dptr(s) for s in st]))
         self.set_particles(w.pos, w.vel)
+        self.set_dispersions(w.disp)
+
+    def set_dispersions(self, disp):
+        disp = np.ascontiguousarray(disp, dtype=np.float64)
+        check(self.lib.galic_set_dispersions(self.h, len(disp), dptr(disp)))
 
     def set_particles(self, pos, vel):
         w = self.w

@@ -65,6 +65,10 @@ int galic_set_targets(galic_ctx *ctx, int type, const double *dgs_mass_target, c
  * type and Vesc.  Copied to device SoA arrays. */
 int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *vel, const double *mass,
                         const double *tint, const int *type, const double *vesc);
+/* Target dispersions of each particle, P[].vr2_target, vt2_target, vp2_target,
+ * vq2_target (set_particles.c:158-161): 4*n doubles, particle-major.  Used by
+ * the device trial draws of galic_fp32_rep. */
+int galic_set_dispersions(galic_ctx *ctx, int n, const double *disp4);
 /* Reads back P[].Vel, P[].Orbits and P[].vr2..vq2; any pointer may be NULL. */
 int galic_get_particles(galic_ctx *ctx, double *vel, int *orbits, double *vr2, double *vt2, double *vp2,
                         double *vq2);
