@@ -58,7 +58,7 @@ EXPORTS = [
     "galic_set_particles", "galic_set_dispersions", "galic_get_particles", "galic_calc_all_response_fields", "galic_rep",
     "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_field_at",
     "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_tree_forces", "galic_tree_fill",
-    "galic_timings", "galic_reset_timings", "galic_bind",
+    "galic_timings", "galic_reset_timings", "galic_copy_probe", "galic_bind",
     "produce_orbit_response_field_mod", "produce_orbit_response_field",
     "forcegrid_get_potential", "forcegrid_get_escape_speed", "forcegrid_get_acceleration", "smooth_stack",
 ]
@@ -116,6 +116,7 @@ def lib():
                             _lp],
         "galic_timings": [_vp, _dp],
         "galic_reset_timings": [_vp],
+        "galic_copy_probe": [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp],
         "galic_bind": [_vp],
     }
     for name, args in sig.items():

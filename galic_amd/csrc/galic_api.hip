@@ -178,6 +178,7 @@ galic::OrbitArgs orbit_args(galic::Ctx &c, int n) {
   A.tall = c.o_tall; A.e0 = c.o_e0; A.e1 = c.o_e1;
   A.norbit = c.o_norbit; A.status = c.o_status; A.steps = c.o_steps; A.bbox = c.o_bbox;
   A.slot_stride = (size_t)galic::kFields * c.Ngrid;
+  A.stats = c.orbit_stats;
   return A;
 }
 
@@ -193,9 +194,26 @@ int orbit_grid(int n, int bs) {
   return std::max(1, std::min(want, need));
 }
 
+// Steps and the first failure of an orbit launch (orbit_response.c:449-456 exits on the step cap).
+int finish_orbit_launch(galic::Ctx &c, int n) {
+  unsigned long long st[3];
+  HIPCHK(hipMemcpy(st, c.orbit_stats, sizeof(st), hipMemcpyDeviceToHost));
+  c.acc_steps += (long long)st[0];
+  c.acc_orbits += n;
+  if(st[1]) {
+    long long status = (long long)st[2];
+    int o = (int)(st[1] - 1);
+    if(status == GALIC_ERR_STEPCAP)
+      return set_err(GALIC_ERR_STEPCAP, "orbit %d exceeded the step cap of %lld steps", o, (long long)c.M.max_steps);
+    return set_err(GALIC_ERR_ARG, "orbit %d has a non-positive or non-finite timestep", o);
+  }
+  return GALIC_OK;
+}
+
 int launch_orbit64(galic::Ctx &c, galic::OrbitArgs A) {
   if(A.n <= 0) return GALIC_OK;
   HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
+  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 3 * sizeof(unsigned long long), c.stream));
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   const int bs = 256;
   hipLaunchKernelGGL(galic::k_orbit64, dim3(orbit_grid(A.n, bs)), dim3(bs), 0, c.stream, c.M, A);
@@ -207,21 +225,7 @@ int launch_orbit64(galic::Ctx &c, galic::OrbitArgs A) {
   c.last_orbit_ms = ms;
   c.acc_orbit_ms += ms;
   c.acc_orbit_launches++;
-  c.acc_orbits += A.n;
-  // orbit status: any step-cap or argument failure is an error (orbit_response.c:449-456)
-  std::vector<int> st(A.n);
-  std::vector<long long> steps(A.n);
-  HIPCHK(hipMemcpy(st.data(), A.status, sizeof(int) * A.n, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(steps.data(), A.steps, sizeof(long long) * A.n, hipMemcpyDeviceToHost));
-  long long tot = 0;
-  for(int o = 0; o < A.n; o++) {
-    tot += steps[o];
-    if(st[o] == GALIC_ERR_STEPCAP)
-      return set_err(GALIC_ERR_STEPCAP, "orbit %d exceeded the step cap of %lld steps", o, (long long)c.M.max_steps);
-    if(st[o] != 0) return set_err(GALIC_ERR_ARG, "orbit %d has a non-positive integration time", o);
-  }
-  c.acc_steps += tot;
-  return GALIC_OK;
+  return finish_orbit_launch(c, A.n);
 }
 
 galic::FitArgs fit_args(galic_ctx *ctx) {
@@ -364,12 +368,14 @@ galic::OrbitArgs32 orbit_args32(galic::Ctx &c, int n) {
   A.omass = c.o_mass; A.otint = c.o_tint; A.otype = c.o_type;
   A.tall = c.o_tall; A.norbit = c.o_norbit; A.status = c.o_status; A.steps = c.o_steps; A.bbox = c.o_bbox;
   A.slot_stride = (size_t)galic::kFields * c.Ngrid;
+  A.stats = c.orbit_stats;
   return A;
 }
 
 int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
   if(A.n <= 0) return GALIC_OK;
   HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
+  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 3 * sizeof(unsigned long long), c.stream));
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   const int bs = 256;
   hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs)), dim3(bs), 0, c.stream, fast_model(c), A);
@@ -381,20 +387,7 @@ int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
   c.last_orbit_ms = ms;
   c.acc_orbit_ms += ms;
   c.acc_orbit_launches++;
-  c.acc_orbits += A.n;
-  std::vector<int> st(A.n);
-  std::vector<long long> steps(A.n);
-  HIPCHK(hipMemcpy(st.data(), A.status, sizeof(int) * A.n, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(steps.data(), A.steps, sizeof(long long) * A.n, hipMemcpyDeviceToHost));
-  long long tot = 0;
-  for(int o = 0; o < A.n; o++) {
-    tot += steps[o];
-    if(st[o] == GALIC_ERR_STEPCAP)
-      return set_err(GALIC_ERR_STEPCAP, "orbit %d exceeded the step cap of %lld steps", o, (long long)c.M.max_steps);
-    if(st[o] != 0) return set_err(GALIC_ERR_ARG, "orbit %d has a non-positive or non-finite timestep", o);
-  }
-  c.acc_steps += tot;
-  return GALIC_OK;
+  return finish_orbit_launch(c, A.n);
 }
 
 // Loc += delta in fixed point, and the FP64 copies the global fit reads.
@@ -419,6 +412,14 @@ size_t slot_budget(galic::Ctx &c) {
 }  // namespace
 
 namespace galic {
+// Plain streaming copy: the reference point that calibrates the HBM byte
+// counters against a kernel whose traffic is known exactly.
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for(; i < n; i += stride) dst[i] = src[i];
+}
+
 int set_error(int code, const char *msg) { return set_err(code, "%s", msg); }
 galic_ctx *bound_ctx() { return g_bound; }
 Ctx &ctx_state(galic_ctx *ctx) { return ctx->c; }
@@ -429,6 +430,34 @@ extern "C" {
 
 const char *galic_last_error(void) { return g_err.c_str(); }
 int galic_version(void) { return 10100; }
+
+int galic_copy_probe(int device, size_t nbytes, int reps, double *ms_per_copy) {
+  if(nbytes < 16 || reps < 1) return set_err(GALIC_ERR_ARG, "copy probe needs >= 16 bytes and >= 1 rep");
+  HIPCHK(hipSetDevice(device));
+  size_t n = nbytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n * 16));
+  HIPCHK(hipMalloc(&b, n * 16));
+  HIPCHK(hipMemset(a, 0, n * 16));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(galic::k_copy, dim3(blocks), dim3(256), 0, 0, (const float4 *)a, b, n);
+  HIPCHK(hipEventRecord(e0, 0));
+  for(int r = 0; r < reps; r++) hipLaunchKernelGGL(galic::k_copy, dim3(blocks), dim3(256), 0, 0, (const float4 *)a, b, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if(ms_per_copy) *ms_per_copy = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return GALIC_OK;
+}
 
 int galic_grid_create(galic_params *p, double outer_frac, double inner_frac) {
   if(!p || !(outer_frac > 0 && outer_frac < 1) || !(inner_frac > 0 && inner_frac < 1))
@@ -541,6 +570,7 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.tovs_dev, GALIC_NTYPES));
   CHK(dalloc(&c.fit, (size_t)GALIC_NTYPES * 16));
   CHK(dalloc(&c.counter, 1));
+  CHK(dalloc(&c.orbit_stats, 3));
 
   // geometry
   ctx->distance.assign(c.Nstack, 0.0);
@@ -610,7 +640,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
-  dfree(c.fit); dfree(c.counter); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk);
+  dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk);
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
@@ -913,7 +943,17 @@ int galic_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const d
     CHK(launch_orbit64(c, A));
     if(!fields) HIPCHK(hipMemsetAsync(c.locA_d, 0, sizeof(double) * GALIC_NTYPES * galic::kFields * c.Ngrid, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
-    if(fields) HIPCHK(hipMemcpy(fields + (size_t)base * galic::kFields * c.Ngrid, c.slots, per * n, hipMemcpyDeviceToHost));
+    if(fields) {
+      // slots are cell-major [cell][field]; the ABI returns 5 grids per orbit
+      std::vector<double> cm((size_t)n * galic::kFields * c.Ngrid);
+      HIPCHK(hipMemcpy(cm.data(), c.slots, per * n, hipMemcpyDeviceToHost));
+      for(int o = 0; o < n; o++) {
+        const double *src = cm.data() + (size_t)o * galic::kFields * c.Ngrid;
+        double *dst = fields + (size_t)(base + o) * galic::kFields * c.Ngrid;
+        for(int cell = 0; cell < c.Ngrid; cell++)
+          for(int f = 0; f < galic::kFields; f++) dst[(size_t)f * c.Ngrid + cell] = src[(size_t)cell * galic::kFields + f];
+      }
+    }
     if(fields) HIPCHK(hipMemset(c.slots, 0, per * n));
     if(t) HIPCHK(hipMemcpy(t + base, c.o_tall, sizeof(double) * n, hipMemcpyDeviceToHost));
     if(orbits) HIPCHK(hipMemcpy(orbits + base, c.o_norbit, sizeof(int) * n, hipMemcpyDeviceToHost));

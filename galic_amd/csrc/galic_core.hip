@@ -45,13 +45,15 @@ __device__ __forceinline__ void run_flush(const Run64 &r, int nb, int ngrid, dou
   if(r.quad < 0) return;
   const int c0 = r.quad, c1 = r.quad + 1, c2 = r.quad + nb, c3 = r.quad + nb + 1;
   if(slot) {
+    // cell-major slot: the 5 fields of a cell are contiguous, so a flush
+    // touches two short row segments instead of ten field rows
+    double *s0 = slot + (size_t)c0 * kFields, *s2 = slot + (size_t)c2 * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      double *s = slot + (size_t)f * ngrid;
-      s[c0] += fac * r.w[f][0];
-      s[c1] += fac * r.w[f][1];
-      s[c2] += fac * r.w[f][2];
-      s[c3] += fac * r.w[f][3];
+      s0[f] += fac * r.w[f][0];
+      s0[kFields + f] += fac * r.w[f][1];
+      s2[f] += fac * r.w[f][2];
+      s2[kFields + f] += fac * r.w[f][3];
     }
   } else {
 #pragma unroll
@@ -172,6 +174,7 @@ __global__ void __launch_bounds__(256) k_orbit64(DevModel M, OrbitArgs A) {
     A.e1[o] = 0.5 * (vx * vx + vy * vy + vz * vz) + potential64(M, x, y, z);
     A.bbox[o] = bb;
     A.status[o] = status;
+    orbit_stats_add(A.stats, steps, status, o);
   }
 }
 
@@ -315,7 +318,7 @@ __global__ void __launch_bounds__(256) k_eval_apply64(DevModel M, EvalArgs A) {
     const double *ratio = (f == F_M) ? nullptr : W + (W_SR + f - 1) * Nstack;
     for(int k = tid; k < ncoarse; k += nt) cs[k] = 0;
     __syncthreads();
-    const double *fo = so + (size_t)f * ngrid, *fn = sn + (size_t)f * ngrid;
+    const double *fo = so + f, *fn = sn + f;  // cell-major slots: field f of cell c at c*5 + f
     for(int k = tid; k < npr * npz; k += nt) {
       int pz = pz0 + k / npr, pr = pr0 + k % npr;
       // children in the order of smooth_stack (grid.c:1256-1260)
@@ -324,7 +327,7 @@ __global__ void __launch_bounds__(256) k_eval_apply64(DevModel M, EvalArgs A) {
       double d[4];
       for(int c = 0; c < 4; c++) {
         int cell = cz[c] * nb + cr[c];
-        d[c] = fn[cell] - fo[cell];
+        d[c] = fn[(size_t)cell * kFields] - fo[(size_t)cell * kFields];
         int node = offL + cell;
         if(d[c] != 0 && den[node] != 0) {
           double b = R[node] - T[node];
@@ -407,10 +410,10 @@ __global__ void __launch_bounds__(256) k_eval_apply64(DevModel M, EvalArgs A) {
   for(int k = tid; k < nr * nzz; k += nt) {
     int cell = (z0 + k / nr) * nb + (r0 + k % nr);
     for(int f = 0; f < kFields; f++) {
-      size_t o = (size_t)f * ngrid + cell;
+      size_t o = (size_t)cell * kFields + f;
       if(acc) {
         double d = wn[o] - wo[o];
-        if(d != 0) atomicAdd(&lad[o], d);
+        if(d != 0) atomicAdd(&lad[(size_t)f * ngrid + cell], d);
       }
       wo[o] = 0;
       wn[o] = 0;

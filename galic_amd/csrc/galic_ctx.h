@@ -7,7 +7,7 @@
 //   Loc grids      [type][field][cell], field = m, r, t, q, p (orbit-averaged)
 //                  and [type][field][cell], field = r, t, p, q (instantaneous)
 //   stacks         [type][which][Nstack], quadtree levels as STACKOFFSET
-//   orbit slots    [orbit][field][cell], one private response per orbit
+//   orbit slots    [orbit][cell][field], one private response per orbit (cell-major)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -119,6 +119,7 @@ struct Ctx {
   long long *o_steps = nullptr;
   int4 *o_bbox = nullptr;
   unsigned int *counter = nullptr;
+  unsigned long long *orbit_stats = nullptr;  // [0] steps of the launch, [1] first failed orbit + 1, [2] its status
 
   // per-trial scratch
   int max_trials = 0;

@@ -36,6 +36,7 @@ struct OrbitArgs32 {
   int *norbit, *status;
   long long *steps;
   int4 *bbox;
+  unsigned long long *stats;
 };
 
 struct EvalArgs32 {

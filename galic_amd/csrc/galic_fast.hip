@@ -172,11 +172,14 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
   if(r.quad < 0) return;
   const int c[4] = {r.quad, r.quad + 1, r.quad + nb, r.quad + nb + 1};
   if(slot) {
+    // cell-major slot (5 fields per cell contiguous): two short row segments per flush
+    float *s0 = slot + (size_t)c[0] * kFields, *s2 = slot + (size_t)c[2] * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      float *s = slot + f * ngrid;
-#pragma unroll
-      for(int k = 0; k < 4; k++) s[c[k]] += fac * r.w[f][k];
+      s0[f] += fac * r.w[f][0];
+      s0[kFields + f] += fac * r.w[f][1];
+      s2[f] += fac * r.w[f][2];
+      s2[kFields + f] += fac * r.w[f][3];
     }
   } else {
 #pragma unroll
@@ -286,6 +289,7 @@ __global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) {
     A.steps[o] = steps;
     A.bbox[o] = bb;
     A.status[o] = status;
+    orbit_stats_add(A.stats, steps, status, o);
   }
 }
 
@@ -516,7 +520,7 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
       const double *ratio = (f == F_M) ? nullptr : W + (W_SR + f - 1) * Nstack;
       for(int q = tid; q < ncoarse; q += nt) cs[q] = 0;
       __syncthreads();
-      const float *fo = so + (size_t)f * ngrid, *fn = sn + (size_t)f * ngrid;
+      const float *fo = so + f, *fn = sn + f;  // field f of cell c at c*5 + f
       for(int q = tid; q < npr * npz; q += nt) {
         int pz = pz0 + q / npr, pr = pr0 + q % npr;
         int cz[4] = {2 * pz, 2 * pz + 1, 2 * pz, 2 * pz + 1};
@@ -524,7 +528,7 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
         double d[4];
         for(int c = 0; c < 4; c++) {
           int cell = cz[c] * nb + cr[c];
-          d[c] = (double)fn[cell] - (double)fo[cell];
+          d[c] = (double)fn[(size_t)cell * kFields] - (double)fo[(size_t)cell * kFields];
           int node = offL + cell;
           if(d[c] != 0 && den[node] != 0) {
             double wgt = ratio ? ratio[node] : 1.0;
@@ -623,9 +627,9 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
     for(int q = tid; q < nr * nzz; q += nt) {
       int cell = (z0 + q / nr) * nb + (r0 + q % nr);
       for(int f = 0; f < kFields; f++) {
-        size_t o = (size_t)f * ngrid + cell;
+        size_t o = (size_t)cell * kFields + f;
         float d0 = sn[o], d1 = so[o];
-        if(d0 != d1) atomic_q(&lad[o], A.qscale[f] * ((double)d0 - (double)d1));
+        if(d0 != d1) atomic_q(&lad[(size_t)f * ngrid + cell], A.qscale[f] * ((double)d0 - (double)d1));
       }
     }
     __syncthreads();
@@ -637,7 +641,7 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
     int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
     for(int q = tid; q < nr * nzz; q += nt) {
       int cell = (b.z + q / nr) * nb + (b.x + q % nr);
-      for(int f = 0; f < kFields; f++) s[(size_t)f * ngrid + cell] = 0.f;
+      for(int f = 0; f < kFields; f++) s[(size_t)cell * kFields + f] = 0.f;
     }
   }
 }

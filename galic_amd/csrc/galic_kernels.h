@@ -32,6 +32,7 @@ struct OrbitArgs {
   int *norbit, *status;
   long long *steps;
   int4 *bbox;          // touched cells: R from .x to .y, z from .z to .w (inclusive)
+  unsigned long long *stats;  // [0] += steps, [1] = failed orbit + 1, [2] = its status
 };
 
 struct EvalArgs {
@@ -65,6 +66,13 @@ struct FitArgs {
 };
 
 __global__ void k_orbit64(DevModel M, OrbitArgs A);
+
+// Per-lane totals of an orbit launch folded into three counters (no per-orbit download).
+__device__ __forceinline__ void orbit_stats_add(unsigned long long *stats, long long steps, int status, int o) {
+  atomicAdd(&stats[0], (unsigned long long)steps);
+  if(status != 0 && atomicCAS(&stats[1], 0ull, (unsigned long long)o + 1) == 0ull)
+    atomicExch(&stats[2], (unsigned long long)(long long)status);
+}
 __global__ void k_inst_init(DevModel M, Particles P, double *locI, double *locM, int ngrid);
 __global__ void k_eval_apply64(DevModel M, EvalArgs A);
 __global__ void k_commit(double *loc, double *delta, size_t n);

@@ -145,6 +145,9 @@ int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *s
  * out[2] fit ms, out[3] accumulated orbit-kernel ms, out[4] orbit-kernel
  * launches, out[5] orbits integrated, out[6] leapfrog steps integrated. */
 int galic_timings(galic_ctx *ctx, double *out7);
+/* A plain device copy of nbytes (reps times): the known-traffic kernel that
+ * calibrates the HBM byte counters; ms_per_copy receives the mean duration. */
+int galic_copy_probe(int device, size_t nbytes, int reps, double *ms_per_copy);
 int galic_reset_timings(galic_ctx *ctx);
 
 /* ---- drop-in entry points with the reference signatures ----

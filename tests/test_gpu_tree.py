@@ -1,0 +1,102 @@
+"""Barnes-Hut table fill on the GPU against the oracle tree.
+
+The FP64 walk builds the same tree as the reference's insertion build
+(forcetree.c:369-495) and walks it in the same order, so it matches the oracle
+tree to rounding (1e-10 relative).  The FP32 walk (FP64 moments) is compared
+with FP64 direct summation on a random subset of targets.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from galic_amd import model
+from galic_amd.runtime import tree_fill, tree_forces
+
+pytestmark = pytest.mark.gpu
+
+
+def sources(n, seed=3):
+    cfg = model.config("composite-4e6")
+    p = model.make_params(cfg)
+    ids = np.arange(n, dtype=np.uint64)
+    nd = n // 5
+    nb = n // 20
+    pos = np.concatenate([model.sample_halo(p, seed, 11, ids[:n - nd - nb]),
+                          model.sample_disk(p, seed, 12, ids[n - nd - nb:n - nb]),
+                          model.sample_halo(p, seed, 13, ids[n - nb:], a=p.Bulge_A)])
+    mass = np.concatenate([np.full(n - nd - nb, p.Halo_Mass / (n - nd - nb)), np.full(nd, p.Disk_Mass / nd),
+                           np.full(nb, p.Bulge_Mass / nb)])
+    return p, np.ascontiguousarray(pos), mass
+
+
+def oracle_forces(oracle_mod, p, spos, smass, tpos):
+    P = ctypes.POINTER(ctypes.c_double)
+    acc = np.zeros((len(tpos), 3))
+    pot = np.zeros(len(tpos))
+    n = oracle_mod.lib().or_tree_forces(ctypes.byref(p), len(spos), oracle_mod.d(spos), oracle_mod.d(smass),
+                                        len(tpos), oracle_mod.d(tpos), acc.ctypes.data_as(P),
+                                        pot.ctypes.data_as(P), 8)
+    return acc, pot, n
+
+
+def test_tree_fp64_matches_oracle_tree(oracle_mod):
+    print("[gpu] tree fp64 vs oracle tree", flush=True)
+    p, spos, smass = sources(20000)
+    tpos = np.random.default_rng(4).normal(size=(3000, 3)) * 8
+    acc_o, pot_o, n_o = oracle_forces(oracle_mod, p, spos, smass, tpos)
+    acc_g, pot_g, n_g = tree_forces(p, spos, smass, tpos, precision=0)
+    assert n_g == n_o
+    np.testing.assert_allclose(acc_g, acc_o, rtol=1e-10, atol=1e-12 * np.abs(acc_o).max())
+    np.testing.assert_allclose(pot_g, pot_o, rtol=1e-10)
+
+
+def test_tree_theta_zero_equals_direct_sum(oracle_mod):
+    p, spos, smass = sources(4000)
+    p.ErrTolTheta = 1e-8
+    tpos = np.random.default_rng(5).normal(size=(500, 3)) * 5
+    P = ctypes.POINTER(ctypes.c_double)
+    acc_d = np.zeros((500, 3))
+    pot_d = np.zeros(500)
+    oracle_mod.lib().or_direct_forces(ctypes.byref(p), len(spos), oracle_mod.d(spos), oracle_mod.d(smass), 500,
+                                      oracle_mod.d(tpos), acc_d.ctypes.data_as(P), pot_d.ctypes.data_as(P), 8)
+    acc_g, pot_g, n = tree_forces(p, spos, smass, tpos, precision=0)
+    assert n == 500 * 4000
+    np.testing.assert_allclose(acc_g, acc_d, rtol=1e-11, atol=1e-13 * np.abs(acc_d).max())
+    np.testing.assert_allclose(pot_g, pot_d, rtol=1e-11)
+
+
+def test_tree_fp32_walk_accuracy(oracle_mod):
+    print("[gpu] tree fp32 walk", flush=True)
+    p, spos, smass = sources(100000)
+    tpos = np.random.default_rng(6).normal(size=(4096, 3)) * 6
+    acc64, pot64, _ = tree_forces(p, spos, smass, tpos, precision=0)
+    acc32, pot32, _ = tree_forces(p, spos, smass, tpos, precision=1)
+    err = np.linalg.norm(acc32 - acc64, axis=1) / np.linalg.norm(acc64, axis=1)
+    assert np.median(err) < 1e-5 and err.max() < 1e-3
+    np.testing.assert_allclose(pot32, pot64, rtol=1e-4)
+    # against FP64 direct summation: the theta = 0.4 monopole error dominates
+    P = ctypes.POINTER(ctypes.c_double)
+    acc_d = np.zeros((4096, 3))
+    pot_d = np.zeros(4096)
+    oracle_mod.lib().or_direct_forces(ctypes.byref(p), len(spos), oracle_mod.d(spos), oracle_mod.d(smass), 4096,
+                                      oracle_mod.d(tpos), acc_d.ctypes.data_as(P), pot_d.ctypes.data_as(P), 8)
+    e = np.linalg.norm(acc32 - acc_d, axis=1) / np.linalg.norm(acc_d, axis=1)
+    print("fp32 walk vs direct: median", np.median(e), "p99", np.quantile(e, 0.99))
+    assert np.median(e) < 5e-3
+
+
+def test_table_fill_matches_oracle(oracle_mod):
+    print("[gpu] table fill", flush=True)
+    cfg = model.config("halo-disk-1e6", fg_nbin=32)
+    p = model.make_params(cfg)
+    p2, spos, smass = sources(20000)
+    P = ctypes.POINTER(ctypes.c_double)
+    nb2 = p.FG_Nbin ** 2
+    po, ro, zo = np.zeros(nb2), np.zeros(nb2), np.zeros(nb2)
+    oracle_mod.lib().or_table_fill(ctypes.byref(p), len(spos), oracle_mod.d(spos), oracle_mod.d(smass),
+                                   po.ctypes.data_as(P), ro.ctypes.data_as(P), zo.ctypes.data_as(P), 8)
+    pg, rg, zg, n = tree_fill(p, spos, smass, precision=0)
+    np.testing.assert_allclose(pg, po, rtol=1e-10)
+    np.testing.assert_allclose(rg, ro, rtol=1e-9, atol=1e-11 * np.abs(ro).max())
+    np.testing.assert_allclose(zg, zo, rtol=1e-9, atol=1e-11 * np.abs(zo).max())
