@@ -100,7 +100,8 @@ def main():
     # ---------------------------------------------------------- FP32 batched
     g = GalicGPU(w)
     g.fp32_calc_all_response_fields()
-    log("[bench] fp32 initial pass done")
+    variant, bpc = g.fp32_autotune(min(w.n, 524288))
+    log(f"[bench] fp32 initial pass done; autotuned orbit kernel: flush variant {variant}, {bpc} workgroups/CU")
     for it in range(args.warmup):
         fp32_iteration(g, perm, args.k, it, args.sigma)
         log(f"[bench] warmup {it} done")
@@ -186,7 +187,8 @@ def main():
                    "n_particles": w.n, "halo_disk_split": [int(w.params.NType[1]), int(w.params.NType[2])],
                    "force_table": [w.params.FG_Nbin, w.params.FG_Nbin], "response_grid": [1 << w.params.DG_MaxLevel] * 2,
                    "trials_per_particle": args.k, "trial_sigma_frac": args.sigma,
-                   "orbits_per_step": w.n * (args.k + 1), "torbit_fac": cfg.torbit_fac},
+                   "orbits_per_step": w.n * (args.k + 1), "torbit_fac": cfg.torbit_fac,
+                   "autotuned": {"slot_flush": ["load-add-store", "atomic"][variant], "workgroups_per_cu": bpc}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": None,
                      "kernel": "k_orbit32", "note": "compute-bound on the FP32 vector ALU (MFMA unused); "

@@ -136,6 +136,11 @@ int alloc_orbit_scratch(galic::Ctx &c, int n) {
 int alloc_trial_scratch(galic::Ctx &c, int n) {
   if(n <= c.max_trials) return GALIC_OK;
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
+  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2);
+  CHK(dalloc(&c.t_key, n));
+  CHK(dalloc(&c.t_key2, n));
+  CHK(dalloc(&c.t_ord, n));
+  CHK(dalloc(&c.t_ord2, n));
   CHK(dalloc(&c.t_idx, n));
   CHK(dalloc(&c.t_v, 3 * (size_t)n));
   CHK(dalloc(&c.t_accept, n));
@@ -184,12 +189,12 @@ galic::OrbitArgs orbit_args(galic::Ctx &c, int n) {
 
 // Persistent grid: enough workgroups to fill every CU several times over, never
 // more than the orbits need.
-int orbit_grid(int n, int bs) {
+int orbit_grid(int n, int bs, int per_cu = 8) {
   int dev;
   (void)hipGetDevice(&dev);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  int want = cus * 8;
+  int want = cus * per_cu;
   int need = nblocks(n, bs);
   return std::max(1, std::min(want, need));
 }
@@ -378,7 +383,12 @@ int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
   HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 3 * sizeof(unsigned long long), c.stream));
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   const int bs = 256;
-  hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs)), dim3(bs), 0, c.stream, fast_model(c), A);
+  if(c.fp32_variant == 1)
+    hipLaunchKernelGGL(galic::k_orbit32_atomic, dim3(orbit_grid(A.n, bs, c.fp32_bpc)), dim3(bs), 0, c.stream,
+                       fast_model(c), A);
+  else
+    hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs, c.fp32_bpc)), dim3(bs), 0, c.stream,
+                       fast_model(c), A);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipEventSynchronize(c.ev1));
@@ -401,12 +411,15 @@ int commit_q(galic::Ctx &c) {
   return GALIC_OK;
 }
 
+// Bytes the orbit slots of one chunk may take: most of the free HBM (a chunk
+// large enough to keep every lane busy until its end), leaving room for
+// another context and the library's other buffers.
 size_t slot_budget(galic::Ctx &c) {
   size_t fr = 0, tot = 0;
   if(hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
-  size_t cap = (size_t)24 << 30;
-  size_t have = c.slot_bytes + fr / 2;
-  return std::min(cap, have);
+  const size_t reserve = (size_t)8 << 30, cap = (size_t)192 << 30;
+  size_t have = c.slot_bytes + (fr > reserve ? (fr - reserve) / 4 * 3 : 0);
+  return std::max<size_t>((size_t)1 << 30, std::min(cap, have));
 }
 
 }  // namespace
@@ -555,6 +568,7 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.DG_CellSize_f, c.Ngrid));
   CHK(dalloc(&c.stacks, (size_t)GALIC_NTYPES * galic::ST_COUNT * c.Nstack));
   CHK(dalloc(&c.weights, (size_t)GALIC_NTYPES * galic::W_COUNT * c.Nstack));
+  CHK(dalloc(&c.merit_bw, (size_t)GALIC_NTYPES * galic::kFields * c.Nstack));
   CHK(dalloc(&c.cutB_of, (size_t)GALIC_NTYPES * c.Ngrid));
   CHK(dalloc(&c.locA, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
   CHK(dalloc(&c.locA_d, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
@@ -637,7 +651,7 @@ int galic_destroy(galic_ctx *ctx) {
   (void)hipStreamSynchronize(c.stream);
   dfree(c.FG_Pot); dfree(c.FG_DPotDR); dfree(c.FG_DPotDz); dfree(c.FG_Vstream);
   dfree(c.FG_Acc_f); dfree(c.FG_Vstream_f); dfree(c.DG_CellSize); dfree(c.DG_CellSize_f);
-  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of);
+  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
   dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk);
@@ -650,6 +664,8 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
   dfree(c.o_steps); dfree(c.o_bbox);
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
+  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.psteps);
+  if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
   if(c.ev1) (void)hipEventDestroy(c.ev1);
@@ -721,8 +737,9 @@ int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
-  dfree(P.type); dfree(P.orbits); dfree(P.disp);
+  dfree(P.type); dfree(P.orbits); dfree(P.disp); dfree(c.psteps);
   P.n = n;
+  CHK(dalloc(&c.psteps, n));
   CHK(dalloc(&P.x, n)); CHK(dalloc(&P.y, n)); CHK(dalloc(&P.z, n));
   CHK(dalloc(&P.vx, n)); CHK(dalloc(&P.vy, n)); CHK(dalloc(&P.vz, n));
   CHK(dalloc(&P.mass, n)); CHK(dalloc(&P.tint, n)); CHK(dalloc(&P.vesc, n));
@@ -1001,6 +1018,8 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx) {
   CHK(launch_orbit32(c, A));
   hipLaunchKernelGGL(galic::k_store_orbits, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.o_norbit,
                      c.P.orbits);
+  hipLaunchKernelGGL(galic::k_store_steps, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n,
+                     (const long long *)c.o_steps, c.psteps);
   hipLaunchKernelGGL(galic::k_inst_init_q, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, c.M, c.P, c.locI_q,
                      c.locM_q, c.Ngrid, c.fp32_iscale, c.fp32_scale[0]);
   HIPCHK(hipGetLastError());
@@ -1012,6 +1031,78 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx) {
   CHK(global_fit(ctx));
   ctx->fp32_initial_done = true;
   ctx->initial_done = false;
+  return GALIC_OK;
+}
+
+int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_per_cu) {
+  if(!ctx || nsample < 1) return set_err(GALIC_ERR_ARG, "bad autotune arguments");
+  galic::Ctx &c = ctx->c;
+  if(!ctx->fp32_initial_done) return set_err(GALIC_ERR_STATE, "galic_fp32_calc_all_response_fields must run first");
+  HIPCHK(hipSetDevice(c.device));
+  const int n = std::min(nsample, c.P.n);
+  const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
+  CHK(alloc_orbit_scratch(c, n));
+  CHK(alloc_slots(c, n, per));
+  // a strided sample of the model's own orbits at their current velocities,
+  // longest expected first as in a rep
+  std::vector<int> idx(n);
+  for(int i = 0; i < n; i++) idx[i] = (int)((long long)i * c.P.n / n);
+  CHK(alloc_trial_scratch(c, n));
+  HIPCHK(hipMemcpy(c.t_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if(galic::sort_trials_by_steps(n, c.t_idx, c.psteps, c.t_key, c.t_key2, c.t_ord, c.t_ord2, &c.sort_tmp,
+                                 &c.sort_tmp_bytes, c.stream) != 0)
+    return set_err(GALIC_ERR_HIP, "sorting the tuning sample failed");
+  hipLaunchKernelGGL(galic::k_prepare_ktrials, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0,
+                     (const int *)c.t_ord2, c.t_idx, (const double *)nullptr, (const double *)nullptr,
+                     (const double *)nullptr, c.P, orbit_in(c));
+  HIPCHK(hipGetLastError());
+  // variants give the same bits; only the launch shape and the slot flush differ.
+  // Grid sizes: half and all of the workgroups a CU holds at once, and twice that.
+  int occ = 4;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(galic::k_orbit32), 256, 0));
+  occ = std::max(1, std::min(occ, 16));
+  const int variants[2] = {0, 1}, bpcs[3] = {std::max(1, occ / 2), occ, 2 * occ};
+  double best = 1e300;
+  int bv = 0, bb = 4;
+  const long long steps0 = c.acc_steps, orbits0 = c.acc_orbits;
+  const double ms0 = c.acc_orbit_ms;
+  const long long launches0 = c.acc_orbit_launches;
+  for(int v : variants)
+    for(int b : bpcs) {
+      c.fp32_variant = v;
+      c.fp32_bpc = b;
+      galic::OrbitArgs32 A = orbit_args32(c, n);
+      A.slots = (float *)c.slots;
+      long long s_before = c.acc_steps;
+      CHK(launch_orbit32(c, A));
+      double t = c.last_orbit_ms / std::max<long long>(1, c.acc_steps - s_before);
+      if(t < best) {
+        best = t;
+        bv = v;
+        bb = b;
+      }
+      HIPCHK(hipMemsetAsync(c.slots, 0, per * n, c.stream));
+    }
+  // tuning launches are not part of the caller's measurements
+  c.acc_steps = steps0;
+  c.acc_orbits = orbits0;
+  c.acc_orbit_ms = ms0;
+  c.acc_orbit_launches = launches0;
+  c.fp32_variant = bv;
+  c.fp32_bpc = bb;
+  c.fp32_tuned = true;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if(variant) *variant = bv;
+  if(blocks_per_cu) *blocks_per_cu = bb;
+  return GALIC_OK;
+}
+
+int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu) {
+  if(!ctx || variant < 0 || variant > 1 || blocks_per_cu < 1 || blocks_per_cu > 32)
+    return set_err(GALIC_ERR_ARG, "bad FP32 launch (variant 0 or 1, 1..32 workgroups per CU)");
+  ctx->c.fp32_variant = variant;
+  ctx->c.fp32_bpc = blocks_per_cu;
+  ctx->c.fp32_tuned = true;
   return GALIC_OK;
 }
 
@@ -1048,16 +1139,32 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
       HIPCHK(hipGetLastError());
     }
   }
+  // Every chunk is evaluated against the grids as they were at the start of the
+  // rep and its deltas are committed after the last chunk, so the order in which
+  // trials are integrated does not change the result: trials go longest
+  // expected orbit first.
+  if(galic::sort_trials_by_steps(ntrial, c.t_idx, c.psteps, c.t_key, c.t_key2, c.t_ord, c.t_ord2, &c.sort_tmp,
+                                 &c.sort_tmp_bytes, c.stream) != 0)
+    return set_err(GALIC_ERR_HIP, "sorting the trials by expected orbit length failed");
+  const int *order = c.t_ord2;
   const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
   int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ntrial, 1), slot_budget(c) / ((K + 1) * per)));
   CHK(alloc_orbit_scratch(c, chunk * (K + 1)));
   CHK(alloc_slots(c, chunk * (K + 1), per));
+  const size_t nbw = (size_t)GALIC_NTYPES * galic::kFields * c.Nstack;
+  hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream, (const double *)c.stacks,
+                     (const double *)c.weights, c.merit_bw, c.Nstack);
+  HIPCHK(hipGetLastError());
+  // all five fields in one pyramid pass when their coarse levels fit in 64 KiB of LDS
+  const size_t ncoarse = (size_t)(c.Nstack - c.Ngrid);
+  const int nf = (galic::kFields * ncoarse * sizeof(double) + 1024 <= 65536) ? galic::kFields : 1;
+  const size_t shmem = sizeof(double) * (nf * ncoarse + 8);
   float eval_ms = 0;
   for(int base = 0; base < ntrial; base += chunk) {
     int nt = std::min(chunk, ntrial - base);
     hipLaunchKernelGGL(galic::k_prepare_ktrials, dim3(nblocks((size_t)nt * (K + 1), 256)), dim3(256), 0, c.stream, nt,
-                       K, c.t_idx + base, vtx + (size_t)base * K, vty + (size_t)base * K, vtz + (size_t)base * K, c.P,
-                       orbit_in(c));
+                       K, order + base, (const int *)c.t_idx, (const double *)vtx, (const double *)vty,
+                       (const double *)vtz, c.P, orbit_in(c));
     HIPCHK(hipGetLastError());
     galic::OrbitArgs32 A = orbit_args32(c, nt * (K + 1));
     A.slots = (float *)c.slots;
@@ -1066,12 +1173,13 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
     memset(&E, 0, sizeof(E));
     E.P = c.P;
     E.K = K;
-    E.tidx = c.t_idx + base;
-    E.tvx = vtx + (size_t)base * K;
-    E.tvy = vty + (size_t)base * K;
-    E.tvz = vtz + (size_t)base * K;
+    E.order = order + base;
+    E.tidx = c.t_idx;
+    E.tvx = vtx;
+    E.tvy = vty;
+    E.tvz = vtz;
     E.stacks = c.stacks;
-    E.weights = c.weights;
+    E.merit_bw = c.merit_bw;
     E.fit = c.fit;
     E.cutB_of = c.cutB_of;
     E.slots = (const float *)c.slots;
@@ -1079,10 +1187,12 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
     E.slot_base = 0;
     E.bbox = c.o_bbox;
     E.norbit = c.o_norbit;
+    E.osteps = c.o_steps;
+    E.psteps = c.psteps;
     E.locA_dq = c.locA_dq;
     E.locI_dq = c.locI_dq;
-    E.accept = c.t_accept + base;
-    E.dF = c.t_dF + base;
+    E.accept = c.t_accept;
+    E.dF = c.t_dF;
     for(int t = 0; t < GALIC_NTYPES; t++) {
       E.MType[t] = c.p.MType[t];
       E.fac[t][0] = c.p.fac_value_rs[t];
@@ -1093,9 +1203,11 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
     E.lowdisp = c.p.LowerDispLimit;
     for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
     E.iscale = c.fp32_iscale;
-    size_t shmem = sizeof(double) * ((c.Nstack - c.Ngrid) + 8 * 256);
     HIPCHK(hipEventRecord(c.ev0, c.stream));
-    hipLaunchKernelGGL(galic::k_eval_apply32, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
+    if(nf == galic::kFields)
+      hipLaunchKernelGGL(galic::k_eval_apply32<galic::kFields>, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
+    else
+      hipLaunchKernelGGL(galic::k_eval_apply32<1>, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, c.stream));
     HIPCHK(hipEventSynchronize(c.ev1));

@@ -18,6 +18,7 @@
 
 #include "galic_ctx.h"
 #include "galic_kernels.h"
+#include "galic_merit.h"
 
 namespace galic {
 
@@ -311,55 +312,16 @@ __global__ void __launch_bounds__(256) k_eval_apply64(DevModel M, EvalArgs A) {
   double phi[kFields];
   const int stR[kFields] = {ST_RM, ST_RSR, ST_RST, ST_RSQ, ST_RSP};
   const int stT[kFields] = {ST_MT, ST_ER, ST_ET, ST_EQ, ST_EP};
+  (void)npr; (void)npz; (void)pr0; (void)pz0; (void)offL; (void)offL1; (void)pr1; (void)pz1;
   for(int f = 0; f < kFields; f++) {
-    phi[f] = 0;
-    const double *R = S + stR[f] * Nstack, *T = S + stT[f] * Nstack;
-    const double *den = W + W_A * Nstack;
-    const double *ratio = (f == F_M) ? nullptr : W + (W_SR + f - 1) * Nstack;
-    for(int k = tid; k < ncoarse; k += nt) cs[k] = 0;
-    __syncthreads();
+    CutTerm C;
+    C.R = S + stR[f] * Nstack;
+    C.T = S + stT[f] * Nstack;
+    C.den = W + W_A * Nstack;
+    C.ratio = (f == F_M) ? nullptr : W + (W_SR + f - 1) * Nstack;
     const double *fo = so + f, *fn = sn + f;  // cell-major slots: field f of cell c at c*5 + f
-    for(int k = tid; k < npr * npz; k += nt) {
-      int pz = pz0 + k / npr, pr = pr0 + k % npr;
-      // children in the order of smooth_stack (grid.c:1256-1260)
-      int cz[4] = {2 * pz, 2 * pz + 1, 2 * pz, 2 * pz + 1};
-      int cr[4] = {2 * pr, 2 * pr, 2 * pr + 1, 2 * pr + 1};
-      double d[4];
-      for(int c = 0; c < 4; c++) {
-        int cell = cz[c] * nb + cr[c];
-        d[c] = fn[(size_t)cell * kFields] - fo[(size_t)cell * kFields];
-        int node = offL + cell;
-        if(d[c] != 0 && den[node] != 0) {
-          double b = R[node] - T[node];
-          double wgt = ratio ? ratio[node] : 1.0;
-          if(wgt != 0) phi[f] += (fabs(b + d[c]) - fabs(b)) * wgt / den[node];
-        }
-      }
-      cs[GALIC_STACKOFFSET(L - 1, pz, pr)] = d[0] + d[1] + d[2] + d[3];
-    }
-    __syncthreads();
-    for(int l = L - 2; l >= 0; l--) {
-      int n = 1 << l;
-      for(int k = tid; k < n * n; k += nt) {
-        int ii = k / n, jj = k % n;
-        cs[GALIC_STACKOFFSET(l, ii, jj)] =
-            cs[GALIC_STACKOFFSET(l + 1, 2 * ii, 2 * jj)] + cs[GALIC_STACKOFFSET(l + 1, 2 * ii + 1, 2 * jj)] +
-            cs[GALIC_STACKOFFSET(l + 1, 2 * ii, 2 * jj + 1)] + cs[GALIC_STACKOFFSET(l + 1, 2 * ii + 1, 2 * jj + 1)];
-      }
-      __syncthreads();
-    }
-    (void)offL1;
-    for(int node = tid; node < ncoarse; node += nt) {
-      double dc = cs[node];
-      if(dc != 0 && den[node] != 0) {
-        double wgt = ratio ? ratio[node] : 1.0;
-        if(wgt != 0) {
-          double b = R[node] - T[node];
-          phi[f] += (fabs(b + dc) - fabs(b)) * wgt / den[node];
-        }
-      }
-    }
-    __syncthreads();
+    phi[f] = cut_sum_field(C, [&](int cell) { return fn[(size_t)cell * kFields] - fo[(size_t)cell * kFields]; }, r0,
+                           r1, z0, z1, L, cs);
   }
   double sums[kFields];
   for(int f = 0; f < kFields; f++) sums[f] = phi[f];

@@ -42,8 +42,8 @@ enum StackId {
 
 // per-node weights and residuals used by the incremental merit
 enum WeightId {
-  W_A = 0,  // 1/min(dist,R200/20) on cut-A nodes (density cut), else 0
-  W_SR,     // (Tm/Er)/min(dist,R200/20) on cut-A nodes with Er>0, else 0
+  W_A = 0,  // min(dist, R200/20) on cut-A nodes (density cut), else 0
+  W_SR,     // Tm/Er on cut-A nodes with Er > 0, else 0 (then W_ST, W_SQ, W_SP)
   W_ST,
   W_SQ,
   W_SP,
@@ -129,6 +129,12 @@ struct Ctx {
   int *t_accept = nullptr;
   double *t_dF = nullptr;
   int *t_best = nullptr;
+  unsigned int *t_key = nullptr, *t_key2 = nullptr;  // trial sort keys (expected orbit steps)
+  int *t_ord = nullptr, *t_ord2 = nullptr;           // trial order, longest expected orbit first
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int *psteps = nullptr;  // per particle: steps of its current orbit (FP32 path's length prediction)
+  double2 *merit_bw = nullptr;  // FP32 merit: (response - target, node weight) [type][5][Nstack]
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -136,6 +142,9 @@ struct Ctx {
   double acc_orbit_ms = 0;
   long long acc_orbit_launches = 0;
   long long acc_orbits = 0, acc_steps = 0;
+  int fp32_variant = 0;   // slot flush of k_orbit32: 0 load-add-store, 1 returnless atomics (autotuned)
+  int fp32_bpc = 4;       // persistent workgroups per CU for k_orbit32 (autotuned)
+  bool fp32_tuned = false;
   double fp32_scale[5];  // fixed-point scales per field (orbit-averaged)
   double fp32_iscale;    // fixed-point scale of the instantaneous fields
 };

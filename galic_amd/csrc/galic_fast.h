@@ -42,15 +42,19 @@ struct OrbitArgs32 {
 struct EvalArgs32 {
   Particles P;
   int K;
+  const int *order;  // trial index of each chunk position, or null (position = trial)
   const int *tidx;
   const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
-  const double *stacks, *weights, *fit;
+  const double *stacks, *fit;
+  const double2 *merit_bw;  // (response - target, weight) per node [type][5][Nstack]
   const int *cutB_of;
   const float *slots;
   size_t slot_stride;
   size_t slot_base;
   const int4 *bbox;
   const int *norbit;
+  const long long *osteps;  // steps of each orbit of the chunk
+  int *psteps;              // per particle: steps of its current orbit (updated)
   long long *locA_dq, *locI_dq;
   int *accept;
   double *dF;
@@ -62,11 +66,17 @@ struct EvalArgs32 {
 };
 
 __global__ void k_orbit32(FastModel F, OrbitArgs32 A);
+__global__ void k_orbit32_atomic(FastModel F, OrbitArgs32 A);
 __global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, const double *disp,
                               const double *vstr_unused, double sigma_frac, double maxvel, uint32_t seed,
                               uint32_t stream, const int *tovs_of_type, double *vtx, double *vty, double *vtz);
-__global__ void k_prepare_ktrials(int ntrial, int K, const int *tidx, const double *vtx, const double *vty,
-                                  const double *vtz, Particles P, OrbitIn O);
+__global__ void k_prepare_ktrials(int ntrial, int K, const int *order, const int *tidx, const double *vtx,
+                                  const double *vty, const double *vtz, Particles P, OrbitIn O);
+__global__ void k_store_steps(int n, const long long *osteps, int *psteps);
+int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigned int *key, unsigned int *key2,
+                         int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
+__global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
+template <int NF>
 __global__ void k_eval_apply32(DevModel M, EvalArgs32 A);
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);

@@ -15,6 +15,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "galic_ctx.h"
 #include "galic_fast.h"
 #include "galic_kernels.h"
@@ -167,19 +169,33 @@ __device__ __forceinline__ void atomic_q(long long *p, double v) {
   atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v));
 }
 
+// Adds a run's weights to the orbit's slot (cell-major: 5 fields per cell
+// contiguous, so a 2x2 block is two short row segments) or, without a slot,
+// to the fixed-point Loc grids.  The slot add is fac*w rounded, then added:
+// a load-add-store (ATOMIC false) and a returnless float atomic (ATOMIC true,
+// the wave does not wait for the line) give the same bits.
+template <bool ATOMIC>
 __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
                                             const FastModel &F) {
   if(r.quad < 0) return;
   const int c[4] = {r.quad, r.quad + 1, r.quad + nb, r.quad + nb + 1};
   if(slot) {
-    // cell-major slot (5 fields per cell contiguous): two short row segments per flush
     float *s0 = slot + (size_t)c[0] * kFields, *s2 = slot + (size_t)c[2] * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      s0[f] += fac * r.w[f][0];
-      s0[kFields + f] += fac * r.w[f][1];
-      s2[f] += fac * r.w[f][2];
-      s2[kFields + f] += fac * r.w[f][3];
+      const float v0 = __fmul_rn(fac, r.w[f][0]), v1 = __fmul_rn(fac, r.w[f][1]);
+      const float v2 = __fmul_rn(fac, r.w[f][2]), v3 = __fmul_rn(fac, r.w[f][3]);
+      if(ATOMIC) {
+        (void)atomicAdd(&s0[f], v0);
+        (void)atomicAdd(&s0[kFields + f], v1);
+        (void)atomicAdd(&s2[f], v2);
+        (void)atomicAdd(&s2[kFields + f], v3);
+      } else {
+        s0[f] = __fadd_rn(s0[f], v0);
+        s0[kFields + f] = __fadd_rn(s0[kFields + f], v1);
+        s2[f] = __fadd_rn(s2[f], v2);
+        s2[kFields + f] = __fadd_rn(s2[kFields + f], v3);
+      }
     }
   } else {
 #pragma unroll
@@ -192,6 +208,9 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
   }
 }
 
+// Trapezoid half-step weights of one end of a step (orbit_response.c:283-319)
+// accumulated in registers while the orbit stays in one 2x2 block.
+template <bool ATOMIC>
 __device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
                                       const FastModel &F, int iR, int iz, float fR, float fz, float dt, float vr2,
                                       float vt2, float vq2, float vp2, int4 &bb) {
@@ -199,7 +218,7 @@ __device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, fl
   float m[4] = {h * (1 - fR) * (1 - fz), h * fR * (1 - fz), h * (1 - fR) * fz, h * fR * fz};
   int quad = iz * nb + iR;
   if(quad != r.quad) {
-    run32_flush(r, nb, ngrid, fac, slot, locq, F);
+    run32_flush<ATOMIC>(r, nb, ngrid, fac, slot, locq, F);
     run32_reset(r, quad);
     bb.x = min(bb.x, iR);
     bb.y = max(bb.y, iR + 1);
@@ -216,82 +235,145 @@ __device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, fl
   }
 }
 
-__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) {
-  const int nb = F.dg_nbin, ngrid = nb * nb;
-  while(true) {
-    int o = (int)atomicAdd(A.counter, 1u);
-    if(o >= A.n) break;
-    const int type = A.otype[o];
-    const int tovs = F.tovs[type];
-    float x = (float)A.ox[o], y = (float)A.oy[o], z = (float)A.oz[o];
-    float vx = (float)A.ovx[o], vy = (float)A.ovy[o], vz = (float)A.ovz[o];
-    const double timespan = A.otint[o];
-    const float fac = (float)(A.omass[o] / timespan);
-    float *slot = A.slots ? A.slots + (size_t)(o - A.slot_base) * A.slot_stride : nullptr;
-    long long *locq = A.locq ? A.locq + (size_t)type * kFields * ngrid : nullptr;
+// State of the orbit a lane is integrating.
+struct Lane32 {
+  int o, type, tovs;
+  double timespan, tall;
+  float fac;
+  float *slot;
+  long long *locq;
+};
 
-    Fg32 g = fg_index32(F, x, y, z);
-    float ax, ay, az;
-    accel32(F, x, y, z, g, ax, ay, az);
-    int iR, iz;
-    float fR, fz;
-    cell32(F, g.R, z, iR, iz, fR, fz);
-    int icell = iz * nb + iR;
-    float vr2, vt2, vq2, vp2;
-    decompose32(F, tovs, type, x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
-    Run32 run;
-    run.quad = -1;
-    int4 bb = make_int4(nb, -1, nb, -1);
-    double tall = 0;
-    float radsign_prev = 0.f;
-    int norbit = 0, flag = 0, status = 0;
-    long long steps = 0;
-    if(!(timespan > 0)) status = GALIC_ERR_ARG;
-    while(!flag && status == 0) {
+// Takes orbits from the launch's counter until one starts cleanly; orbits with a
+// non-positive time span are finished at once with GALIC_ERR_ARG, as the FP64
+// path does.  Returns false when the launch has no orbits left.
+__device__ __forceinline__ bool take_orbit32(const OrbitArgs32 &A, int nb, Lane32 &L, float &x, float &y, float &z,
+                                             float &vx, float &vy, float &vz) {
+  const int ngrid = nb * nb;
+  while(true) {
+    unsigned int q = atomicAdd(A.counter, 1u);
+    if(q >= (unsigned int)A.n) return false;
+    const int o = (int)q;
+    L.o = o;
+    L.type = A.otype[o];
+    L.timespan = A.otint[o];
+    L.tall = 0;
+    x = (float)A.ox[o];
+    y = (float)A.oy[o];
+    z = (float)A.oz[o];
+    vx = (float)A.ovx[o];
+    vy = (float)A.ovy[o];
+    vz = (float)A.ovz[o];
+    L.fac = (float)(A.omass[o] / L.timespan);
+    L.slot = A.slots ? A.slots + (size_t)(o - A.slot_base) * A.slot_stride : nullptr;
+    L.locq = A.locq ? A.locq + (size_t)L.type * kFields * ngrid : nullptr;
+    if(L.timespan > 0) return true;
+    A.tall[o] = 0;
+    A.norbit[o] = 0;
+    A.steps[o] = 0;
+    A.bbox[o] = make_int4(nb, -1, nb, -1);
+    A.status[o] = GALIC_ERR_ARG;
+    orbit_stats_add(A.stats, 0, GALIC_ERR_ARG, o);
+  }
+}
+
+// Persistent orbit integration with per-lane refill: a lane whose orbit ends
+// writes it out and takes the next one at once, so the lanes of a wave stay
+// busy however uneven the orbit lengths are.  A freshly taken orbit spends one
+// pass of the loop on the lookups at its start point (the pre-loop lookups of
+// orbit_response.c:199-210); every later pass is one step of :211-458.
+template <bool ATOMIC>
+__device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs32 &A) {
+  const int nb = F.dg_nbin, ngrid = nb * nb;
+  Lane32 L;
+  float x, y, z, vx, vy, vz;
+  if(!take_orbit32(A, nb, L, x, y, z, vx, vy, vz)) return;
+  L.tovs = F.tovs[L.type];
+  bool fresh = true;
+  float ax = 0, ay = 0, az = 0;
+  int iR = 0, iz = 0, icell = 0;
+  float fR = 0, fz = 0, vr2 = 0, vt2 = 0, vq2 = 0, vp2 = 0;
+  Run32 run;
+  run.quad = -1;
+  int4 bb = make_int4(nb, -1, nb, -1);
+  float radsign_prev = 0.f;
+  int norbit = 0, flag = 0, status = 0;
+  long long steps = 0;
+  float dt = 0.f, hdt = 0.f;
+  while(true) {
+    if(!fresh) {
       float inv_v = fast_rsq(vx * vx + vy * vy + vz * vz);
       float inv_a = fast_rsq(ax * ax + ay * ay + az * az);
-      float dt = fminf(F.fo_v200 * inv_a, F.fc * F.cellsize[icell] * inv_v);
-      if(timespan < tall + (double)dt) {
-        dt = (float)(timespan - tall);
+      dt = fminf(F.fo_v200 * inv_a, F.fc * F.cellsize[icell] * inv_v);
+      if(L.timespan < L.tall + (double)dt) {
+        dt = (float)(L.timespan - L.tall);
         flag = 1;
       }
       if(!(dt >= 0.f && dt < 3.0e38f)) {
         status = GALIC_ERR_ARG;
-        break;
+      } else {
+        bin32<ATOMIC>(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+        hdt = 0.5f * dt;
+        vx += hdt * ax;
+        vy += hdt * ay;
+        vz += hdt * az;
+        x += dt * vx;
+        y += dt * vy;
+        z += dt * vz;
       }
-      bin32(run, nb, ngrid, fac, slot, locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
-      float hdt = 0.5f * dt;
+    }
+    Fg32 g = fg_index32(F, x, y, z);
+    float nax, nay, naz;
+    accel32(F, x, y, z, g, nax, nay, naz);
+    ax = nax;
+    ay = nay;
+    az = naz;
+    if(!fresh && status == 0) {
       vx += hdt * ax;
       vy += hdt * ay;
       vz += hdt * az;
-      x += dt * vx;
-      y += dt * vy;
-      z += dt * vz;
-      g = fg_index32(F, x, y, z);
-      accel32(F, x, y, z, g, ax, ay, az);
-      vx += hdt * ax;
-      vy += hdt * ay;
-      vz += hdt * az;
-      cell32(F, g.R, z, iR, iz, fR, fz);
-      icell = iz * nb + iR;
-      decompose32(F, tovs, type, x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
-      bin32(run, nb, ngrid, fac, slot, locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
-      tall += (double)dt;
+    }
+    cell32(F, g.R, z, iR, iz, fR, fz);
+    icell = iz * nb + iR;
+    decompose32(F, L.tovs, L.type, x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
+    if(fresh) {
+      fresh = false;
+      continue;
+    }
+    if(status == 0) {
+      bin32<ATOMIC>(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+      L.tall += (double)dt;
       float radsign = vx * x + vy * y + vz * z;
       if(radsign > 0 && radsign_prev < 0) norbit++;
       radsign_prev = radsign;
       steps++;
       if(steps > F.max_steps) status = GALIC_ERR_STEPCAP;
     }
-    run32_flush(run, nb, ngrid, fac, slot, locq, F);
-    A.tall[o] = tall;
-    A.norbit[o] = norbit;
-    A.steps[o] = steps;
-    A.bbox[o] = bb;
-    A.status[o] = status;
-    orbit_stats_add(A.stats, steps, status, o);
+    if(flag || status) {
+      run32_flush<ATOMIC>(run, nb, ngrid, L.fac, L.slot, L.locq, F);
+      const int o = L.o;
+      A.tall[o] = L.tall;
+      A.norbit[o] = norbit;
+      A.steps[o] = steps;
+      A.bbox[o] = bb;
+      A.status[o] = status;
+      orbit_stats_add(A.stats, steps, status, o);
+      if(!take_orbit32(A, nb, L, x, y, z, vx, vy, vz)) break;
+      L.tovs = F.tovs[L.type];
+      fresh = true;
+      run.quad = -1;
+      bb = make_int4(nb, -1, nb, -1);
+      radsign_prev = 0.f;
+      norbit = 0;
+      flag = 0;
+      status = 0;
+      steps = 0;
+    }
   }
 }
+
+__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body<false>(F, A); }
+__global__ void __launch_bounds__(256) k_orbit32_atomic(FastModel F, OrbitArgs32 A) { orbit32_body<true>(F, A); }
 
 // ------------------------------------------------------------ trial draws --
 
@@ -388,13 +470,15 @@ __global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, c
   vtz[q] = oz;
 }
 
-// Orbits of a K-trial rep: orbit i*(K+1) is particle tidx[i] at its current
-// velocity, orbits i*(K+1)+1+k its trials.
-__global__ void k_prepare_ktrials(int ntrial, int K, const int *tidx, const double *vtx, const double *vty,
-                                  const double *vtz, Particles P, OrbitIn O) {
+// Orbits of a K-trial chunk: orbit j*(K+1) is the particle of trial i = order[j]
+// at its current velocity, orbits j*(K+1)+1+k its trials (i*K+k).  Without an
+// order the chunk is trials 0..ntrial-1 in turn.
+__global__ void k_prepare_ktrials(int ntrial, int K, const int *order, const int *tidx, const double *vtx,
+                                  const double *vty, const double *vtz, Particles P, OrbitIn O) {
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if(o >= ntrial * (K + 1)) return;
-  int i = o / (K + 1), k = o % (K + 1);
+  int j = o / (K + 1), k = o % (K + 1);
+  int i = order ? order[j] : j;
   int p = tidx[i];
   O.pidx[o] = p;
   O.ox[o] = P.x[p];
@@ -403,7 +487,7 @@ __global__ void k_prepare_ktrials(int ntrial, int K, const int *tidx, const doub
   O.omass[o] = P.mass[p];
   O.otint[o] = P.tint[p];
   O.otype[o] = P.type[p];
-  if(k == 0) {
+  if(k == 0) {  // (also the whole K = 0 case, where the trial arrays are unused)
     O.ovx[o] = P.vx[p];
     O.ovy[o] = P.vy[p];
     O.ovz[o] = P.vz[p];
@@ -415,21 +499,45 @@ __global__ void k_prepare_ktrials(int ntrial, int K, const int *tidx, const doub
   }
 }
 
-// -------------------------------------------------------------- FP32 merit --
-
-template <int NV>
-__device__ void block_sum_f(double (&v)[NV], double *lds) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for(int k = 0; k < NV; k++) lds[k * nt + tid] = v[k];
-  __syncthreads();
-  for(int s = nt / 2; s > 0; s >>= 1) {
-    if(tid < s)
-      for(int k = 0; k < NV; k++) lds[k * nt + tid] += lds[k * nt + tid + s];
-    __syncthreads();
-  }
-  for(int k = 0; k < NV; k++) v[k] = lds[k * nt];
-  __syncthreads();
+// Expected orbit length of each trial's particle: the steps its current orbit took.
+__global__ void k_step_keys(int ntrial, const int *tidx, const int *psteps, unsigned int *key, int *ord) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= ntrial) return;
+  key[i] = (unsigned int)psteps[tidx[i]];
+  ord[i] = i;
 }
+
+__global__ void k_store_steps(int n, const long long *osteps, int *psteps) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if(o < n) psteps[o] = (int)min(osteps[o], (long long)0x7fffffff);
+}
+
+// Trials in order of decreasing expected orbit length (stable, so the order is
+// reproducible): chunks then hold orbits of similar length and the longest
+// start first.
+int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigned int *key, unsigned int *key2,
+                         int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream) {
+  if(ntrial <= 0) return 0;
+  hipLaunchKernelGGL(k_step_keys, dim3((ntrial + 255) / 256), dim3(256), 0, stream, ntrial, tidx, psteps, key, ord);
+  if(hipGetLastError() != hipSuccess) return -1;
+  size_t need = 0;
+  if(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, key, key2, ord, ord2, ntrial, 0, 32, stream) !=
+     hipSuccess)
+    return -1;
+  if(need > *tmp_bytes) {
+    if(*tmp) (void)hipFree(*tmp);
+    *tmp = nullptr;
+    *tmp_bytes = 0;
+    if(hipMalloc(tmp, need) != hipSuccess) return -1;
+    *tmp_bytes = need;
+  }
+  if(hipcub::DeviceRadixSort::SortPairsDescending(*tmp, need, key, key2, ord, ord2, ntrial, 0, 32, stream) !=
+     hipSuccess)
+    return -1;
+  return 0;
+}
+
+// -------------------------------------------------------------- FP32 merit --
 
 // Change of the instantaneous merit terms, as inst_delta of the FP64 path.
 __device__ double inst_delta_q(const DevModel &M, const double *S, const int *cutB_of, int t, double x, double y,
@@ -481,104 +589,169 @@ __device__ double inst_delta_q(const DevModel &M, const double *S, const int *cu
 // One workgroup per particle: the change of the merit for each of its K trials
 // (FP32 slots, FP64 sums on the fixed cuts), the best trial taken when it lowers
 // the merit, deltas added as fixed point, and all K+1 slots cleared.
+// Node terms of the orbit-averaged merit, refreshed after every commit: b =
+// response - target and w = the node's weight, 1/min(dist, R200/20) for the
+// mass term and (Tm/E_x)/min(dist, R200/20) for an S term on the cut nodes, 0
+// elsewhere (calc_stack_difference_mod flags 0 and 2, grid.c:1115-1151).
+__global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if(q >= GALIC_NTYPES * kFields * Nstack) return;
+  const int node = q % Nstack, f = (q / Nstack) % kFields, t = q / (Nstack * kFields);
+  const int stR[kFields] = {ST_RM, ST_RSR, ST_RST, ST_RSQ, ST_RSP};
+  const int stT[kFields] = {ST_MT, ST_ER, ST_ET, ST_EQ, ST_EP};
+  const double *S = stacks + (size_t)t * ST_COUNT * Nstack;
+  const double *W = weights + (size_t)t * W_COUNT * Nstack;
+  const double den = W[W_A * Nstack + node];
+  const double ratio = (f == F_M) ? 1.0 : W[(W_SR + f - 1) * Nstack + node];
+  double2 v;
+  v.x = S[stR[f] * Nstack + node] - S[stT[f] * Nstack + node];
+  v.y = (den != 0 && ratio != 0) ? ratio / den : 0.0;
+  bw[q] = v;
+}
+
+__device__ __forceinline__ double merit_term(const double2 *bw, int node, double d) {
+  const double2 v = bw[node];
+  return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
+}
+
+// Per-thread partial of sum_f coef[f] * sum_nodes w (|b + D| - |b|) for fields
+// f0..f0+NF-1, D = new - old over the box r0..r1, z0..z1 of fine cells and its
+// ancestors (all threads call it).  Slots are cell-major, 5 floats per cell.
+// cs: LDS of NF * ncoarse doubles.
+template <int NF>
+__device__ double merit_change32(const double2 *bw, int Nstack, const double (&coef)[kFields], int f0,
+                                 const float *fo, const float *fn, int r0, int r1, int z0, int z1, int L, double *cs) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nb = 1 << L, ngrid = nb * nb, ncoarse = Nstack - ngrid;
+  const int offL = GALIC_STACKOFFSET(L, 0, 0);
+  double acc = 0;
+  if(r1 < r0 || z1 < z0) {
+    __syncthreads();
+    return 0.0;
+  }
+  int a0 = r0 >> 1, a1 = r1 >> 1, c0 = z0 >> 1, c1 = z1 >> 1;
+  int na = a1 - a0 + 1, nc = c1 - c0 + 1;
+  for(int k = tid; k < na * nc; k += nt) {
+    const int pz = c0 + k / na, pr = a0 + k % na;
+    double s[NF];
+#pragma unroll
+    for(int f = 0; f < NF; f++) s[f] = 0;
+#pragma unroll
+    for(int c = 0; c < 4; c++) {  // child order of smooth_stack
+      const int cell = (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1);
+      const float *po = fo + (size_t)cell * kFields + f0, *pn = fn + (size_t)cell * kFields + f0;
+#pragma unroll
+      for(int f = 0; f < NF; f++) {
+        const double d = (double)pn[f] - (double)po[f];
+        s[f] += d;
+        acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, offL + cell, d);
+      }
+    }
+    const int node = GALIC_STACKOFFSET(L - 1, pz, pr);
+#pragma unroll
+    for(int f = 0; f < NF; f++) {
+      cs[f * ncoarse + node] = s[f];
+      acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, s[f]);
+    }
+  }
+  __syncthreads();
+  // coarser levels over the box's ancestors; children outside the previous
+  // region hold stale LDS and are read as 0
+  for(int l = L - 2; l >= 0; l--) {
+    const int b0 = a0 >> 1, b1 = a1 >> 1, e0 = c0 >> 1, e1 = c1 >> 1;
+    const int nbx = b1 - b0 + 1, nez = e1 - e0 + 1;
+    for(int k = tid; k < nbx * nez; k += nt) {
+      const int i = e0 + k / nbx, j = b0 + k % nbx;
+      double s[NF];
+#pragma unroll
+      for(int f = 0; f < NF; f++) s[f] = 0;
+#pragma unroll
+      for(int c = 0; c < 4; c++) {
+        const int ci = 2 * i + (c & 1), cj = 2 * j + (c >> 1);  // child order of smooth_stack
+        if(ci >= c0 && ci <= c1 && cj >= a0 && cj <= a1) {
+          const int cn = GALIC_STACKOFFSET(l + 1, ci, cj);
+#pragma unroll
+          for(int f = 0; f < NF; f++) s[f] += cs[f * ncoarse + cn];
+        }
+      }
+      const int node = GALIC_STACKOFFSET(l, i, j);
+#pragma unroll
+      for(int f = 0; f < NF; f++) {
+        cs[f * ncoarse + node] = s[f];
+        acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, s[f]);
+      }
+    }
+    a0 = b0;
+    a1 = b1;
+    c0 = e0;
+    c1 = e1;
+    __syncthreads();
+  }
+  return acc;
+}
+
+// Sum over the block of one double; every thread gets the total.
+__device__ __forceinline__ double block_sum1(double v, double *red) {
+  for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0;
+  for(int k = 0; k < nw; k++) s += red[k];
+  __syncthreads();
+  return s;
+}
+
+// One particle per workgroup: the merit change of each of its K trials (the
+// orbit-averaged terms on the fixed density cut, the instantaneous terms on the
+// fixed cut of EGs_MassResponse), the best improving trial taken, its deltas
+// added in fixed point, and the K+1 slots cleared for the next chunk.
+template <int NF>
 __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds[];
-  const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x, K = A.K;
+  const int tid = threadIdx.x, nt = blockDim.x, K = A.K;
   const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
-  const int ncoarse = Nstack - ngrid;
   double *cs = lds;
-  double *red = lds + ncoarse;
+  double *red = lds + NF * (Nstack - ngrid);
   __shared__ double s_dF[64];
   __shared__ int s_best;
 
-  const int p = A.tidx[i];
+  const int j = blockIdx.x;                  // position in the chunk (orbits j*(K+1)..)
+  const int ti = A.order ? A.order[j] : j;   // trial index (trial arrays, accept, dF)
+  const int p = A.tidx[ti];
   const int t = A.P.type[p];
   const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
-  const double *W = A.weights + (size_t)t * W_COUNT * Nstack;
-  const size_t base = (size_t)i * (K + 1) - A.slot_base;
+  const double2 *bw = A.merit_bw + (size_t)t * kFields * Nstack;
+  const size_t base = (size_t)j * (K + 1) - A.slot_base;
   const float *so = A.slots + base * A.slot_stride;
-  const int4 bo = A.bbox[(size_t)i * (K + 1)];
-  const int offL = GALIC_STACKOFFSET(L, 0, 0);
-  const int stR[kFields] = {ST_RM, ST_RSR, ST_RST, ST_RSQ, ST_RSP};
-  const int stT[kFields] = {ST_MT, ST_ER, ST_ET, ST_EQ, ST_EP};
-  const double *den = W + W_A * Nstack;
+  const int4 bo = A.bbox[(size_t)j * (K + 1)];
   const double x = A.P.x[p], y = A.P.y[p], z = A.P.z[p], mass = A.P.mass[p];
   const double dold[4] = {A.P.vr2[p], A.P.vt2[p], A.P.vp2[p], A.P.vq2[p]};
   const double MT = A.MType[t];
+  // eval_fit_mod's combination (main.c:1215-1216): D/M + (f_rs V_rs + f_ts V_ts + f_qs V_qs + f_ps V_ps)/M/4
+  const double coef[kFields] = {1.0 / MT, A.fac[t][0] / MT / 4, A.fac[t][1] / MT / 4, A.fac[t][2] / MT / 4,
+                                A.fac[t][3] / MT / 4};
 
   for(int k = 0; k < K; k++) {
     const float *sn = so + (size_t)(1 + k) * A.slot_stride;
-    const int4 bn = A.bbox[(size_t)i * (K + 1) + 1 + k];
-    int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
-    const int npr = (r1 < r0) ? 0 : (r1 >> 1) - (r0 >> 1) + 1;
-    const int npz = (z1 < z0) ? 0 : (z1 >> 1) - (z0 >> 1) + 1;
-    const int pr0 = r0 >> 1, pz0 = z0 >> 1;
-    double phi[kFields];
-    for(int f = 0; f < kFields; f++) {
-      phi[f] = 0;
-      const double *R = S + stR[f] * Nstack, *T = S + stT[f] * Nstack;
-      const double *ratio = (f == F_M) ? nullptr : W + (W_SR + f - 1) * Nstack;
-      for(int q = tid; q < ncoarse; q += nt) cs[q] = 0;
-      __syncthreads();
-      const float *fo = so + f, *fn = sn + f;  // field f of cell c at c*5 + f
-      for(int q = tid; q < npr * npz; q += nt) {
-        int pz = pz0 + q / npr, pr = pr0 + q % npr;
-        int cz[4] = {2 * pz, 2 * pz + 1, 2 * pz, 2 * pz + 1};
-        int cr[4] = {2 * pr, 2 * pr, 2 * pr + 1, 2 * pr + 1};
-        double d[4];
-        for(int c = 0; c < 4; c++) {
-          int cell = cz[c] * nb + cr[c];
-          d[c] = (double)fn[(size_t)cell * kFields] - (double)fo[(size_t)cell * kFields];
-          int node = offL + cell;
-          if(d[c] != 0 && den[node] != 0) {
-            double wgt = ratio ? ratio[node] : 1.0;
-            if(wgt != 0) {
-              double b = R[node] - T[node];
-              phi[f] += (fabs(b + d[c]) - fabs(b)) * wgt / den[node];
-            }
-          }
-        }
-        cs[GALIC_STACKOFFSET(L - 1, pz, pr)] = d[0] + d[1] + d[2] + d[3];
-      }
-      __syncthreads();
-      for(int l = L - 2; l >= 0; l--) {
-        int n = 1 << l;
-        for(int q = tid; q < n * n; q += nt) {
-          int ii = q / n, jj = q % n;
-          cs[GALIC_STACKOFFSET(l, ii, jj)] =
-              cs[GALIC_STACKOFFSET(l + 1, 2 * ii, 2 * jj)] + cs[GALIC_STACKOFFSET(l + 1, 2 * ii + 1, 2 * jj)] +
-              cs[GALIC_STACKOFFSET(l + 1, 2 * ii, 2 * jj + 1)] + cs[GALIC_STACKOFFSET(l + 1, 2 * ii + 1, 2 * jj + 1)];
-        }
-        __syncthreads();
-      }
-      for(int node = tid; node < ncoarse; node += nt) {
-        double dc = cs[node];
-        if(dc != 0 && den[node] != 0) {
-          double wgt = ratio ? ratio[node] : 1.0;
-          if(wgt != 0) {
-            double b = R[node] - T[node];
-            phi[f] += (fabs(b + dc) - fabs(b)) * wgt / den[node];
-          }
-        }
-      }
-      __syncthreads();
-    }
-    block_sum_f<kFields>(phi, red);
+    const int4 bn = A.bbox[(size_t)j * (K + 1) + 1 + k];
+    const int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
+    double part = 0;
+    for(int f0 = 0; f0 < kFields; f0 += NF)
+      part += merit_change32<NF>(bw, Nstack, coef, f0, so, sn, r0, r1, z0, z1, L, cs);
+    const double orbit_terms = block_sum1(part, red);
     if(tid == 0) {
-      const size_t q = (size_t)i * K + k;
+      const size_t q = (size_t)ti * K + k;
       double dnew[4];
       disp_components64(M, t, x, y, z, A.tvx[q], A.tvy[q], A.tvz[q], dnew[0], dnew[1], dnew[2], dnew[3]);
       double cw[4];
       int cc[4];
       double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
       const double *fit = A.fit + (size_t)t * 16;
-      s_dF[k] = phi[F_M] / MT +
-                (A.fac[t][0] * phi[F_R] + A.fac[t][1] * phi[F_T] + A.fac[t][2] * phi[F_Q] + A.fac[t][3] * phi[F_P]) /
-                    MT / 4 +
-                fit[5] / fit[6] * vinst;
+      s_dF[k] = orbit_terms + fit[5] / fit[6] * vinst;
     }
-    __syncthreads();
   }
+  __syncthreads();
   if(tid == 0) {
     int best = -1;
     double bv = 0;
@@ -588,12 +761,13 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
         best = k;
       }
     s_best = best;
-    A.accept[i] = best + 1;
-    A.dF[i] = (best >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
-    const int o0 = i * (K + 1);
-    A.P.orbits[p] = (best >= 0) ? A.norbit[o0 + 1 + best] : A.norbit[o0];
+    A.accept[ti] = best + 1;
+    A.dF[ti] = (best >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
+    const int o0 = j * (K + 1), oc = (best >= 0) ? o0 + 1 + best : o0;
+    A.P.orbits[p] = A.norbit[oc];
+    A.psteps[p] = (int)min(A.osteps[oc], (long long)0x7fffffff);
     if(best >= 0) {
-      const size_t q = (size_t)i * K + best;
+      const size_t q = (size_t)ti * K + best;
       const double vt[3] = {A.tvx[q], A.tvy[q], A.tvz[q]};
       double dnew[4];
       disp_components64(M, t, x, y, z, vt[0], vt[1], vt[2], dnew[0], dnew[1], dnew[2], dnew[3]);
@@ -621,7 +795,7 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
   long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
   if(best >= 0) {
     const float *sn = so + (size_t)(1 + best) * A.slot_stride;
-    const int4 bn = A.bbox[(size_t)i * (K + 1) + 1 + best];
+    const int4 bn = A.bbox[(size_t)j * (K + 1) + 1 + best];
     int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
     int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
     for(int q = tid; q < nr * nzz; q += nt) {
@@ -637,7 +811,7 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
   // clear the K+1 slots inside their boxes
   for(int k = 0; k <= K; k++) {
     float *s = (float *)(so + (size_t)k * A.slot_stride);
-    const int4 b = A.bbox[(size_t)i * (K + 1) + k];
+    const int4 b = A.bbox[(size_t)j * (K + 1) + k];
     int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
     for(int q = tid; q < nr * nzz; q += nt) {
       int cell = (b.z + q / nr) * nb + (b.x + q % nr);
@@ -645,6 +819,9 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
     }
   }
 }
+
+template __global__ void k_eval_apply32<1>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_apply32<kFields>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 

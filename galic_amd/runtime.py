@@ -78,6 +78,15 @@ class GalicGPU:
         check(self.lib.galic_fp32_rep(self.h, len(idx), iptr(idx), k, dptr(vt), sigma_frac, seed, iptr(acc)))
         return acc
 
+    def fp32_autotune(self, nsample=65536):
+        v = np.zeros(1, dtype=np.int32)
+        b = np.zeros(1, dtype=np.int32)
+        check(self.lib.galic_fp32_autotune(self.h, nsample, iptr(v), iptr(b)))
+        return int(v[0]), int(b[0])
+
+    def fp32_set_launch(self, variant, blocks_per_cu):
+        check(self.lib.galic_fp32_set_launch(self.h, int(variant), int(blocks_per_cu)))
+
     # ---------------------------------------------------------------- reads --
     def stack(self, t, which):
         out = np.zeros(self.nstack)

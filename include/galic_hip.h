@@ -122,6 +122,16 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx);
 int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int k, const double *vtry, double sigma_frac,
                    unsigned long long seed, int *accept);
 
+/* Times the FP32 orbit kernel's variants (slot flush by load-add-store, 0, or
+ * by returnless atomics, 1) and persistent grid sizes (half, all and twice the
+ * workgroups a CU holds at once) on nsample orbits of the model and keeps the
+ * fastest for later reps; reports the choice.  The variants give bit-identical
+ * results.  nsample should exceed the lanes the GPU holds at once. */
+int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_per_cu);
+
+/* Sets the FP32 orbit launch by hand (variant 0 or 1, 1..32 workgroups per CU). */
+int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu);
+
 /* ---- Barnes-Hut tree fill of the force tables ---- */
 
 /* Accelerations and potentials (times G) at ntarget points from nsrc sources

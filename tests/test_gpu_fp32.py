@@ -54,6 +54,25 @@ def test_fp32_grids_are_bit_deterministic(w):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("launch", [(0, 2), (1, 4), (1, 8)])
+def test_fp32_launch_variants_give_identical_bits(w, launch):
+    """The slot flush (load-add-store or returnless atomics) and the persistent
+    grid size change the schedule only: grids, velocities and accept flags are
+    bit-identical to the reference launch (variant 0, 4 workgroups per CU)."""
+    print(f"[gpu] fp32 launch variant {launch}", flush=True)
+    perm = model.permutation(w, 1)
+    runs = []
+    for variant, bpc in [(0, 4), launch]:
+        g = GalicGPU(w)
+        g.fp32_calc_all_response_fields()
+        g.fp32_set_launch(variant, bpc)
+        acc = g.fp32_rep(perm[: w.n // 2], k=4, sigma_frac=0.1, seed=11)
+        runs.append([acc] + [g.stack(t, wh) for t in w.types for wh in range(10)] + [g.particles()[0]])
+        g.close()
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_fp32_history_tracks_fp64(w):
     """Same trials (K=1) on both paths for several reps."""
     print("[gpu] fp32 vs fp64 history", flush=True)
@@ -81,3 +100,36 @@ def test_fp32_history_tracks_fp64(w):
         assert abs(s32 - s64) / s64 < 1e-3, (t, s32, s64)
     g64.close()
     g32.close()
+
+
+def test_device_trial_draws_match_host_philox(w):
+    """Trials drawn on the device (k_draw_trials) equal the same Philox draws on
+    the host: every accepted velocity is v + sigma_frac * (g0 s_r e_r + g1 s_t e_t
+    + g2 s_p e_p) for the chosen trial k (basis of main.c:383-405)."""
+    from galic_amd.rng import gaussians
+    print("[gpu] device trial draws", flush=True)
+    g = GalicGPU(w)
+    g.fp32_calc_all_response_fields()
+    v0 = g.particles()[0].copy()
+    idx = model.permutation(w, 1)[:3000]
+    K, sig, seed = 4, 0.1, 77
+    acc = g.fp32_rep(idx, k=K, sigma_frac=sig, seed=seed)
+    v1 = g.particles()[0]
+    g.close()
+    sel = np.nonzero(acc > 0)[0]
+    assert len(sel) > 100
+    p = w.params
+    for i in sel[:400]:
+        pi = idx[i]
+        k = acc[i] - 1
+        x = w.pos[pi]
+        tovs = p.TypeOfVelocityStructure[w.ptype[pi]]
+        R, T, Q = model.local_basis(x[None, :], tovs == 2)
+        s = sig * np.sqrt(w.disp[pi, :3])
+        lim = p.MaxVelInUnitsVesc * w.vesc[pi]
+        for attempt in range(64):
+            gg = gaussians(seed, 3000, np.array([pi], dtype=np.uint64), attempt, k)[:, 0]
+            t = v0[pi] + gg[0] * s[0] * R[0] + gg[1] * s[1] * T[0] + gg[2] * s[2] * Q[0]
+            if np.linalg.norm(t) < lim:
+                break
+        np.testing.assert_allclose(v1[pi], t, rtol=1e-12, atol=1e-9)
