@@ -4,18 +4,21 @@ Workload: the precision-form model -- a beta=0.3 Hernquist halo (M=100,
 a=30 kpc) plus an exponential sech^2 disk (Md=4, Rd=3.5, z0=0.35 kpc),
 N=1e6 (80% halo, 20% disk), 256^2 force tables, 64^2 response grids.
 
-A step is one optimisation iteration of the batched FP32 path: every particle
-gets K=8 trial velocities (Gaussian perturbations of 10% of its target
-dispersions, drawn on the device), its current orbit and the K trial orbits
-are integrated and binned, the merit change of each trial is evaluated, the
-best improving trial is taken, and the grids are committed and re-fitted.
-orbits/s counts every integrated and binned orbit (current + trials).
+A step is one optimisation iteration of the batched FP32 path over all N
+particles in reps of 1e5 (each rep evaluated against the grids at its start and
+committed before the next, main.c:93-100): every particle gets K=8 trial
+velocities (Gaussian perturbations of 10% of its target dispersions, drawn on
+the device), its current orbit and the K trial orbits are integrated and
+binned, the merit change of each trial is evaluated, the best improving trial
+is taken, and each rep's deltas are committed and re-fitted.  orbits/s counts
+every integrated and binned orbit (current + trials).
 
 Beside it the line reports the plain FP64 path on the same model with the
-reference's method (one fresh trial per particle, reps of 1e5), the relative
-difference of the final chi^2 (Sall) between the FP32 and FP64 paths run on
-identical trials, the roofline of the FP32 orbit kernel, and the CPU oracle
-timed on the host's cores on a bounded sample.
+reference's method (one fresh trial per particle, reps of 1e5), the final
+chi^2 (Sall) of the FP32 and FP64 paths run on identical trials, the chi^2
+reached by each method after the same number of iterations, the roofline of
+the FP32 orbit kernel, and the CPU oracle timed on the host's cores on a
+bounded sample.
 """
 import argparse
 import json
@@ -29,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from galic_amd import _abi, model  # noqa: E402
-from galic_amd.runtime import GalicGPU  # noqa: E402
+from galic_amd.runtime import GalicGPU, direct_forces, tree_fill, tree_forces, tree_last_timings  # noqa: E402
 
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 rate
@@ -40,9 +43,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fp32_iteration(g, perm, k, it, sigma):
-    acc = g.fp32_rep(perm, k=k, sigma_frac=sigma, seed=1000 + it)
+def fp32_iteration(g, perm, k, it, sigma, rep):
+    acc = g.fp32_iterate(perm, k=k, rep_size=rep, sigma_frac=sigma, seed=1000 + it)
     return int((acc > 0).sum())
+
+
+def fp64_trials(w, perm, rep, it):
+    """Fresh trial velocities of one iteration of the plain method (main.c:407-435), per rep."""
+    return [(perm[a:a + rep], model.trial_velocities(w, perm[a:a + rep], 5 + it, r))
+            for r, a in enumerate(range(0, len(perm), rep))]
+
+
+def fp64_iteration(g, trials):
+    return np.concatenate([g.rep(idx, v)[0] for idx, v in trials])
 
 
 def cpu_baseline(cfg, seconds, nthreads):
@@ -72,6 +85,55 @@ def cpu_baseline(cfg, seconds, nthreads):
     return 2 * ntr / dt, dt, ntr
 
 
+def tree_sources(p, n, seed):
+    """The composite's mass (halo 75%, disk 20%, bulge 5% of the particles) sampled
+    from its profiles, as forcedensitygrid_calculate's sources (grid.c:659-678)."""
+    ids = np.arange(n, dtype=np.uint64)
+    nd, nb = n // 5, n // 20
+    nh = n - nd - nb
+    pos = np.concatenate([model.sample_halo(p, seed, 11, ids[:nh]), model.sample_disk(p, seed, 12, ids[nh:nh + nd]),
+                          model.sample_halo(p, seed, 13, ids[nh + nd:], a=p.Bulge_A)])
+    mass = np.concatenate([np.full(nh, p.Halo_Mass / nh), np.full(nd, p.Disk_Mass / nd), np.full(nb, p.Bulge_Mass / nb)])
+    return np.ascontiguousarray(pos), mass
+
+
+def tree_leg(n, nbin, ncheck):
+    """Force-field fill: Barnes-Hut table fill over n composite sources (seed 3),
+    theta = 0.4, FG_Nbin^2 x 8 sections of targets, FP32 walk with FP64 moments;
+    force error against FP64 direct summation on ncheck random table nodes."""
+    cfg = model.config("composite-4e6", fg_nbin=nbin, theta=0.4)
+    p = model.make_params(cfg)
+    t0 = time.perf_counter()
+    spos, smass = tree_sources(p, n, 3)
+    log(f"[bench] tree: {n} sources sampled in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    tree_fill(p, spos, smass, precision=1)
+    wall = time.perf_counter() - t0
+    tm = tree_last_timings()
+    ntarget = 8 * nbin * nbin
+    # error check on random table nodes (section, z row, R column)
+    rng = np.random.default_rng(3)
+    s, k, j = rng.integers(0, 8, ncheck), rng.integers(0, nbin, ncheck), rng.integers(0, nbin, ncheck)
+    r1 = p.FG_Rmin * (p.FG_Fac ** j - 1.0)
+    z1 = p.FG_Rmin * (p.FG_Fac ** k - 1.0)
+    phi = 2 * np.pi / 8 * s
+    tpos = np.stack([r1 * np.cos(phi), r1 * np.sin(phi), z1], axis=1)
+    acc32, pot32, _ = tree_forces(p, spos, smass, tpos, precision=1)
+    accd, potd = direct_forces(p, spos, smass, tpos)
+    err = np.linalg.norm(acc32 - accd, axis=1) / np.linalg.norm(accd, axis=1)
+    perr = np.abs(pot32 / potd - 1)
+    out = dict(sources=n, targets=ntarget, fg_nbin=nbin, theta=0.4, wall_ms=wall * 1e3, build_ms=tm["build_ms"],
+               walk_ms=tm["walk_ms"], interactions=tm["interactions"],
+               interactions_per_s=tm["interactions"] / (tm["walk_ms"] * 1e-3),
+               walk_tflops=21 * tm["interactions"] / (tm["walk_ms"] * 1e-3) / 1e12,
+               force_err_vs_direct_fp64=dict(nodes=ncheck, median=float(np.median(err)),
+                                             p99=float(np.quantile(err, 0.99)), max=float(err.max())),
+               pot_err_vs_direct_fp64=dict(median=float(np.median(perr)), max=float(perr.max())))
+    log(f"[bench] tree fill: {wall * 1e3:.0f} ms wall (build {tm['build_ms']:.0f} ms, walk {tm['walk_ms']:.0f} ms, "
+        f"{tm['interactions']:.3g} interactions), force error vs direct median {np.median(err):.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,9 +142,13 @@ def main():
     ap.add_argument("--n", type=int, default=1000000)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--sigma", type=float, default=0.1)
-    ap.add_argument("--fp64-reps", type=int, default=10, help="reps of the FP64 iteration (0: skip)")
+    ap.add_argument("--rep", type=int, default=10000, help="trials per rep (both paths)")
+    ap.add_argument("--fp64-iters", type=int, default=-1,
+                    help="iterations of the plain FP64 path (default warmup+steps; 0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget in seconds (0: skip)")
     ap.add_argument("--target-samples", type=int, default=4000000)
+    ap.add_argument("--tree-n", type=int, default=10000000, help="sources of the table-fill leg (0: skip)")
+    ap.add_argument("--tree-nbin", type=int, default=512)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,21 +166,23 @@ def main():
     # ---------------------------------------------------------- FP32 batched
     g = GalicGPU(w)
     g.fp32_calc_all_response_fields()
+    hist32 = [{t: float(g.fit(t)[7]) for t in w.types}]
     variant, bpc = g.fp32_autotune(min(w.n, 524288))
-    log(f"[bench] fp32 initial pass done; autotuned orbit kernel: flush variant {variant}, {bpc} workgroups/CU")
+    log(f"[bench] fp32 initial pass done; autotuned orbit kernel: {bpc} workgroups/CU")
     for it in range(args.warmup):
-        fp32_iteration(g, perm, args.k, it, args.sigma)
+        fp32_iteration(g, perm, args.k, it, args.sigma, args.rep)
+        hist32.append({t: float(g.fit(t)[7]) for t in w.types})
         log(f"[bench] warmup {it} done")
     g.reset_timings()
     t0 = time.perf_counter()
     accepted = 0
     for it in range(args.steps):
-        accepted += fp32_iteration(g, perm, args.k, args.warmup + it, args.sigma)
+        accepted += fp32_iteration(g, perm, args.k, args.warmup + it, args.sigma, args.rep)
+        hist32.append({t: float(g.fit(t)[7]) for t in w.types})
         log(f"[bench] step {it} done")
     wall = time.perf_counter() - t0
     tm = g.timings()
     orbits = args.steps * w.n * (args.k + 1)
-    sall32 = {t: float(g.fit(t)[7]) for t in w.types}
     g.close()
     value = orbits / wall
     steps_total = tm["steps"]
@@ -125,33 +193,65 @@ def main():
     # ------------------------------------------------------ FP64 plain path
     fp64 = None
     chi2 = None
-    if args.fp64_reps > 0:
-        ntr = w.n // args.fp64_reps
+    iters64 = args.warmup + args.steps if args.fp64_iters < 0 else args.fp64_iters
+    if iters64 > 0:
+        trials = [fp64_trials(w, perm, args.rep, it) for it in range(iters64)]
+        g64 = GalicGPU(w)
+        g64.calc_all_response_fields()
+        hist64 = [{t: float(g64.fit(t)[7]) for t in w.types}]
+        g64.reset_timings()
+        wall64 = 0.0
+        for it in range(iters64):
+            t1 = time.perf_counter()
+            fp64_iteration(g64, trials[it])
+            wall64 += time.perf_counter() - t1
+            hist64.append({t: float(g64.fit(t)[7]) for t in w.types})
+            log(f"[bench] fp64 iteration {it} done")
+        del trials
+        tm64 = g64.timings()
+        g64.close()
+        fp64 = dict(orbits_per_s=2 * w.n * iters64 / wall64, ms_per_iteration=wall64 / iters64 * 1e3,
+                    steps_per_s=tm64["steps"] / wall64, orbit_kernel_ms=tm64["orbit_ms"], iterations=iters64)
+        log(f"[bench] fp64: {fp64['orbits_per_s']:.4g} orbits/s")
+
+        # identical trials on both paths (K=1): the precision of the FP32 path alone
         g64 = GalicGPU(w)
         g64.calc_all_response_fields()
         g32 = GalicGPU(w)
         g32.fp32_calc_all_response_fields()
-        reps = [perm[r * ntr:(r + 1) * ntr] for r in range(args.fp64_reps)]
-        trials = [model.trial_velocities(w, idx, 5, r) for r, idx in enumerate(reps)]
         init = {t: {"fp64": float(g64.fit(t)[7]), "fp32": float(g32.fit(t)[7])} for t in w.types}
-        g64.reset_timings()
-        t1 = time.perf_counter()
-        acc64 = [g64.rep(idx, v)[0] for idx, v in zip(reps, trials)]
-        wall64 = time.perf_counter() - t1
-        tm64 = g64.timings()
-        agree = []
-        for (idx, v), a64 in zip(zip(reps, trials), acc64):
+        agree, drift = [], []
+        for r, a in enumerate(range(0, w.n, args.rep)):
+            idx = perm[a:a + args.rep]
+            v = model.trial_velocities(w, idx, 5, r)
+            a64 = g64.rep(idx, v)[0]
             a32 = g32.fp32_rep(idx, k=1, vtry=v)
-            agree.append(float(np.mean((a32 > 0) == (a64 > 0))))
-        log(f"[bench] chi2 check: initial Sall {init}, decision agreement per rep {agree}")
-        chi2 = {t: {"fp64": float(g64.fit(t)[7]), "fp32": float(g32.fit(t)[7])} for t in w.types}
-        worst = max(abs(c["fp32"] - c["fp64"]) / abs(c["fp64"]) for c in chi2.values())
-        fp64 = dict(orbits_per_s=2 * ntr * args.fp64_reps / wall64, ms_per_iteration=wall64 * 1e3,
-                    steps_per_s=tm64["steps"] / wall64, orbit_kernel_ms=tm64["orbit_ms"])
+            agree.append(round(float(np.mean((a32 > 0) == (a64 > 0))), 5))
+            drift.append({t: float(g32.fit(t)[7] / g64.fit(t)[7] - 1) for t in w.types})
+        same = {t: {"fp64": float(g64.fit(t)[7]), "fp32": float(g32.fit(t)[7])} for t in w.types}
         g64.close()
         g32.close()
-        log(f"[bench] fp64: {fp64['orbits_per_s']:.4g} orbits/s; chi2 rel diff fp32 vs fp64 {worst:.2e}")
-        chi2 = dict(per_type=chi2, max_rel_diff=worst, initial=init, decision_agreement=agree)
+        worst = max(abs(c["fp32"] - c["fp64"]) / abs(c["fp64"]) for c in same.values())
+        every = max(1, len(agree) // 10)
+        agree, drift = agree[every - 1::every], drift[every - 1::every]
+        log(f"[bench] identical trials: initial Sall {init}, final {same}, rel diff {worst:.2e}, "
+            f"decision agreement every {every} reps {agree}, Sall fp32/fp64 - 1 every {every} reps {drift}")
+        n_cmp = min(len(hist32), len(hist64)) - 1
+        # time to reach the plain path's final chi^2: iterations of the batched path
+        # until every type's Sall is at or below the plain path's after iters64
+        goal = hist64[-1]
+        reach = next((i for i in range(1, len(hist32)) if all(hist32[i][t] <= goal[t] for t in w.types)), None)
+        ms32 = wall / args.steps * 1e3
+        to_chi2 = dict(plain_fp64_iterations=iters64, plain_fp64_ms=fp64["ms_per_iteration"] * iters64,
+                       batched_fp32_iterations=reach, batched_fp32_ms=(reach * ms32) if reach else None,
+                       speedup=(fp64["ms_per_iteration"] * iters64 / (reach * ms32)) if reach else None)
+        chi2 = dict(identical_trials=dict(per_type=same, max_rel_diff=worst, initial=init, decision_agreement=agree,
+                                          rel_diff_per_rep=drift, sampled_every_reps=every, iterations=1,
+                                          rep=args.rep),
+                    methods=dict(iterations=n_cmp, batched_fp32=hist32[n_cmp], plain_fp64=hist64[n_cmp],
+                                 history_batched_fp32=hist32, history_plain_fp64=hist64),
+                    time_to_plain_final_chi2=to_chi2)
+        log(f"[bench] time to the plain path's final chi^2: {to_chi2}")
 
     # ------------------------------------------------------------- CPU
     cpu = None
@@ -163,6 +263,9 @@ def main():
                          f"optimize()) on a 20000-particle realisation of this model, {nthreads} OpenMP threads, "
                          f"{dt:.1f} s"}
         log(f"[bench] cpu: {rate:.4g} orbits/s on {nthreads} threads ({ntr} trials, {dt:.1f} s)")
+
+    # ------------------------------------------------------ table fill
+    tree = tree_leg(args.tree_n, args.tree_nbin, 4096) if args.tree_n > 0 else None
 
     # ---------------------------------------------------- HBM calibration
     copy_ms = np.zeros(1)
@@ -187,8 +290,8 @@ def main():
                    "n_particles": w.n, "halo_disk_split": [int(w.params.NType[1]), int(w.params.NType[2])],
                    "force_table": [w.params.FG_Nbin, w.params.FG_Nbin], "response_grid": [1 << w.params.DG_MaxLevel] * 2,
                    "trials_per_particle": args.k, "trial_sigma_frac": args.sigma,
-                   "orbits_per_step": w.n * (args.k + 1), "torbit_fac": cfg.torbit_fac,
-                   "autotuned": {"slot_flush": ["load-add-store", "atomic"][variant], "workgroups_per_cu": bpc}},
+                   "orbits_per_step": w.n * (args.k + 1), "rep": args.rep, "torbit_fac": cfg.torbit_fac,
+                   "autotuned": {"workgroups_per_cu": bpc}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": None,
                      "kernel": "k_orbit32", "note": "compute-bound on the FP32 vector ALU (MFMA unused); "
@@ -196,11 +299,11 @@ def main():
         "steps_per_s": steps_total / wall,
         "orbit_kernel_ms_per_step": tm["orbit_ms"] / args.steps,
         "accept_frac": accepted / (args.steps * w.n),
-        "final_sall": sall32,
         "fp64_plain": fp64,
         "fp32_vs_fp64_speedup": (value / fp64["orbits_per_s"]) if fp64 else None,
         "chi2_check": chi2,
         "cpu_baseline": cpu,
+        "tree_fill": tree,
         "copy_probe_GBs": copy_gbs,
     }
     print(json.dumps(line), flush=True)

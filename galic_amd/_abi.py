@@ -56,8 +56,8 @@ EXPORTS = [
     "galic_last_error", "galic_version", "galic_grid_create", "galic_density_geometry",
     "galic_create", "galic_destroy", "galic_set_force_tables", "galic_set_vstream", "galic_set_targets",
     "galic_set_particles", "galic_set_dispersions", "galic_get_particles", "galic_calc_all_response_fields", "galic_rep",
-    "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_field_at",
-    "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_tree_forces", "galic_tree_fill",
+    "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_fp32_orbit_responses", "galic_field_at",
+    "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_tree_last_timings",
     "galic_timings", "galic_reset_timings", "galic_copy_probe", "galic_bind",
     "produce_orbit_response_field_mod", "produce_orbit_response_field",
     "forcegrid_get_potential", "forcegrid_get_escape_speed", "forcegrid_get_acceleration", "smooth_stack",
@@ -107,15 +107,20 @@ def lib():
         "galic_get_stack": [_vp, ctypes.c_int, ctypes.c_int, _dp],
         "galic_get_fit": [_vp, ctypes.c_int, _dp],
         "galic_orbit_responses": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _lp, _dp, _dp],
+        "galic_fp32_orbit_responses": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _lp, _dp, _dp],
         "galic_field_at": [_vp, ctypes.c_int, _dp, _dp, _dp],
         "galic_fp32_calc_all_response_fields": [_vp],
         "galic_fp32_rep": [_vp, ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_ulonglong, _ip],
+        "galic_fp32_iterate": [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                               ctypes.c_ulonglong, _ip],
         "galic_fp32_autotune": [_vp, ctypes.c_int, _ip, _ip],
         "galic_fp32_set_launch": [_vp, ctypes.c_int, ctypes.c_int],
         "galic_tree_forces": [_P(GalicParams), ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                               ctypes.c_int, _lp],
         "galic_tree_fill": [_P(GalicParams), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                             _lp],
+        "galic_direct_forces": [_P(GalicParams), ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp],
+        "galic_tree_last_timings": [_dp],
         "galic_timings": [_vp, _dp],
         "galic_reset_timings": [_vp],
         "galic_copy_probe": [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp],

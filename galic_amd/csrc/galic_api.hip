@@ -50,6 +50,12 @@ int set_err(int code, const char *fmt, ...) {
                      #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
   } while(0)
 
+// HIP status as a library status (for paths that must release memory before returning)
+int hip_ok(hipError_t e) {
+  if(e == hipSuccess) return GALIC_OK;
+  return set_err(e == hipErrorOutOfMemory ? GALIC_ERR_NOMEM : GALIC_ERR_HIP, "%s", hipGetErrorString(e));
+}
+
 #define CHK(expr)          \
   do {                     \
     int r_ = (expr);       \
@@ -136,7 +142,8 @@ int alloc_orbit_scratch(galic::Ctx &c, int n) {
 int alloc_trial_scratch(galic::Ctx &c, int n) {
   if(n <= c.max_trials) return GALIC_OK;
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
-  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2);
+  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv);
+  CHK(dalloc(&c.t_inv, n));
   CHK(dalloc(&c.t_key, n));
   CHK(dalloc(&c.t_key2, n));
   CHK(dalloc(&c.t_ord, n));
@@ -383,12 +390,8 @@ int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
   HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 3 * sizeof(unsigned long long), c.stream));
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   const int bs = 256;
-  if(c.fp32_variant == 1)
-    hipLaunchKernelGGL(galic::k_orbit32_atomic, dim3(orbit_grid(A.n, bs, c.fp32_bpc)), dim3(bs), 0, c.stream,
-                       fast_model(c), A);
-  else
-    hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs, c.fp32_bpc)), dim3(bs), 0, c.stream,
-                       fast_model(c), A);
+  hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs, c.fp32_bpc)), dim3(bs), 0, c.stream, fast_model(c),
+                     A);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipEventSynchronize(c.ev1));
@@ -664,7 +667,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
   dfree(c.o_steps); dfree(c.o_bbox);
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
-  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.psteps);
+  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv); dfree(c.psteps);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -981,6 +984,71 @@ int galic_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const d
   return GALIC_OK;
 }
 
+int galic_fp32_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const double *vel, const double *mass,
+                               const double *tint, const int *type, double *fields, double *t, int *orbits,
+                               long long *steps, double *e0, double *e1) {
+  if(!ctx || norbit < 0 || (norbit > 0 && (!pos || !vel || !mass || !tint || !type)))
+    return set_err(GALIC_ERR_ARG, "bad orbit arguments");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  for(int o = 0; o < norbit; o++)
+    if(type[o] < 1 || type[o] > 3) return set_err(GALIC_ERR_ARG, "orbit %d: unknown type %d", o, type[o]);
+  const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
+  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(norbit, 1), (1ull << 30) / per));
+  CHK(alloc_orbit_scratch(c, chunk));
+  CHK(alloc_slots(c, chunk, per));
+  float *fin = nullptr;
+  HIPCHK(hipMalloc(&fin, sizeof(float) * 6 * chunk));
+  std::vector<double> tmp(chunk);
+  std::vector<float> cm(fields ? (size_t)chunk * galic::kFields * c.Ngrid : 0);
+  int rc = GALIC_OK;
+  for(int base = 0; base < norbit && rc == GALIC_OK; base += chunk) {
+    const int n = std::min(chunk, norbit - base);
+    double *d3[3] = {c.o_x, c.o_y, c.o_z};
+    double *v3[3] = {c.o_vx, c.o_vy, c.o_vz};
+    for(int k = 0; k < 3 && rc == GALIC_OK; k++) {
+      for(int o = 0; o < n; o++) tmp[o] = pos[3 * (size_t)(base + o) + k];
+      rc = hip_ok(hipMemcpy(d3[k], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      for(int o = 0; o < n; o++) tmp[o] = vel[3 * (size_t)(base + o) + k];
+      if(rc == GALIC_OK) rc = hip_ok(hipMemcpy(v3[k], tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    if(rc == GALIC_OK) rc = hip_ok(hipMemcpy(c.o_mass, mass + base, sizeof(double) * n, hipMemcpyHostToDevice));
+    if(rc == GALIC_OK) rc = hip_ok(hipMemcpy(c.o_tint, tint + base, sizeof(double) * n, hipMemcpyHostToDevice));
+    if(rc == GALIC_OK) rc = hip_ok(hipMemcpy(c.o_type, type + base, sizeof(int) * n, hipMemcpyHostToDevice));
+    if(rc == GALIC_OK) rc = hip_ok(hipMemsetAsync(c.slots, 0, per * n, c.stream));
+    if(rc != GALIC_OK) break;
+    galic::OrbitArgs32 A = orbit_args32(c, n);
+    A.slots = (float *)c.slots;
+    A.fin = fin;
+    rc = launch_orbit32(c, A);
+    if(rc != GALIC_OK) break;
+    if(e0 || e1) {
+      hipLaunchKernelGGL(galic::k_energy32, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, c.M, n,
+                         (const double *)c.o_x, (const double *)c.o_y, (const double *)c.o_z, (const double *)c.o_vx,
+                         (const double *)c.o_vy, (const double *)c.o_vz, (const float *)fin, c.o_e0, c.o_e1);
+      rc = hip_ok(hipGetLastError());
+    }
+    if(rc == GALIC_OK) rc = hip_ok(hipStreamSynchronize(c.stream));
+    if(rc == GALIC_OK && fields) {
+      rc = hip_ok(hipMemcpy(cm.data(), c.slots, per * n, hipMemcpyDeviceToHost));
+      for(int o = 0; o < n && rc == GALIC_OK; o++) {
+        const float *src = cm.data() + (size_t)o * galic::kFields * c.Ngrid;
+        double *dst = fields + (size_t)(base + o) * galic::kFields * c.Ngrid;
+        for(int cell = 0; cell < c.Ngrid; cell++)
+          for(int f = 0; f < galic::kFields; f++) dst[(size_t)f * c.Ngrid + cell] = src[(size_t)cell * galic::kFields + f];
+      }
+    }
+    if(rc == GALIC_OK) rc = hip_ok(hipMemset(c.slots, 0, per * n));
+    if(rc == GALIC_OK && t) rc = hip_ok(hipMemcpy(t + base, c.o_tall, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if(rc == GALIC_OK && orbits) rc = hip_ok(hipMemcpy(orbits + base, c.o_norbit, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if(rc == GALIC_OK && steps) rc = hip_ok(hipMemcpy(steps + base, c.o_steps, sizeof(long long) * n, hipMemcpyDeviceToHost));
+    if(rc == GALIC_OK && e0) rc = hip_ok(hipMemcpy(e0 + base, c.o_e0, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if(rc == GALIC_OK && e1) rc = hip_ok(hipMemcpy(e1 + base, c.o_e1, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(fin);
+  return rc;
+}
+
 int galic_set_dispersions(galic_ctx *ctx, int n, const double *disp4) {
   if(!ctx || !disp4) return set_err(GALIC_ERR_ARG, "null argument");
   galic::Ctx &c = ctx->c;
@@ -1056,12 +1124,12 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
                      (const int *)c.t_ord2, c.t_idx, (const double *)nullptr, (const double *)nullptr,
                      (const double *)nullptr, c.P, orbit_in(c));
   HIPCHK(hipGetLastError());
-  // variants give the same bits; only the launch shape and the slot flush differ.
-  // Grid sizes: half and all of the workgroups a CU holds at once, and twice that.
+  // grid sizes: half and all of the workgroups a CU holds at once, and twice that
+  // (the results are bit-identical; only the schedule differs)
   int occ = 4;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(galic::k_orbit32), 256, 0));
   occ = std::max(1, std::min(occ, 16));
-  const int variants[2] = {0, 1}, bpcs[3] = {std::max(1, occ / 2), occ, 2 * occ};
+  const int variants[1] = {0}, bpcs[3] = {std::max(1, occ / 2), occ, 2 * occ};
   double best = 1e300;
   int bv = 0, bb = 4;
   const long long steps0 = c.acc_steps, orbits0 = c.acc_orbits;
@@ -1098,18 +1166,28 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
 }
 
 int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu) {
-  if(!ctx || variant < 0 || variant > 1 || blocks_per_cu < 1 || blocks_per_cu > 32)
-    return set_err(GALIC_ERR_ARG, "bad FP32 launch (variant 0 or 1, 1..32 workgroups per CU)");
+  if(!ctx || variant != 0 || blocks_per_cu < 1 || blocks_per_cu > 32)
+    return set_err(GALIC_ERR_ARG, "bad FP32 launch (variant 0, 1..32 workgroups per CU)");
   ctx->c.fp32_variant = variant;
   ctx->c.fp32_bpc = blocks_per_cu;
   ctx->c.fp32_tuned = true;
   return GALIC_OK;
 }
 
-int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const double *vtry, double sigma_frac,
-                   unsigned long long seed, int *accept) {
-  if(!ctx || ntrial < 0 || (ntrial > 0 && !idx) || K < 1 || K > 64)
-    return set_err(GALIC_ERR_ARG, "bad FP32 rep arguments (K must be 1..64)");
+namespace {
+
+// Reps of the batched FP32 path.  ntrial trials (K per particle idx[i]) are
+// taken in reps of rep_size: every trial of a rep is evaluated against the grids
+// as they were at the start of the rep, and the rep's deltas are committed
+// before the next rep (main.c:93-100).  An orbit depends only on its particle's
+// position and velocity, and particles are distinct within a group of reps, so
+// the orbits of as many whole reps as the slot budget holds are integrated in one
+// launch, longest expected orbit first; the reps are then evaluated and
+// committed in turn.  The result is that of one rep at a time.
+int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, const double *vtry, double sigma_frac,
+             unsigned long long seed, int *accept) {
+  if(!ctx || ntrial < 0 || (ntrial > 0 && !idx) || K < 1 || K > 64 || rep_size < 1)
+    return set_err(GALIC_ERR_ARG, "bad FP32 rep arguments (K must be 1..64, rep_size >= 1)");
   galic::Ctx &c = ctx->c;
   if(!ctx->fp32_initial_done) return set_err(GALIC_ERR_STATE, "galic_fp32_calc_all_response_fields must run first");
   if(!vtry && !c.P.disp) return set_err(GALIC_ERR_STATE, "device trial draws need galic_set_dispersions");
@@ -1131,95 +1209,162 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
       for(size_t q = 0; q < (size_t)ntrial * K; q++)
         for(int k = 0; k < 3; k++) tv[k * nk + q] = vtry[3 * q + k];
       HIPCHK(hipMemcpy(c.t_vk, tv.data(), sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
-    } else {
-      hipLaunchKernelGGL(galic::k_draw_trials, dim3(nblocks((size_t)ntrial * K, 256)), dim3(256), 0, c.stream, ntrial,
-                         K, c.t_idx, c.P, (const double *)c.P.disp, (const double *)nullptr, sigma_frac,
-                         c.p.MaxVelInUnitsVesc, (uint32_t)seed, (uint32_t)(3000 + (seed >> 32)),
-                         (const int *)c.tovs_dev, vtx, vty, vtz);
-      HIPCHK(hipGetLastError());
     }
   }
-  // Every chunk is evaluated against the grids as they were at the start of the
-  // rep and its deltas are committed after the last chunk, so the order in which
-  // trials are integrated does not change the result: trials go longest
-  // expected orbit first.
-  if(galic::sort_trials_by_steps(ntrial, c.t_idx, c.psteps, c.t_key, c.t_key2, c.t_ord, c.t_ord2, &c.sort_tmp,
-                                 &c.sort_tmp_bytes, c.stream) != 0)
-    return set_err(GALIC_ERR_HIP, "sorting the trials by expected orbit length failed");
-  const int *order = c.t_ord2;
   const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
-  int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ntrial, 1), slot_budget(c) / ((K + 1) * per)));
-  CHK(alloc_orbit_scratch(c, chunk * (K + 1)));
-  CHK(alloc_slots(c, chunk * (K + 1), per));
+  const int cap = (int)std::max<size_t>(
+      1, std::min<size_t>((size_t)std::max(ntrial, 1), slot_budget(c) / ((K + 1) * per)));
+  const int nrep = (ntrial + rep_size - 1) / rep_size;
+  // rep r covers trials [r*rep_size, min((r+1)*rep_size, ntrial)); a group is a run of
+  // whole reps whose particles are distinct and whose orbits fit the slots (or a
+  // part of one rep when a single rep does not fit)
+  std::vector<char> seen(c.P.n, 0);
+  std::vector<int> touched;
   const size_t nbw = (size_t)GALIC_NTYPES * galic::kFields * c.Nstack;
-  hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream, (const double *)c.stacks,
-                     (const double *)c.weights, c.merit_bw, c.Nstack);
-  HIPCHK(hipGetLastError());
-  // all five fields in one pyramid pass when their coarse levels fit in 64 KiB of LDS
   const size_t ncoarse = (size_t)(c.Nstack - c.Ngrid);
+  // all five fields in one pyramid pass when their coarse levels fit in 64 KiB of LDS
   const int nf = (galic::kFields * ncoarse * sizeof(double) + 1024 <= 65536) ? galic::kFields : 1;
   const size_t shmem = sizeof(double) * (nf * ncoarse + 8);
+  bool packed = false;
   float eval_ms = 0;
-  for(int base = 0; base < ntrial; base += chunk) {
-    int nt = std::min(chunk, ntrial - base);
-    hipLaunchKernelGGL(galic::k_prepare_ktrials, dim3(nblocks((size_t)nt * (K + 1), 256)), dim3(256), 0, c.stream, nt,
-                       K, order + base, (const int *)c.t_idx, (const double *)vtx, (const double *)vty,
-                       (const double *)vtz, c.P, orbit_in(c));
+  int s0 = 0;
+  while(s0 < ntrial) {
+    int s1;
+    const int r0 = s0 / rep_size;
+    const int rep_end0 = std::min(ntrial, (r0 + 1) * rep_size);
+    if(rep_end0 - s0 > cap || s0 % rep_size != 0) {
+      s1 = std::min(rep_end0, s0 + cap);  // (part of) one rep
+    } else {
+      s1 = rep_end0;
+      for(int t : touched) seen[t] = 0;
+      touched.clear();
+      for(int i = s0; i < s1; i++)
+        if(!seen[idx[i]]) {
+          seen[idx[i]] = 1;
+          touched.push_back(idx[i]);
+        }
+      while(s1 < ntrial) {
+        const int e = std::min(ntrial, s1 + rep_size);
+        if(e - s0 > cap) break;
+        bool distinct = true;
+        for(int i = s1; i < e && distinct; i++) distinct = !seen[idx[i]];
+        if(!distinct) break;
+        for(int i = s1; i < e; i++)
+          if(!seen[idx[i]]) {
+            seen[idx[i]] = 1;
+            touched.push_back(idx[i]);
+          }
+        s1 = e;
+      }
+    }
+    const int ng = s1 - s0;
+    if(!vtry) {
+      // trials drawn from the particles' current velocities (unchanged by earlier reps
+      // of the group: its particles are distinct)
+      hipLaunchKernelGGL(galic::k_draw_trials, dim3(nblocks((size_t)ng * K, 256)), dim3(256), 0, c.stream, ng, K,
+                         c.t_idx + s0, c.P, (const double *)c.P.disp, (const double *)nullptr, sigma_frac,
+                         c.p.MaxVelInUnitsVesc, (uint32_t)seed, (uint32_t)(3000 + (seed >> 32)),
+                         (const int *)c.tovs_dev, vtx + (size_t)s0 * K, vty + (size_t)s0 * K, vtz + (size_t)s0 * K);
+      HIPCHK(hipGetLastError());
+    }
+    if(galic::sort_trials_by_steps(ng, c.t_idx + s0, c.psteps, c.t_key, c.t_key2, c.t_ord, c.t_ord2, &c.sort_tmp,
+                                   &c.sort_tmp_bytes, c.stream) != 0)
+      return set_err(GALIC_ERR_HIP, "sorting the trials by expected orbit length failed");
+    hipLaunchKernelGGL(galic::k_invert_order, dim3(nblocks(ng, 256)), dim3(256), 0, c.stream, ng,
+                       (const int *)c.t_ord2, c.t_inv);
+    CHK(alloc_orbit_scratch(c, ng * (K + 1)));
+    CHK(alloc_slots(c, ng * (K + 1), per));
+    hipLaunchKernelGGL(galic::k_prepare_ktrials, dim3(nblocks((size_t)ng * (K + 1), 256)), dim3(256), 0, c.stream, ng,
+                       K, (const int *)c.t_ord2, (const int *)(c.t_idx + s0), (const double *)(vtx + (size_t)s0 * K),
+                       (const double *)(vty + (size_t)s0 * K), (const double *)(vtz + (size_t)s0 * K), c.P,
+                       orbit_in(c));
     HIPCHK(hipGetLastError());
-    galic::OrbitArgs32 A = orbit_args32(c, nt * (K + 1));
+    galic::OrbitArgs32 A = orbit_args32(c, ng * (K + 1));
     A.slots = (float *)c.slots;
     CHK(launch_orbit32(c, A));
-    galic::EvalArgs32 E;
-    memset(&E, 0, sizeof(E));
-    E.P = c.P;
-    E.K = K;
-    E.order = order + base;
-    E.tidx = c.t_idx;
-    E.tvx = vtx;
-    E.tvy = vty;
-    E.tvz = vtz;
-    E.stacks = c.stacks;
-    E.merit_bw = c.merit_bw;
-    E.fit = c.fit;
-    E.cutB_of = c.cutB_of;
-    E.slots = (const float *)c.slots;
-    E.slot_stride = A.slot_stride;
-    E.slot_base = 0;
-    E.bbox = c.o_bbox;
-    E.norbit = c.o_norbit;
-    E.osteps = c.o_steps;
-    E.psteps = c.psteps;
-    E.locA_dq = c.locA_dq;
-    E.locI_dq = c.locI_dq;
-    E.accept = c.t_accept;
-    E.dF = c.t_dF;
-    for(int t = 0; t < GALIC_NTYPES; t++) {
-      E.MType[t] = c.p.MType[t];
-      E.fac[t][0] = c.p.fac_value_rs[t];
-      E.fac[t][1] = c.p.fac_value_ts[t];
-      E.fac[t][2] = c.p.fac_value_qs[t];
-      E.fac[t][3] = c.p.fac_value_ps[t];
+    // evaluate the reps of the group in turn, committing each completed rep
+    for(int r = r0; r < nrep && r * rep_size < s1; r++) {
+      const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
+      const int a = std::max(ra, s0), b = std::min(rb, s1);
+      if(!packed) {
+        hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream,
+                           (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.Nstack);
+        HIPCHK(hipGetLastError());
+        packed = true;
+      }
+      galic::EvalArgs32 E;
+      memset(&E, 0, sizeof(E));
+      E.P = c.P;
+      E.K = K;
+      E.trial_base = a;
+      E.slot_pos = c.t_inv + (a - s0);
+      E.tidx = c.t_idx;
+      E.tvx = vtx;
+      E.tvy = vty;
+      E.tvz = vtz;
+      E.stacks = c.stacks;
+      E.merit_bw = c.merit_bw;
+      E.fit = c.fit;
+      E.cutB_of = c.cutB_of;
+      E.slots = (const float *)c.slots;
+      E.slot_stride = A.slot_stride;
+      E.slot_base = 0;
+      E.bbox = c.o_bbox;
+      E.norbit = c.o_norbit;
+      E.osteps = c.o_steps;
+      E.psteps = c.psteps;
+      E.locA_dq = c.locA_dq;
+      E.locI_dq = c.locI_dq;
+      E.accept = c.t_accept;
+      E.dF = c.t_dF;
+      for(int t = 0; t < GALIC_NTYPES; t++) {
+        E.MType[t] = c.p.MType[t];
+        E.fac[t][0] = c.p.fac_value_rs[t];
+        E.fac[t][1] = c.p.fac_value_ts[t];
+        E.fac[t][2] = c.p.fac_value_qs[t];
+        E.fac[t][3] = c.p.fac_value_ps[t];
+      }
+      E.lowdisp = c.p.LowerDispLimit;
+      for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
+      E.iscale = c.fp32_iscale;
+      HIPCHK(hipEventRecord(c.ev0, c.stream));
+      if(nf == galic::kFields)
+        hipLaunchKernelGGL(galic::k_eval_apply32<galic::kFields>, dim3(b - a), dim3(256), shmem, c.stream, c.M, E);
+      else
+        hipLaunchKernelGGL(galic::k_eval_apply32<1>, dim3(b - a), dim3(256), shmem, c.stream, c.M, E);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c.ev1, c.stream));
+      HIPCHK(hipEventSynchronize(c.ev1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+      eval_ms += ms;
+      if(b == rb) {
+        CHK(commit_q(c));
+        CHK(global_fit(ctx));
+        packed = false;
+      }
     }
-    E.lowdisp = c.p.LowerDispLimit;
-    for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
-    E.iscale = c.fp32_iscale;
-    HIPCHK(hipEventRecord(c.ev0, c.stream));
-    if(nf == galic::kFields)
-      hipLaunchKernelGGL(galic::k_eval_apply32<galic::kFields>, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
-    else
-      hipLaunchKernelGGL(galic::k_eval_apply32<1>, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c.ev1, c.stream));
-    HIPCHK(hipEventSynchronize(c.ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-    eval_ms += ms;
+    s0 = s1;
+  }
+  if(ntrial == 0) {
+    CHK(commit_q(c));
+    CHK(global_fit(ctx));
   }
   c.last_eval_ms = eval_ms;
-  CHK(commit_q(c));
-  CHK(global_fit(ctx));
   if(ntrial > 0 && accept) HIPCHK(hipMemcpy(accept, c.t_accept, sizeof(int) * ntrial, hipMemcpyDeviceToHost));
   return GALIC_OK;
+}
+
+}  // namespace
+
+int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const double *vtry, double sigma_frac,
+                   unsigned long long seed, int *accept) {
+  return fp32_run(ctx, ntrial, idx, K, std::max(ntrial, 1), vtry, sigma_frac, seed, accept);
+}
+
+int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, const double *vtry,
+                       double sigma_frac, unsigned long long seed, int *accept) {
+  return fp32_run(ctx, ntrial, idx, K, rep_size, vtry, sigma_frac, seed, accept);
 }
 
 int galic_timings(galic_ctx *ctx, double *out7) {

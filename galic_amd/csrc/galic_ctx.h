@@ -131,6 +131,7 @@ struct Ctx {
   int *t_best = nullptr;
   unsigned int *t_key = nullptr, *t_key2 = nullptr;  // trial sort keys (expected orbit steps)
   int *t_ord = nullptr, *t_ord2 = nullptr;           // trial order, longest expected orbit first
+  int *t_inv = nullptr;                              // chunk position of each trial
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int *psteps = nullptr;  // per particle: steps of its current orbit (FP32 path's length prediction)
@@ -142,7 +143,7 @@ struct Ctx {
   double acc_orbit_ms = 0;
   long long acc_orbit_launches = 0;
   long long acc_orbits = 0, acc_steps = 0;
-  int fp32_variant = 0;   // slot flush of k_orbit32: 0 load-add-store, 1 returnless atomics (autotuned)
+  int fp32_variant = 0;   // orbit kernel variant (one so far)
   int fp32_bpc = 4;       // persistent workgroups per CU for k_orbit32 (autotuned)
   bool fp32_tuned = false;
   double fp32_scale[5];  // fixed-point scales per field (orbit-averaged)

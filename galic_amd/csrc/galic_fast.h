@@ -37,12 +37,14 @@ struct OrbitArgs32 {
   long long *steps;
   int4 *bbox;
   unsigned long long *stats;
+  float *fin;  // final x, y, z, vx, vy, vz per orbit, or null
 };
 
 struct EvalArgs32 {
   Particles P;
   int K;
-  const int *order;  // trial index of each chunk position, or null (position = trial)
+  int trial_base;       // block b evaluates trial trial_base + b
+  const int *slot_pos;  // ... whose orbits sit at chunk position slot_pos[b] (null: b)
   const int *tidx;
   const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
   const double *stacks, *fit;
@@ -66,13 +68,15 @@ struct EvalArgs32 {
 };
 
 __global__ void k_orbit32(FastModel F, OrbitArgs32 A);
-__global__ void k_orbit32_atomic(FastModel F, OrbitArgs32 A);
 __global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, const double *disp,
                               const double *vstr_unused, double sigma_frac, double maxvel, uint32_t seed,
                               uint32_t stream, const int *tovs_of_type, double *vtx, double *vty, double *vtz);
 __global__ void k_prepare_ktrials(int ntrial, int K, const int *order, const int *tidx, const double *vtx,
                                   const double *vty, const double *vtz, Particles P, OrbitIn O);
 __global__ void k_store_steps(int n, const long long *osteps, int *psteps);
+__global__ void k_invert_order(int n, const int *ord, int *inv);
+__global__ void k_energy32(DevModel M, int n, const double *ox, const double *oy, const double *oz, const double *ovx,
+                           const double *ovy, const double *ovz, const float *fin, double *e0, double *e1);
 int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigned int *key, unsigned int *key2,
                          int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
 __global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);

@@ -171,10 +171,7 @@ __device__ __forceinline__ void atomic_q(long long *p, double v) {
 
 // Adds a run's weights to the orbit's slot (cell-major: 5 fields per cell
 // contiguous, so a 2x2 block is two short row segments) or, without a slot,
-// to the fixed-point Loc grids.  The slot add is fac*w rounded, then added:
-// a load-add-store (ATOMIC false) and a returnless float atomic (ATOMIC true,
-// the wave does not wait for the line) give the same bits.
-template <bool ATOMIC>
+// to the fixed-point Loc grids.
 __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
                                             const FastModel &F) {
   if(r.quad < 0) return;
@@ -183,19 +180,10 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
     float *s0 = slot + (size_t)c[0] * kFields, *s2 = slot + (size_t)c[2] * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      const float v0 = __fmul_rn(fac, r.w[f][0]), v1 = __fmul_rn(fac, r.w[f][1]);
-      const float v2 = __fmul_rn(fac, r.w[f][2]), v3 = __fmul_rn(fac, r.w[f][3]);
-      if(ATOMIC) {
-        (void)atomicAdd(&s0[f], v0);
-        (void)atomicAdd(&s0[kFields + f], v1);
-        (void)atomicAdd(&s2[f], v2);
-        (void)atomicAdd(&s2[kFields + f], v3);
-      } else {
-        s0[f] = __fadd_rn(s0[f], v0);
-        s0[kFields + f] = __fadd_rn(s0[kFields + f], v1);
-        s2[f] = __fadd_rn(s2[f], v2);
-        s2[kFields + f] = __fadd_rn(s2[kFields + f], v3);
-      }
+      s0[f] += fac * r.w[f][0];
+      s0[kFields + f] += fac * r.w[f][1];
+      s2[f] += fac * r.w[f][2];
+      s2[kFields + f] += fac * r.w[f][3];
     }
   } else {
 #pragma unroll
@@ -210,7 +198,6 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
 
 // Trapezoid half-step weights of one end of a step (orbit_response.c:283-319)
 // accumulated in registers while the orbit stays in one 2x2 block.
-template <bool ATOMIC>
 __device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
                                       const FastModel &F, int iR, int iz, float fR, float fz, float dt, float vr2,
                                       float vt2, float vq2, float vp2, int4 &bb) {
@@ -218,7 +205,7 @@ __device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, fl
   float m[4] = {h * (1 - fR) * (1 - fz), h * fR * (1 - fz), h * (1 - fR) * fz, h * fR * fz};
   int quad = iz * nb + iR;
   if(quad != r.quad) {
-    run32_flush<ATOMIC>(r, nb, ngrid, fac, slot, locq, F);
+    run32_flush(r, nb, ngrid, fac, slot, locq, F);
     run32_reset(r, quad);
     bb.x = min(bb.x, iR);
     bb.y = max(bb.y, iR + 1);
@@ -282,7 +269,6 @@ __device__ __forceinline__ bool take_orbit32(const OrbitArgs32 &A, int nb, Lane3
 // busy however uneven the orbit lengths are.  A freshly taken orbit spends one
 // pass of the loop on the lookups at its start point (the pre-loop lookups of
 // orbit_response.c:199-210); every later pass is one step of :211-458.
-template <bool ATOMIC>
 __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs32 &A) {
   const int nb = F.dg_nbin, ngrid = nb * nb;
   Lane32 L;
@@ -312,7 +298,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       if(!(dt >= 0.f && dt < 3.0e38f)) {
         status = GALIC_ERR_ARG;
       } else {
-        bin32<ATOMIC>(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+        bin32(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
         hdt = 0.5f * dt;
         vx += hdt * ax;
         vy += hdt * ay;
@@ -341,7 +327,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       continue;
     }
     if(status == 0) {
-      bin32<ATOMIC>(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+      bin32(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
       L.tall += (double)dt;
       float radsign = vx * x + vy * y + vz * z;
       if(radsign > 0 && radsign_prev < 0) norbit++;
@@ -350,13 +336,22 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       if(steps > F.max_steps) status = GALIC_ERR_STEPCAP;
     }
     if(flag || status) {
-      run32_flush<ATOMIC>(run, nb, ngrid, L.fac, L.slot, L.locq, F);
+      run32_flush(run, nb, ngrid, L.fac, L.slot, L.locq, F);
       const int o = L.o;
       A.tall[o] = L.tall;
       A.norbit[o] = norbit;
       A.steps[o] = steps;
       A.bbox[o] = bb;
       A.status[o] = status;
+      if(A.fin) {
+        float *fo = A.fin + (size_t)o * 6;
+        fo[0] = x;
+        fo[1] = y;
+        fo[2] = z;
+        fo[3] = vx;
+        fo[4] = vy;
+        fo[5] = vz;
+      }
       orbit_stats_add(A.stats, steps, status, o);
       if(!take_orbit32(A, nb, L, x, y, z, vx, vy, vz)) break;
       L.tovs = F.tovs[L.type];
@@ -372,8 +367,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   }
 }
 
-__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body<false>(F, A); }
-__global__ void __launch_bounds__(256) k_orbit32_atomic(FastModel F, OrbitArgs32 A) { orbit32_body<true>(F, A); }
+__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body(F, A); }
 
 // ------------------------------------------------------------ trial draws --
 
@@ -497,6 +491,24 @@ __global__ void k_prepare_ktrials(int ntrial, int K, const int *order, const int
     O.ovy[o] = vty[q];
     O.ovz[o] = vtz[q];
   }
+}
+
+// Energy at the start and at the end of each FP32 orbit, with the FP64
+// potential (orbit_response.c:201-208, 464-473).
+__global__ void k_energy32(DevModel M, int n, const double *ox, const double *oy, const double *oz, const double *ovx,
+                           const double *ovy, const double *ovz, const float *fin, double *e0, double *e1) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if(o >= n) return;
+  e0[o] = 0.5 * (ovx[o] * ovx[o] + ovy[o] * ovy[o] + ovz[o] * ovz[o]) + potential64(M, ox[o], oy[o], oz[o]);
+  const float *f = fin + (size_t)o * 6;
+  const double x = f[0], y = f[1], z = f[2], vx = f[3], vy = f[4], vz = f[5];
+  e1[o] = 0.5 * (vx * vx + vy * vy + vz * vz) + potential64(M, x, y, z);
+}
+
+// inv[ord[j]] = j: chunk position of each trial.
+__global__ void k_invert_order(int n, const int *ord, int *inv) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if(j < n) inv[ord[j]] = j;
 }
 
 // Expected orbit length of each trial's particle: the steps its current orbit took.
@@ -716,8 +728,8 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
   __shared__ double s_dF[64];
   __shared__ int s_best;
 
-  const int j = blockIdx.x;                  // position in the chunk (orbits j*(K+1)..)
-  const int ti = A.order ? A.order[j] : j;   // trial index (trial arrays, accept, dF)
+  const int ti = A.trial_base + blockIdx.x;                    // trial (trial arrays, accept, dF)
+  const int j = A.slot_pos ? A.slot_pos[blockIdx.x] : (int)blockIdx.x;  // its orbits j*(K+1).. in the chunk
   const int p = A.tidx[ti];
   const int t = A.P.type[p];
   const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;

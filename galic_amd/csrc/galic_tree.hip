@@ -423,6 +423,43 @@ __global__ void k_project(int nb, const double *ax, const double *ay, const doub
   pot[i] = p / GALIC_FG_SECTIONS;
 }
 
+// Direct summation in FP64 with the walk's interaction (the check of the
+// tree's force error): one target per lane, sources staged through LDS.
+__global__ void __launch_bounds__(256) k_direct64(int nsrc, const double *sx, const double *sy, const double *sz,
+                                                   const double *sm, int ntarget, const double *tx, const double *ty,
+                                                   const double *tz, double h, double G, double *ax, double *ay,
+                                                   double *az, double *pot) {
+  __shared__ double4 tile[256];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < ntarget;
+  const double px = live ? tx[t] : 0, py = live ? ty[t] : 0, pz = live ? tz[t] : 0;
+  double acc_x = 0, acc_y = 0, acc_z = 0, potv = 0, fac, wp;
+  for(int base = 0; base < nsrc; base += 256) {
+    const int q = base + threadIdx.x;
+    tile[threadIdx.x] = (q < nsrc) ? make_double4(sx[q], sy[q], sz[q], sm[q]) : make_double4(0, 0, 0, 0);
+    __syncthreads();
+    const int nt = min(256, nsrc - base);
+    for(int k = 0; k < nt; k++) {
+      const double4 s = tile[k];
+      if(s.w != 0) {
+        const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+        interact64(s.w, dx * dx + dy * dy + dz * dz, h, fac, wp);
+        acc_x += dx * fac;
+        acc_y += dy * fac;
+        acc_z += dz * fac;
+        potv += wp;
+      }
+    }
+    __syncthreads();
+  }
+  if(live) {
+    ax[t] = acc_x * G;
+    ay[t] = acc_y * G;
+    az[t] = acc_z * G;
+    pot[t] = potv * G;
+  }
+}
+
 }  // namespace tree
 }  // namespace galic
 
@@ -451,6 +488,10 @@ int terr(int code, const char *fmt, ...) {
 
 inline int nb256(size_t n) { return (int)((n + 255) / 256); }
 
+// Device times of the last tree call of this process (ms): build (sort, split,
+// moments, threading), walk, and the interaction count.
+double g_last_build_ms = 0, g_last_walk_ms = 0, g_last_interactions = 0;
+
 struct DevBufs {
   std::vector<void *> ptrs;
   template <typename T>
@@ -476,8 +517,11 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
   int *idx_in = nullptr, *idx = nullptr;
   void *tmp = nullptr;
   size_t tmp_bytes = 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, eb = nullptr;
+  float build_ms = 0;
   T.nsrc = nsrc;
+  TCHK(hipEventCreate(&eb));
+  TCHK(hipEventRecord(eb, st));
   TCHK(B.alloc(&keys_in, nsrc));
   TCHK(B.alloc(&T.key, nsrc));
   TCHK(B.alloc(&idx_in, nsrc));
@@ -605,6 +649,8 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
     TCHK(hipEventCreate(&e0));
     TCHK(hipEventCreate(&e1));
     TCHK(hipEventRecord(e0, st));
+    TCHK(hipEventSynchronize(e0));
+    TCHK(hipEventElapsedTime(&build_ms, eb, e0));
     if(ntarget > 0) {
       if(precision == 0)
         hipLaunchKernelGGL(k_walk64, dim3(nb256(ntarget)), dim3(256), 0, st, W);
@@ -618,10 +664,14 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
     unsigned long long ni = 0;
     TCHK(hipMemcpy(&ni, nint, sizeof(ni), hipMemcpyDeviceToHost));
     if(ninteractions) *ninteractions = (long long)ni;
+    g_last_build_ms = build_ms;
+    g_last_walk_ms = walk_ms ? *walk_ms : 0;
+    g_last_interactions = (double)ni;
   }
 done:
   if(e0) (void)hipEventDestroy(e0);
   if(e1) (void)hipEventDestroy(e1);
+  if(eb) (void)hipEventDestroy(eb);
   return rc;
 }
 
@@ -755,6 +805,50 @@ int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *s
 done:
   if(st) (void)hipStreamDestroy(st);
   return rc;
+}
+
+int galic_direct_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                        int ntarget, const double *tpos, double *acc, double *pot) {
+  if(!p || nsrc < 0 || ntarget < 0 || (nsrc > 0 && (!spos || !smass)) || (ntarget > 0 && (!tpos || !acc || !pot)))
+    return terr(GALIC_ERR_ARG, "bad direct-summation arguments");
+  int rc = GALIC_OK;
+  hipStream_t st = nullptr;
+  DevBufs B;
+  double *px, *py, *pz, *pm, *tx, *ty, *tz, *ax, *ay, *az, *pt;
+  std::vector<double> h;
+  TCHK(hipSetDevice(device));
+  TCHK(hipStreamCreate(&st));
+  TCHK(B.alloc(&px, nsrc)); TCHK(B.alloc(&py, nsrc)); TCHK(B.alloc(&pz, nsrc)); TCHK(B.alloc(&pm, nsrc));
+  TCHK(B.alloc(&tx, ntarget)); TCHK(B.alloc(&ty, ntarget)); TCHK(B.alloc(&tz, ntarget));
+  TCHK(B.alloc(&ax, ntarget)); TCHK(B.alloc(&ay, ntarget)); TCHK(B.alloc(&az, ntarget)); TCHK(B.alloc(&pt, ntarget));
+  if((rc = upload_soa(st, nsrc, spos, px, py, pz)) != GALIC_OK) goto done;
+  if((rc = upload_soa(st, ntarget, tpos, tx, ty, tz)) != GALIC_OK) goto done;
+  if(nsrc > 0) TCHK(hipMemcpy(pm, smass, sizeof(double) * nsrc, hipMemcpyHostToDevice));
+  if(ntarget > 0) {
+    hipLaunchKernelGGL(galic::tree::k_direct64, dim3(nb256(ntarget)), dim3(256), 0, st, nsrc, (const double *)px,
+                       (const double *)py, (const double *)pz, (const double *)pm, ntarget, (const double *)tx,
+                       (const double *)ty, (const double *)tz, p->ForceSoftening, p->G, ax, ay, az, pt);
+    TCHK(hipGetLastError());
+    TCHK(hipStreamSynchronize(st));
+    h.resize(ntarget);
+    double *src[3] = {ax, ay, az};
+    for(int k = 0; k < 3; k++) {
+      TCHK(hipMemcpy(h.data(), src[k], sizeof(double) * ntarget, hipMemcpyDeviceToHost));
+      for(int i = 0; i < ntarget; i++) acc[3 * (size_t)i + k] = h[i];
+    }
+    TCHK(hipMemcpy(pot, pt, sizeof(double) * ntarget, hipMemcpyDeviceToHost));
+  }
+done:
+  if(st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
+int galic_tree_last_timings(double *out3) {
+  if(!out3) return terr(GALIC_ERR_ARG, "null argument");
+  out3[0] = g_last_build_ms;
+  out3[1] = g_last_walk_ms;
+  out3[2] = g_last_interactions;
+  return GALIC_OK;
 }
 
 }  // extern "C"

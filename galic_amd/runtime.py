@@ -78,6 +78,15 @@ class GalicGPU:
         check(self.lib.galic_fp32_rep(self.h, len(idx), iptr(idx), k, dptr(vt), sigma_frac, seed, iptr(acc)))
         return acc
 
+    def fp32_iterate(self, idx, k=1, rep_size=100000, vtry=None, sigma_frac=0.1, seed=1):
+        """Trials in reps of rep_size, each rep committed before the next."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        acc = np.zeros(len(idx), dtype=np.int32)
+        vt = None if vtry is None else np.ascontiguousarray(vtry, dtype=np.float64)
+        check(self.lib.galic_fp32_iterate(self.h, len(idx), iptr(idx), k, int(rep_size), dptr(vt), sigma_frac, seed,
+                                          iptr(acc)))
+        return acc
+
     def fp32_autotune(self, nsample=65536):
         v = np.zeros(1, dtype=np.int32)
         b = np.zeros(1, dtype=np.int32)
@@ -106,7 +115,9 @@ class GalicGPU:
         check(self.lib.galic_get_particles(self.h, dptr(vel), iptr(orb), *[dptr(x) for x in v]))
         return vel, orb, np.stack(v, -1)
 
-    def orbit_responses(self, pos, vel, mass, tint, ptype, fields=True):
+    def orbit_responses(self, pos, vel, mass, tint, ptype, fields=True, precision=64):
+        """One orbit per row: response fields [n][5][Ngrid], t, pericentre count,
+        steps, E0, E1.  precision=32 integrates with the FP32 path."""
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
         n = len(pos)
@@ -119,8 +130,9 @@ class GalicGPU:
         steps = np.zeros(n, dtype=np.int64)
         e0 = np.zeros(n)
         e1 = np.zeros(n)
-        check(self.lib.galic_orbit_responses(self.h, n, dptr(pos), dptr(vel), dptr(mass), dptr(tint), iptr(ptype),
-                                             dptr(f), dptr(t), iptr(orb), lptr(steps), dptr(e0), dptr(e1)))
+        fn = self.lib.galic_fp32_orbit_responses if precision == 32 else self.lib.galic_orbit_responses
+        check(fn(self.h, n, dptr(pos), dptr(vel), dptr(mass), dptr(tint), iptr(ptype), dptr(f), dptr(t), iptr(orb),
+                 lptr(steps), dptr(e0), dptr(e1)))
         return dict(fields=f, t=t, orbits=orb, steps=steps, e0=e0, e1=e1)
 
     def field_at(self, pos):
@@ -164,3 +176,22 @@ def tree_fill(params, spos, smass, precision=0, device=0):
     check(_abi.lib().galic_tree_fill(ctypes.byref(params), device, len(spos), dptr(spos), dptr(smass), dptr(pot),
                                      dptr(dr), dptr(dz), precision, lptr(nint)))
     return pot, dr, dz, int(nint[0])
+
+
+def direct_forces(params, spos, smass, tpos, device=0):
+    """FP64 direct summation with the tree's interaction: (acc [n][3], pot [n])."""
+    spos = np.ascontiguousarray(spos, dtype=np.float64).reshape(-1, 3)
+    smass = np.ascontiguousarray(smass, dtype=np.float64)
+    tpos = np.ascontiguousarray(tpos, dtype=np.float64).reshape(-1, 3)
+    acc = np.zeros_like(tpos)
+    pot = np.zeros(len(tpos))
+    check(_abi.lib().galic_direct_forces(ctypes.byref(params), device, len(spos), dptr(spos), dptr(smass), len(tpos),
+                                         dptr(tpos), dptr(acc), dptr(pot)))
+    return acc, pot
+
+
+def tree_last_timings():
+    """Device ms of the last tree call's build and walk, and its interaction count."""
+    out = np.zeros(3)
+    check(_abi.lib().galic_tree_last_timings(dptr(out)))
+    return dict(build_ms=out[0], walk_ms=out[1], interactions=out[2])

@@ -104,6 +104,13 @@ int galic_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const d
                           const double *tint, const int *type, double *fields, double *t, int *orbits,
                           long long *steps, double *e0, double *e1);
 
+/* The orbits of galic_orbit_responses integrated by the FP32 path (FP32 state
+ * and tables, FP64 time): the same outputs, for measuring its response error
+ * and energy drift against the FP64 path.  E0 and E1 use the FP64 potential. */
+int galic_fp32_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, const double *vel, const double *mass,
+                               const double *tint, const int *type, double *fields, double *t, int *orbits,
+                               long long *steps, double *e0, double *e1);
+
 /* Lookups at n points (pos: 3*n): acceleration (3*n) and potential (n) of
  * forcegrid_get_acceleration / forcegrid_get_potential (grid.c:59-197). */
 int galic_field_at(galic_ctx *ctx, int n, const double *pos, double *acc, double *pot);
@@ -122,14 +129,21 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx);
 int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int k, const double *vtry, double sigma_frac,
                    unsigned long long seed, int *accept);
 
-/* Times the FP32 orbit kernel's variants (slot flush by load-add-store, 0, or
- * by returnless atomics, 1) and persistent grid sizes (half, all and twice the
+/* ntrial trials in reps of rep_size (main.c:93-100: each rep evaluated against
+ * the grids at its start, then committed), with the arguments of galic_fp32_rep.
+ * Equivalent to galic_fp32_rep on consecutive slices of idx; the orbits of as
+ * many reps as fit in device memory are integrated in one launch. */
+int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int k, int rep_size, const double *vtry,
+                       double sigma_frac, unsigned long long seed, int *accept);
+
+/* Times the FP32 orbit kernel's persistent grid sizes (half, all and twice the
  * workgroups a CU holds at once) on nsample orbits of the model and keeps the
- * fastest for later reps; reports the choice.  The variants give bit-identical
- * results.  nsample should exceed the lanes the GPU holds at once. */
+ * fastest for later reps; reports the choice (variant: the kernel variant, 0).
+ * The launches give bit-identical results.  nsample should exceed the lanes the
+ * GPU holds at once. */
 int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_per_cu);
 
-/* Sets the FP32 orbit launch by hand (variant 0 or 1, 1..32 workgroups per CU). */
+/* Sets the FP32 orbit launch by hand (variant 0, 1..32 workgroups per CU). */
 int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu);
 
 /* ---- Barnes-Hut tree fill of the force tables ---- */
@@ -147,6 +161,16 @@ int galic_tree_forces(const galic_params *p, int device, int nsrc, const double 
  * Writes FG_Nbin^2 doubles each of pot, dpotdr, dpotdz. */
 int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
                     double *pot, double *dpotdr, double *dpotdz, int precision, long long *ninteractions);
+
+/* Direct summation in FP64 with the tree's softened interaction
+ * (forcetree_walk.c:162-183), times G: the reference point of the tree's force
+ * error.  O(nsrc * ntarget). */
+int galic_direct_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                        int ntarget, const double *tpos, double *acc, double *pot);
+
+/* Device times of the last galic_tree_forces / galic_tree_fill call of this
+ * process: out[0] tree build ms, out[1] walk ms, out[2] interactions. */
+int galic_tree_last_timings(double *out3);
 
 /* ---- measurement ---- */
 

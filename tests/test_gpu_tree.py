@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from galic_amd import model
-from galic_amd.runtime import tree_fill, tree_forces
+from galic_amd.runtime import direct_forces, tree_fill, tree_forces, tree_last_timings
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +100,22 @@ def test_table_fill_matches_oracle(oracle_mod):
     np.testing.assert_allclose(pg, po, rtol=1e-10)
     np.testing.assert_allclose(rg, ro, rtol=1e-9, atol=1e-11 * np.abs(ro).max())
     np.testing.assert_allclose(zg, zo, rtol=1e-9, atol=1e-11 * np.abs(zo).max())
+
+
+def test_direct_forces_match_oracle(oracle_mod):
+    """galic_direct_forces (the FP64 check of the tree at full size) against the
+    oracle's direct summation with the same spline kernel."""
+    print("[gpu] direct summation", flush=True)
+    p, spos, smass = sources(30000)
+    tpos = np.random.default_rng(7).normal(size=(700, 3)) * 4
+    P = ctypes.POINTER(ctypes.c_double)
+    acc_d = np.zeros((700, 3))
+    pot_d = np.zeros(700)
+    oracle_mod.lib().or_direct_forces(ctypes.byref(p), len(spos), oracle_mod.d(spos), oracle_mod.d(smass), 700,
+                                      oracle_mod.d(tpos), acc_d.ctypes.data_as(P), pot_d.ctypes.data_as(P), 8)
+    acc_g, pot_g = direct_forces(p, spos, smass, tpos)
+    np.testing.assert_allclose(acc_g, acc_d, rtol=1e-11, atol=1e-13 * np.abs(acc_d).max())
+    np.testing.assert_allclose(pot_g, pot_d, rtol=1e-11)
+    tree_forces(p, spos, smass, tpos, precision=1)
+    tm = tree_last_timings()
+    assert tm["interactions"] > 0 and tm["walk_ms"] > 0 and tm["build_ms"] > 0
