@@ -35,6 +35,9 @@ from galic_amd import _abi, model  # noqa: E402
 from galic_amd.runtime import GalicGPU, direct_forces, tree_fill, tree_forces, tree_last_timings  # noqa: E402
 
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
+# HBM bytes per leapfrog step of k_orbit32 in the iteration's launches, from the FETCH_SIZE and
+# WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_halo-disk-1e6.txt)
+HBM_BYTES_PER_STEP = 67.1
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 rate
 PEAK_HBM = 8000.0     # GB/s
 
@@ -293,9 +296,13 @@ def main():
                    "orbits_per_step": w.n * (args.k + 1), "rep": args.rep, "torbit_fac": cfg.torbit_fac,
                    "autotuned": {"workgroups_per_cu": bpc}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
-                     "kernel": "k_orbit32", "note": "compute-bound on the FP32 vector ALU (MFMA unused); "
-                     f"achieved = {F_STEP} algorithmic flop/step x leapfrog steps / HIP-event time of the kernel"},
+                     "frac": achieved / PEAK_FP32,
+                     "traffic": HBM_BYTES_PER_STEP * steps_total / max(1, tm["orbit_launches"]),
+                     "kernel": "k_orbit32",
+                     "note": f"roof: the FP32 vector ALU (no step is a contraction, MFMA unused); achieved = {F_STEP} "
+                             "algorithmic flop/step x leapfrog steps / HIP-event time of the kernel; traffic = HBM "
+                             f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE): "
+                             "the kernel is held back by the read-modify-write of its orbit slots, not by the ALU"},
         "steps_per_s": steps_total / wall,
         "orbit_kernel_ms_per_step": tm["orbit_ms"] / args.steps,
         "accept_frac": accepted / (args.steps * w.n),
