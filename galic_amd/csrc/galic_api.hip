@@ -1261,8 +1261,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     if(!vtry) {
       // trials drawn from the particles' current velocities (unchanged by earlier reps
       // of the group: its particles are distinct)
-      hipLaunchKernelGGL(galic::k_draw_trials, dim3(nblocks((size_t)ng * K, 256)), dim3(256), 0, c.stream, ng, K,
-                         c.t_idx + s0, c.P, (const double *)c.P.disp, (const double *)nullptr, sigma_frac,
+      hipLaunchKernelGGL(galic::k_draw_trials, dim3(nblocks((size_t)ng * K, 256)), dim3(256), 0, c.stream, c.M, ng, K,
+                         std::min(c.fp32_nfresh, K), c.t_idx + s0, c.P, (const double *)c.P.disp, sigma_frac,
                          c.p.MaxVelInUnitsVesc, (uint32_t)seed, (uint32_t)(3000 + (seed >> 32)),
                          (const int *)c.tovs_dev, vtx + (size_t)s0 * K, vty + (size_t)s0 * K, vtz + (size_t)s0 * K);
       HIPCHK(hipGetLastError());
@@ -1356,6 +1356,12 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
 }
 
 }  // namespace
+
+int galic_fp32_set_fresh_trials(galic_ctx *ctx, int nfresh) {
+  if(!ctx || nfresh < 0 || nfresh > 64) return set_err(GALIC_ERR_ARG, "nfresh must be 0..64");
+  ctx->c.fp32_nfresh = nfresh;
+  return GALIC_OK;
+}
 
 int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const double *vtry, double sigma_frac,
                    unsigned long long seed, int *accept) {

@@ -145,6 +145,7 @@ struct Ctx {
   long long acc_orbits = 0, acc_steps = 0;
   int fp32_variant = 0;   // orbit kernel variant (one so far)
   int fp32_bpc = 4;       // persistent workgroups per CU for k_orbit32 (autotuned)
+  int fp32_nfresh = 0;    // device trials per particle drawn fresh (optimize()'s distribution)
   bool fp32_tuned = false;
   double fp32_scale[5];  // fixed-point scales per field (orbit-averaged)
   double fp32_iscale;    // fixed-point scale of the instantaneous fields

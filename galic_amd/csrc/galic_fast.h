@@ -68,9 +68,9 @@ struct EvalArgs32 {
 };
 
 __global__ void k_orbit32(FastModel F, OrbitArgs32 A);
-__global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, const double *disp,
-                              const double *vstr_unused, double sigma_frac, double maxvel, uint32_t seed,
-                              uint32_t stream, const int *tovs_of_type, double *vtx, double *vty, double *vtz);
+__global__ void k_draw_trials(DevModel M, int ntrial, int K, int nfresh, const int *tidx, Particles P,
+                              const double *disp, double sigma_frac, double maxvel, uint32_t seed, uint32_t stream,
+                              const int *tovs_of_type, double *vtx, double *vty, double *vtz);
 __global__ void k_prepare_ktrials(int ntrial, int K, const int *order, const int *tidx, const double *vtx,
                                   const double *vty, const double *vtz, Particles P, OrbitIn O);
 __global__ void k_store_steps(int n, const long long *osteps, int *psteps);

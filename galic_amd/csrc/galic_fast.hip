@@ -412,20 +412,23 @@ __device__ void gauss4(uint64_t id, uint32_t attempt, uint32_t block, uint32_t s
   g[3] = r2 * sin(2 * M_PI * u[3]);
 }
 
-// K perturbations of the current velocity per particle: Gaussian in the local
-// basis of optimize() (main.c:383-405) with sigma_frac times the target
-// dispersions, redrawn while |v| >= MaxVelInUnitsVesc * Vesc (main.c:415-435);
-// after 64 attempts the trial is the current velocity (a no-op trial).
-__global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, const double *disp,
-                              const double *vstr_unused, double sigma_frac, double maxvel, uint32_t seed,
-                              uint32_t stream, const int *tovs_of_type, double *vtx, double *vty, double *vtz) {
+// K trial velocities per particle.  Trials k < nfresh are fresh draws from the
+// distribution of optimize() (main.c:383-435): Gaussians of the target
+// dispersions in the local basis, the third around the streaming velocity.
+// The other trials perturb the current velocity by sigma_frac times the same
+// dispersions.  Both redraw while |v| >= MaxVelInUnitsVesc * Vesc; after 64
+// attempts the trial is the current velocity (a no-op trial).
+__global__ void k_draw_trials(DevModel M, int ntrial, int K, int nfresh, const int *tidx, Particles P,
+                              const double *disp, double sigma_frac, double maxvel, uint32_t seed, uint32_t stream,
+                              const int *tovs_of_type, double *vtx, double *vty, double *vtz) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if(q >= ntrial * K) return;
   int i = q / K, k = q % K;
   int p = tidx[i];
   double x = P.x[p], y = P.y[p], z = P.z[p];
   double vx = P.vx[p], vy = P.vy[p], vz = P.vz[p];
-  int tovs = tovs_of_type[P.type[p]];
+  const int type = P.type[p];
+  int tovs = tovs_of_type[type];
   double phi = atan2(y, x);
   double sp = sin(phi), cp = cos(phi);
   double e1[3], e2[3], e3[3];
@@ -440,18 +443,28 @@ __global__ void k_draw_trials(int ntrial, int K, const int *tidx, Particles P, c
     e2[0] = -ct * cp; e2[1] = -ct * sp; e2[2] = st;
     e3[0] = -sp; e3[1] = cp; e3[2] = 0;
   }
-  double s1 = sigma_frac * sqrt(disp[4 * (size_t)p + 0]);
-  double s2 = sigma_frac * sqrt(disp[4 * (size_t)p + 1]);
-  double s3 = sigma_frac * sqrt(disp[4 * (size_t)p + 2]);
+  const bool fresh = k < nfresh;
+  const double f = fresh ? 1.0 : sigma_frac;
+  double s1 = f * sqrt(disp[4 * (size_t)p + 0]);
+  double s2 = f * sqrt(disp[4 * (size_t)p + 1]);
+  double s3 = f * sqrt(disp[4 * (size_t)p + 2]);
+  // fresh draws are centred on (0, 0, v_str) in the local basis, perturbations on v
+  double cx = vx, cy = vy, cz = vz;
+  if(fresh) {
+    double vstr = vstream64(M, fg_index64(M, x, y, z), type);
+    cx = vstr * e3[0];
+    cy = vstr * e3[1];
+    cz = vstr * e3[2];
+  }
   double lim = maxvel * P.vesc[p];
   double ox = vx, oy = vy, oz = vz;
   for(uint32_t attempt = 0; attempt < 64; attempt++) {
     double g[4];
     gauss4((uint64_t)p, attempt, (uint32_t)k, seed, stream, g);
     double a = g[0] * s1, b = g[1] * s2, c = g[2] * s3;
-    double tx = vx + a * e1[0] + b * e2[0] + c * e3[0];
-    double ty = vy + a * e1[1] + b * e2[1] + c * e3[1];
-    double tz = vz + a * e1[2] + b * e2[2] + c * e3[2];
+    double tx = cx + a * e1[0] + b * e2[0] + c * e3[0];
+    double ty = cy + a * e1[1] + b * e2[1] + c * e3[1];
+    double tz = cz + a * e1[2] + b * e2[2] + c * e3[2];
     if(sqrt(tx * tx + ty * ty + tz * tz) < lim) {
       ox = tx;
       oy = ty;

@@ -96,6 +96,10 @@ class GalicGPU:
     def fp32_set_launch(self, variant, blocks_per_cu):
         check(self.lib.galic_fp32_set_launch(self.h, int(variant), int(blocks_per_cu)))
 
+    def fp32_set_fresh_trials(self, nfresh):
+        """The first nfresh device trials per particle are fresh draws (main.c:383-435)."""
+        check(self.lib.galic_fp32_set_fresh_trials(self.h, int(nfresh)))
+
     # ---------------------------------------------------------------- reads --
     def stack(self, t, which):
         out = np.zeros(self.nstack)

@@ -129,6 +129,12 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx);
 int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int k, const double *vtry, double sigma_frac,
                    unsigned long long seed, int *accept);
 
+/* Of the k trials a particle gets from the device draws of galic_fp32_rep /
+ * galic_fp32_iterate, the first nfresh are fresh draws from optimize()'s trial
+ * distribution (main.c:383-435) and the rest perturbations of the current
+ * velocity (default nfresh = 0). */
+int galic_fp32_set_fresh_trials(galic_ctx *ctx, int nfresh);
+
 /* ntrial trials in reps of rep_size (main.c:93-100: each rep evaluated against
  * the grids at its start, then committed), with the arguments of galic_fp32_rep.
  * Equivalent to galic_fp32_rep on consecutive slices of idx; the orbits of as

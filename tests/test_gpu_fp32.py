@@ -152,11 +152,13 @@ def test_fp32_history_tracks_fp64(w):
 def test_device_trial_draws_match_host_philox(w):
     """Trials drawn on the device (k_draw_trials) equal the same Philox draws on
     the host: every accepted velocity is v + sigma_frac * (g0 s_r e_r + g1 s_t e_t
-    + g2 s_p e_p) for the chosen trial k (basis of main.c:383-405)."""
+    + g2 s_p e_p) for a perturbation trial k, and v_str e_p + g0 s_r e_r + g1 s_t
+    e_t + g2 s_p e_p for the fresh trial (basis of main.c:383-405)."""
     from galic_amd.rng import gaussians
     print("[gpu] device trial draws", flush=True)
     g = GalicGPU(w)
     g.fp32_calc_all_response_fields()
+    g.fp32_set_fresh_trials(1)
     v0 = g.particles()[0].copy()
     idx = model.permutation(w, 1)[:3000]
     K, sig, seed = 4, 0.1, 77
@@ -166,17 +168,23 @@ def test_device_trial_draws_match_host_philox(w):
     sel = np.nonzero(acc > 0)[0]
     assert len(sel) > 100
     p = w.params
-    for i in sel[:400]:
+    kinds = set()
+    for i in sel[:1500]:
         pi = idx[i]
         k = acc[i] - 1
         x = w.pos[pi]
         tovs = p.TypeOfVelocityStructure[w.ptype[pi]]
         R, T, Q = model.local_basis(x[None, :], tovs == 2)
-        s = sig * np.sqrt(w.disp[pi, :3])
+        # trial 0 is a fresh draw around (0, 0, v_str) with the full dispersions
+        # (main.c:383-435), the others perturb the current velocity by sig * sigma
+        s = (1.0 if k == 0 else sig) * np.sqrt(w.disp[pi, :3])
+        centre = w.vstr[pi] * Q[0] if k == 0 else v0[pi]
         lim = p.MaxVelInUnitsVesc * w.vesc[pi]
         for attempt in range(64):
             gg = gaussians(seed, 3000, np.array([pi], dtype=np.uint64), attempt, k)[:, 0]
-            t = v0[pi] + gg[0] * s[0] * R[0] + gg[1] * s[1] * T[0] + gg[2] * s[2] * Q[0]
+            t = centre + gg[0] * s[0] * R[0] + gg[1] * s[1] * T[0] + gg[2] * s[2] * Q[0]
             if np.linalg.norm(t) < lim:
                 break
-        np.testing.assert_allclose(v1[pi], t, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(v1[pi], t, rtol=1e-10, atol=1e-9)
+        kinds.add(k == 0)
+    assert kinds == {True, False}
