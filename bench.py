@@ -194,8 +194,12 @@ def main():
     value = orbits / wall
     steps_total = tm["steps"]
     achieved = F_STEP * steps_total / (tm["orbit_ms"] * 1e-3) / 1e12
+    breakdown = dict(orbit_kernel=tm["orbit_ms"] / args.steps, merit_kernel=tm["eval_ms"] / args.steps,
+                     fit=tm["fit_ms"] / args.steps,
+                     other=(wall * 1e3 - tm["orbit_ms"] - tm["eval_ms"] - tm["fit_ms"]) / args.steps)
     log(f"[bench] fp32: {value:.4g} orbits/s, {steps_total / wall:.4g} steps/s, orbit kernel {tm['orbit_ms']:.1f} ms "
-        f"of {wall * 1e3:.1f} ms, {achieved:.2f} TFLOP/s algorithmic")
+        f"of {wall * 1e3:.1f} ms, {achieved:.2f} TFLOP/s algorithmic; ms per step {breakdown}, "
+        f"log relaunches {tm['log_relaunches']:.0f}")
 
     # ------------------------------------------------------ FP64 plain path
     fp64 = None
@@ -312,7 +316,7 @@ def main():
                              f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE): "
                              "the kernel is held back by the read-modify-write of its orbit slots, not by the ALU"},
         "steps_per_s": steps_total / wall,
-        "orbit_kernel_ms_per_step": tm["orbit_ms"] / args.steps,
+        "ms_per_step_breakdown": breakdown,
         "accept_frac": accepted / (args.steps * w.n),
         "fp64_plain": fp64,
         "fp32_vs_fp64_speedup": (value / fp64["orbits_per_s"]) if fp64 else None,

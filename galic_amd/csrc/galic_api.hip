@@ -225,7 +225,7 @@ int finish_orbit_launch(galic::Ctx &c, int n) {
 int launch_orbit64(galic::Ctx &c, galic::OrbitArgs A) {
   if(A.n <= 0) return GALIC_OK;
   HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
-  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 3 * sizeof(unsigned long long), c.stream));
+  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 4 * sizeof(unsigned long long), c.stream));
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   const int bs = 256;
   hipLaunchKernelGGL(galic::k_orbit64, dim3(orbit_grid(A.n, bs)), dim3(bs), 0, c.stream, c.M, A);
@@ -326,6 +326,7 @@ int global_fit(galic_ctx *ctx) {
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipEventSynchronize(c.ev1));
   HIPCHK(hipEventElapsedTime(&c.last_fit_ms, c.ev0, c.ev1));
+  c.acc_fit_ms += c.last_fit_ms;
   return GALIC_OK;
 }
 
@@ -387,7 +388,7 @@ galic::OrbitArgs32 orbit_args32(galic::Ctx &c, int n) {
 int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
   if(A.n <= 0) return GALIC_OK;
   HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
-  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 3 * sizeof(unsigned long long), c.stream));
+  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 4 * sizeof(unsigned long long), c.stream));
   HIPCHK(hipEventRecord(c.ev0, c.stream));
   const int bs = 256;
   hipLaunchKernelGGL(galic::k_orbit32, dim3(orbit_grid(A.n, bs, c.fp32_bpc)), dim3(bs), 0, c.stream, fast_model(c),
@@ -401,6 +402,105 @@ int launch_orbit32(galic::Ctx &c, galic::OrbitArgs32 A) {
   c.acc_orbit_ms += ms;
   c.acc_orbit_launches++;
   return finish_orbit_launch(c, A.n);
+}
+
+// Exact sums of the response logs: the merit kernel turns each FP32 record
+// value into an int64 at log_scale[f] = 2^k units per response unit and adds
+// integers, so D = new - old is exact and independent of the order.  k is the
+// largest power that keeps an orbit's total of a field (the particle's mass,
+// times at most vmax^2 for a moment) below 2^60.
+void log_setup(galic::Ctx &c) {
+  const double vmax = c.max_vesc > 0 ? 2 * c.max_vesc : 10 * c.p.V200;
+  for(int f = 0; f < galic::kFields; f++) {
+    const double bound = std::max(c.max_mass, 1e-300) * (f == 0 ? 1.0 : vmax * vmax);
+    int e = 0;
+    frexp(bound, &e);  // bound < 2^e
+    const int k = std::max(-120, std::min(120, 60 - e));
+    c.log_scale[f] = ldexpf(1.0f, k);
+    c.log_unit[f] = ldexp(1.0, -k);
+    c.log_to_loc[f] = c.fp32_scale[f] * ldexp(1.0, -k);
+  }
+}
+
+size_t log_block_bytes() { return (size_t)galic::LOG_B * (sizeof(int) + galic::kFields * sizeof(float4)); }
+
+// The log arena (the `slots` buffer): reallocated only when it is below half of
+// what a call could use, so repeated calls reuse one allocation.
+int ensure_arena(galic::Ctx &c, size_t want, size_t need) {
+  if(c.slot_bytes >= need && 2 * c.slot_bytes >= want) return GALIC_OK;
+  if(c.slots) (void)hipFree(c.slots);
+  c.slots = nullptr;
+  c.slot_bytes = 0;
+  const size_t bytes = std::max(want, need);
+  HIPCHK(hipMalloc(&c.slots, bytes));
+  c.slot_bytes = bytes;
+  return GALIC_OK;
+}
+
+// Blocks of the arena a launch of n orbits is planned with.
+double log_blocks_planned(const galic::Ctx &c, int n) { return (double)n * (c.log_est_blocks + 1.0) * 1.25 + 64; }
+
+constexpr int kLogArenaFull = 1;  // launch_orbit32_log: the arena ran out, plan a smaller launch
+
+// One launch of the logging orbit kernel over A.n orbits into the whole arena.
+// If an orbit needs more blocks than its table holds, the table grows and the
+// launch is repeated (the orbits and their logs do not depend on the launch, so
+// the result is the same).  If the arena runs out, the block estimate doubles and
+// kLogArenaFull asks the caller for a smaller launch.
+int launch_orbit32_log(galic::Ctx &c, galic::OrbitArgs32 A) {
+  if(A.n <= 0) return GALIC_OK;
+  const size_t bb = log_block_bytes();
+  for(int attempt = 0; attempt < 8; attempt++) {
+    const size_t cap = c.slot_bytes / bb;
+    if(cap < 64) return set_err(GALIC_ERR_NOMEM, "no room for orbit response logs");
+    const size_t nt = (size_t)A.n * c.log_M;
+    if(nt > c.log_blocks_len) {
+      dfree(c.log_blocks);
+      CHK(dalloc(&c.log_blocks, nt));
+      c.log_blocks_len = nt;
+    }
+    if(A.n > c.log_n_len) {
+      dfree(c.log_n);
+      CHK(dalloc(&c.log_n, (size_t)A.n));
+      c.log_n_len = A.n;
+    }
+    A.log_blocks = c.log_blocks;
+    A.log_n = c.log_n;
+    A.log_M = c.log_M;
+    A.log_q = c.log_q = (int *)c.slots;
+    A.log_v = c.log_v = (float4 *)((int *)c.slots + cap * galic::LOG_B);
+    A.log_next = c.log_next;
+    A.log_cap = (unsigned int)std::min<size_t>(cap, 0xffffff00u);
+    HIPCHK(hipMemsetAsync(c.log_next, 0, sizeof(unsigned int), c.stream));
+    HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
+    HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 4 * sizeof(unsigned long long), c.stream));
+    HIPCHK(hipEventRecord(c.ev0, c.stream));
+    hipLaunchKernelGGL(galic::k_orbit32_log, dim3(orbit_grid(A.n, 256, c.fp32_bpc)), dim3(256), 0, c.stream,
+                       fast_model(c), A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, c.stream));
+    HIPCHK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    unsigned long long st[4];
+    unsigned int used = 0;
+    HIPCHK(hipMemcpy(st, c.orbit_stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&used, c.log_next, sizeof(used), hipMemcpyDeviceToHost));
+    if(st[3] == 0) {
+      c.last_orbit_ms = ms;
+      c.acc_orbit_ms += ms;
+      c.acc_orbit_launches++;
+      c.log_est_blocks = std::max(1.0, (double)std::min<size_t>(used, cap) / A.n);
+      return finish_orbit_launch(c, A.n);
+    }
+    c.acc_log_relaunches++;
+    if((size_t)used >= cap) {
+      c.log_est_blocks *= 2;
+      return kLogArenaFull;
+    }
+    c.log_M *= 2;  // an orbit needed more blocks than its table holds
+  }
+  return set_err(GALIC_ERR_NOMEM, "orbit response logs still overflow after growing the block tables");
 }
 
 // Loc += delta in fixed point, and the FP64 copies the global fit reads.
@@ -587,7 +687,8 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.tovs_dev, GALIC_NTYPES));
   CHK(dalloc(&c.fit, (size_t)GALIC_NTYPES * 16));
   CHK(dalloc(&c.counter, 1));
-  CHK(dalloc(&c.orbit_stats, 3));
+  CHK(dalloc(&c.orbit_stats, 4));
+  CHK(dalloc(&c.log_next, 1));
 
   // geometry
   ctx->distance.assign(c.Nstack, 0.0);
@@ -668,6 +769,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.o_steps); dfree(c.o_bbox);
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
   dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv); dfree(c.psteps);
+  dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_next);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -748,6 +850,12 @@ int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *
   CHK(dalloc(&P.mass, n)); CHK(dalloc(&P.tint, n)); CHK(dalloc(&P.vesc, n));
   CHK(dalloc(&P.vr2, n)); CHK(dalloc(&P.vt2, n)); CHK(dalloc(&P.vp2, n)); CHK(dalloc(&P.vq2, n));
   CHK(dalloc(&P.type, n)); CHK(dalloc(&P.orbits, n));
+  c.max_mass = 0;
+  c.max_vesc = 0;
+  for(int i = 0; i < n; i++) {
+    c.max_mass = std::max(c.max_mass, mass[i]);
+    c.max_vesc = std::max(c.max_vesc, vesc[i]);
+  }
   std::vector<double> tmp(n);
   double *dst3[3] = {P.x, P.y, P.z};
   for(int k = 0; k < 3; k++) {
@@ -1083,7 +1191,12 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx) {
   HIPCHK(hipGetLastError());
   galic::OrbitArgs32 A = orbit_args32(c, n);
   A.locq = c.locA_q;
+  const long long steps0 = c.acc_steps;
   CHK(launch_orbit32(c, A));
+  // the response logs of later reps: fixed point bounds, and a first estimate of
+  // their length (about one flushed 2x2 run per 4 to 6 steps)
+  log_setup(c);
+  c.log_est_blocks = 0.25 * (double)(c.acc_steps - steps0) / n / galic::LOG_B + 1.0;
   hipLaunchKernelGGL(galic::k_store_orbits, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.o_norbit,
                      c.P.orbits);
   hipLaunchKernelGGL(galic::k_store_steps, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n,
@@ -1108,9 +1221,7 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
   if(!ctx->fp32_initial_done) return set_err(GALIC_ERR_STATE, "galic_fp32_calc_all_response_fields must run first");
   HIPCHK(hipSetDevice(c.device));
   const int n = std::min(nsample, c.P.n);
-  const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
   CHK(alloc_orbit_scratch(c, n));
-  CHK(alloc_slots(c, n, per));
   // a strided sample of the model's own orbits at their current velocities,
   // longest expected first as in a rep
   std::vector<int> idx(n);
@@ -1127,7 +1238,7 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
   // grid sizes: half and all of the workgroups a CU holds at once, and twice that
   // (the results are bit-identical; only the schedule differs)
   int occ = 4;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(galic::k_orbit32), 256, 0));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(galic::k_orbit32_log), 256, 0));
   occ = std::max(1, std::min(occ, 16));
   const int variants[1] = {0}, bpcs[3] = {std::max(1, occ / 2), occ, 2 * occ};
   double best = 1e300;
@@ -1139,17 +1250,21 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
     for(int b : bpcs) {
       c.fp32_variant = v;
       c.fp32_bpc = b;
-      galic::OrbitArgs32 A = orbit_args32(c, n);
-      A.slots = (float *)c.slots;
       long long s_before = c.acc_steps;
-      CHK(launch_orbit32(c, A));
+      int lr = kLogArenaFull;
+      for(int attempt = 0; attempt < 6 && lr == kLogArenaFull; attempt++) {
+        const size_t bb = log_block_bytes();
+        CHK(ensure_arena(c, (size_t)(log_blocks_planned(c, n) * bb), (size_t)(log_blocks_planned(c, n) * bb)));
+        lr = launch_orbit32_log(c, orbit_args32(c, n));
+      }
+      if(lr == kLogArenaFull) return set_err(GALIC_ERR_NOMEM, "no room for the tuning sample's response logs");
+      CHK(lr);
       double t = c.last_orbit_ms / std::max<long long>(1, c.acc_steps - s_before);
       if(t < best) {
         best = t;
         bv = v;
         bb = b;
       }
-      HIPCHK(hipMemsetAsync(c.slots, 0, per * n, c.stream));
     }
   // tuning launches are not part of the caller's measurements
   c.acc_steps = steps0;
@@ -1211,9 +1326,20 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       HIPCHK(hipMemcpy(c.t_vk, tv.data(), sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
     }
   }
-  const size_t per = sizeof(float) * galic::kFields * c.Ngrid;
-  const int cap = (int)std::max<size_t>(
-      1, std::min<size_t>((size_t)std::max(ntrial, 1), slot_budget(c) / ((K + 1) * per)));
+  // the log arena: as much of the free memory as the call could use, kept across calls
+  {
+    const size_t bb = log_block_bytes();
+    const size_t all = (size_t)(log_blocks_planned(c, std::max(ntrial, 1) * (K + 1)) * bb);
+    const size_t one = (size_t)(log_blocks_planned(c, std::min(std::max(ntrial, 1), 1024) * (K + 1)) * bb);
+    CHK(ensure_arena(c, std::min(all, slot_budget(c)), one));
+  }
+  // trials whose orbits' logs fit in the arena at the current estimate
+  auto group_cap = [&]() {
+    const double blocks = (double)(c.slot_bytes / log_block_bytes());
+    const double per_trial = (K + 1) * (c.log_est_blocks + 1.0) * 1.25;
+    return (int)std::max(1.0, std::min((double)std::max(ntrial, 1), (blocks - 64) / per_trial));
+  };
+  int cap = group_cap();
   const int nrep = (ntrial + rep_size - 1) / rep_size;
   // rep r covers trials [r*rep_size, min((r+1)*rep_size, ntrial)); a group is a run of
   // whole reps whose particles are distinct and whose orbits fit the slots (or a
@@ -1221,10 +1347,17 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   std::vector<char> seen(c.P.n, 0);
   std::vector<int> touched;
   const size_t nbw = (size_t)GALIC_NTYPES * galic::kFields * c.Nstack;
-  const size_t ncoarse = (size_t)(c.Nstack - c.Ngrid);
-  // all five fields in one pyramid pass when their coarse levels fit in 64 KiB of LDS
-  const int nf = (galic::kFields * ncoarse * sizeof(double) + 1024 <= 65536) ? galic::kFields : 1;
-  const size_t shmem = sizeof(double) * (nf * ncoarse + 8);
+  // all five fields in one pass when their fine and coarse grids fit in the LDS of one workgroup
+  size_t lds_max = 65536;
+  {
+    int v = 0;
+    if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess && v > 0)
+      lds_max = (size_t)v;
+  }
+  auto eval_lds = [&](int nf) {
+    return 16 * sizeof(double) + sizeof(long long) * (size_t)nf * c.Nstack + sizeof(int) * (size_t)(K + 1) * c.log_M;
+  };
+  const int nf = (eval_lds(galic::kFields) + 1024 <= lds_max) ? galic::kFields : 1;
   bool packed = false;
   float eval_ms = 0;
   int s0 = 0;
@@ -1273,15 +1406,23 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     hipLaunchKernelGGL(galic::k_invert_order, dim3(nblocks(ng, 256)), dim3(256), 0, c.stream, ng,
                        (const int *)c.t_ord2, c.t_inv);
     CHK(alloc_orbit_scratch(c, ng * (K + 1)));
-    CHK(alloc_slots(c, ng * (K + 1), per));
     hipLaunchKernelGGL(galic::k_prepare_ktrials, dim3(nblocks((size_t)ng * (K + 1), 256)), dim3(256), 0, c.stream, ng,
                        K, (const int *)c.t_ord2, (const int *)(c.t_idx + s0), (const double *)(vtx + (size_t)s0 * K),
                        (const double *)(vty + (size_t)s0 * K), (const double *)(vtz + (size_t)s0 * K), c.P,
                        orbit_in(c));
     HIPCHK(hipGetLastError());
-    galic::OrbitArgs32 A = orbit_args32(c, ng * (K + 1));
-    A.slots = (float *)c.slots;
-    CHK(launch_orbit32(c, A));
+    {
+      const int lr = launch_orbit32_log(c, orbit_args32(c, ng * (K + 1)));
+      if(lr == kLogArenaFull) {
+        // the logs were longer than planned: plan this group again, smaller
+        if(ng <= 1) return set_err(GALIC_ERR_NOMEM, "the response logs of one trial do not fit in device memory");
+        cap = group_cap();
+        continue;
+      }
+      CHK(lr);
+    }
+    cap = group_cap();
+    const size_t shmem = eval_lds(nf);
     // evaluate the reps of the group in turn, committing each completed rep
     for(int r = r0; r < nrep && r * rep_size < s1; r++) {
       const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
@@ -1306,9 +1447,16 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.merit_bw = c.merit_bw;
       E.fit = c.fit;
       E.cutB_of = c.cutB_of;
-      E.slots = (const float *)c.slots;
-      E.slot_stride = A.slot_stride;
-      E.slot_base = 0;
+      E.log_blocks = c.log_blocks;
+      E.log_n = c.log_n;
+      E.log_M = c.log_M;
+      E.log_q = c.log_q;
+      E.log_v = c.log_v;
+      for(int f = 0; f < galic::kFields; f++) {
+        E.log_scale[f] = c.log_scale[f];
+        E.log_unit[f] = c.log_unit[f];
+        E.log_to_loc[f] = c.log_to_loc[f];
+      }
       E.bbox = c.o_bbox;
       E.norbit = c.o_norbit;
       E.osteps = c.o_steps;
@@ -1329,15 +1477,16 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.iscale = c.fp32_iscale;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
       if(nf == galic::kFields)
-        hipLaunchKernelGGL(galic::k_eval_apply32<galic::kFields>, dim3(b - a), dim3(256), shmem, c.stream, c.M, E);
+        hipLaunchKernelGGL(galic::k_eval_log32<galic::kFields>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       else
-        hipLaunchKernelGGL(galic::k_eval_apply32<1>, dim3(b - a), dim3(256), shmem, c.stream, c.M, E);
+        hipLaunchKernelGGL(galic::k_eval_log32<1>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c.ev1, c.stream));
       HIPCHK(hipEventSynchronize(c.ev1));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
       eval_ms += ms;
+      c.acc_eval_ms += ms;
       if(b == rb) {
         CHK(commit_q(c));
         CHK(global_fit(ctx));
@@ -1373,16 +1522,19 @@ int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int K, int re
   return fp32_run(ctx, ntrial, idx, K, rep_size, vtry, sigma_frac, seed, accept);
 }
 
-int galic_timings(galic_ctx *ctx, double *out7) {
-  if(!ctx || !out7) return set_err(GALIC_ERR_ARG, "null argument");
+int galic_timings(galic_ctx *ctx, double *out10) {
+  if(!ctx || !out10) return set_err(GALIC_ERR_ARG, "null argument");
   galic::Ctx &c = ctx->c;
-  out7[0] = c.last_orbit_ms;
-  out7[1] = c.last_eval_ms;
-  out7[2] = c.last_fit_ms;
-  out7[3] = c.acc_orbit_ms;
-  out7[4] = (double)c.acc_orbit_launches;
-  out7[5] = (double)c.acc_orbits;
-  out7[6] = (double)c.acc_steps;
+  out10[0] = c.last_orbit_ms;
+  out10[1] = c.last_eval_ms;
+  out10[2] = c.last_fit_ms;
+  out10[3] = c.acc_orbit_ms;
+  out10[4] = (double)c.acc_orbit_launches;
+  out10[5] = (double)c.acc_orbits;
+  out10[6] = (double)c.acc_steps;
+  out10[7] = c.acc_eval_ms;
+  out10[8] = c.acc_fit_ms;
+  out10[9] = (double)c.acc_log_relaunches;
   return GALIC_OK;
 }
 
@@ -1393,6 +1545,9 @@ int galic_reset_timings(galic_ctx *ctx) {
   c.acc_orbit_launches = 0;
   c.acc_orbits = 0;
   c.acc_steps = 0;
+  c.acc_eval_ms = 0;
+  c.acc_fit_ms = 0;
+  c.acc_log_relaunches = 0;
   return GALIC_OK;
 }
 

@@ -119,7 +119,8 @@ struct Ctx {
   long long *o_steps = nullptr;
   int4 *o_bbox = nullptr;
   unsigned int *counter = nullptr;
-  unsigned long long *orbit_stats = nullptr;  // [0] steps of the launch, [1] first failed orbit + 1, [2] its status
+  unsigned long long *orbit_stats = nullptr;  // [0] steps of the launch, [1] first failed orbit + 1, [2] its status,
+                                              // [3] orbits whose response log overflowed
 
   // per-trial scratch
   int max_trials = 0;
@@ -136,11 +137,26 @@ struct Ctx {
   size_t sort_tmp_bytes = 0;
   int *psteps = nullptr;  // per particle: steps of its current orbit (FP32 path's length prediction)
   double2 *merit_bw = nullptr;  // FP32 merit: (response - target, node weight) [type][5][Nstack]
+  // response logs of the batched FP32 path (arena carved from `slots`)
+  int *log_blocks = nullptr;     // [orbit][log_M]
+  size_t log_blocks_len = 0;
+  int *log_n = nullptr;          // [orbit]
+  int log_n_len = 0;
+  int log_M = 64;                // block-table entries per orbit (doubled when an orbit needs more)
+  unsigned int *log_next = nullptr;
+  int *log_q = nullptr;          // the last launch's arena: quads, then values
+  float4 *log_v = nullptr;
+  double log_est_blocks = 12;    // running estimate of arena blocks per orbit
+  float log_scale[5] = {0, 0, 0, 0, 0};  // record value -> exact int64 sum units (a power of two)
+  double log_unit[5] = {0, 0, 0, 0, 0};   // its inverse
+  double log_to_loc[5] = {0, 0, 0, 0, 0}; // int64 sum units -> Loc fixed point
+  double max_mass = 0, max_vesc = 0;  // of the particles, bound the log's fixed point
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_orbit_ms = 0, last_eval_ms = 0, last_fit_ms = 0;
-  double acc_orbit_ms = 0;
+  double acc_orbit_ms = 0, acc_eval_ms = 0, acc_fit_ms = 0;
+  long long acc_log_relaunches = 0;
   long long acc_orbit_launches = 0;
   long long acc_orbits = 0, acc_steps = 0;
   int fp32_variant = 0;   // orbit kernel variant (one so far)

@@ -38,7 +38,19 @@ struct OrbitArgs32 {
   int4 *bbox;
   unsigned long long *stats;
   float *fin;  // final x, y, z, vx, vy, vz per orbit, or null
+  // response logs (k_orbit32_log): every flushed 2x2 run is appended as a record
+  // (quad index, 4 cells x 5 fields of fac * weight in FP32) to the orbit's blocks
+  int *log_blocks;       // [orbit][log_M] arena block of each log block
+  int *log_n;            // [orbit] records written (-1: the log overflowed)
+  int log_M;             // block-table entries per orbit
+  int *log_q;            // arena: [block][LOG_B] quad index
+  float4 *log_v;         // arena: [block][5][LOG_B] values
+  unsigned int *log_next;  // next free arena block
+  unsigned int log_cap;    // arena blocks
 };
+
+// records per log block
+constexpr int LOG_B = 64;
 
 struct EvalArgs32 {
   Particles P;
@@ -50,9 +62,13 @@ struct EvalArgs32 {
   const double *stacks, *fit;
   const double2 *merit_bw;  // (response - target, weight) per node [type][5][Nstack]
   const int *cutB_of;
-  const float *slots;
-  size_t slot_stride;
-  size_t slot_base;
+  const int *log_blocks, *log_n;  // the chunk's response logs (OrbitArgs32)
+  int log_M;
+  const int *log_q;
+  const float4 *log_v;
+  float log_scale[kFields];   // record value -> exact int64 sum units (a power of two)
+  double log_unit[kFields];   // int64 sum units -> response units (its inverse)
+  double log_to_loc[kFields]; // int64 sum units -> Loc fixed point
   const int4 *bbox;
   const int *norbit;
   const long long *osteps;  // steps of each orbit of the chunk
@@ -68,6 +84,7 @@ struct EvalArgs32 {
 };
 
 __global__ void k_orbit32(FastModel F, OrbitArgs32 A);
+__global__ void k_orbit32_log(FastModel F, OrbitArgs32 A);
 __global__ void k_draw_trials(DevModel M, int ntrial, int K, int nfresh, const int *tidx, Particles P,
                               const double *disp, double sigma_frac, double maxvel, uint32_t seed, uint32_t stream,
                               const int *tovs_of_type, double *vtx, double *vty, double *vtz);
@@ -81,7 +98,7 @@ int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigne
                          int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
 __global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
 template <int NF>
-__global__ void k_eval_apply32(DevModel M, EvalArgs32 A);
+__global__ void k_eval_log32(DevModel M, EvalArgs32 A);
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);
 __global__ void k_inst_init_q(DevModel M, Particles P, long long *locIq, long long *locMq, int ngrid, double iscale,

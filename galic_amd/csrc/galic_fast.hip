@@ -196,16 +196,47 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
   }
 }
 
+// Append-only response log of one orbit: the lane never reads its log back,
+// so a flush is a handful of stores and the wave does not wait on memory.
+struct Log32 {
+  int n;     // records written
+  int blk;   // current arena block
+  bool full; // out of block-table entries or arena blocks
+};
+
+__device__ __forceinline__ void log32_flush(const Run32 &r, float fac, Log32 &lg, int o, const OrbitArgs32 &A) {
+  if(r.quad < 0) return;
+  const int j = lg.n & (LOG_B - 1);
+  if(!lg.full && j == 0) {
+    const int b = lg.n / LOG_B;
+    unsigned int nb = b < A.log_M ? atomicAdd(A.log_next, 1u) : 0xffffffffu;
+    if(nb >= A.log_cap) {
+      lg.full = true;
+    } else {
+      lg.blk = (int)nb;
+      A.log_blocks[(size_t)o * A.log_M + b] = (int)nb;
+    }
+  }
+  if(!lg.full) {
+    A.log_q[(size_t)lg.blk * LOG_B + j] = r.quad;
+    float4 *v = A.log_v + (size_t)lg.blk * kFields * LOG_B + j;
+#pragma unroll
+    for(int f = 0; f < kFields; f++)
+      v[f * LOG_B] = make_float4(fac * r.w[f][0], fac * r.w[f][1], fac * r.w[f][2], fac * r.w[f][3]);
+  }
+  lg.n++;
+}
+
 // Trapezoid half-step weights of one end of a step (orbit_response.c:283-319)
 // accumulated in registers while the orbit stays in one 2x2 block.
-__device__ __forceinline__ void bin32(Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
-                                      const FastModel &F, int iR, int iz, float fR, float fz, float dt, float vr2,
-                                      float vt2, float vq2, float vp2, int4 &bb) {
+template <typename Flush>
+__device__ __forceinline__ void bin32(Run32 &r, int nb, Flush flush, int iR, int iz, float fR, float fz, float dt,
+                                      float vr2, float vt2, float vq2, float vp2, int4 &bb) {
   float h = 0.5f * dt;
   float m[4] = {h * (1 - fR) * (1 - fz), h * fR * (1 - fz), h * (1 - fR) * fz, h * fR * fz};
   int quad = iz * nb + iR;
   if(quad != r.quad) {
-    run32_flush(r, nb, ngrid, fac, slot, locq, F);
+    flush(r);
     run32_reset(r, quad);
     bb.x = min(bb.x, iR);
     bb.y = max(bb.y, iR + 1);
@@ -260,6 +291,7 @@ __device__ __forceinline__ bool take_orbit32(const OrbitArgs32 &A, int nb, Lane3
     A.steps[o] = 0;
     A.bbox[o] = make_int4(nb, -1, nb, -1);
     A.status[o] = GALIC_ERR_ARG;
+    if(A.log_n) A.log_n[o] = 0;
     orbit_stats_add(A.stats, 0, GALIC_ERR_ARG, o);
   }
 }
@@ -269,9 +301,20 @@ __device__ __forceinline__ bool take_orbit32(const OrbitArgs32 &A, int nb, Lane3
 // busy however uneven the orbit lengths are.  A freshly taken orbit spends one
 // pass of the loop on the lookups at its start point (the pre-loop lookups of
 // orbit_response.c:199-210); every later pass is one step of :211-458.
+template <bool LOG>
 __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs32 &A) {
   const int nb = F.dg_nbin, ngrid = nb * nb;
   Lane32 L;
+  Log32 lg;
+  lg.n = 0;
+  lg.blk = 0;
+  lg.full = false;
+  auto flush = [&](const Run32 &r) {
+    if(LOG)
+      log32_flush(r, L.fac, lg, L.o, A);
+    else
+      run32_flush(r, nb, ngrid, L.fac, L.slot, L.locq, F);
+  };
   float x, y, z, vx, vy, vz;
   if(!take_orbit32(A, nb, L, x, y, z, vx, vy, vz)) return;
   L.tovs = F.tovs[L.type];
@@ -298,7 +341,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       if(!(dt >= 0.f && dt < 3.0e38f)) {
         status = GALIC_ERR_ARG;
       } else {
-        bin32(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+        bin32(run, nb, flush, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
         hdt = 0.5f * dt;
         vx += hdt * ax;
         vy += hdt * ay;
@@ -327,7 +370,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       continue;
     }
     if(status == 0) {
-      bin32(run, nb, ngrid, L.fac, L.slot, L.locq, F, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+      bin32(run, nb, flush, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
       L.tall += (double)dt;
       float radsign = vx * x + vy * y + vz * z;
       if(radsign > 0 && radsign_prev < 0) norbit++;
@@ -336,8 +379,14 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       if(steps > F.max_steps) status = GALIC_ERR_STEPCAP;
     }
     if(flag || status) {
-      run32_flush(run, nb, ngrid, L.fac, L.slot, L.locq, F);
+      flush(run);
       const int o = L.o;
+      if(LOG) {
+        A.log_n[o] = lg.full ? -1 : lg.n;
+        if(lg.full) atomicAdd(&A.stats[3], 1ull);
+        lg.n = 0;
+        lg.full = false;
+      }
       A.tall[o] = L.tall;
       A.norbit[o] = norbit;
       A.steps[o] = steps;
@@ -367,7 +416,8 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   }
 }
 
-__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body(F, A); }
+__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body<false>(F, A); }
+__global__ void __launch_bounds__(256) k_orbit32_log(FastModel F, OrbitArgs32 A) { orbit32_body<true>(F, A); }
 
 // ------------------------------------------------------------ trial draws --
 
@@ -639,13 +689,38 @@ __device__ __forceinline__ double merit_term(const double2 *bw, int node, double
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
 
-// Per-thread partial of sum_f coef[f] * sum_nodes w (|b + D| - |b|) for fields
-// f0..f0+NF-1, D = new - old over the box r0..r1, z0..z1 of fine cells and its
-// ancestors (all threads call it).  Slots are cell-major, 5 floats per cell.
-// cs: LDS of NF * ncoarse doubles.
+// Adds sign * (record values of fields f0..f0+NF-1) of one orbit's log into the
+// LDS grid G [NF][ngrid].  Each FP32 value times a power of two is rounded to an
+// int64 and added with an integer atomic, so the sums are exact and do not
+// depend on the order of the records.  tbl: the orbit's arena blocks.
 template <int NF>
-__device__ double merit_change32(const double2 *bw, int Nstack, const double (&coef)[kFields], int f0,
-                                 const float *fo, const float *fn, int r0, int r1, int z0, int z1, int L, double *cs) {
+__device__ __forceinline__ void log_scatter(const EvalArgs32 &A, const int *tbl, int n, bool neg, int f0, int nb,
+                                           int ngrid, unsigned long long *G) {
+  for(int r = threadIdx.x; r < n; r += blockDim.x) {
+    const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
+    const int quad = A.log_q[(size_t)blk * LOG_B + jj];
+    const float4 *v = A.log_v + (size_t)blk * kFields * LOG_B + jj;
+#pragma unroll
+    for(int f = 0; f < NF; f++) {
+      const float4 x = v[(f0 + f) * LOG_B];
+      const float sc = neg ? -A.log_scale[f0 + f] : A.log_scale[f0 + f];
+      unsigned long long *g = G + f * ngrid + quad;
+      atomicAdd(g, (unsigned long long)__float2ll_rn(x.x * sc));
+      atomicAdd(g + 1, (unsigned long long)__float2ll_rn(x.y * sc));
+      atomicAdd(g + nb, (unsigned long long)__float2ll_rn(x.z * sc));
+      atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x.w * sc));
+    }
+  }
+}
+
+// Per-thread partial of sum_f coef[f] * sum_nodes w (|b + D| - |b|) for fields
+// f0..f0+NF-1, D = new - old held exactly in G (int64 sum units) over the box
+// r0..r1, z0..z1 of fine cells, and its ancestors built in cs [NF][ncoarse].
+// All threads call it.
+template <int NF>
+__device__ double merit_log32(const double2 *bw, int Nstack, const double (&coef)[kFields],
+                              const double (&unit)[kFields], int f0, const long long *G, long long *cs, int r0,
+                              int r1, int z0, int z1, int L) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int nb = 1 << L, ngrid = nb * nb, ncoarse = Nstack - ngrid;
   const int offL = GALIC_STACKOFFSET(L, 0, 0);
@@ -658,25 +733,25 @@ __device__ double merit_change32(const double2 *bw, int Nstack, const double (&c
   int na = a1 - a0 + 1, nc = c1 - c0 + 1;
   for(int k = tid; k < na * nc; k += nt) {
     const int pz = c0 + k / na, pr = a0 + k % na;
-    double s[NF];
+    long long s[NF];
 #pragma unroll
     for(int f = 0; f < NF; f++) s[f] = 0;
 #pragma unroll
     for(int c = 0; c < 4; c++) {  // child order of smooth_stack
       const int cell = (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1);
-      const float *po = fo + (size_t)cell * kFields + f0, *pn = fn + (size_t)cell * kFields + f0;
 #pragma unroll
       for(int f = 0; f < NF; f++) {
-        const double d = (double)pn[f] - (double)po[f];
+        const long long d = G[f * ngrid + cell];
         s[f] += d;
-        acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, offL + cell, d);
+        if(d) acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, offL + cell, (double)d * unit[f0 + f]);
       }
     }
     const int node = GALIC_STACKOFFSET(L - 1, pz, pr);
 #pragma unroll
     for(int f = 0; f < NF; f++) {
       cs[f * ncoarse + node] = s[f];
-      acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, s[f]);
+      if(s[f])
+        acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, (double)s[f] * unit[f0 + f]);
     }
   }
   __syncthreads();
@@ -687,12 +762,12 @@ __device__ double merit_change32(const double2 *bw, int Nstack, const double (&c
     const int nbx = b1 - b0 + 1, nez = e1 - e0 + 1;
     for(int k = tid; k < nbx * nez; k += nt) {
       const int i = e0 + k / nbx, j = b0 + k % nbx;
-      double s[NF];
+      long long s[NF];
 #pragma unroll
       for(int f = 0; f < NF; f++) s[f] = 0;
 #pragma unroll
       for(int c = 0; c < 4; c++) {
-        const int ci = 2 * i + (c & 1), cj = 2 * j + (c >> 1);  // child order of smooth_stack
+        const int ci = 2 * i + (c & 1), cj = 2 * j + (c >> 1);
         if(ci >= c0 && ci <= c1 && cj >= a0 && cj <= a1) {
           const int cn = GALIC_STACKOFFSET(l + 1, ci, cj);
 #pragma unroll
@@ -703,7 +778,8 @@ __device__ double merit_change32(const double2 *bw, int Nstack, const double (&c
 #pragma unroll
       for(int f = 0; f < NF; f++) {
         cs[f * ncoarse + node] = s[f];
-        acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, s[f]);
+        if(s[f])
+          acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, (double)s[f] * unit[f0 + f]);
       }
     }
     a0 = b0;
@@ -713,6 +789,17 @@ __device__ double merit_change32(const double2 *bw, int Nstack, const double (&c
     __syncthreads();
   }
   return acc;
+}
+
+// Zeroes fields 0..NF-1 of G over the box (the cells any of the logs touched).
+template <int NF>
+__device__ __forceinline__ void clear_box(long long *G, int ngrid, int nb, int r0, int r1, int z0, int z1) {
+  const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
+  for(int q = threadIdx.x; q < nr * nzz; q += blockDim.x) {
+    const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
+#pragma unroll
+    for(int f = 0; f < NF; f++) G[f * ngrid + cell] = 0;
+  }
 }
 
 // Sum over the block of one double; every thread gets the total.
@@ -729,41 +816,62 @@ __device__ __forceinline__ double block_sum1(double v, double *red) {
 
 // One particle per workgroup: the merit change of each of its K trials (the
 // orbit-averaged terms on the fixed density cut, the instantaneous terms on the
-// fixed cut of EGs_MassResponse), the best improving trial taken, its deltas
-// added in fixed point, and the K+1 slots cleared for the next chunk.
+// fixed cut of EGs_MassResponse), the best improving trial taken and its deltas
+// added in fixed point.  D = new - old of a trial is rebuilt exactly from the
+// two orbits' logs in LDS, NF fields per pass.
+// LDS: red [16] double | G [NF][ngrid] int64 | cs [NF][ncoarse] int64 | tbl [K+1][log_M] int
 template <int NF>
-__global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) {
-  extern __shared__ double lds[];
+__global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
+  extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K;
-  const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
-  double *cs = lds;
-  double *red = lds + NF * (Nstack - ngrid);
+  const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L), ncoarse = Nstack - ngrid;
+  double *red = lds_d;
+  long long *G = (long long *)(lds_d + 16);
+  long long *cs = G + NF * ngrid;
+  int *tbl = (int *)(cs + NF * ncoarse);
   __shared__ double s_dF[64];
-  __shared__ int s_best;
+  __shared__ int s_best, s_n[65];
 
-  const int ti = A.trial_base + blockIdx.x;                    // trial (trial arrays, accept, dF)
-  const int j = A.slot_pos ? A.slot_pos[blockIdx.x] : (int)blockIdx.x;  // its orbits j*(K+1).. in the chunk
+  const int ti = A.trial_base + blockIdx.x;
+  const int j = A.slot_pos ? A.slot_pos[blockIdx.x] : (int)blockIdx.x;
   const int p = A.tidx[ti];
   const int t = A.P.type[p];
   const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
   const double2 *bw = A.merit_bw + (size_t)t * kFields * Nstack;
-  const size_t base = (size_t)j * (K + 1) - A.slot_base;
-  const float *so = A.slots + base * A.slot_stride;
-  const int4 bo = A.bbox[(size_t)j * (K + 1)];
+  const size_t o0 = (size_t)j * (K + 1);
+  const int4 bo = A.bbox[o0];
   const double x = A.P.x[p], y = A.P.y[p], z = A.P.z[p], mass = A.P.mass[p];
   const double dold[4] = {A.P.vr2[p], A.P.vt2[p], A.P.vp2[p], A.P.vq2[p]};
   const double MT = A.MType[t];
   // eval_fit_mod's combination (main.c:1215-1216): D/M + (f_rs V_rs + f_ts V_ts + f_qs V_qs + f_ps V_ps)/M/4
   const double coef[kFields] = {1.0 / MT, A.fac[t][0] / MT / 4, A.fac[t][1] / MT / 4, A.fac[t][2] / MT / 4,
                                 A.fac[t][3] / MT / 4};
+  double unit[kFields];
+#pragma unroll
+  for(int f = 0; f < kFields; f++) unit[f] = A.log_unit[f];
+
+  // block tables of the K+1 logs, and a clean grid
+  for(int q = tid; q <= K; q += nt) s_n[q] = A.log_n[o0 + q];
+  __syncthreads();
+  for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
+    const int o = q / A.log_M, b = q % A.log_M;
+    if(b * LOG_B < s_n[o]) tbl[q] = A.log_blocks[(o0 + o) * A.log_M + b];
+  }
+  for(int q = tid; q < NF * ngrid; q += nt) G[q] = 0;
+  __syncthreads();
 
   for(int k = 0; k < K; k++) {
-    const float *sn = so + (size_t)(1 + k) * A.slot_stride;
-    const int4 bn = A.bbox[(size_t)j * (K + 1) + 1 + k];
+    const int4 bn = A.bbox[o0 + 1 + k];
     const int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
     double part = 0;
-    for(int f0 = 0; f0 < kFields; f0 += NF)
-      part += merit_change32<NF>(bw, Nstack, coef, f0, so, sn, r0, r1, z0, z1, L, cs);
+    for(int f0 = 0; f0 < kFields; f0 += NF) {
+      log_scatter<NF>(A, tbl + (1 + k) * A.log_M, s_n[1 + k], false, f0, nb, ngrid, (unsigned long long *)G);
+      log_scatter<NF>(A, tbl, s_n[0], true, f0, nb, ngrid, (unsigned long long *)G);
+      __syncthreads();
+      part += merit_log32<NF>(bw, Nstack, coef, unit, f0, G, cs, r0, r1, z0, z1, L);
+      clear_box<NF>(G, ngrid, nb, r0, r1, z0, z1);
+      __syncthreads();
+    }
     const double orbit_terms = block_sum1(part, red);
     if(tid == 0) {
       const size_t q = (size_t)ti * K + k;
@@ -788,7 +896,7 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
     s_best = best;
     A.accept[ti] = best + 1;
     A.dF[ti] = (best >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
-    const int o0 = j * (K + 1), oc = (best >= 0) ? o0 + 1 + best : o0;
+    const size_t oc = (best >= 0) ? o0 + 1 + best : o0;
     A.P.orbits[p] = A.norbit[oc];
     A.psteps[p] = (int)min(A.osteps[oc], (long long)0x7fffffff);
     if(best >= 0) {
@@ -817,36 +925,33 @@ __global__ void __launch_bounds__(256) k_eval_apply32(DevModel M, EvalArgs32 A) 
   }
   __syncthreads();
   const int best = s_best;
-  long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
   if(best >= 0) {
-    const float *sn = so + (size_t)(1 + best) * A.slot_stride;
-    const int4 bn = A.bbox[(size_t)j * (K + 1) + 1 + best];
-    int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
-    int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
-    for(int q = tid; q < nr * nzz; q += nt) {
-      int cell = (z0 + q / nr) * nb + (r0 + q % nr);
-      for(int f = 0; f < kFields; f++) {
-        size_t o = (size_t)cell * kFields + f;
-        float d0 = sn[o], d1 = so[o];
-        if(d0 != d1) atomic_q(&lad[(size_t)f * ngrid + cell], A.qscale[f] * ((double)d0 - (double)d1));
+    // the orbit-averaged deltas of the taken trial, exact in fixed point
+    long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
+    const int4 bn = A.bbox[o0 + 1 + best];
+    const int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
+    const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
+    for(int f0 = 0; f0 < kFields; f0 += NF) {
+      log_scatter<NF>(A, tbl + (1 + best) * A.log_M, s_n[1 + best], false, f0, nb, ngrid, (unsigned long long *)G);
+      log_scatter<NF>(A, tbl, s_n[0], true, f0, nb, ngrid, (unsigned long long *)G);
+      __syncthreads();
+      for(int q = tid; q < nr * nzz; q += nt) {
+        const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
+#pragma unroll
+        for(int f = 0; f < NF; f++) {
+          const long long d = G[f * ngrid + cell];
+          if(d) atomic_q(&lad[(size_t)(f0 + f) * ngrid + cell], (double)d * A.log_to_loc[f0 + f]);
+        }
       }
-    }
-    __syncthreads();
-  }
-  // clear the K+1 slots inside their boxes
-  for(int k = 0; k <= K; k++) {
-    float *s = (float *)(so + (size_t)k * A.slot_stride);
-    const int4 b = A.bbox[(size_t)j * (K + 1) + k];
-    int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
-    for(int q = tid; q < nr * nzz; q += nt) {
-      int cell = (b.z + q / nr) * nb + (b.x + q % nr);
-      for(int f = 0; f < kFields; f++) s[(size_t)cell * kFields + f] = 0.f;
+      __syncthreads();
+      clear_box<NF>(G, ngrid, nb, r0, r1, z0, z1);
+      __syncthreads();
     }
   }
 }
 
-template __global__ void k_eval_apply32<1>(DevModel M, EvalArgs32 A);
-template __global__ void k_eval_apply32<kFields>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<1>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<kFields>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 

@@ -147,9 +147,10 @@ class GalicGPU:
         return acc, pot
 
     def timings(self):
-        out = np.zeros(7)
+        out = np.zeros(10)
         check(self.lib.galic_timings(self.h, dptr(out)))
-        keys = ["last_orbit_ms", "last_eval_ms", "last_fit_ms", "orbit_ms", "orbit_launches", "orbits", "steps"]
+        keys = ["last_orbit_ms", "last_eval_ms", "last_fit_ms", "orbit_ms", "orbit_launches", "orbits", "steps",
+                "eval_ms", "fit_ms", "log_relaunches"]
         return dict(zip(keys, out.tolist()))
 
     def reset_timings(self):

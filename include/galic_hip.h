@@ -183,8 +183,10 @@ int galic_tree_last_timings(double *out3);
 /* Device time of the last launches, measured with HIP events on the context's
  * stream: out[0] orbit kernel ms (last launch), out[1] merit kernel ms,
  * out[2] fit ms, out[3] accumulated orbit-kernel ms, out[4] orbit-kernel
- * launches, out[5] orbits integrated, out[6] leapfrog steps integrated. */
-int galic_timings(galic_ctx *ctx, double *out7);
+ * launches, out[5] orbits integrated, out[6] leapfrog steps integrated,
+ * out[7] accumulated merit-kernel ms, out[8] accumulated fit ms, out[9] orbit
+ * launches repeated to give the response logs more room. */
+int galic_timings(galic_ctx *ctx, double *out10);
 /* A plain device copy of nbytes (reps times): the known-traffic kernel that
  * calibrates the HBM byte counters; ms_per_copy receives the mean duration. */
 int galic_copy_probe(int device, size_t nbytes, int reps, double *ms_per_copy);
