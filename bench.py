@@ -36,9 +36,9 @@ from galic_amd import _abi, model  # noqa: E402
 from galic_amd.runtime import GalicGPU, direct_forces, tree_fill, tree_forces, tree_last_timings  # noqa: E402
 
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
-# HBM bytes per leapfrog step of k_orbit32 in the iteration's launches, from the FETCH_SIZE and
-# WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_halo-disk-1e6.txt)
-HBM_BYTES_PER_STEP = 67.1
+# HBM bytes per leapfrog step of k_orbit32_log in the iteration's launches, from the FETCH_SIZE and
+# WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_halo-disk-1e6.txt)
+HBM_BYTES_PER_STEP = 42.8
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 rate
 PEAK_HBM = 8000.0     # GB/s
 
@@ -310,11 +310,12 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32,
                      "traffic": HBM_BYTES_PER_STEP * steps_total / max(1, tm["orbit_launches"]),
-                     "kernel": "k_orbit32",
+                     "kernel": "k_orbit32_log",
                      "note": f"roof: the FP32 vector ALU (no step is a contraction, MFMA unused); achieved = {F_STEP} "
                              "algorithmic flop/step x leapfrog steps / HIP-event time of the kernel; traffic = HBM "
-                             f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE): "
-                             "the kernel is held back by the read-modify-write of its orbit slots, not by the ALU"},
+                             f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE, "
+                             "almost all of it response-log writes); waves spend most cycles waiting on memory "
+                             "(SQ_WAIT_ANY), not on the ALU"},
         "steps_per_s": steps_total / wall,
         "ms_per_step_breakdown": breakdown,
         "accept_frac": accepted / (args.steps * w.n),
