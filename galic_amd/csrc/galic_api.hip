@@ -1354,10 +1354,11 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess && v > 0)
       lds_max = (size_t)v;
   }
-  auto eval_lds = [&](int nf) {
-    return 16 * sizeof(double) + sizeof(long long) * (size_t)nf * c.Nstack + sizeof(int) * (size_t)(K + 1) * c.log_M;
+  // the merit kernel's LDS: two int64 grids, block sums and the K+1 block tables
+  auto eval_lds = [&]() {
+    return (16 + 64) * sizeof(double) + 2 * sizeof(long long) * (size_t)c.Ngrid + sizeof(int) * (size_t)(K + 1) * c.log_M;
   };
-  const int nf = (eval_lds(galic::kFields) + 1024 <= lds_max) ? galic::kFields : 1;
+  if(c.L > 7) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to level 7");
   bool packed = false;
   float eval_ms = 0;
   int s0 = 0;
@@ -1422,7 +1423,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       CHK(lr);
     }
     cap = group_cap();
-    const size_t shmem = eval_lds(nf);
+    const size_t shmem = eval_lds();
+    if(shmem + 1024 > lds_max)
+      return set_err(GALIC_ERR_ARG, "the merit kernel needs %zu bytes of LDS, the device gives %zu", shmem, lds_max);
     // evaluate the reps of the group in turn, committing each completed rep
     for(int r = r0; r < nrep && r * rep_size < s1; r++) {
       const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
@@ -1476,10 +1479,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
       E.iscale = c.fp32_iscale;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
-      if(nf == galic::kFields)
-        hipLaunchKernelGGL(galic::k_eval_log32<galic::kFields>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
-      else
-        hipLaunchKernelGGL(galic::k_eval_log32<1>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+      hipLaunchKernelGGL(galic::k_eval_log32, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c.ev1, c.stream));
       HIPCHK(hipEventSynchronize(c.ev1));

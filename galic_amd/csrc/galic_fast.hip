@@ -689,117 +689,43 @@ __device__ __forceinline__ double merit_term(const double2 *bw, int node, double
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
 
-// Adds sign * (record values of fields f0..f0+NF-1) of one orbit's log into the
-// LDS grid G [NF][ngrid].  Each FP32 value times a power of two is rounded to an
-// int64 and added with an integer atomic, so the sums are exact and do not
-// depend on the order of the records.  tbl: the orbit's arena blocks.
-template <int NF>
-__device__ __forceinline__ void log_scatter(const EvalArgs32 &A, const int *tbl, int n, bool neg, int f0, int nb,
-                                           int ngrid, unsigned long long *G) {
-  for(int r = threadIdx.x; r < n; r += blockDim.x) {
-    const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
-    const int quad = A.log_q[(size_t)blk * LOG_B + jj];
-    const float4 *v = A.log_v + (size_t)blk * kFields * LOG_B + jj;
+// Adds sign * (field f of the record values) of one orbit's log into the LDS
+// grid G.  Each FP32 value times a power of two is rounded to an int64 and
+// added with an integer atomic, so the sums are exact and independent of the
+// order of the records.  A thread loads up to four records before adding them.
+// tbl: the orbit's arena blocks.
+__device__ __forceinline__ void log_scatter(const EvalArgs32 &A, const int *tbl, int n, bool neg, int f, int nb,
+                                           unsigned long long *G) {
+  const float sc = neg ? -A.log_scale[f] : A.log_scale[f];
+  for(int r0 = threadIdx.x; r0 < n; r0 += 4 * blockDim.x) {
+    int quad[4];
+    float4 x[4];
 #pragma unroll
-    for(int f = 0; f < NF; f++) {
-      const float4 x = v[(f0 + f) * LOG_B];
-      const float sc = neg ? -A.log_scale[f0 + f] : A.log_scale[f0 + f];
-      unsigned long long *g = G + f * ngrid + quad;
-      atomicAdd(g, (unsigned long long)__float2ll_rn(x.x * sc));
-      atomicAdd(g + 1, (unsigned long long)__float2ll_rn(x.y * sc));
-      atomicAdd(g + nb, (unsigned long long)__float2ll_rn(x.z * sc));
-      atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x.w * sc));
+    for(int u = 0; u < 4; u++) {
+      const int r = r0 + u * blockDim.x;
+      quad[u] = -1;
+      if(r < n) {
+        const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
+        quad[u] = A.log_q[(size_t)blk * LOG_B + jj];
+        x[u] = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < 4; u++) {
+      if(quad[u] < 0) continue;
+      unsigned long long *g = G + quad[u];
+      atomicAdd(g, (unsigned long long)__float2ll_rn(x[u].x * sc));
+      atomicAdd(g + 1, (unsigned long long)__float2ll_rn(x[u].y * sc));
+      atomicAdd(g + nb, (unsigned long long)__float2ll_rn(x[u].z * sc));
+      atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x[u].w * sc));
     }
   }
 }
 
-// Per-thread partial of sum_f coef[f] * sum_nodes w (|b + D| - |b|) for fields
-// f0..f0+NF-1, D = new - old held exactly in G (int64 sum units) over the box
-// r0..r1, z0..z1 of fine cells, and its ancestors built in cs [NF][ncoarse].
-// All threads call it.
-template <int NF>
-__device__ double merit_log32(const double2 *bw, int Nstack, const double (&coef)[kFields],
-                              const double (&unit)[kFields], int f0, const long long *G, long long *cs, int r0,
-                              int r1, int z0, int z1, int L) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int nb = 1 << L, ngrid = nb * nb, ncoarse = Nstack - ngrid;
-  const int offL = GALIC_STACKOFFSET(L, 0, 0);
-  double acc = 0;
-  if(r1 < r0 || z1 < z0) {
-    __syncthreads();
-    return 0.0;
-  }
-  int a0 = r0 >> 1, a1 = r1 >> 1, c0 = z0 >> 1, c1 = z1 >> 1;
-  int na = a1 - a0 + 1, nc = c1 - c0 + 1;
-  for(int k = tid; k < na * nc; k += nt) {
-    const int pz = c0 + k / na, pr = a0 + k % na;
-    long long s[NF];
-#pragma unroll
-    for(int f = 0; f < NF; f++) s[f] = 0;
-#pragma unroll
-    for(int c = 0; c < 4; c++) {  // child order of smooth_stack
-      const int cell = (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1);
-#pragma unroll
-      for(int f = 0; f < NF; f++) {
-        const long long d = G[f * ngrid + cell];
-        s[f] += d;
-        if(d) acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, offL + cell, (double)d * unit[f0 + f]);
-      }
-    }
-    const int node = GALIC_STACKOFFSET(L - 1, pz, pr);
-#pragma unroll
-    for(int f = 0; f < NF; f++) {
-      cs[f * ncoarse + node] = s[f];
-      if(s[f])
-        acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, (double)s[f] * unit[f0 + f]);
-    }
-  }
-  __syncthreads();
-  // coarser levels over the box's ancestors; children outside the previous
-  // region hold stale LDS and are read as 0
-  for(int l = L - 2; l >= 0; l--) {
-    const int b0 = a0 >> 1, b1 = a1 >> 1, e0 = c0 >> 1, e1 = c1 >> 1;
-    const int nbx = b1 - b0 + 1, nez = e1 - e0 + 1;
-    for(int k = tid; k < nbx * nez; k += nt) {
-      const int i = e0 + k / nbx, j = b0 + k % nbx;
-      long long s[NF];
-#pragma unroll
-      for(int f = 0; f < NF; f++) s[f] = 0;
-#pragma unroll
-      for(int c = 0; c < 4; c++) {
-        const int ci = 2 * i + (c & 1), cj = 2 * j + (c >> 1);
-        if(ci >= c0 && ci <= c1 && cj >= a0 && cj <= a1) {
-          const int cn = GALIC_STACKOFFSET(l + 1, ci, cj);
-#pragma unroll
-          for(int f = 0; f < NF; f++) s[f] += cs[f * ncoarse + cn];
-        }
-      }
-      const int node = GALIC_STACKOFFSET(l, i, j);
-#pragma unroll
-      for(int f = 0; f < NF; f++) {
-        cs[f * ncoarse + node] = s[f];
-        if(s[f])
-          acc += coef[f0 + f] * merit_term(bw + (size_t)(f0 + f) * Nstack, node, (double)s[f] * unit[f0 + f]);
-      }
-    }
-    a0 = b0;
-    a1 = b1;
-    c0 = e0;
-    c1 = e1;
-    __syncthreads();
-  }
-  return acc;
-}
-
-// Zeroes fields 0..NF-1 of G over the box (the cells any of the logs touched).
-template <int NF>
-__device__ __forceinline__ void clear_box(long long *G, int ngrid, int nb, int r0, int r1, int z0, int z1) {
-  const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
-  for(int q = threadIdx.x; q < nr * nzz; q += blockDim.x) {
-    const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
-#pragma unroll
-    for(int f = 0; f < NF; f++) G[f * ngrid + cell] = 0;
-  }
+// Zeroes G over the box of fine cells r0..r1, z0..z1.
+__device__ __forceinline__ void clear_box(long long *G, int nb, int4 b) {
+  const int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
+  for(int q = threadIdx.x; q < nr * nzz; q += blockDim.x) G[(b.z + q / nr) * nb + (b.x + q % nr)] = 0;
 }
 
 // Sum over the block of one double; every thread gets the total.
@@ -814,21 +740,100 @@ __device__ __forceinline__ double block_sum1(double v, double *red) {
   return s;
 }
 
+__device__ __forceinline__ int morton_x(int lane) { return (lane & 1) | ((lane >> 1) & 2) | ((lane >> 2) & 4); }
+__device__ __forceinline__ int morton_y(int lane) { return ((lane >> 1) & 1) | ((lane >> 2) & 2) | ((lane >> 3) & 4); }
+
+__device__ __forceinline__ double term_at(const double2 &v, double d) {
+  return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
+}
+
+// Per-thread partial of coef * sum over nodes of w (|b + D| - |b|) for one
+// field, D = G - O (O null: D = G) in exact int64 units times `unit`.
+// The grid is cut into blocks of 16 x 16 fine cells (the whole grid below L = 4);
+// a wave owns a block: each lane a 2x2 cell node, lanes in Morton order so that
+// groups of 4, 16 and 64 lanes are the node's parent, grandparent and the block
+// node, summed with shuffles.  The block sums then give the levels above.
+// box: the fine cells D can be non-zero in.  topb: LDS of (nb/16)^2 int64.
+__device__ double merit_waves(const double2 *bw, int L, double coef, double unit, const long long *G,
+                              const long long *O, int4 box, long long *topb) {
+  const int nb = 1 << L, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int LB = L < 4 ? L : 4, bside = 1 << LB, nblk = nb / bside, lblk = L - LB;
+  const int offL = GALIC_STACKOFFSET(L, 0, 0);
+  double acc = 0;
+  const int jx = morton_x(lane), jy = morton_y(lane);
+  for(int blk = wave; blk < nblk * nblk; blk += nw) {
+    const int bz = blk / nblk, br = blk % nblk;
+    const bool hit = !(box.y < br * bside || box.x > br * bside + bside - 1 || box.w < bz * bside ||
+                       box.z > bz * bside + bside - 1) &&
+                     box.y >= box.x && box.w >= box.z;
+    long long s = 0;
+    if(hit) {
+      const int pz = bz * (bside >> 1) + jy, pr = br * (bside >> 1) + jx;  // level L-1 node of the lane
+      const bool valid = 2 * jy < bside && 2 * jx < bside;
+      if(valid) {
+        double2 w[5];
+        int cell[4];
+#pragma unroll
+        for(int c = 0; c < 4; c++) {
+          cell[c] = (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1);
+          w[c] = bw[offL + cell[c]];
+        }
+        const int node = GALIC_STACKOFFSET(L - 1, pz, pr);
+        w[4] = bw[node];
+#pragma unroll
+        for(int c = 0; c < 4; c++) {
+          const long long d = G[cell[c]] - (O ? O[cell[c]] : 0ll);
+          s += d;
+          acc += term_at(w[c], (double)d * unit);
+        }
+        acc += term_at(w[4], (double)s * unit);
+      }
+      // levels L-2 .. lblk: groups of 4^k lanes
+      for(int l = L - 2, g = 1; l >= lblk; l--, g += 2) {
+        s += __shfl_xor(s, 1 << (g - 1), 64);
+        s += __shfl_xor(s, 1 << g, 64);
+        const int mask = (1 << (g + 1)) - 1;
+        const int sh = L - 1 - l;
+        if((lane & mask) == 0 && valid) {
+          const int node = GALIC_STACKOFFSET(l, pz >> sh, pr >> sh);
+          acc += term_at(bw[node], (double)s * unit);
+        }
+      }
+    }
+    if(lane == 0) topb[blk] = s;  // 0 for blocks the box misses
+  }
+  __syncthreads();
+  // levels above the blocks (at most 8 x 8 blocks): one wave, lanes in Morton order
+  if(lblk > 0 && wave == 0) {
+    const int bx = morton_x(lane), by = morton_y(lane);
+    const bool in = bx < nblk && by < nblk;
+    long long s = in ? topb[by * nblk + bx] : 0;
+    for(int l = lblk - 1, g = 1; l >= 0; l--, g += 2) {
+      s += __shfl_xor(s, 1 << (g - 1), 64);
+      s += __shfl_xor(s, 1 << g, 64);
+      const int mask = (1 << (g + 1)) - 1;
+      const int sh = lblk - l;
+      if((lane & mask) == 0 && in) acc += term_at(bw[GALIC_STACKOFFSET(l, by >> sh, bx >> sh)], (double)s * unit);
+    }
+  }
+  return coef * acc;
+}
+
 // One particle per workgroup: the merit change of each of its K trials (the
 // orbit-averaged terms on the fixed density cut, the instantaneous terms on the
 // fixed cut of EGs_MassResponse), the best improving trial taken and its deltas
-// added in fixed point.  D = new - old of a trial is rebuilt exactly from the
-// two orbits' logs in LDS, NF fields per pass.
-// LDS: red [16] double | G [NF][ngrid] int64 | cs [NF][ncoarse] int64 | tbl [K+1][log_M] int
-template <int NF>
+// added in fixed point.  Field by field, the current orbit's log is summed into
+// O once and each trial's into G, so D = G - O exactly.
+// LDS: red [16] double | topb [64] int64 | G [ngrid] int64 | O [ngrid] int64 | tbl [K+1][log_M] int
 __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K;
-  const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L), ncoarse = Nstack - ngrid;
+  const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
   double *red = lds_d;
-  long long *G = (long long *)(lds_d + 16);
-  long long *cs = G + NF * ngrid;
-  int *tbl = (int *)(cs + NF * ncoarse);
+  long long *topb = (long long *)(lds_d + 16);
+  long long *G = topb + 64;
+  long long *O = G + ngrid;
+  int *tbl = (int *)(O + ngrid);
   __shared__ double s_dF[64];
   __shared__ int s_best, s_n[65];
 
@@ -846,43 +851,44 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   // eval_fit_mod's combination (main.c:1215-1216): D/M + (f_rs V_rs + f_ts V_ts + f_qs V_qs + f_ps V_ps)/M/4
   const double coef[kFields] = {1.0 / MT, A.fac[t][0] / MT / 4, A.fac[t][1] / MT / 4, A.fac[t][2] / MT / 4,
                                 A.fac[t][3] / MT / 4};
-  double unit[kFields];
-#pragma unroll
-  for(int f = 0; f < kFields; f++) unit[f] = A.log_unit[f];
 
-  // block tables of the K+1 logs, and a clean grid
+  // block tables of the K+1 logs, clean grids, zero merit sums
   for(int q = tid; q <= K; q += nt) s_n[q] = A.log_n[o0 + q];
+  for(int q = tid; q < K; q += nt) s_dF[q] = 0;
   __syncthreads();
   for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
     const int o = q / A.log_M, b = q % A.log_M;
     if(b * LOG_B < s_n[o]) tbl[q] = A.log_blocks[(o0 + o) * A.log_M + b];
   }
-  for(int q = tid; q < NF * ngrid; q += nt) G[q] = 0;
+  for(int q = tid; q < 2 * ngrid; q += nt) G[q] = 0;
   __syncthreads();
 
-  for(int k = 0; k < K; k++) {
-    const int4 bn = A.bbox[o0 + 1 + k];
-    const int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
-    double part = 0;
-    for(int f0 = 0; f0 < kFields; f0 += NF) {
-      log_scatter<NF>(A, tbl + (1 + k) * A.log_M, s_n[1 + k], false, f0, nb, ngrid, (unsigned long long *)G);
-      log_scatter<NF>(A, tbl, s_n[0], true, f0, nb, ngrid, (unsigned long long *)G);
+  for(int f = 0; f < kFields; f++) {
+    log_scatter(A, tbl, s_n[0], false, f, nb, (unsigned long long *)O);
+    for(int k = 0; k < K; k++) {
+      const int4 bn = A.bbox[o0 + 1 + k];
+      const int4 box = make_int4(min(bo.x, bn.x), max(bo.y, bn.y), min(bo.z, bn.z), max(bo.w, bn.w));
+      log_scatter(A, tbl + (1 + k) * A.log_M, s_n[1 + k], false, f, nb, (unsigned long long *)G);
       __syncthreads();
-      part += merit_log32<NF>(bw, Nstack, coef, unit, f0, G, cs, r0, r1, z0, z1, L);
-      clear_box<NF>(G, ngrid, nb, r0, r1, z0, z1);
+      const double part = merit_waves(bw + (size_t)f * Nstack, L, coef[f], A.log_unit[f], G, O, box, topb);
+      const double total = block_sum1(part, red);  // (its barriers also end the reads of G)
+      if(tid == 0) s_dF[k] += total;
+      clear_box(G, nb, bn);
       __syncthreads();
     }
-    const double orbit_terms = block_sum1(part, red);
-    if(tid == 0) {
-      const size_t q = (size_t)ti * K + k;
-      double dnew[4];
-      disp_components64(M, t, x, y, z, A.tvx[q], A.tvy[q], A.tvz[q], dnew[0], dnew[1], dnew[2], dnew[3]);
-      double cw[4];
-      int cc[4];
-      double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
-      const double *fit = A.fit + (size_t)t * 16;
-      s_dF[k] = orbit_terms + fit[5] / fit[6] * vinst;
-    }
+    clear_box(O, nb, bo);
+    __syncthreads();
+  }
+  // instantaneous terms, one trial per thread
+  for(int k = tid; k < K; k += nt) {
+    const size_t q = (size_t)ti * K + k;
+    double dnew[4];
+    disp_components64(M, t, x, y, z, A.tvx[q], A.tvy[q], A.tvz[q], dnew[0], dnew[1], dnew[2], dnew[3]);
+    double cw[4];
+    int cc[4];
+    double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
+    const double *fit = A.fit + (size_t)t * 16;
+    s_dF[k] += fit[5] / fit[6] * vinst;
   }
   __syncthreads();
   if(tid == 0) {
@@ -931,27 +937,22 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int4 bn = A.bbox[o0 + 1 + best];
     const int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
     const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
-    for(int f0 = 0; f0 < kFields; f0 += NF) {
-      log_scatter<NF>(A, tbl + (1 + best) * A.log_M, s_n[1 + best], false, f0, nb, ngrid, (unsigned long long *)G);
-      log_scatter<NF>(A, tbl, s_n[0], true, f0, nb, ngrid, (unsigned long long *)G);
+    for(int f = 0; f < kFields; f++) {
+      log_scatter(A, tbl + (1 + best) * A.log_M, s_n[1 + best], false, f, nb, (unsigned long long *)G);
+      log_scatter(A, tbl, s_n[0], false, f, nb, (unsigned long long *)O);
       __syncthreads();
       for(int q = tid; q < nr * nzz; q += nt) {
         const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
-#pragma unroll
-        for(int f = 0; f < NF; f++) {
-          const long long d = G[f * ngrid + cell];
-          if(d) atomic_q(&lad[(size_t)(f0 + f) * ngrid + cell], (double)d * A.log_to_loc[f0 + f]);
-        }
+        const long long d = G[cell] - O[cell];
+        if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
       }
       __syncthreads();
-      clear_box<NF>(G, ngrid, nb, r0, r1, z0, z1);
+      clear_box(G, nb, bn);
+      clear_box(O, nb, bo);
       __syncthreads();
     }
   }
 }
-
-template __global__ void k_eval_log32<1>(DevModel M, EvalArgs32 A);
-template __global__ void k_eval_log32<kFields>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 
