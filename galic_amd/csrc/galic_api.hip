@@ -1354,11 +1354,13 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess && v > 0)
       lds_max = (size_t)v;
   }
-  // the merit kernel's LDS: two int64 grids, block sums and the K+1 block tables
+  // the merit kernel's LDS: per-wave partials, block sums, the coarse nodes' terms, two int64
+  // grids and the K+1 block tables
   auto eval_lds = [&]() {
-    return (16 + 64) * sizeof(double) + 2 * sizeof(long long) * (size_t)c.Ngrid + sizeof(int) * (size_t)(K + 1) * c.log_M;
+    return (8 * (size_t)K + 16) * sizeof(double) + sizeof(double2) * (size_t)GALIC_NSTACK(std::max(c.L - 2, 0)) +
+           2 * sizeof(long long) * (size_t)c.Ngrid + sizeof(int) * (size_t)(K + 1) * c.log_M;
   };
-  if(c.L > 7) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to level 7");
+  if(c.L > 6) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to 64 x 64 (level 6)");
   bool packed = false;
   float eval_ms = 0;
   int s0 = 0;

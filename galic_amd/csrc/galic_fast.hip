@@ -689,55 +689,55 @@ __device__ __forceinline__ double merit_term(const double2 *bw, int node, double
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
 
-// Adds sign * (field f of the record values) of one orbit's log into the LDS
-// grid G.  Each FP32 value times a power of two is rounded to an int64 and
-// added with an integer atomic, so the sums are exact and independent of the
-// order of the records.  A thread loads up to four records before adding them.
-// tbl: the orbit's arena blocks.
-__device__ __forceinline__ void log_scatter(const EvalArgs32 &A, const int *tbl, int n, bool neg, int f, int nb,
-                                           unsigned long long *G) {
-  const float sc = neg ? -A.log_scale[f] : A.log_scale[f];
-  for(int r0 = threadIdx.x; r0 < n; r0 += 4 * blockDim.x) {
-    int quad[4];
-    float4 x[4];
+// Records of one orbit's log held by a thread: r = tid + u * blockDim.x.
+constexpr int LOG_RPT = 2;
+struct LogRegs {
+  int quad[LOG_RPT];
+  float4 x[LOG_RPT];
+};
+
+// Loads field f of the first LOG_RPT * blockDim.x records of a log (tbl: the
+// orbit's arena blocks) into registers.
+__device__ __forceinline__ void log_load(const EvalArgs32 &A, const int *tbl, int n, int f, LogRegs &R) {
 #pragma unroll
-    for(int u = 0; u < 4; u++) {
-      const int r = r0 + u * blockDim.x;
-      quad[u] = -1;
-      if(r < n) {
-        const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
-        quad[u] = A.log_q[(size_t)blk * LOG_B + jj];
-        x[u] = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
-      }
-    }
-#pragma unroll
-    for(int u = 0; u < 4; u++) {
-      if(quad[u] < 0) continue;
-      unsigned long long *g = G + quad[u];
-      atomicAdd(g, (unsigned long long)__float2ll_rn(x[u].x * sc));
-      atomicAdd(g + 1, (unsigned long long)__float2ll_rn(x[u].y * sc));
-      atomicAdd(g + nb, (unsigned long long)__float2ll_rn(x[u].z * sc));
-      atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x[u].w * sc));
+  for(int u = 0; u < LOG_RPT; u++) {
+    const int r = threadIdx.x + u * blockDim.x;
+    R.quad[u] = -1;
+    if(r < n) {
+      const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
+      R.quad[u] = A.log_q[(size_t)blk * LOG_B + jj];
+      R.x[u] = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
     }
   }
 }
 
-// Zeroes G over the box of fine cells r0..r1, z0..z1.
+// Each FP32 value times a power of two is rounded to an int64 and added with an
+// integer atomic, so the sums are exact and independent of the order.
+__device__ __forceinline__ void add_record(unsigned long long *G, int quad, float4 x, float sc, int nb) {
+  unsigned long long *g = G + quad;
+  atomicAdd(g, (unsigned long long)__float2ll_rn(x.x * sc));
+  atomicAdd(g + 1, (unsigned long long)__float2ll_rn(x.y * sc));
+  atomicAdd(g + nb, (unsigned long long)__float2ll_rn(x.z * sc));
+  atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x.w * sc));
+}
+
+// Adds field f of a whole log into G: the records held in R, then the rest.
+__device__ __forceinline__ void log_add(const EvalArgs32 &A, const int *tbl, int n, int f, const LogRegs &R, int nb,
+                                        unsigned long long *G) {
+  const float sc = A.log_scale[f];
+#pragma unroll
+  for(int u = 0; u < LOG_RPT; u++)
+    if(R.quad[u] >= 0) add_record(G, R.quad[u], R.x[u], sc, nb);
+  for(int r = threadIdx.x + LOG_RPT * blockDim.x; r < n; r += blockDim.x) {
+    const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
+    add_record(G, A.log_q[(size_t)blk * LOG_B + jj], A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb);
+  }
+}
+
+// Zeroes G over the box of fine cells.
 __device__ __forceinline__ void clear_box(long long *G, int nb, int4 b) {
   const int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
   for(int q = threadIdx.x; q < nr * nzz; q += blockDim.x) G[(b.z + q / nr) * nb + (b.x + q % nr)] = 0;
-}
-
-// Sum over the block of one double; every thread gets the total.
-__device__ __forceinline__ double block_sum1(double v, double *red) {
-  for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  if((threadIdx.x & 63) == 0) red[wave] = v;
-  __syncthreads();
-  double s = 0;
-  for(int k = 0; k < nw; k++) s += red[k];
-  __syncthreads();
-  return s;
 }
 
 __device__ __forceinline__ int morton_x(int lane) { return (lane & 1) | ((lane >> 1) & 2) | ((lane >> 2) & 4); }
@@ -747,91 +747,112 @@ __device__ __forceinline__ double term_at(const double2 &v, double d) {
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
 
-// Per-thread partial of coef * sum over nodes of w (|b + D| - |b|) for one
-// field, D = G - O (O null: D = G) in exact int64 units times `unit`.
-// The grid is cut into blocks of 16 x 16 fine cells (the whole grid below L = 4);
-// a wave owns a block: each lane a 2x2 cell node, lanes in Morton order so that
-// groups of 4, 16 and 64 lanes are the node's parent, grandparent and the block
-// node, summed with shuffles.  The block sums then give the levels above.
-// box: the fine cells D can be non-zero in.  topb: LDS of (nb/16)^2 int64.
-__device__ double merit_waves(const double2 *bw, int L, double coef, double unit, const long long *G,
-                              const long long *O, int4 box, long long *topb) {
+// The part of the merit pyramid one lane evaluates, for grids of at most 64^2
+// cells.  The grid is cut into blocks of 16 x 16 fine cells (the whole grid
+// below L = 4); a wave takes blocks wave, wave + nwaves, ...  In a block each
+// lane owns a 2x2 cell node, lanes in Morton order, so that groups of 4, 16 and
+// 64 lanes are the node's parent, grandparent and the block node, summed with
+// shuffles.  Wave 0 then forms the levels above the blocks from their sums.
+// The (residual, weight) pairs of fine cells and level L-1 nodes come from
+// memory (bw), those of the coarser levels from LDS (bwc, indexed by node).
+__device__ __forceinline__ void grid_blocks(int L, int &bside, int &nblk, int &lblk) {
+  const int LB = L < 4 ? L : 4;
+  bside = 1 << LB;
+  nblk = (1 << L) / bside;
+  lblk = L - LB;
+}
+
+// Block phase: this lane's terms of D = G - O (G cleared as it is read), the
+// block sums to topb.  Returns the lane's partial (without coef).
+__device__ __forceinline__ double merit_block(const double2 *bw, const double2 *bwc, int L, double unit, long long *G,
+                                              const long long *O, int4 box, long long *topb) {
   const int nb = 1 << L, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int LB = L < 4 ? L : 4, bside = 1 << LB, nblk = nb / bside, lblk = L - LB;
+  int bside, nblk, lblk;
+  grid_blocks(L, bside, nblk, lblk);
   const int offL = GALIC_STACKOFFSET(L, 0, 0);
-  double acc = 0;
   const int jx = morton_x(lane), jy = morton_y(lane);
+  const bool valid = 2 * jy < bside && 2 * jx < bside;
+  double acc = 0;
   for(int blk = wave; blk < nblk * nblk; blk += nw) {
     const int bz = blk / nblk, br = blk % nblk;
-    const bool hit = !(box.y < br * bside || box.x > br * bside + bside - 1 || box.w < bz * bside ||
-                       box.z > bz * bside + bside - 1) &&
-                     box.y >= box.x && box.w >= box.z;
+    const bool hit = box.y >= box.x && box.w >= box.z &&
+                     !(box.y < br * bside || box.x > br * bside + bside - 1 || box.w < bz * bside ||
+                       box.z > bz * bside + bside - 1);
     long long s = 0;
     if(hit) {
-      const int pz = bz * (bside >> 1) + jy, pr = br * (bside >> 1) + jx;  // level L-1 node of the lane
-      const bool valid = 2 * jy < bside && 2 * jx < bside;
+      const int pz = bz * (bside >> 1) + jy, pr = br * (bside >> 1) + jx;
       if(valid) {
-        double2 w[5];
         int cell[4];
+        double2 w[5];
 #pragma unroll
         for(int c = 0; c < 4; c++) {
           cell[c] = (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1);
           w[c] = bw[offL + cell[c]];
         }
-        const int node = GALIC_STACKOFFSET(L - 1, pz, pr);
-        w[4] = bw[node];
+        w[4] = bw[GALIC_STACKOFFSET(L - 1, pz, pr)];
 #pragma unroll
         for(int c = 0; c < 4; c++) {
-          const long long d = G[cell[c]] - (O ? O[cell[c]] : 0ll);
+          const long long d = G[cell[c]] - O[cell[c]];
+          G[cell[c]] = 0;
           s += d;
           acc += term_at(w[c], (double)d * unit);
         }
         acc += term_at(w[4], (double)s * unit);
       }
-      // levels L-2 .. lblk: groups of 4^k lanes
       for(int l = L - 2, g = 1; l >= lblk; l--, g += 2) {
         s += __shfl_xor(s, 1 << (g - 1), 64);
         s += __shfl_xor(s, 1 << g, 64);
-        const int mask = (1 << (g + 1)) - 1;
-        const int sh = L - 1 - l;
-        if((lane & mask) == 0 && valid) {
-          const int node = GALIC_STACKOFFSET(l, pz >> sh, pr >> sh);
-          acc += term_at(bw[node], (double)s * unit);
-        }
+        const int mask = (1 << (g + 1)) - 1, sh = L - 1 - l;
+        if((lane & mask) == 0 && valid) acc += term_at(bwc[GALIC_STACKOFFSET(l, pz >> sh, pr >> sh)], (double)s * unit);
       }
     }
     if(lane == 0) topb[blk] = s;  // 0 for blocks the box misses
   }
-  __syncthreads();
-  // levels above the blocks (at most 8 x 8 blocks): one wave, lanes in Morton order
-  if(lblk > 0 && wave == 0) {
-    const int bx = morton_x(lane), by = morton_y(lane);
-    const bool in = bx < nblk && by < nblk;
-    long long s = in ? topb[by * nblk + bx] : 0;
-    for(int l = lblk - 1, g = 1; l >= 0; l--, g += 2) {
-      s += __shfl_xor(s, 1 << (g - 1), 64);
-      s += __shfl_xor(s, 1 << g, 64);
-      const int mask = (1 << (g + 1)) - 1;
-      const int sh = lblk - l;
-      if((lane & mask) == 0 && in) acc += term_at(bw[GALIC_STACKOFFSET(l, by >> sh, bx >> sh)], (double)s * unit);
-    }
-  }
-  return coef * acc;
+  return acc;
 }
 
-// One particle per workgroup: the merit change of each of its K trials (the
-// orbit-averaged terms on the fixed density cut, the instantaneous terms on the
-// fixed cut of EGs_MassResponse), the best improving trial taken and its deltas
-// added in fixed point.  Field by field, the current orbit's log is summed into
-// O once and each trial's into G, so D = G - O exactly.
-// LDS: red [16] double | topb [64] int64 | G [ngrid] int64 | O [ngrid] int64 | tbl [K+1][log_M] int
+// Levels above the blocks (wave 0; at most 4 x 4 blocks).
+__device__ __forceinline__ double merit_top(const double2 *bwc, int L, double unit, const long long *topb) {
+  const int lane = threadIdx.x & 63;
+  int bside, nblk, lblk;
+  grid_blocks(L, bside, nblk, lblk);
+  if(lblk == 0 || (threadIdx.x >> 6) != 0) return 0.0;
+  const int bx = morton_x(lane), by = morton_y(lane);
+  const bool in = bx < nblk && by < nblk;
+  long long s = in ? topb[by * nblk + bx] : 0;
+  double acc = 0;
+  for(int l = lblk - 1, g = 1; l >= 0; l--, g += 2) {
+    s += __shfl_xor(s, 1 << (g - 1), 64);
+    s += __shfl_xor(s, 1 << g, 64);
+    const int mask = (1 << (g + 1)) - 1, sh = lblk - l;
+    if((lane & mask) == 0 && in) acc += term_at(bwc[GALIC_STACKOFFSET(l, by >> sh, bx >> sh)], (double)s * unit);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One particle per workgroup (16 waves): the merit change of each of its K
+// trials (the orbit-averaged terms on the fixed density cut, the instantaneous
+// terms on the fixed cut of EGs_MassResponse), the best improving trial taken
+// and its deltas added in fixed point.  Field by field, the current orbit's log
+// is summed into O once and each trial's into G, so D = G - O exactly; the next
+// trial's records are loaded while the current one is evaluated.  Grids up to
+// 64^2 cells.
+// LDS: part [K][8] double | topb [16] int64 | bwc [coarse nodes] double2 | G [ngrid] int64 | O [ngrid] int64 |
+//      tbl [K+1][log_M] int
 __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
-  const int tid = threadIdx.x, nt = blockDim.x, K = A.K;
+  const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
   const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
-  double *red = lds_d;
-  long long *topb = (long long *)(lds_d + 16);
-  long long *G = topb + 64;
+  const int ncb = GALIC_NSTACK(L - 2);  // nodes of levels 0..L-2
+  double *part = lds_d;
+  long long *topb = (long long *)(part + 8 * K);
+  double2 *bwc = (double2 *)(topb + 16);
+  long long *G = (long long *)(bwc + ncb);
   long long *O = G + ngrid;
   int *tbl = (int *)(O + ngrid);
   __shared__ double s_dF[64];
@@ -854,7 +875,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
 
   // block tables of the K+1 logs, clean grids, zero merit sums
   for(int q = tid; q <= K; q += nt) s_n[q] = A.log_n[o0 + q];
-  for(int q = tid; q < K; q += nt) s_dF[q] = 0;
+  for(int q = tid; q < 8 * K; q += nt) part[q] = 0;
   __syncthreads();
   for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
     const int o = q / A.log_M, b = q % A.log_M;
@@ -863,23 +884,27 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   for(int q = tid; q < 2 * ngrid; q += nt) G[q] = 0;
   __syncthreads();
 
+  LogRegs R;
   for(int f = 0; f < kFields; f++) {
-    log_scatter(A, tbl, s_n[0], false, f, nb, (unsigned long long *)O);
+    for(int q = tid; q < ncb; q += nt) bwc[q] = bw[(size_t)f * Nstack + q];
+    log_load(A, tbl, s_n[0], f, R);
+    log_add(A, tbl, s_n[0], f, R, nb, (unsigned long long *)O);
     for(int k = 0; k < K; k++) {
       const int4 bn = A.bbox[o0 + 1 + k];
       const int4 box = make_int4(min(bo.x, bn.x), max(bo.y, bn.y), min(bo.z, bn.z), max(bo.w, bn.w));
-      log_scatter(A, tbl + (1 + k) * A.log_M, s_n[1 + k], false, f, nb, (unsigned long long *)G);
+      log_load(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R);
+      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, (unsigned long long *)G);
       __syncthreads();
-      const double part = merit_waves(bw + (size_t)f * Nstack, L, coef[f], A.log_unit[f], G, O, box, topb);
-      const double total = block_sum1(part, red);  // (its barriers also end the reads of G)
-      if(tid == 0) s_dF[k] += total;
-      clear_box(G, nb, bn);
+      double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, A.log_unit[f], G, O, box, topb);
       __syncthreads();
+      acc += merit_top(bwc, L, A.log_unit[f], topb);
+      acc = wave_sum(coef[f] * acc);
+      if(lane == 0) part[k * 8 + wave] += acc;
     }
     clear_box(O, nb, bo);
     __syncthreads();
   }
-  // instantaneous terms, one trial per thread
+  // instantaneous terms, one trial per thread; then the total per trial
   for(int k = tid; k < K; k += nt) {
     const size_t q = (size_t)ti * K + k;
     double dnew[4];
@@ -888,7 +913,9 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     int cc[4];
     double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
     const double *fit = A.fit + (size_t)t * 16;
-    s_dF[k] += fit[5] / fit[6] * vinst;
+    double orbit_terms = 0;
+    for(int w = 0; w < 8; w++) orbit_terms += part[k * 8 + w];
+    s_dF[k] = orbit_terms + fit[5] / fit[6] * vinst;
   }
   __syncthreads();
   if(tid == 0) {
@@ -938,8 +965,10 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int r0 = min(bo.x, bn.x), r1 = max(bo.y, bn.y), z0 = min(bo.z, bn.z), z1 = max(bo.w, bn.w);
     const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
     for(int f = 0; f < kFields; f++) {
-      log_scatter(A, tbl + (1 + best) * A.log_M, s_n[1 + best], false, f, nb, (unsigned long long *)G);
-      log_scatter(A, tbl, s_n[0], false, f, nb, (unsigned long long *)O);
+      log_load(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R);
+      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, (unsigned long long *)G);
+      log_load(A, tbl, s_n[0], f, R);
+      log_add(A, tbl, s_n[0], f, R, nb, (unsigned long long *)O);
       __syncthreads();
       for(int q = tid; q < nr * nzz; q += nt) {
         const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
