@@ -227,12 +227,11 @@ __device__ __forceinline__ void log32_flush(const Run32 &r, float fac, Log32 &lg
   lg.n++;
 }
 
-// Trapezoid half-step weights of one end of a step (orbit_response.c:283-319)
+// Trapezoid weight h of one point of the orbit (orbit_response.c:283-319)
 // accumulated in registers while the orbit stays in one 2x2 block.
 template <typename Flush>
-__device__ __forceinline__ void bin32(Run32 &r, int nb, Flush flush, int iR, int iz, float fR, float fz, float dt,
+__device__ __forceinline__ void bin32(Run32 &r, int nb, Flush flush, int iR, int iz, float fR, float fz, float h,
                                       float vr2, float vt2, float vq2, float vp2, int4 &bb) {
-  float h = 0.5f * dt;
   float m[4] = {h * (1 - fR) * (1 - fz), h * fR * (1 - fz), h * (1 - fR) * fz, h * fR * fz};
   int quad = iz * nb + iR;
   if(quad != r.quad) {
@@ -329,6 +328,11 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   int norbit = 0, flag = 0, status = 0;
   long long steps = 0;
   float dt = 0.f, hdt = 0.f;
+  // The trapezoid rule weights each point of the orbit by half of each step it
+  // ends or starts (orbit_response.c:283-319).  A point is binned once, with
+  // both halves: hprev, the half of the step that ended there, is added when
+  // the next step's dt is known.
+  float hprev = 0.f;
   while(true) {
     if(!fresh) {
       float inv_v = fast_rsq(vx * vx + vy * vy + vz * vz);
@@ -340,9 +344,10 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       }
       if(!(dt >= 0.f && dt < 3.0e38f)) {
         status = GALIC_ERR_ARG;
+        if(hprev > 0.f) bin32(run, nb, flush, iR, iz, fR, fz, hprev, vr2, vt2, vq2, vp2, bb);
       } else {
-        bin32(run, nb, flush, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
         hdt = 0.5f * dt;
+        bin32(run, nb, flush, iR, iz, fR, fz, hprev + hdt, vr2, vt2, vq2, vp2, bb);
         vx += hdt * ax;
         vy += hdt * ay;
         vz += hdt * az;
@@ -370,7 +375,8 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       continue;
     }
     if(status == 0) {
-      bin32(run, nb, flush, iR, iz, fR, fz, dt, vr2, vt2, vq2, vp2, bb);
+      hprev = hdt;
+      if(flag) bin32(run, nb, flush, iR, iz, fR, fz, hprev, vr2, vt2, vq2, vp2, bb);  // the orbit's last point
       L.tall += (double)dt;
       float radsign = vx * x + vy * y + vz * z;
       if(radsign > 0 && radsign_prev < 0) norbit++;
@@ -412,12 +418,15 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       flag = 0;
       status = 0;
       steps = 0;
+      hprev = 0.f;
     }
   }
 }
 
 __global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body<false>(F, A); }
-__global__ void __launch_bounds__(256) k_orbit32_log(FastModel F, OrbitArgs32 A) { orbit32_body<true>(F, A); }
+__global__ void __launch_bounds__(256) k_orbit32_log(FastModel F, OrbitArgs32 A) {
+  orbit32_body<true>(F, A);
+}
 
 // ------------------------------------------------------------ trial draws --
 
