@@ -1354,13 +1354,13 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess && v > 0)
       lds_max = (size_t)v;
   }
-  // the merit kernel's LDS: per-wave partials, block sums, the coarse nodes' terms, two int64
-  // grids and the K+1 block tables
-  auto eval_lds = [&]() {
-    return (8 * (size_t)K + 16) * sizeof(double) + sizeof(double2) * (size_t)GALIC_NSTACK(std::max(c.L - 2, 0)) +
-           2 * sizeof(long long) * (size_t)c.Ngrid + sizeof(int) * (size_t)(K + 1) * c.log_M;
+  // the merit kernel's LDS: per-wave partials, block sums, the coarse nodes' terms, one or two
+  // int64 grids and the K+1 block tables; two grids where they fit
+  auto eval_lds = [&](int ngrids) {
+    return (8 * (size_t)K + 64) * sizeof(double) + sizeof(double2) * (size_t)GALIC_NSTACK(std::max(c.L - 2, 0)) +
+           ngrids * sizeof(long long) * (size_t)c.Ngrid + sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
   };
-  if(c.L > 6) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to 64 x 64 (level 6)");
+  if(c.L > 7) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to 128 x 128 (level 7)");
   bool packed = false;
   float eval_ms = 0;
   int s0 = 0;
@@ -1425,7 +1425,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       CHK(lr);
     }
     cap = group_cap();
-    const size_t shmem = eval_lds();
+    const bool use_o = eval_lds(2) <= lds_max;
+    const size_t shmem = eval_lds(use_o ? 2 : 1) - 1024;
     if(shmem + 1024 > lds_max)
       return set_err(GALIC_ERR_ARG, "the merit kernel needs %zu bytes of LDS, the device gives %zu", shmem, lds_max);
     // evaluate the reps of the group in turn, committing each completed rep
@@ -1481,7 +1482,10 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
       E.iscale = c.fp32_iscale;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
-      hipLaunchKernelGGL(galic::k_eval_log32, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+      if(use_o)
+        hipLaunchKernelGGL(galic::k_eval_log32<true>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+      else
+        hipLaunchKernelGGL(galic::k_eval_log32<false>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c.ev1, c.stream));
       HIPCHK(hipEventSynchronize(c.ev1));

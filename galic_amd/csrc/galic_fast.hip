@@ -730,10 +730,10 @@ __device__ __forceinline__ void add_record(unsigned long long *G, int quad, floa
   atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x.w * sc));
 }
 
-// Adds field f of a whole log into G: the records held in R, then the rest.
+// Adds sign * field f of a whole log into G: the records held in R, then the rest.
 __device__ __forceinline__ void log_add(const EvalArgs32 &A, const int *tbl, int n, int f, const LogRegs &R, int nb,
-                                        unsigned long long *G) {
-  const float sc = A.log_scale[f];
+                                        float sign, unsigned long long *G) {
+  const float sc = sign * A.log_scale[f];
 #pragma unroll
   for(int u = 0; u < LOG_RPT; u++)
     if(R.quad[u] >= 0) add_record(G, R.quad[u], R.x[u], sc, nb);
@@ -756,7 +756,7 @@ __device__ __forceinline__ double term_at(const double2 &v, double d) {
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
 
-// The part of the merit pyramid one lane evaluates, for grids of at most 64^2
+// The part of the merit pyramid one lane evaluates, for grids of at most 128^2
 // cells.  The grid is cut into blocks of 16 x 16 fine cells (the whole grid
 // below L = 4); a wave takes blocks wave, wave + nwaves, ...  In a block each
 // lane owns a 2x2 cell node, lanes in Morton order, so that groups of 4, 16 and
@@ -771,8 +771,8 @@ __device__ __forceinline__ void grid_blocks(int L, int &bside, int &nblk, int &l
   lblk = L - LB;
 }
 
-// Block phase: this lane's terms of D = G - O (G cleared as it is read), the
-// block sums to topb.  Returns the lane's partial (without coef).
+// Block phase: this lane's terms of D = G - O (D = G without O; G cleared as it
+// is read), the block sums to topb.  Returns the lane's partial (without coef).
 __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *bwc, int L, double unit, long long *G,
                                               const long long *O, int4 box, long long *topb) {
   const int nb = 1 << L, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -801,7 +801,7 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
         w[4] = bw[GALIC_STACKOFFSET(L - 1, pz, pr)];
 #pragma unroll
         for(int c = 0; c < 4; c++) {
-          const long long d = G[cell[c]] - O[cell[c]];
+          const long long d = O ? G[cell[c]] - O[cell[c]] : G[cell[c]];
           G[cell[c]] = 0;
           s += d;
           acc += term_at(w[c], (double)d * unit);
@@ -820,7 +820,7 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
   return acc;
 }
 
-// Levels above the blocks (wave 0; at most 4 x 4 blocks).
+// Levels above the blocks (wave 0; at most 8 x 8 blocks).
 __device__ __forceinline__ double merit_top(const double2 *bwc, int L, double unit, const long long *topb) {
   const int lane = threadIdx.x & 63;
   int bside, nblk, lblk;
@@ -844,15 +844,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One particle per workgroup (16 waves): the merit change of each of its K
-// trials (the orbit-averaged terms on the fixed density cut, the instantaneous
-// terms on the fixed cut of EGs_MassResponse), the best improving trial taken
-// and its deltas added in fixed point.  Field by field, the current orbit's log
-// is summed into O once and each trial's into G, so D = G - O exactly; the next
-// trial's records are loaded while the current one is evaluated.  Grids up to
-// 64^2 cells.
-// LDS: part [K][8] double | topb [16] int64 | bwc [coarse nodes] double2 | G [ngrid] int64 | O [ngrid] int64 |
-//      tbl [K+1][log_M] int
+// One particle per workgroup: the merit change of each of its K trials (the
+// orbit-averaged terms on the fixed density cut, the instantaneous terms on the
+// fixed cut of EGs_MassResponse), the best improving trial taken and its deltas
+// added in fixed point.  Field by field, the current orbit's log is summed into
+// O once and each trial's into G, so D = G - O exactly (USE_O); where two grids
+// do not fit in LDS (128^2 cells) both logs go into G for every trial.
+// LDS: part [K][8] double | topb [64] int64 | bwc [coarse nodes] double2 | G [ngrid] int64 |
+//      O [ngrid] int64 (USE_O) | tbl [K+1][log_M] int
+template <bool USE_O>
 __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
@@ -860,10 +860,10 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   const int ncb = GALIC_NSTACK(L - 2);  // nodes of levels 0..L-2
   double *part = lds_d;
   long long *topb = (long long *)(part + 8 * K);
-  double2 *bwc = (double2 *)(topb + 16);
+  double2 *bwc = (double2 *)(topb + 64);
   long long *G = (long long *)(bwc + ncb);
-  long long *O = G + ngrid;
-  int *tbl = (int *)(O + ngrid);
+  long long *O = USE_O ? G + ngrid : nullptr;
+  int *tbl = (int *)(G + (USE_O ? 2 : 1) * ngrid);
   __shared__ double s_dF[64];
   __shared__ int s_best, s_n[65];
 
@@ -890,19 +890,25 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int o = q / A.log_M, b = q % A.log_M;
     if(b * LOG_B < s_n[o]) tbl[q] = A.log_blocks[(o0 + o) * A.log_M + b];
   }
-  for(int q = tid; q < 2 * ngrid; q += nt) G[q] = 0;
+  for(int q = tid; q < (USE_O ? 2 : 1) * ngrid; q += nt) G[q] = 0;
   __syncthreads();
 
   LogRegs R;
   for(int f = 0; f < kFields; f++) {
     for(int q = tid; q < ncb; q += nt) bwc[q] = bw[(size_t)f * Nstack + q];
-    log_load(A, tbl, s_n[0], f, R);
-    log_add(A, tbl, s_n[0], f, R, nb, (unsigned long long *)O);
+    if(USE_O) {
+      log_load(A, tbl, s_n[0], f, R);
+      log_add(A, tbl, s_n[0], f, R, nb, 1.0f, (unsigned long long *)O);
+    }
     for(int k = 0; k < K; k++) {
       const int4 bn = A.bbox[o0 + 1 + k];
       const int4 box = make_int4(min(bo.x, bn.x), max(bo.y, bn.y), min(bo.z, bn.z), max(bo.w, bn.w));
       log_load(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R);
-      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, (unsigned long long *)G);
+      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, 1.0f, (unsigned long long *)G);
+      if(!USE_O) {
+        log_load(A, tbl, s_n[0], f, R);
+        log_add(A, tbl, s_n[0], f, R, nb, -1.0f, (unsigned long long *)G);
+      }
       __syncthreads();
       double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, A.log_unit[f], G, O, box, topb);
       __syncthreads();
@@ -910,7 +916,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       acc = wave_sum(coef[f] * acc);
       if(lane == 0) part[k * 8 + wave] += acc;
     }
-    clear_box(O, nb, bo);
+    if(USE_O) clear_box(O, nb, bo);
     __syncthreads();
   }
   // instantaneous terms, one trial per thread; then the total per trial
@@ -975,22 +981,23 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
     for(int f = 0; f < kFields; f++) {
       log_load(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R);
-      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, (unsigned long long *)G);
+      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, 1.0f, (unsigned long long *)G);
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, (unsigned long long *)O);
+      log_add(A, tbl, s_n[0], f, R, nb, -1.0f, (unsigned long long *)G);
       __syncthreads();
       for(int q = tid; q < nr * nzz; q += nt) {
         const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
-        const long long d = G[cell] - O[cell];
+        const long long d = G[cell];
+        G[cell] = 0;
         if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
       }
-      __syncthreads();
-      clear_box(G, nb, bn);
-      clear_box(O, nb, bo);
       __syncthreads();
     }
   }
 }
+
+template __global__ void k_eval_log32<true>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<false>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 

@@ -188,3 +188,37 @@ def test_device_trial_draws_match_host_philox(w):
         np.testing.assert_allclose(v1[pi], t, rtol=1e-10, atol=1e-9)
         kinds.add(k == 0)
     assert kinds == {True, False}
+
+
+def test_fp32_level7_grids():
+    """128 x 128 response grids (the level of the reference's parameter files):
+    the merit kernel sums both logs into one LDS grid.  Reps in one launch equal
+    rep-by-rep bit for bit, and decisions follow the FP64 path."""
+    print("[gpu] fp32 level-7 grids", flush=True)
+    cfg = model.config("halo-disk-1e6", level=7, max_steps=2000000)
+    w7 = model.make_workload(cfg, n=3000, target_samples=200000)
+    perm = model.permutation(w7, 4)
+    out = []
+    for mode in ("iterate", "reps"):
+        g = GalicGPU(w7)
+        g.fp32_calc_all_response_fields()
+        if mode == "iterate":
+            acc = g.fp32_iterate(perm, k=3, rep_size=500, sigma_frac=0.1, seed=9)
+        else:
+            acc = np.concatenate([g.fp32_rep(perm[a:a + 500], k=3, sigma_frac=0.1, seed=9)
+                                  for a in range(0, len(perm), 500)])
+        out.append([acc] + [g.stack(t, wh) for t in w7.types for wh in range(10)] + [g.particles()[0]])
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    g64 = GalicGPU(w7)
+    g64.calc_all_response_fields()
+    g32 = GalicGPU(w7)
+    g32.fp32_calc_all_response_fields()
+    idx = perm[:1000]
+    v = model.trial_velocities(w7, idx, 5, 0)
+    a64 = g64.rep(idx, v)[0]
+    a32 = g32.fp32_rep(idx, k=1, vtry=v)
+    g64.close()
+    g32.close()
+    assert np.mean((a32 > 0) == (a64 > 0)) > 0.97
