@@ -153,6 +153,8 @@ def main():
                     help="iterations of the plain FP64 path (default warmup+steps; 0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget in seconds (0: skip)")
     ap.add_argument("--target-samples", type=int, default=4000000)
+    ap.add_argument("--precision-sample", type=int, default=2000,
+                    help="orbits integrated by both paths for the FP32 error figures (0: skip)")
     ap.add_argument("--tree-n", type=int, default=10000000, help="sources of the table-fill leg (0: skip)")
     ap.add_argument("--tree-nbin", type=int, default=512)
     args = ap.parse_args()
@@ -268,6 +270,31 @@ def main():
                     time_to_plain_final_chi2=to_chi2)
         log(f"[bench] time to the plain path's final chi^2: {to_chi2}")
 
+    # ------------------------------------------- FP32 orbits against FP64
+    precision = None
+    if args.precision_sample > 0:
+        g = GalicGPU(w)
+        sel = np.random.default_rng(7).choice(w.n, args.precision_sample, replace=False)
+        errs, d32, d64, dsteps = [], [], [], []
+        for a in range(0, len(sel), 500):
+            s = sel[a:a + 500]
+            orb = (w.pos[s], w.vel[s], w.mass[s], w.tint[s], w.ptype[s])
+            r64 = g.orbit_responses(*orb)
+            r32 = g.orbit_responses(*orb, precision=32)
+            errs.append(np.abs(r32["fields"] - r64["fields"]).sum(axis=(1, 2)) / np.abs(r64["fields"]).sum(axis=(1, 2)))
+            d32.append(np.abs(r32["e1"] / r32["e0"] - 1))
+            d64.append(np.abs(r64["e1"] / r64["e0"] - 1))
+            dsteps.append(np.abs(r32["steps"] - r64["steps"]) / r64["steps"])
+        g.close()
+        errs, d32, d64, dsteps = map(np.concatenate, (errs, d32, d64, dsteps))
+        precision = dict(orbits=len(sel), response_l1_rel_err=dict(median=float(np.median(errs)),
+                                                                  p99=float(np.quantile(errs, 0.99))),
+                         energy_drift=dict(fp32_median=float(np.median(d32)), fp64_median=float(np.median(d64)),
+                                           fp32_p99=float(np.quantile(d32, 0.99)),
+                                           fp64_p99=float(np.quantile(d64, 0.99))),
+                         step_count_rel_diff_median=float(np.median(dsteps)))
+        log(f"[bench] fp32 vs fp64 orbits: {precision}")
+
     # ------------------------------------------------------------- CPU
     cpu = None
     if args.cpu_seconds > 0:
@@ -322,6 +349,7 @@ def main():
         "fp64_plain": fp64,
         "fp32_vs_fp64_speedup": (value / fp64["orbits_per_s"]) if fp64 else None,
         "chi2_check": chi2,
+        "fp32_orbit_precision": precision,
         "cpu_baseline": cpu,
         "tree_fill": tree,
         "copy_probe_GBs": copy_gbs,
