@@ -365,7 +365,7 @@ galic::FastModel fast_model(galic::Ctx &c) {
   F.fc = (float)p.TimeStepFactorCellCross;
   F.max_steps = c.M.max_steps;
   F.acc = c.FG_Acc_f;
-  for(int t = 0; t < GALIC_NTYPES; t++) F.vstream[t] = c.FG_Vstream_f + (size_t)t * c.FG_Ngrid;
+  F.vstream = c.FG_Vstream_f;
   F.cellsize = c.DG_CellSize_f;
   for(int f = 0; f < 5; f++) F.qscale[f] = c.fp32_scale[f];
   return F;
@@ -438,7 +438,12 @@ int ensure_arena(galic::Ctx &c, size_t want, size_t need) {
 }
 
 // Blocks of the arena a launch of n orbits is planned with.
-double log_blocks_planned(const galic::Ctx &c, int n) { return (double)n * (c.log_est_blocks + 1.0) * 1.25 + 64; }
+// Arena blocks planned for a launch of n orbits: the running estimate per orbit
+// with a margin, and the blocks the lanes may claim and leave unused.
+double log_blocks_planned(const galic::Ctx &c, int n) {
+  const double lanes = (double)orbit_grid(n, 256, c.fp32_bpc) * 256;
+  return (double)n * (c.log_est_blocks + 1.0) * 1.25 + lanes * galic::LOG_CHUNK + 64;
+}
 
 constexpr int kLogArenaFull = 1;  // launch_orbit32_log: the arena ran out, plan a smaller launch
 
@@ -470,7 +475,7 @@ int launch_orbit32_log(galic::Ctx &c, galic::OrbitArgs32 A) {
     A.log_q = c.log_q = (int *)c.slots;
     A.log_v = c.log_v = (float4 *)((int *)c.slots + cap * galic::LOG_B);
     A.log_next = c.log_next;
-    A.log_cap = (unsigned int)std::min<size_t>(cap, 0xffffff00u);
+    A.log_cap = (unsigned int)std::min<size_t>(cap, 0xf0000000u);  // claims past the cap stay below 2^32
     HIPCHK(hipMemsetAsync(c.log_next, 0, sizeof(unsigned int), c.stream));
     HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
     HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 4 * sizeof(unsigned long long), c.stream));
@@ -1337,7 +1342,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   auto group_cap = [&]() {
     const double blocks = (double)(c.slot_bytes / log_block_bytes());
     const double per_trial = (K + 1) * (c.log_est_blocks + 1.0) * 1.25;
-    return (int)std::max(1.0, std::min((double)std::max(ntrial, 1), (blocks - 64) / per_trial));
+    const double lanes = (double)orbit_grid(std::max(ntrial, 1) * (K + 1), 256, c.fp32_bpc) * 256;
+    return (int)std::max(1.0, std::min((double)std::max(ntrial, 1),
+                                       (blocks - 64 - lanes * galic::LOG_CHUNK) / per_trial));
   };
   int cap = group_cap();
   const int nrep = (ntrial + rep_size - 1) / rep_size;

@@ -18,7 +18,7 @@ struct FastModel {
   float fo_v200, fc;
   long long max_steps;
   const float2 *acc;  // (DPotDR, DPotDz) per node
-  const float *vstream[GALIC_NTYPES];
+  const float *vstream;  // [type][node] streaming velocity
   const float *cellsize;
   double qscale[kFields];  // fixed-point scale per orbit-averaged field
 };
@@ -51,6 +51,9 @@ struct OrbitArgs32 {
 
 // records per log block
 constexpr int LOG_B = 64;
+// arena blocks a lane claims at once (the blocks a lane has not used when the
+// launch ends are left empty: at most LOG_CHUNK - 1 per lane)
+constexpr int LOG_CHUNK = 8;
 
 struct EvalArgs32 {
   Particles P;

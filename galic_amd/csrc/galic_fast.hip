@@ -47,19 +47,49 @@ __device__ __forceinline__ Fg32 fg_index32(const FastModel &F, float x, float y,
   return g;
 }
 
-__device__ __forceinline__ void accel32(const FastModel &F, float x, float y, float z, const Fg32 &g, float &ax,
-                                        float &ay, float &az) {
+// Every table value one point of the orbit needs, loaded together so a step
+// waits on memory once: the four force-table nodes and the four streaming
+// velocity nodes around the point (both on the force grid, indices clamped
+// into the table; the off-grid force below does not use the force taps), and
+// the density-grid cell size that sets the next step's dt.
+struct Taps32 {
+  float2 a00, a01, a10, a11;  // (DPotDR, DPotDz) at (z row, R column) +(0,0), (0,1), (1,0), (1,1)
+  float v00, v01, v10, v11;   // streaming velocity at the same nodes
+  float cell;                 // DG_CellSize of the point's density cell
+  int jb, ib;                 // force-grid node left of / below the point (unclamped)
+};
+
+__device__ __forceinline__ Taps32 load_taps32(const FastModel &F, const Fg32 &g, int type, int icell) {
+  Taps32 t;
+  const int nb = F.fg_nbin;
+  t.jb = max((int)g.jd, 0);
+  t.ib = max((int)g.id, 0);
+  const int k = min(t.ib, nb - 2) * nb + min(t.jb, nb - 2);
+  const float2 *T = F.acc + k;
+  const float *V = F.vstream + (size_t)type * nb * nb + k;
+  t.a00 = T[0];
+  t.a01 = T[1];
+  t.a10 = T[nb];
+  t.a11 = T[nb + 1];
+  t.v00 = V[0];
+  t.v01 = V[1];
+  t.v10 = V[nb];
+  t.v11 = V[nb + 1];
+  t.cell = F.cellsize[icell];
+  return t;
+}
+
+__device__ __forceinline__ void accel32(const FastModel &F, float x, float y, float z, const Fg32 &g,
+                                        const Taps32 &t, float &ax, float &ay, float &az) {
   ax = ay = az = 0.f;
   if(F.use_table) {
     const int nb = F.fg_nbin;
-    int jb = (int)g.jd, ib = (int)g.id;
+    const int jb = t.jb, ib = t.ib;
     if(ib < nb - 1 && jb < nb - 1) {
       float jf = g.jd - jb, fi = g.id - ib;
-      const float2 *T = F.acc + ib * nb + jb;
-      float2 t00 = T[0], t01 = T[1], t10 = T[nb], t11 = T[nb + 1];
       float w00 = (1 - fi) * (1 - jf), w01 = (1 - fi) * jf, w10 = fi * (1 - jf), w11 = fi * jf;
-      float accR = t00.x * w00 + t01.x * w01 + t10.x * w10 + t11.x * w11;
-      float accz = t00.y * w00 + t01.y * w01 + t10.y * w10 + t11.y * w11;
+      float accR = t.a00.x * w00 + t.a01.x * w01 + t.a10.x * w10 + t.a11.x * w11;
+      float accz = t.a00.y * w00 + t.a01.y * w01 + t.a10.y * w10 + t.a11.y * w11;
       if(z < 0) accz = -accz;
       float c = 1.f, s = 0.f;
       if(g.R > 0) {
@@ -88,21 +118,14 @@ __device__ __forceinline__ void accel32(const FastModel &F, float x, float y, fl
   }
 }
 
-__device__ __forceinline__ float vstream32(const FastModel &F, const Fg32 &g, int type) {
+// Bilinear streaming velocity; beyond the last node the outermost column / row
+// is used (fraction 1 of the clamped cell).
+__device__ __forceinline__ float vstream32(const FastModel &F, const Fg32 &g, const Taps32 &t) {
   const int nb = F.fg_nbin;
-  float br = g.jd, bz = g.id;
-  int iR = (int)br, iz = (int)bz;
-  float fR = br - iR, fz = bz - iz;
-  if(iR >= nb - 1) {
-    iR = nb - 2;
-    fR = 1.f;
-  }
-  if(iz >= nb - 1) {
-    iz = nb - 2;
-    fz = 1.f;
-  }
-  const float *V = F.vstream[type] + iz * nb + iR;
-  return V[0] * (1 - fR) * (1 - fz) + V[nb] * (1 - fR) * fz + V[1] * fR * (1 - fz) + V[nb + 1] * fR * fz;
+  float fR = g.jd - t.jb, fz = g.id - t.ib;
+  if(t.jb >= nb - 1) fR = 1.f;
+  if(t.ib >= nb - 1) fz = 1.f;
+  return t.v00 * (1 - fR) * (1 - fz) + t.v10 * (1 - fR) * fz + t.v01 * fR * (1 - fz) + t.v11 * fR * fz;
 }
 
 __device__ __forceinline__ void cell32(const FastModel &F, float R, float z, int &iR, int &iz, float &fR,
@@ -128,13 +151,11 @@ __device__ __forceinline__ void cell32(const FastModel &F, float R, float z, int
 
 // vr2, vt2, vq2, vp2 of orbit_response.c:222-273 in closed form:
 // spherical basis Q = (-y, x, 0), T = (-zx, -zy, R^2), |Q| = R, |T| = R r.
-__device__ __forceinline__ void decompose32(const FastModel &F, int tovs, int type, float x, float y, float z,
-                                            float vx, float vy, float vz, const Fg32 &g, float &vr2, float &vt2,
-                                            float &vq2, float &vp2) {
+__device__ __forceinline__ void decompose32(int tovs, float vstr, float x, float y, float z, float vx, float vy,
+                                            float vz, const Fg32 &g, float &vr2, float &vt2, float &vq2, float &vp2) {
   float R2 = x * x + y * y;
   float vq = (vy * x - vx * y) * g.invR;
   vq2 = vq * vq;
-  float vstr = vstream32(F, g, type);
   vp2 = (vq - vstr) * (vq - vstr);
   if(tovs == 2) {
     float vdx = vx * x + vy * y;
@@ -198,10 +219,15 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
 
 // Append-only response log of one orbit: the lane never reads its log back,
 // so a flush is a handful of stores and the wave does not wait on memory.
+// Arena blocks are claimed LOG_CHUNK at a time per lane and handed to the
+// lane's orbits in turn: a claim is an atomic with return on one counter that
+// every lane of the chip shares, and its round trip stalls the whole wave.
 struct Log32 {
-  int n;     // records written
-  int blk;   // current arena block
-  bool full; // out of block-table entries or arena blocks
+  int n;              // records written
+  int blk;            // current arena block
+  bool full;          // out of block-table entries or arena blocks
+  unsigned int next;  // the lane's claimed blocks [next, end)
+  unsigned int end;
 };
 
 __device__ __forceinline__ void log32_flush(const Run32 &r, float fac, Log32 &lg, int o, const OrbitArgs32 &A) {
@@ -209,7 +235,14 @@ __device__ __forceinline__ void log32_flush(const Run32 &r, float fac, Log32 &lg
   const int j = lg.n & (LOG_B - 1);
   if(!lg.full && j == 0) {
     const int b = lg.n / LOG_B;
-    unsigned int nb = b < A.log_M ? atomicAdd(A.log_next, 1u) : 0xffffffffu;
+    unsigned int nb = 0xffffffffu;
+    if(b < A.log_M) {
+      if(lg.next == lg.end) {
+        lg.next = atomicAdd(A.log_next, (unsigned int)LOG_CHUNK);
+        lg.end = lg.next + LOG_CHUNK;
+      }
+      nb = lg.next++;
+    }
     if(nb >= A.log_cap) {
       lg.full = true;
     } else {
@@ -308,6 +341,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   lg.n = 0;
   lg.blk = 0;
   lg.full = false;
+  lg.next = lg.end = 0;
   auto flush = [&](const Run32 &r) {
     if(LOG)
       log32_flush(r, L.fac, lg, L.o, A);
@@ -319,7 +353,8 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   L.tovs = F.tovs[L.type];
   bool fresh = true;
   float ax = 0, ay = 0, az = 0;
-  int iR = 0, iz = 0, icell = 0;
+  int iR = 0, iz = 0;
+  float cellsize = 0.f;  // of the current point's density cell
   float fR = 0, fz = 0, vr2 = 0, vt2 = 0, vq2 = 0, vp2 = 0;
   Run32 run;
   run.quad = -1;
@@ -337,7 +372,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
     if(!fresh) {
       float inv_v = fast_rsq(vx * vx + vy * vy + vz * vz);
       float inv_a = fast_rsq(ax * ax + ay * ay + az * az);
-      dt = fminf(F.fo_v200 * inv_a, F.fc * F.cellsize[icell] * inv_v);
+      dt = fminf(F.fo_v200 * inv_a, F.fc * cellsize * inv_v);
       if(L.timespan < L.tall + (double)dt) {
         dt = (float)(L.timespan - L.tall);
         flag = 1;
@@ -357,8 +392,10 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       }
     }
     Fg32 g = fg_index32(F, x, y, z);
+    cell32(F, g.R, z, iR, iz, fR, fz);
+    const Taps32 taps = load_taps32(F, g, L.type, iz * nb + iR);
     float nax, nay, naz;
-    accel32(F, x, y, z, g, nax, nay, naz);
+    accel32(F, x, y, z, g, taps, nax, nay, naz);
     ax = nax;
     ay = nay;
     az = naz;
@@ -367,9 +404,8 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       vy += hdt * ay;
       vz += hdt * az;
     }
-    cell32(F, g.R, z, iR, iz, fR, fz);
-    icell = iz * nb + iR;
-    decompose32(F, L.tovs, L.type, x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
+    cellsize = taps.cell;
+    decompose32(L.tovs, vstream32(F, g, taps), x, y, z, vx, vy, vz, g, vr2, vt2, vq2, vp2);
     if(fresh) {
       fresh = false;
       continue;
