@@ -437,7 +437,6 @@ int ensure_arena(galic::Ctx &c, size_t want, size_t need) {
   return GALIC_OK;
 }
 
-// Blocks of the arena a launch of n orbits is planned with.
 // Arena blocks planned for a launch of n orbits: the running estimate per orbit
 // with a margin, and the blocks the lanes may claim and leave unused.
 double log_blocks_planned(const galic::Ctx &c, int n) {
@@ -1365,7 +1364,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   // int64 grids and the K+1 block tables; two grids where they fit
   auto eval_lds = [&](int ngrids) {
     return (8 * (size_t)K + 64) * sizeof(double) + sizeof(double2) * (size_t)GALIC_NSTACK(std::max(c.L - 2, 0)) +
-           ngrids * sizeof(long long) * (size_t)c.Ngrid + sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
+           ngrids * sizeof(long long) * (size_t)c.Nbin * (c.Nbin + (ngrids == 2 ? galic::MERIT_ROW_PAD : 0)) +
+           sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
   };
   if(c.L > 7) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to 128 x 128 (level 7)");
   bool packed = false;

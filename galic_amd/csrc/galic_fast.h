@@ -54,6 +54,10 @@ constexpr int LOG_B = 64;
 // arena blocks a lane claims at once (the blocks a lane has not used when the
 // launch ends are left empty: at most LOG_CHUNK - 1 per lane)
 constexpr int LOG_CHUNK = 8;
+// Where two grids fit in LDS, the merit kernel's grids hold row iz of the
+// response grid at iz * (nb + MERIT_ROW_PAD): a record's cells one row apart
+// then fall in different LDS banks.
+constexpr int MERIT_ROW_PAD = 2;
 
 struct EvalArgs32 {
   Particles P;

@@ -758,31 +758,34 @@ __device__ __forceinline__ void log_load(const EvalArgs32 &A, const int *tbl, in
 
 // Each FP32 value times a power of two is rounded to an int64 and added with an
 // integer atomic, so the sums are exact and independent of the order.
-__device__ __forceinline__ void add_record(unsigned long long *G, int quad, float4 x, float sc, int nb) {
-  unsigned long long *g = G + quad;
+// quad = iz * nb + iR names the 2x2 cells (iz..iz+1, iR..iR+1), nb = 2^L; the
+// LDS grid holds cell (iz, iR) at iz * rs + iR.
+__device__ __forceinline__ void add_record(unsigned long long *G, int quad, float4 x, float sc, int nb, int L, int rs) {
+  unsigned long long *g = G + (quad >> L) * rs + (quad & (nb - 1));
   atomicAdd(g, (unsigned long long)__float2ll_rn(x.x * sc));
   atomicAdd(g + 1, (unsigned long long)__float2ll_rn(x.y * sc));
-  atomicAdd(g + nb, (unsigned long long)__float2ll_rn(x.z * sc));
-  atomicAdd(g + nb + 1, (unsigned long long)__float2ll_rn(x.w * sc));
+  atomicAdd(g + rs, (unsigned long long)__float2ll_rn(x.z * sc));
+  atomicAdd(g + rs + 1, (unsigned long long)__float2ll_rn(x.w * sc));
 }
 
 // Adds sign * field f of a whole log into G: the records held in R, then the rest.
 __device__ __forceinline__ void log_add(const EvalArgs32 &A, const int *tbl, int n, int f, const LogRegs &R, int nb,
-                                        float sign, unsigned long long *G) {
+                                        int L, int rs, float sign, unsigned long long *G) {
   const float sc = sign * A.log_scale[f];
 #pragma unroll
   for(int u = 0; u < LOG_RPT; u++)
-    if(R.quad[u] >= 0) add_record(G, R.quad[u], R.x[u], sc, nb);
+    if(R.quad[u] >= 0) add_record(G, R.quad[u], R.x[u], sc, nb, L, rs);
   for(int r = threadIdx.x + LOG_RPT * blockDim.x; r < n; r += blockDim.x) {
     const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
-    add_record(G, A.log_q[(size_t)blk * LOG_B + jj], A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb);
+    add_record(G, A.log_q[(size_t)blk * LOG_B + jj], A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb, L,
+               rs);
   }
 }
 
-// Zeroes G over the box of fine cells.
-__device__ __forceinline__ void clear_box(long long *G, int nb, int4 b) {
+// Zeroes G (row stride rs) over the box of fine cells.
+__device__ __forceinline__ void clear_box(long long *G, int rs, int4 b) {
   const int nr = (b.y < b.x) ? 0 : b.y - b.x + 1, nzz = (b.w < b.z) ? 0 : b.w - b.z + 1;
-  for(int q = threadIdx.x; q < nr * nzz; q += blockDim.x) G[(b.z + q / nr) * nb + (b.x + q % nr)] = 0;
+  for(int q = threadIdx.x; q < nr * nzz; q += blockDim.x) G[(b.z + q / nr) * rs + b.x + q % nr] = 0;
 }
 
 __device__ __forceinline__ int morton_x(int lane) { return (lane & 1) | ((lane >> 1) & 2) | ((lane >> 2) & 4); }
@@ -809,8 +812,8 @@ __device__ __forceinline__ void grid_blocks(int L, int &bside, int &nblk, int &l
 
 // Block phase: this lane's terms of D = G - O (D = G without O; G cleared as it
 // is read), the block sums to topb.  Returns the lane's partial (without coef).
-__device__ __forceinline__ double merit_block(const double2 *bw, const double2 *bwc, int L, double unit, long long *G,
-                                              const long long *O, int4 box, long long *topb) {
+__device__ __forceinline__ double merit_block(const double2 *bw, const double2 *bwc, int L, int rs, double unit,
+                                              long long *G, const long long *O, int4 box, long long *topb) {
   const int nb = 1 << L, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   int bside, nblk, lblk;
   grid_blocks(L, bside, nblk, lblk);
@@ -831,8 +834,8 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
         double2 w[5];
 #pragma unroll
         for(int c = 0; c < 4; c++) {
-          cell[c] = (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1);
-          w[c] = bw[offL + cell[c]];
+          cell[c] = (2 * pz + (c & 1)) * rs + 2 * pr + (c >> 1);
+          w[c] = bw[offL + (2 * pz + (c & 1)) * nb + 2 * pr + (c >> 1)];
         }
         w[4] = bw[GALIC_STACKOFFSET(L - 1, pz, pr)];
 #pragma unroll
@@ -886,8 +889,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // added in fixed point.  Field by field, the current orbit's log is summed into
 // O once and each trial's into G, so D = G - O exactly (USE_O); where two grids
 // do not fit in LDS (128^2 cells) both logs go into G for every trial.
-// LDS: part [K][8] double | topb [64] int64 | bwc [coarse nodes] double2 | G [ngrid] int64 |
-//      O [ngrid] int64 (USE_O) | tbl [K+1][log_M] int
+// LDS: part [K][8] double | topb [64] int64 | bwc [coarse nodes] double2 | G [nb rows] int64 |
+//      O [nb rows] int64 (USE_O) | tbl [K+1][log_M] int; a row of G and O holds rs cells
 template <bool USE_O>
 __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
@@ -898,8 +901,9 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   long long *topb = (long long *)(part + 8 * K);
   double2 *bwc = (double2 *)(topb + 64);
   long long *G = (long long *)(bwc + ncb);
-  long long *O = USE_O ? G + ngrid : nullptr;
-  int *tbl = (int *)(G + (USE_O ? 2 : 1) * ngrid);
+  const int rs = nb + (USE_O ? MERIT_ROW_PAD : 0), nlds = nb * rs;  // LDS row stride, cells of one LDS grid
+  long long *O = USE_O ? G + nlds : nullptr;
+  int *tbl = (int *)(G + (USE_O ? 2 : 1) * nlds);
   __shared__ double s_dF[64];
   __shared__ int s_best, s_n[65];
 
@@ -926,7 +930,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int o = q / A.log_M, b = q % A.log_M;
     if(b * LOG_B < s_n[o]) tbl[q] = A.log_blocks[(o0 + o) * A.log_M + b];
   }
-  for(int q = tid; q < (USE_O ? 2 : 1) * ngrid; q += nt) G[q] = 0;
+  for(int q = tid; q < (USE_O ? 2 : 1) * nlds; q += nt) G[q] = 0;
   __syncthreads();
 
   LogRegs R;
@@ -934,25 +938,28 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     for(int q = tid; q < ncb; q += nt) bwc[q] = bw[(size_t)f * Nstack + q];
     if(USE_O) {
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, 1.0f, (unsigned long long *)O);
+      log_add(A, tbl, s_n[0], f, R, nb, L, rs, 1.0f, (unsigned long long *)O);
     }
+    // each trial's first records are loaded before the previous trial's pyramid,
+    // so their latency overlaps it
+    if(K > 0) log_load(A, tbl + A.log_M, s_n[1], f, R);
     for(int k = 0; k < K; k++) {
       const int4 bn = A.bbox[o0 + 1 + k];
       const int4 box = make_int4(min(bo.x, bn.x), max(bo.y, bn.y), min(bo.z, bn.z), max(bo.w, bn.w));
-      log_load(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R);
-      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, 1.0f, (unsigned long long *)G);
+      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, L, rs, 1.0f, (unsigned long long *)G);
       if(!USE_O) {
         log_load(A, tbl, s_n[0], f, R);
-        log_add(A, tbl, s_n[0], f, R, nb, -1.0f, (unsigned long long *)G);
+        log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G);
       }
+      if(k + 1 < K) log_load(A, tbl + (2 + k) * A.log_M, s_n[2 + k], f, R);
       __syncthreads();
-      double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, A.log_unit[f], G, O, box, topb);
+      double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, rs, A.log_unit[f], G, O, box, topb);
       __syncthreads();
       acc += merit_top(bwc, L, A.log_unit[f], topb);
       acc = wave_sum(coef[f] * acc);
       if(lane == 0) part[k * 8 + wave] += acc;
     }
-    if(USE_O) clear_box(O, nb, bo);
+    if(USE_O) clear_box(O, rs, bo);
     __syncthreads();
   }
   // instantaneous terms, one trial per thread; then the total per trial
@@ -1017,14 +1024,14 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
     for(int f = 0; f < kFields; f++) {
       log_load(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R);
-      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, 1.0f, (unsigned long long *)G);
+      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, L, rs, 1.0f, (unsigned long long *)G);
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, -1.0f, (unsigned long long *)G);
+      log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G);
       __syncthreads();
       for(int q = tid; q < nr * nzz; q += nt) {
-        const int cell = (z0 + q / nr) * nb + (r0 + q % nr);
-        const long long d = G[cell];
-        G[cell] = 0;
+        const int cell = (z0 + q / nr) * nb + (r0 + q % nr), lc = (z0 + q / nr) * rs + (r0 + q % nr);
+        const long long d = G[lc];
+        G[lc] = 0;
         if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
       }
       __syncthreads();
