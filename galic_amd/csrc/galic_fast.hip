@@ -193,7 +193,7 @@ __device__ __forceinline__ void atomic_q(long long *p, double v) {
 // Adds a run's weights to the orbit's slot (cell-major: 5 fields per cell
 // contiguous, so a 2x2 block is two short row segments) or, without a slot,
 // to the fixed-point Loc grids.
-__device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, float fac, float *slot, long long *locq,
+__device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, float *slot, long long *locq,
                                             const FastModel &F) {
   if(r.quad < 0) return;
   const int c[4] = {r.quad, r.quad + 1, r.quad + nb, r.quad + nb + 1};
@@ -201,16 +201,16 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
     float *s0 = slot + (size_t)c[0] * kFields, *s2 = slot + (size_t)c[2] * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      s0[f] += fac * r.w[f][0];
-      s0[kFields + f] += fac * r.w[f][1];
-      s2[f] += fac * r.w[f][2];
-      s2[kFields + f] += fac * r.w[f][3];
+      s0[f] += r.w[f][0];
+      s0[kFields + f] += r.w[f][1];
+      s2[f] += r.w[f][2];
+      s2[kFields + f] += r.w[f][3];
     }
   } else {
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
       long long *s = locq + (size_t)f * ngrid;
-      double sc = F.qscale[f] * (double)fac;
+      double sc = F.qscale[f];
 #pragma unroll
       for(int k = 0; k < 4; k++) atomic_q(&s[c[k]], sc * (double)r.w[f][k]);
     }
@@ -230,7 +230,7 @@ struct Log32 {
   unsigned int end;
 };
 
-__device__ __forceinline__ void log32_flush(const Run32 &r, float fac, Log32 &lg, int o, const OrbitArgs32 &A) {
+__device__ __forceinline__ void log32_flush(const Run32 &r, Log32 &lg, int o, const OrbitArgs32 &A) {
   if(r.quad < 0) return;
   const int j = lg.n & (LOG_B - 1);
   if(!lg.full && j == 0) {
@@ -255,12 +255,13 @@ __device__ __forceinline__ void log32_flush(const Run32 &r, float fac, Log32 &lg
     float4 *v = A.log_v + (size_t)lg.blk * kFields * LOG_B + j;
 #pragma unroll
     for(int f = 0; f < kFields; f++)
-      v[f * LOG_B] = make_float4(fac * r.w[f][0], fac * r.w[f][1], fac * r.w[f][2], fac * r.w[f][3]);
+      v[f * LOG_B] = make_float4(r.w[f][0], r.w[f][1], r.w[f][2], r.w[f][3]);
   }
   lg.n++;
 }
 
-// Trapezoid weight h of one point of the orbit (orbit_response.c:283-319)
+// Trapezoid weight h of one point of the orbit (orbit_response.c:283-319),
+// already multiplied by mass / timespan (the final scaling of :475-486),
 // accumulated in registers while the orbit stays in one 2x2 block.
 template <typename Flush>
 __device__ __forceinline__ void bin32(Run32 &r, int nb, Flush flush, int iR, int iz, float fR, float fz, float h,
@@ -289,7 +290,7 @@ __device__ __forceinline__ void bin32(Run32 &r, int nb, Flush flush, int iR, int
 struct Lane32 {
   int o, type, tovs;
   double timespan, tall;
-  float fac;
+  float fac;  // mass / timespan, folded into every trapezoid weight
   float *slot;
   long long *locq;
 };
@@ -344,9 +345,9 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   lg.next = lg.end = 0;
   auto flush = [&](const Run32 &r) {
     if(LOG)
-      log32_flush(r, L.fac, lg, L.o, A);
+      log32_flush(r, lg, L.o, A);
     else
-      run32_flush(r, nb, ngrid, L.fac, L.slot, L.locq, F);
+      run32_flush(r, nb, ngrid, L.slot, L.locq, F);
   };
   float x, y, z, vx, vy, vz;
   if(!take_orbit32(A, nb, L, x, y, z, vx, vy, vz)) return;
@@ -379,10 +380,10 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       }
       if(!(dt >= 0.f && dt < 3.0e38f)) {
         status = GALIC_ERR_ARG;
-        if(hprev > 0.f) bin32(run, nb, flush, iR, iz, fR, fz, hprev, vr2, vt2, vq2, vp2, bb);
+        if(hprev > 0.f) bin32(run, nb, flush, iR, iz, fR, fz, L.fac * hprev, vr2, vt2, vq2, vp2, bb);
       } else {
         hdt = 0.5f * dt;
-        bin32(run, nb, flush, iR, iz, fR, fz, hprev + hdt, vr2, vt2, vq2, vp2, bb);
+        bin32(run, nb, flush, iR, iz, fR, fz, L.fac * (hprev + hdt), vr2, vt2, vq2, vp2, bb);
         vx += hdt * ax;
         vy += hdt * ay;
         vz += hdt * az;
@@ -412,7 +413,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
     }
     if(status == 0) {
       hprev = hdt;
-      if(flag) bin32(run, nb, flush, iR, iz, fR, fz, hprev, vr2, vt2, vq2, vp2, bb);  // the orbit's last point
+      if(flag) bin32(run, nb, flush, iR, iz, fR, fz, L.fac * hprev, vr2, vt2, vq2, vp2, bb);  // the orbit's last point
       L.tall += (double)dt;
       float radsign = vx * x + vy * y + vz * z;
       if(radsign > 0 && radsign_prev < 0) norbit++;
