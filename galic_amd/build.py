@@ -27,6 +27,7 @@ UNITS = {
     "galic_dropin.hip": ["-ffp-contract=off"],
     "galic_fast.hip": ["-ffp-contract=fast"],
     "galic_tree.hip": ["-ffp-contract=off"],
+    "galic_targets.hip": ["-ffp-contract=off"],
 }
 
 

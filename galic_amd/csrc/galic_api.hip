@@ -8,12 +8,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
 #include "../../include/galic_hip.h"
 #include "galic_ctx.h"
 #include "galic_fast.h"
+#include "galic_targets.h"
 #include "galic_kernels.h"
 
 struct galic_ctx {
@@ -774,6 +776,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
   dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv); dfree(c.psteps);
   dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_next);
+  for(int t = 0; t < GALIC_NTYPES; t++) dfree(c.disp_tab[t]);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -830,6 +833,138 @@ int galic_set_targets(galic_ctx *ctx, int type, const double *dgs_mass_target, c
                    sizeof(double) * c.Nstack, hipMemcpyHostToDevice));
   c.targets_set[type] = true;
   c.cuts_ready = false;
+  return GALIC_OK;
+}
+
+int galic_set_disp_tables(galic_ctx *ctx, int type, double beta, const double *disp_r, const double *disp_z,
+                          const double *disp_phi, const double *tilt_r, const double *tilt_theta) {
+  if(!ctx || type < 1 || type > 3) return set_err(GALIC_ERR_ARG, "bad type %d", type);
+  galic::Ctx &c = ctx->c;
+  const int tovs = c.p.TypeOfVelocityStructure[type];
+  const bool need[5] = {tovs <= 1, tovs == 2, tovs >= 2, tovs == 3, tovs == 3};
+  const double *src[5] = {disp_r, disp_z, disp_phi, tilt_r, tilt_theta};
+  const char *names[5] = {"disp_r", "disp_z", "disp_phi", "tilt_r", "tilt_theta"};
+  for(int k = 0; k < 5; k++)
+    if(need[k] && !src[k])
+      return set_err(GALIC_ERR_ARG, "velocity structure %d of type %d needs the %s table", tovs, type, names[k]);
+  HIPCHK(hipSetDevice(c.device));
+  const size_t nb = c.p.FG_Nbin, n2 = nb * nb, total = nb + 4 * n2;
+  std::vector<double> h(total, 0.0);
+  double bound = 0;
+  size_t off = 0;
+  for(int k = 0; k < 5; k++) {
+    const size_t len = k == 0 ? nb : n2;
+    if(src[k])
+      for(size_t i = 0; i < len; i++) {
+        if(!std::isfinite(src[k][i])) return set_err(GALIC_ERR_ARG, "%s[%zu] is not finite", names[k], i);
+        h[off + i] = src[k][i];
+        bound = std::max(bound, fabs(src[k][i]));
+      }
+    off += len;
+  }
+  if(!c.disp_tab[type]) CHK(dalloc(&c.disp_tab[type], total));
+  HIPCHK(hipMemcpy(c.disp_tab[type], h.data(), total * sizeof(double), hipMemcpyHostToDevice));
+  c.disp_beta[type] = beta;
+  // (1 - beta) scales the tangential dispersions of structure 1; a slope-dependent beta lies in [0.05, 0.65]
+  c.disp_bound[type] = bound * (1 + (beta < 1 ? fabs(beta) : 1.0));
+  return GALIC_OK;
+}
+
+int galic_sample_targets(galic_ctx *ctx, int type, long long nsamples, unsigned int seed, unsigned int stream,
+                         double stretch, double *dgs_mass_target, double *egs_targets, double *points,
+                         long long *nbinned, float *kernel_ms) {
+  if(!ctx || type < 1 || type > 3) return set_err(GALIC_ERR_ARG, "bad type %d", type);
+  galic::Ctx &c = ctx->c;
+  if(!dgs_mass_target && !egs_targets) return set_err(GALIC_ERR_ARG, "no target requested");
+  // int64 sums of up to 2^32 units per point stay below 2^62
+  if(nsamples < 1 || nsamples > (1ll << 30)) return set_err(GALIC_ERR_ARG, "nsamples must be in [1, 2^30]");
+  if(!(stretch > 0)) return set_err(GALIC_ERR_ARG, "stretch must be positive");
+  if(!(c.p.MType[type] > 0)) return set_err(GALIC_ERR_ARG, "type %d has no mass", type);
+  if(egs_targets && !c.disp_tab[type])
+    return set_err(GALIC_ERR_ARG, "the dispersion targets of type %d need galic_set_disp_tables first", type);
+  HIPCHK(hipSetDevice(c.device));
+  const int tovs = c.p.TypeOfVelocityStructure[type];
+  const size_t ng = c.Ngrid, nb = c.p.FG_Nbin, n2 = nb * nb;
+  long long *maps = nullptr;
+  unsigned long long *counts = nullptr;
+  CHK(dalloc(&maps, 6 * ng));
+  CHK(dalloc(&counts, 2));
+  galic::TargetArgs A;
+  memset(&A, 0, sizeof(A));
+  A.M = c.M;
+  A.type = type;
+  A.tovs = tovs;
+  A.n = nsamples;
+  A.seed = seed;
+  A.stream = stream;
+  A.Rmax = c.p.Rmax;
+  A.Disk_H = c.p.Disk_H;
+  A.Disk_Z0 = c.p.Disk_Z0;
+  A.stretch = stretch;
+  A.beta = c.disp_beta[type];
+  if(c.disp_tab[type]) {
+    A.disp_r = c.disp_tab[type];
+    A.disp_z = A.disp_r + nb;
+    A.disp_phi = A.disp_z + n2;
+    A.tilt_r = A.disp_phi + n2;
+    A.tilt_theta = A.tilt_r + n2;
+  }
+  A.dg = dgs_mass_target ? maps : nullptr;
+  A.eg = egs_targets ? maps + ng : nullptr;
+  A.qdg = ldexp(1.0, 32);
+  A.qeg[0] = ldexp(1.0, 32);
+  if(egs_targets) {
+    // every dispersion value stays below 2^e: the q field adds the squared streaming velocity
+    std::vector<double> v(n2);
+    HIPCHK(hipMemcpy(v.data(), c.FG_Vstream + (size_t)type * n2, n2 * sizeof(double), hipMemcpyDeviceToHost));
+    double vmax = 0;
+    for(double x : v) vmax = std::max(vmax, fabs(x));
+    int e = 0;
+    frexp(std::max(c.disp_bound[type] + vmax * vmax, 1e-300), &e);
+    for(int f = 1; f < 5; f++) A.qeg[f] = ldexp(1.0, 32 - e);
+  }
+  A.counts = counts;
+  double *dpts = nullptr;
+  if(points) {
+    CHK(dalloc(&dpts, 3 * (size_t)nsamples));
+    A.pos_out = dpts;
+  }
+  const int grid = orbit_grid((int)std::min<long long>(nsamples, 1 << 30), 256, 8);
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  if(galic::launch_sample_targets(A, grid, c.stream) != 0) {
+    dfree(maps);
+    dfree(counts);
+    dfree(dpts);
+    return set_err(GALIC_ERR_HIP, "target sampling launch failed");
+  }
+  HIPCHK(hipEventRecord(c.ev1, c.stream));
+  HIPCHK(hipEventSynchronize(c.ev1));
+  HIPCHK(hipEventElapsedTime(&c.last_target_ms, c.ev0, c.ev1));
+  std::vector<long long> hq(6 * ng);
+  unsigned long long hc[2];
+  HIPCHK(hipMemcpy(hq.data(), maps, 6 * ng * sizeof(long long), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hc, counts, sizeof(hc), hipMemcpyDeviceToHost));
+  if(points) HIPCHK(hipMemcpy(points, dpts, 3 * (size_t)nsamples * sizeof(double), hipMemcpyDeviceToHost));
+  dfree(maps);
+  dfree(counts);
+  dfree(dpts);
+  if(hc[1] != 0) return set_err(GALIC_ERR_ARG, "%llu samples found no point inside Rmax", hc[1]);
+  if(hc[0] == 0) return set_err(GALIC_ERR_ARG, "no sample fell inside the grid");
+  // normalisation to the component's mass (grid.c:603-611, 1051-1062), then the pyramid
+  const double norm = c.p.MType[type] / (double)hc[0];
+  const size_t off = GALIC_STACKOFFSET(c.L, 0, 0);
+  std::vector<double> st(c.Nstack);
+  auto finish = [&](const long long *q, double unit, double *out) {
+    std::fill(st.begin(), st.end(), 0.0);
+    for(size_t i = 0; i < ng; i++) st[off + i] = (double)q[i] / unit * norm;
+    smooth_stack_host(st.data(), c.L);
+    memcpy(out, st.data(), sizeof(double) * c.Nstack);
+  };
+  if(dgs_mass_target) finish(hq.data(), A.qdg, dgs_mass_target);
+  if(egs_targets)
+    for(int f = 0; f < 5; f++) finish(hq.data() + (1 + f) * ng, A.qeg[f], egs_targets + (size_t)f * c.Nstack);
+  if(nbinned) *nbinned = (long long)hc[0];
+  if(kernel_ms) *kernel_ms = c.last_target_ms;
   return GALIC_OK;
 }
 

@@ -152,6 +152,13 @@ struct Ctx {
   double log_to_loc[5] = {0, 0, 0, 0, 0}; // int64 sum units -> Loc fixed point
   double max_mass = 0, max_vesc = 0;  // of the particles, bound the log's fixed point
 
+  // target dispersion tables of the sampled targets, per type: FG_Disp_r [FG_Nbin], then FG_DispZ,
+  // FG_DispPhi, FG_tilted_vR2_prime, FG_tilted_vz2_prime [FG_Nbin^2 each] (disp_fields.c)
+  double *disp_tab[GALIC_NTYPES] = {nullptr, nullptr, nullptr, nullptr};
+  double disp_beta[GALIC_NTYPES] = {0, 0, 0, 0};   // HaloBetaParameter / BulgeBetaParameter
+  double disp_bound[GALIC_NTYPES] = {0, 0, 0, 0};  // bound of every dispersion the tables give
+  float last_target_ms = 0;                          // kernel time of the last target sampling
+
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_orbit_ms = 0, last_eval_ms = 0, last_fit_ms = 0;

@@ -477,6 +477,26 @@ def cell_np(x, y, z, Rmin, Fac, nb):
     return iR, iz, fR, fz
 
 
+def disp_tables(p, cfg, t):
+    """Target dispersion tables of type t on the force-grid nodes, as get_disp_rtp
+    (disp_fields.c:66-198) interpolates them: FG_Disp_r at radii R_j for the spherical
+    structures, FG_DispZ and FG_DispPhi at (R_j, z_k) for structure 2.
+    Returns (beta parameter, dict of tables for GalicGPU.set_disp_tables)."""
+    tovs = p.TypeOfVelocityStructure[t]
+    beta = cfg.halo_beta if t == 1 else 0.0
+    nodes = fg_nodes(p)
+    if tovs >= 2:
+        nb = p.FG_Nbin
+        R = nodes[None, :].repeat(nb, 0).ravel()   # [k][j]: z row k, R column j
+        z = nodes[:, None].repeat(nb, 1).ravel()
+        d, _ = disp_at(p, t, tovs, beta, R, np.zeros_like(R), z, {})
+        if tovs == 2:
+            return beta, dict(disp_z=d[:, 1].copy(), disp_phi=d[:, 2].copy())
+        return beta, dict(tilt_r=d[:, 0].copy(), tilt_theta=d[:, 1].copy(), disp_phi=d[:, 2].copy())
+    d, _ = disp_at(p, t, tovs, beta, nodes, np.zeros_like(nodes), np.zeros_like(nodes), {})
+    return beta, dict(disp_r=d[:, 0].copy())
+
+
 def vstream_table(p, t, cfg):
     if t != 2 or p.TypeOfVelocityStructure[2] < 2:
         return np.zeros(p.FG_Nbin * p.FG_Nbin)

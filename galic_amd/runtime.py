@@ -37,6 +37,30 @@ class GalicGPU:
         disp = np.ascontiguousarray(disp, dtype=np.float64)
         check(self.lib.galic_set_dispersions(self.h, len(disp), dptr(disp)))
 
+    def set_disp_tables(self, t, beta=0.0, disp_r=None, disp_z=None, disp_phi=None, tilt_r=None, tilt_theta=None):
+        """Target dispersion tables of type t as get_disp_rtp reads them (disp_fields.c:66-198)."""
+        tabs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+                for a in (disp_r, disp_z, disp_phi, tilt_r, tilt_theta)]
+        check(self.lib.galic_set_disp_tables(self.h, t, float(beta), *[dptr(a) for a in tabs]))
+
+    def sample_targets(self, t, nsamples, seed=1, stream=None, stretch=1.0, density=True, energy=True, points=False):
+        """Sampled DGs_MassTarget and EGs targets of type t (grid.c:536-617, 941-1076).
+
+        Returns (dgs_mass_target or None, [5, Nstack] energy targets or None, points binned, kernel ms), and the
+        [nsamples, 3] sample points as a fifth item if points is set."""
+        dg = np.zeros(self.nstack) if density else None
+        eg = np.zeros((5, self.nstack)) if energy else None
+        pts = np.zeros((int(nsamples), 3)) if points else None
+        nb = np.zeros(1, dtype=np.int64)
+        ms = ctypes.c_float(0)
+        stream = 100 + t if stream is None else stream
+        check(self.lib.galic_sample_targets(self.h, t, int(nsamples), int(seed) & 0xFFFFFFFF, int(stream) & 0xFFFFFFFF,
+                                            float(stretch), dptr(dg), dptr(eg), dptr(pts), lptr(nb),
+                                            ctypes.byref(ms)))
+        if points:
+            return dg, eg, int(nb[0]), float(ms.value), pts
+        return dg, eg, int(nb[0]), float(ms.value)
+
     def set_particles(self, pos, vel):
         w = self.w
         pos = np.ascontiguousarray(pos, dtype=np.float64)

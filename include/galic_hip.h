@@ -61,6 +61,32 @@ int galic_set_vstream(galic_ctx *ctx, int type, const double *vstream);
  * EGs_EgyTarget_{r,t,p,q} (grid.c:830-877, 941-1076). */
 int galic_set_targets(galic_ctx *ctx, int type, const double *dgs_mass_target, const double *egs_mass_target,
                       const double *egs_r, const double *egs_t, const double *egs_p, const double *egs_q);
+/* Target dispersion tables of one component, as get_disp_rtp reads them
+ * (disp_fields.c:66-115): FG_Disp_r [FG_Nbin] for velocity structures 0 and 1
+ * (disp_fields.c:375-398), FG_DispZ [FG_Nbin^2] for structure 2, FG_DispPhi
+ * [FG_Nbin^2] for structures 2 and 3, FG_tilted_vR2_prime and
+ * FG_tilted_vz2_prime [FG_Nbin^2] for structure 3 (disp_fields.c:134-198);
+ * NULL where the structure does not use the table.  beta is
+ * HaloBetaParameter / BulgeBetaParameter (>= 1: the slope-dependent beta of
+ * disp_fields.c:26-64).  Copied to the device. */
+int galic_set_disp_tables(galic_ctx *ctx, int type, double beta, const double *disp_r, const double *disp_z,
+                          const double *disp_phi, const double *tilt_r, const double *tilt_theta);
+/* Sampled target maps of one component, computed on the device:
+ * densitygrid_sample_targetresponse (grid.c:536-617) into dgs_mass_target
+ * [Nstack] and calc_energy_grid_mass_maps (grid.c:941-1076) into egs_targets
+ * [5][Nstack] (EGs_MassTarget, EGs_EgyTarget_r, _t, _p, _q); either may be NULL.
+ * Points come from halo_/disk_/bulge_get_fresh_coordinate (halo.c:17-40,
+ * disk.c:17-51, bulge.c:17-38) driven by Philox draws of (sample id, round) under
+ * the key (seed, stream) instead of one GSL stream; nsamples ids are drawn (at
+ * most 2^30), those with sqrt(x^2+z^2) < Rmax are binned and counted
+ * (*nbinned), and the maps are normalised by MType/count as the reference
+ * normalises by its count.  stretch is HaloStretch / BulgeStretch.  The results
+ * are returned, not installed: galic_set_targets installs them.  points
+ * (optional, [nsamples][3]) receives the sample points (zeros for an id whose
+ * draws never fell inside Rmax); kernel_ms (optional) the sampling kernel's time. */
+int galic_sample_targets(galic_ctx *ctx, int type, long long nsamples, unsigned int seed, unsigned int stream,
+                         double stretch, double *dgs_mass_target, double *egs_targets, double *points,
+                         long long *nbinned, float *kernel_ms);
 /* Particle state P[] (allvars.h:844-868): positions, velocities, mass, Tint,
  * type and Vesc.  Copied to device SoA arrays. */
 int galic_set_particles(galic_ctx *ctx, int n, const double *pos, const double *vel, const double *mass,

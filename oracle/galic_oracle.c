@@ -54,6 +54,12 @@ struct oracle_model {
   double *pos, *vel, *mass, *tint, *vesc;
   int *type, *orbits;
   double *vr2, *vt2, *vp2, *vq2;
+
+  /* target dispersion tables (disp_fields.c): FG_Disp_r [FG_Nbin], FG_DispZ, FG_DispPhi,
+   * FG_tilted_vR2_prime, FG_tilted_vz2_prime [FG_Nbin^2]; beta per type */
+  double *Disp_r[GALIC_NTYPES], *DispZ[GALIC_NTYPES], *DispPhi[GALIC_NTYPES];
+  double *TiltR[GALIC_NTYPES], *TiltTheta[GALIC_NTYPES];
+  double beta[GALIC_NTYPES];
 };
 
 /* ------------------------------------------------------------------------- */
@@ -228,6 +234,7 @@ void or_model_free(oracle_model *m)
     free(m->EG_RLoc_d[t]); free(m->EG_TLoc_d[t]); free(m->EG_PLoc_d[t]); free(m->EG_QLoc_d[t]);
     free(m->EG_RLocS[t]); free(m->EG_TLocS[t]); free(m->EG_QLocS[t]); free(m->EG_PLocS[t]);
     free(m->EG_RLocS_d[t]); free(m->EG_TLocS_d[t]); free(m->EG_QLocS_d[t]); free(m->EG_PLocS_d[t]);
+    free(m->Disp_r[t]); free(m->DispZ[t]); free(m->DispPhi[t]); free(m->TiltR[t]); free(m->TiltTheta[t]);
   }
   free(m->pos); free(m->vel); free(m->mass); free(m->tint); free(m->vesc);
   free(m->type); free(m->orbits);
@@ -1670,3 +1677,213 @@ long long or_table_fill(const galic_params *p, int nsrc, const double *spos, con
   free(pot);
   return nint;
 }
+
+/* ------------------------------------------------------------------------- */
+/* sampled targets                                                           */
+/* ------------------------------------------------------------------------- */
+
+/* One round of halo_get_fresh_coordinate (halo.c:17-40), disk_get_fresh_coordinate
+ * (disk.c:17-51) or bulge_get_fresh_coordinate (bulge.c:17-38) with the round's
+ * uniforms u[0..2] taken in the order the reference draws them.  Returns 1 if the
+ * reference would keep the point, 0 if it would draw again.  The halo and the bulge
+ * are spherical here (the stretch divides z as in the reference). */
+int or_sample_point(const galic_params *p, int type, double stretch, const double *u, double *pos)
+{
+  if(type == 2) {
+    double q, f, f_, R, Rold, phi;
+    pos[2] = p->Disk_Z0 / 2 * log(u[0] / (1 - u[0]));
+    q = u[1];
+    R = 1.0;
+    do {
+      f = (1 + R) * exp(-R) + q - 1;
+      f_ = -R * exp(-R);
+      Rold = R;
+      R = R - f / f_;
+    } while(fabs(R - Rold) / R > 1e-7);
+    R *= p->Disk_H;
+    phi = u[2] * M_PI * 2;
+    pos[0] = R * cos(phi);
+    pos[1] = R * sin(phi);
+    return !(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2] > p->Rmax * p->Rmax);
+  } else {
+    double a = type == 1 ? p->Halo_A : p->Bulge_A;
+    double q = u[0], r, phi, theta;
+    r = q > 0 ? a * (q + sqrt(q)) / (1 - q) : 0;
+    if(type == 3 && r > p->Rmax)
+      return 0;
+    phi = u[1] * M_PI * 2;
+    theta = acos(u[2] * 2 - 1);
+    pos[0] = r * sin(theta) * cos(phi);
+    pos[1] = r * sin(theta) * sin(phi);
+    pos[2] = r * cos(theta) / stretch;
+    if(type == 3)
+      return 1;
+    r = sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+    return !(r > p->Rmax);
+  }
+}
+
+static double *copy_or_null(const double *src, size_t n)
+{
+  double *d;
+  if(!src)
+    return NULL;
+  d = (double *)malloc(n * sizeof(double));
+  memcpy(d, src, n * sizeof(double));
+  return d;
+}
+
+void or_set_disp_tables(oracle_model *m, int type, double beta, const double *disp_r, const double *disp_z,
+                        const double *disp_phi, const double *tilt_r, const double *tilt_theta)
+{
+  size_t nb = (size_t)m->p.FG_Nbin, n2 = nb * nb;
+  free(m->Disp_r[type]); free(m->DispZ[type]); free(m->DispPhi[type]); free(m->TiltR[type]); free(m->TiltTheta[type]);
+  m->Disp_r[type] = copy_or_null(disp_r, nb);
+  m->DispZ[type] = copy_or_null(disp_z, n2);
+  m->DispPhi[type] = copy_or_null(disp_phi, n2);
+  m->TiltR[type] = copy_or_null(tilt_r, n2);
+  m->TiltTheta[type] = copy_or_null(tilt_theta, n2);
+  m->beta[type] = beta;
+}
+
+/* bilinear value of a force-grid table, as get_z_disp_cylindrical (disp_fields.c:134-147) */
+static double fg_table(const oracle_model *m, const double *T, const double *pos)
+{
+  int iz, iR, nb = m->p.FG_Nbin;
+  double fR, fz;
+  or_forcegrid_get_cell(m, pos, &iR, &iz, &fR, &fz);
+  return T[iz * nb + iR] * (1 - fR) * (1 - fz) + T[(iz + 1) * nb + iR] * (1 - fR) * (fz) +
+         T[iz * nb + (iR + 1)] * (fR) * (1 - fz) + T[(iz + 1) * nb + (iR + 1)] * (fR) * (fz);
+}
+
+/* disp_fields.c:375-398 */
+static double radial_disp_spherical(const oracle_model *m, const double *pos, int type)
+{
+  double r = sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+  double br = (log(r / m->p.FG_Rmin + 1.0) / log(m->p.FG_Fac));
+  int binr;
+  double fR;
+  if(br < 0)
+    br = 0;
+  binr = (int)br;
+  fR = br - binr;
+  if(binr >= m->p.FG_Nbin - 1) {
+    binr = m->p.FG_Nbin - 2;
+    fR = 1;
+  }
+  return m->Disp_r[type][binr] * (1 - fR) + m->Disp_r[type][binr + 1] * (fR);
+}
+
+/* disp_fields.c:26-64 */
+static double beta_of_type(const oracle_model *m, const double *pos, int type)
+{
+  double beta = 0;
+  if(type == 1 || type == 3) {
+    beta = m->beta[type];
+    if(beta >= 1) {
+      double a = type == 1 ? m->p.Halo_A : m->p.Bulge_A;
+      double r = sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+      double dlogrhodr = -1.0 - 3.0 * r / (r + a);
+      beta = -0.15 - 0.20 * dlogrhodr;
+    }
+  }
+  return beta;
+}
+
+/* disp_fields.c:66-115 */
+void or_get_disp_rtp(const oracle_model *m, const double *pos, int type, double *dr, double *dt, double *dp,
+                     double *dq)
+{
+  int tovs = m->p.TypeOfVelocityStructure[type];
+  if(tovs == 0) {
+    *dr = radial_disp_spherical(m, pos, type);
+    *dt = *dr;
+    *dp = *dr;
+    *dq = *dr;
+  } else if(tovs == 1) {
+    *dr = radial_disp_spherical(m, pos, type);
+    *dt = (1 - beta_of_type(m, pos, type)) * (*dr);
+    *dp = *dt;
+    *dq = *dt;
+  } else {
+    double vstr;
+    if(tovs == 2) {
+      *dt = fg_table(m, m->DispZ[type], pos);
+      *dr = *dt;
+    } else {
+      *dr = fg_table(m, m->TiltR[type], pos);
+      *dt = fg_table(m, m->TiltTheta[type], pos);
+    }
+    *dp = fg_table(m, m->DispPhi[type], pos);
+    vstr = or_get_vstream(m, pos, type);
+    *dq = (*dp) + vstr * vstr;
+  }
+}
+
+/* The binning half of densitygrid_sample_targetresponse (grid.c:536-617) and
+ * calc_energy_grid_mass_maps (grid.c:941-1076) for given points: points with
+ * sqrt(x^2 + z^2) < Rmax are binned and counted, the maps are scaled by
+ * MType/count and smoothed.  dg [DG_Nstack] and eg [5][EG_Nstack] may be NULL.
+ * Returns the count. */
+long long or_bin_targets(const oracle_model *m, int type, long long n, const double *pos, double *dg, double *eg)
+{
+  long long i, count = 0;
+  int k, nb = m->DG_Nbin, ne = m->EG_Nbin;
+  double *mfield = (double *)calloc((size_t)m->DG_Ngrid, sizeof(double));
+  double *efield[5];
+  for(k = 0; k < 5; k++)
+    efield[k] = (double *)calloc((size_t)m->EG_Ngrid, sizeof(double));
+  for(i = 0; i < n; i++) {
+    const double *x = pos + 3 * i;
+    double r = sqrt(x[0] * x[0] + x[2] * x[2]);
+    int iR, iz;
+    double fR, fz;
+    if(!(r < m->p.Rmax))
+      continue;
+    if(dg) {
+      or_densitygrid_get_cell(m, x, &iR, &iz, &fR, &fz);
+      mfield[iz * nb + iR] += (1 - fR) * (1 - fz);
+      mfield[iz * nb + (iR + 1)] += (fR) * (1 - fz);
+      mfield[(iz + 1) * nb + iR] += (1 - fR) * (fz);
+      mfield[(iz + 1) * nb + (iR + 1)] += (fR) * (fz);
+    }
+    if(eg) {
+      double d[4];
+      or_energygrid_get_cell(m, x, &iR, &iz, &fR, &fz);
+      or_get_disp_rtp(m, x, type, &d[0], &d[1], &d[2], &d[3]);
+      efield[0][iz * ne + iR] += (1 - fR) * (1 - fz);
+      efield[0][iz * ne + (iR + 1)] += (fR) * (1 - fz);
+      efield[0][(iz + 1) * ne + iR] += (1 - fR) * (fz);
+      efield[0][(iz + 1) * ne + (iR + 1)] += (fR) * (fz);
+      for(k = 0; k < 4; k++) {
+        efield[1 + k][iz * ne + iR] += (1 - fR) * (1 - fz) * d[k];
+        efield[1 + k][iz * ne + (iR + 1)] += (fR) * (1 - fz) * d[k];
+        efield[1 + k][(iz + 1) * ne + iR] += (1 - fR) * (fz) * d[k];
+        efield[1 + k][(iz + 1) * ne + (iR + 1)] += (fR) * (fz) * d[k];
+      }
+    }
+    count++;
+  }
+  if(dg) {
+    int off = GALIC_STACKOFFSET(m->p.DG_MaxLevel, 0, 0);
+    memset(dg, 0, sizeof(double) * (size_t)m->DG_Nstack);
+    for(i = 0; i < m->DG_Ngrid; i++)
+      dg[off + i] = mfield[i] * (m->p.MType[type] / (double)count);
+    or_smooth_stack(dg, m->p.DG_MaxLevel);
+  }
+  if(eg) {
+    int off = GALIC_STACKOFFSET(m->p.EG_MaxLevel, 0, 0);
+    for(k = 0; k < 5; k++) {
+      double *st = eg + (size_t)k * m->EG_Nstack;
+      memset(st, 0, sizeof(double) * (size_t)m->EG_Nstack);
+      for(i = 0; i < m->EG_Ngrid; i++)
+        st[off + i] = efield[k][i] * (m->p.MType[type] / (double)count);
+      or_smooth_stack(st, m->p.EG_MaxLevel);
+    }
+  }
+  free(mfield);
+  for(k = 0; k < 5; k++)
+    free(efield[k]);
+  return count;
+}
+

@@ -107,6 +107,20 @@ void or_rep(oracle_model *m, int ntrial, const int *idx, const double *vtry, int
 void or_eval_trials(oracle_model *m, int ntrial, const int *idx, const double *vtry, double *fbase, double *ftry,
                     int nthreads);
 
+/* ---- sampled targets (grid.c:536-617, 941-1076; halo.c:17-40; disk.c:17-51; bulge.c:17-38;
+ *      disp_fields.c:26-198, 375-398) ---- */
+/* One round of a component's fresh-coordinate draw with uniforms u[0..2] in the
+ * reference's order; returns 1 if the point is kept, 0 if the reference draws again. */
+int    or_sample_point(const galic_params *p, int type, double stretch, const double *u, double *pos);
+/* Target dispersion tables of a type (copied; NULL where unused) and its beta parameter. */
+void   or_set_disp_tables(oracle_model *m, int type, double beta, const double *disp_r, const double *disp_z,
+                          const double *disp_phi, const double *tilt_r, const double *tilt_theta);
+void   or_get_disp_rtp(const oracle_model *m, const double *pos, int type, double *dr, double *dt, double *dp,
+                       double *dq);
+/* Bins n points into DGs_MassTarget (dg, [DG_Nstack]) and the EGs targets (eg, [5][EG_Nstack]:
+ * mass, r, t, p, q), either may be NULL; normalised by MType/count and smoothed.  Returns the count. */
+long long or_bin_targets(const oracle_model *m, int type, long long n, const double *pos, double *dg, double *eg);
+
 /* ---- Barnes-Hut tree (forcetree.c:139-722, forcetree_walk.c:12-222, gravtree.c:455-463) ---- */
 /* Accelerations and potentials (already multiplied by G) at ntarget points due to nsrc sources.
  * The domain cube is the bounding cube of sources and targets (domain_box.c:20-65).

@@ -76,6 +76,10 @@ def lib():
                                                ctypes.c_int]),
         "or_direct_forces": (None, [pp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
         "or_table_fill": (ctypes.c_longlong, [pp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]),
+        "or_sample_point": (ctypes.c_int, [pp, ctypes.c_int, ctypes.c_double, _dp, _dp]),
+        "or_set_disp_tables": (None, [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]),
+        "or_get_disp_rtp": (None, [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+        "or_bin_targets": (ctypes.c_longlong, [_vp, ctypes.c_int, ctypes.c_longlong, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -182,6 +186,24 @@ class Oracle:
         lib().or_get_particles(self.h, d(vel), orb.ctypes.data_as(_ip), *[d(x) for x in v])
         return vel, orb, np.stack(v, -1)
 
+    def set_disp_tables(self, t, beta=0.0, disp_r=None, disp_z=None, disp_phi=None, tilt_r=None, tilt_theta=None):
+        tabs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+                for a in (disp_r, disp_z, disp_phi, tilt_r, tilt_theta)]
+        lib().or_set_disp_tables(self.h, t, float(beta), *[d(a) for a in tabs])
+
+    def disp_rtp(self, pos, t):
+        out = [ctypes.c_double(0) for _ in range(4)]
+        lib().or_get_disp_rtp(self.h, d(np.asarray(pos, float)), int(t), *[ctypes.byref(o) for o in out])
+        return np.array([o.value for o in out])
+
+    def bin_targets(self, t, pos, density=True, energy=True):
+        """DGs_MassTarget [Nstack] and the EGs targets [5, Nstack] of the given points, and the count binned."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        dg = np.zeros(self.nstack) if density else None
+        eg = np.zeros((5, self.nstack)) if energy else None
+        n = lib().or_bin_targets(self.h, int(t), len(pos), d(pos), d(dg), d(eg))
+        return dg, eg, int(n)
+
     def cellsize(self):
         return np.ctypeslib.as_array(lib().or_cellsize(self.h), shape=(self.ngrid,)).copy()
 
@@ -195,3 +217,26 @@ class Oracle:
 
     def potential(self, pos):
         return lib().or_forcegrid_get_potential(self.h, d(np.asarray(pos, float)))
+
+
+def sample_points(params, t, ids, uniforms, seed, stream, stretch=1.0, max_attempts=1000):
+    """Points of a component's fresh-coordinate draw (halo.c:17-40, disk.c:17-51, bulge.c:17-38) for the
+    given sample ids: round a of id i uses uniforms(seed, stream, [i], a)[0:3] in the reference's order,
+    until the reference would keep the point.  Returns [len(ids), 3] (rows of zeros if no round kept one)."""
+    L = lib()
+    ids = np.asarray(ids, dtype=np.uint64)
+    out = np.zeros((len(ids), 3))
+    todo = np.arange(len(ids))
+    pos = np.zeros(3)
+    for attempt in range(max_attempts):
+        if len(todo) == 0:
+            break
+        u = np.ascontiguousarray(uniforms(seed, stream, ids[todo], attempt).T)
+        keep = np.zeros(len(todo), dtype=bool)
+        for k in range(len(todo)):
+            if L.or_sample_point(ctypes.byref(params), int(t), float(stretch), d(u[k]), d(pos)):
+                out[todo[k]] = pos
+                keep[k] = True
+        todo = todo[~keep]
+    return out
+
