@@ -138,6 +138,52 @@ def tree_leg(n, nbin, ncheck):
     return out
 
 
+def target_leg(cfg, n, ncpu):
+    """Sampled targets of the bench model (grid.c:536-617, 941-1076), as the reference builds them:
+    n fresh points per component for the density map and another n for the five energy-grid maps.
+    CPU baseline: the same kind of points drawn by the host sampler and binned by the oracle
+    (one thread), ncpu points per map."""
+    w = model.make_workload(cfg, n=2000, target_samples=20000)   # parameters and tables only
+    p = w.params
+    g = GalicGPU(w)
+    per_type = {}
+    for t in w.types:
+        beta, tabs = model.disp_tables(p, cfg, t)
+        g.set_disp_tables(t, beta, **tabs)
+        g.sample_targets(t, 1 << 20, seed=2)   # warm-up
+        t0 = time.perf_counter()
+        _, _, nd, ms_d = g.sample_targets(t, n, seed=1, stream=300 + t, energy=False)
+        _, _, ne, ms_e = g.sample_targets(t, n, seed=1, stream=400 + t, density=False)
+        wall = time.perf_counter() - t0
+        per_type[t] = dict(points=2 * n, binned=nd + ne, density_kernel_ms=ms_d, energy_kernel_ms=ms_e,
+                           wall_ms=wall * 1e3, points_per_s=2 * n / ((ms_d + ms_e) * 1e-3))
+        log(f"[bench] targets type {t}: 2 x {n:.3g} points, kernels {ms_d:.1f} + {ms_e:.1f} ms, wall {wall * 1e3:.0f} ms")
+    g.close()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker, used here only as the CPU baseline
+    o = oracle.Oracle(w, set_particles=False)
+    cpu_s = 0.0
+    for t in w.types:
+        beta, tabs = model.disp_tables(p, cfg, t)
+        o.set_disp_tables(t, beta, **tabs)
+        ids = np.arange(ncpu, dtype=np.uint64)
+        t0 = time.perf_counter()
+        pos = model.sample_type(p, t, 1, 300 + t, ids)
+        o.bin_targets(t, pos, energy=False)
+        pos = model.sample_type(p, t, 1, 400 + t, ids)
+        o.bin_targets(t, pos, density=False)
+        cpu_s += time.perf_counter() - t0
+    o.close()
+    cpu_rate = 2 * ncpu * len(w.types) / cpu_s
+    gpu_rate = sum(v["points"] for v in per_type.values()) / sum(
+        (v["density_kernel_ms"] + v["energy_kernel_ms"]) * 1e-3 for v in per_type.values())
+    log(f"[bench] targets: GPU {gpu_rate:.4g} points/s, CPU {cpu_rate:.4g} points/s (1 thread)")
+    return dict(points_per_map=n, per_type=per_type, points_per_s=gpu_rate,
+                cpu=dict(points_per_s=cpu_rate, threads=1, kind="port",
+                         sample=f"{ncpu} points per map and type: host sampler (numpy) + oracle binning"),
+                gpu_over_cpu=gpu_rate / cpu_rate)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +203,8 @@ def main():
                     help="orbits integrated by both paths for the FP32 error figures (0: skip)")
     ap.add_argument("--tree-n", type=int, default=10000000, help="sources of the table-fill leg (0: skip)")
     ap.add_argument("--tree-nbin", type=int, default=512)
+    ap.add_argument("--target-points", type=int, default=100000000,
+                    help="points per map and component of the sampled-target leg (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -308,6 +356,8 @@ def main():
 
     # ------------------------------------------------------ table fill
     tree = tree_leg(args.tree_n, args.tree_nbin, 4096) if args.tree_n > 0 else None
+    # ------------------------------------------------- sampled targets
+    targets = target_leg(cfg, args.target_points, 1000000) if args.target_points > 0 else None
 
     # ---------------------------------------------------- HBM calibration
     copy_ms = np.zeros(1)
@@ -352,6 +402,7 @@ def main():
         "fp32_orbit_precision": precision,
         "cpu_baseline": cpu,
         "tree_fill": tree,
+        "target_fill": targets,
         "copy_probe_GBs": copy_gbs,
     }
     print(json.dumps(line), flush=True)
