@@ -56,6 +56,7 @@ EXPORTS = [
     "galic_last_error", "galic_version", "galic_grid_create", "galic_density_geometry",
     "galic_create", "galic_destroy", "galic_set_force_tables", "galic_set_vstream", "galic_set_targets",
     "galic_set_disp_tables", "galic_sample_targets",
+    "galic_write_fit_line", "galic_write_densfield", "galic_write_snapshot",
     "galic_set_particles", "galic_set_dispersions", "galic_get_particles", "galic_calc_all_response_fields", "galic_rep",
     "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_fp32_orbit_responses", "galic_field_at",
     "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_fp32_set_fresh_trials", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_tree_last_timings",
@@ -101,6 +102,10 @@ def lib():
         "galic_set_vstream": [_vp, ctypes.c_int, _dp],
         "galic_set_targets": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp],
         "galic_set_disp_tables": [_vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp],
+        "galic_write_fit_line": [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_double, ctypes.c_longlong,
+                                 ctypes.c_longlong],
+        "galic_write_densfield": [_vp, ctypes.c_int, ctypes.c_char_p],
+        "galic_write_snapshot": [_vp, ctypes.c_char_p, ctypes.c_int, _dp],
         "galic_sample_targets": [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_uint, ctypes.c_uint, ctypes.c_double,
                                  _dp, _dp, _dp, _lp, _P(ctypes.c_float)],
         "galic_set_particles": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _dp],

@@ -28,6 +28,7 @@ UNITS = {
     "galic_fast.hip": ["-ffp-contract=fast"],
     "galic_tree.hip": ["-ffp-contract=off"],
     "galic_targets.hip": ["-ffp-contract=off"],
+    "galic_io.hip": ["-ffp-contract=off"],
 }
 
 

@@ -17,17 +17,7 @@
 #include "galic_fast.h"
 #include "galic_targets.h"
 #include "galic_kernels.h"
-
-struct galic_ctx {
-  galic::Ctx c;
-  // host copies of the targets and of the geometry, used to build the cuts
-  std::vector<double> targets[GALIC_NTYPES][6];
-  std::vector<double> distance, cellsize;
-  std::vector<double> weights;  // [type][W_COUNT][Nstack]
-  std::vector<int> cutB_of;     // [type][Ngrid]
-  bool initial_done = false;
-  bool fp32_initial_done = false;
-};
+#include "galic_handle.h"
 
 namespace {
 
@@ -1150,10 +1140,11 @@ int galic_rep(galic_ctx *ctx, int ntrial, const int *idx, const double *vtry, in
 }
 
 int galic_get_stack(galic_ctx *ctx, int type, int which, double *out) {
-  if(!ctx || type < 1 || type > 3 || which < 0 || which > 10 || !out) return set_err(GALIC_ERR_ARG, "bad arguments");
+  if(!ctx || type < 1 || type > 3 || which < 0 || which > 15 || !out) return set_err(GALIC_ERR_ARG, "bad arguments");
   galic::Ctx &c = ctx->c;
-  const int map[11] = {galic::ST_RM,  galic::ST_EMR, galic::ST_IR,  galic::ST_IT,  galic::ST_IP, galic::ST_IQ,
-                       galic::ST_RSR, galic::ST_RST, galic::ST_RSQ, galic::ST_RSP, galic::ST_MT};
+  const int map[16] = {galic::ST_RM,  galic::ST_EMR, galic::ST_IR,  galic::ST_IT,  galic::ST_IP,  galic::ST_IQ,
+                       galic::ST_RSR, galic::ST_RST, galic::ST_RSQ, galic::ST_RSP, galic::ST_MT,  galic::ST_EMT,
+                       galic::ST_ER,  galic::ST_ET,  galic::ST_EP,  galic::ST_EQ};
   HIPCHK(hipSetDevice(c.device));
   HIPCHK(hipStreamSynchronize(c.stream));
   HIPCHK(hipMemcpy(out, c.stacks + ((size_t)type * galic::ST_COUNT + map[which]) * c.Nstack,

@@ -6,6 +6,7 @@ independent trials followed by a commit and a global fit.  All arithmetic runs
 in libgalic_hip.so on the GPU; this module only moves arrays across the ABI.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -36,6 +37,19 @@ class GalicGPU:
     def set_dispersions(self, disp):
         disp = np.ascontiguousarray(disp, dtype=np.float64)
         check(self.lib.galic_set_dispersions(self.h, len(disp), dptr(disp)))
+
+    def write_fit_line(self, t, path, opt_frac, tries, changes):
+        """Appends one fit_<type>.txt line (main.c:1221-1288)."""
+        check(self.lib.galic_write_fit_line(self.h, t, os.fsencode(path), float(opt_frac), int(tries), int(changes)))
+
+    def write_densfield(self, t, path):
+        """densfield_<type>_<num>.dat (io.c:36-457)."""
+        check(self.lib.galic_write_densfield(self.h, t, os.fsencode(path)))
+
+    def write_snapshot(self, path, double_precision=False, vel_theo=None):
+        """One-file Gadget format-1 snapshot (io.c:460-1240)."""
+        vt = None if vel_theo is None else np.ascontiguousarray(vel_theo, dtype=np.float64)
+        check(self.lib.galic_write_snapshot(self.h, os.fsencode(path), int(bool(double_precision)), dptr(vt)))
 
     def set_disp_tables(self, t, beta=0.0, disp_r=None, disp_z=None, disp_phi=None, tilt_r=None, tilt_theta=None):
         """Target dispersion tables of type t as get_disp_rtp reads them (disp_fields.c:66-198)."""

@@ -112,12 +112,34 @@ int galic_calc_all_response_fields(galic_ctx *ctx);
 int galic_rep(galic_ctx *ctx, int ntrial, const int *idx, const double *vtry, int *accept, double *dF);
 
 /* Stacks: which = 0 DGs_MassResponse, 1 EGs_MassResponse, 2..5 EGs_EgyResponse_{r,t,p,q},
- * 6..9 EGs_EgyResponse{R,T,Q,P}S, 10 DGs_MassTarget.  out has Nstack doubles. */
+ * 6..9 EGs_EgyResponse{R,T,Q,P}S, 10 DGs_MassTarget, 11 EGs_MassTarget,
+ * 12..15 EGs_EgyTarget_{r,t,p,q}.  out has Nstack doubles. */
 int galic_get_stack(galic_ctx *ctx, int type, int which, double *out);
 
 /* calc_global_fit results of one type: out[0..7] = S, Sdisp_r, Sdisp_t, Sdisp_p,
  * Sdisp_q, Srelfac, Srelfac_count, Sall (main.c:294-359, 1270). */
 int galic_get_fit(galic_ctx *ctx, int type, double *out8);
+
+/* ---- the reference's output files, from the current state ---- */
+/* Appends one line of fit_<type>.txt as log_message writes it (main.c:1221-1288):
+ * opt-frac, S, vavg, Sdisp_r, Sdisp_t, Sdisp_p, Sdisp_q, Sall, Chg-Frac.  opt_frac
+ * is the caller's Noptimized / TotNumPart, tries and changes its counts of the type
+ * since the last line; vavg is the rms speed of the type's particles.  Writes
+ * nothing for a type without particles, as the reference. */
+int galic_write_fit_line(galic_ctx *ctx, int type, const char *path, double opt_frac, long long tries,
+                         long long changes);
+/* densfield_<type>_<num>.dat as output_density_field writes it (io.c:36-457, the
+ * build without VER_1_1_GNUPLOT_LOG): the DGs_MassTarget stack level by level,
+ * calc_smoothed_stack maps (grid.c:1217-1244) of the targets and responses, the
+ * fine DGs_MassResponse, the cell-centre coordinates of both grids. */
+int galic_write_densfield(galic_ctx *ctx, int type, const char *path);
+/* One-file Gadget format-1 snapshot as output_particles/write_file write it
+ * (io.c:460-1240, SnapFormat 1, NumFilesPerSnapshot 1): header, then the POS,
+ * VEL, ID (1..N in the particles' order, set_particles.c:65), MASS and VELTHEO
+ * blocks, particles grouped by type.  Single precision unless double_precision
+ * (OUTPUT_IN_DOUBLEPRECISION); vel_theo (3 per particle, the comparison
+ * velocities of set_particles.c:94-109) may be NULL for zeros. */
+int galic_write_snapshot(galic_ctx *ctx, const char *path, int double_precision, const double *vel_theo);
 
 /* Orbit responses of many independent orbits (produce_orbit_response_field_mod,
  * orbit_response.c:155-491, once per orbit): orbit o starts at pos[3*o..] with
