@@ -33,7 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from galic_amd import _abi, model  # noqa: E402
-from galic_amd.runtime import GalicGPU, direct_forces, tree_fill, tree_forces, tree_last_timings  # noqa: E402
+from galic_amd.runtime import (TREE_FP32, TREE_PLUMMER, TREE_QUADRUPOLE, GalicGPU, direct_forces,  # noqa: E402
+                               tree_fill, tree_forces, tree_last_timings)
 
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
 # HBM bytes per leapfrog step of k_orbit32_log in the iteration's launches, from the FETCH_SIZE and
@@ -102,18 +103,16 @@ def tree_sources(p, n, seed):
 
 
 def tree_leg(n, nbin, ncheck):
-    """Force-field fill: Barnes-Hut table fill over n composite sources (seed 3),
-    theta = 0.4, FG_Nbin^2 x 8 sections of targets, FP32 walk with FP64 moments;
-    force error against FP64 direct summation on ncheck random table nodes."""
+    """Force-field fill: Barnes-Hut table fill over n composite sources (seed 3), theta = 0.4,
+    FG_Nbin^2 x 8 sections of targets.  Two walks: the force-field configuration (FP32 walk,
+    FP64 quadrupole moments, Plummer softening 0.05) and the reference's kernel (monopole,
+    spline softening) in FP32; force errors against FP64 direct summation with the same
+    softening on ncheck random table nodes."""
     cfg = model.config("composite-4e6", fg_nbin=nbin, theta=0.4)
     p = model.make_params(cfg)
     t0 = time.perf_counter()
     spos, smass = tree_sources(p, n, 3)
     log(f"[bench] tree: {n} sources sampled in {time.perf_counter() - t0:.1f}s")
-    t0 = time.perf_counter()
-    tree_fill(p, spos, smass, precision=1)
-    wall = time.perf_counter() - t0
-    tm = tree_last_timings()
     ntarget = 8 * nbin * nbin
     # error check on random table nodes (section, z row, R column)
     rng = np.random.default_rng(3)
@@ -122,19 +121,25 @@ def tree_leg(n, nbin, ncheck):
     z1 = p.FG_Rmin * (p.FG_Fac ** k - 1.0)
     phi = 2 * np.pi / 8 * s
     tpos = np.stack([r1 * np.cos(phi), r1 * np.sin(phi), z1], axis=1)
-    acc32, pot32, _ = tree_forces(p, spos, smass, tpos, precision=1)
-    accd, potd = direct_forces(p, spos, smass, tpos)
-    err = np.linalg.norm(acc32 - accd, axis=1) / np.linalg.norm(accd, axis=1)
-    perr = np.abs(pot32 / potd - 1)
-    out = dict(sources=n, targets=ntarget, fg_nbin=nbin, theta=0.4, wall_ms=wall * 1e3, build_ms=tm["build_ms"],
-               walk_ms=tm["walk_ms"], interactions=tm["interactions"],
-               interactions_per_s=tm["interactions"] / (tm["walk_ms"] * 1e-3),
-               walk_tflops=21 * tm["interactions"] / (tm["walk_ms"] * 1e-3) / 1e12,
-               force_err_vs_direct_fp64=dict(nodes=ncheck, median=float(np.median(err)),
-                                             p99=float(np.quantile(err, 0.99)), max=float(err.max())),
-               pot_err_vs_direct_fp64=dict(median=float(np.median(perr)), max=float(perr.max())))
-    log(f"[bench] tree fill: {wall * 1e3:.0f} ms wall (build {tm['build_ms']:.0f} ms, walk {tm['walk_ms']:.0f} ms, "
-        f"{tm['interactions']:.3g} interactions), force error vs direct median {np.median(err):.2e}")
+    out = dict(sources=n, targets=ntarget, fg_nbin=nbin, theta=0.4, softening=cfg.softening)
+    for name, mode in (("quadrupole_plummer_fp32", TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER),
+                       ("reference_kernel_fp32", TREE_FP32)):
+        t0 = time.perf_counter()
+        tree_fill(p, spos, smass, precision=mode)
+        wall = time.perf_counter() - t0
+        tm = tree_last_timings()
+        acc32, pot32, _ = tree_forces(p, spos, smass, tpos, precision=mode)
+        accd, potd = direct_forces(p, spos, smass, tpos, plummer=bool(mode & TREE_PLUMMER))
+        err = np.linalg.norm(acc32 - accd, axis=1) / np.linalg.norm(accd, axis=1)
+        perr = np.abs(pot32 / potd - 1)
+        out[name] = dict(wall_ms=wall * 1e3, build_ms=tm["build_ms"], walk_ms=tm["walk_ms"],
+                         interactions=tm["interactions"], interactions_per_s=tm["interactions"] / (tm["walk_ms"] * 1e-3),
+                         force_err_vs_direct_fp64=dict(nodes=ncheck, median=float(np.median(err)),
+                                                       p99=float(np.quantile(err, 0.99)), max=float(err.max())),
+                         pot_err_vs_direct_fp64=dict(median=float(np.median(perr)), max=float(perr.max())))
+        log(f"[bench] tree fill {name}: {wall * 1e3:.0f} ms wall (build {tm['build_ms']:.0f} ms, walk "
+            f"{tm['walk_ms']:.0f} ms, {tm['interactions']:.3g} interactions), force error vs direct median "
+            f"{np.median(err):.2e}")
     return out
 
 

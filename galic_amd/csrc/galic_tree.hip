@@ -50,6 +50,8 @@ struct Tree {
   int *first = nullptr, *count = nullptr, *depth = nullptr;
   int *child = nullptr;  // [node][8]: -1, particle (< nsrc) or nsrc + node
   double *mass = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  double *quad = nullptr;  // [6][node] traceless quadrupole xx, yy, zz, xy, xz, yz (with quadrupole moments)
+  float4 *quadf = nullptr; // [node][2] the same in FP32 for the FP32 walk: (xx, yy, zz, xy), (xz, yz, 0, 0)
   float *len = nullptr;
   int *sibling = nullptr, *nextnode = nullptr, *bucket = nullptr;
   int *counter = nullptr;
@@ -143,10 +145,24 @@ __global__ void k_split(int lo, int hi, int d, int nsrc, int cap, const unsigned
   }
 }
 
+// Traceless quadrupole m (3 d_i d_j - d^2 delta_ij) of a point mass at offset d,
+// added to q (xx, yy, zz, xy, xz, yz).
+__device__ __forceinline__ void add_quad(double q[6], double m, double dx, double dy, double dz) {
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  q[0] += m * (3 * dx * dx - d2);
+  q[1] += m * (3 * dy * dy - d2);
+  q[2] += m * (3 * dz * dz - d2);
+  q[3] += m * 3 * dx * dy;
+  q[4] += m * 3 * dx * dz;
+  q[5] += m * 3 * dy * dz;
+}
+
 // Monopole moments of the nodes of one depth (forcetree.c:619-697), children in order 0..7.
-__global__ void k_moments(int lo, int hi, int nsrc, const int *child, const int *bucket, const int *first,
+// With quad (nnodes per component), also the traceless quadrupole about the
+// centre of mass, from the children's own quadrupoles shifted to it.
+__global__ void k_moments(int lo, int hi, int nsrc, int nnodes, const int *child, const int *bucket, const int *first,
                           const int *count, const double *x, const double *y, const double *z, const double *m,
-                          double *mass, double *sx, double *sy, double *sz) {
+                          double *mass, double *sx, double *sy, double *sz, double *quad) {
   int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if(nd >= hi) return;
   double ms = 0, s0 = 0, s1 = 0, s2 = 0;
@@ -185,6 +201,25 @@ __global__ void k_moments(int lo, int hi, int nsrc, const int *child, const int 
   sx[nd] = s0;
   sy[nd] = s1;
   sz[nd] = s2;
+  if(!quad) return;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  if(bucket[nd]) {
+    for(int p = first[nd]; p < first[nd] + count[nd]; p++) add_quad(q, m[p], x[p] - s0, y[p] - s1, z[p] - s2);
+  } else {
+    const int *ch = child + (size_t)nd * 8;
+    for(int k = 0; k < 8; k++) {
+      int c = ch[k];
+      if(c < 0) continue;
+      if(c < nsrc) {
+        add_quad(q, m[c], x[c] - s0, y[c] - s1, z[c] - s2);
+      } else {
+        int r = c - nsrc;
+        for(int j = 0; j < 6; j++) q[j] += quad[(size_t)j * nnodes + r];
+        add_quad(q, mass[r], sx[r] - s0, sy[r] - s1, sz[r] - s2);
+      }
+    }
+  }
+  for(int j = 0; j < 6; j++) quad[(size_t)j * nnodes + nd] = q[j];
 }
 
 // nextnode = first child; the sibling of a child is the next child, the last
@@ -213,6 +248,15 @@ __global__ void k_thread(int lo, int hi, int nsrc, const int *child, const int *
   }
 }
 
+// FP32 copy of the quadrupoles, two 16-byte loads per node in the FP32 walk
+__global__ void k_quadf(int nnodes, const double *quad, float4 *quadf) {
+  int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if(nd >= nnodes) return;
+  const size_t nn = nnodes;
+  quadf[2 * nd] = make_float4((float)quad[nd], (float)quad[nn + nd], (float)quad[2 * nn + nd], (float)quad[3 * nn + nd]);
+  quadf[2 * nd + 1] = make_float4((float)quad[4 * nn + nd], (float)quad[5 * nn + nd], 0.f, 0.f);
+}
+
 __global__ void k_len(int lo, int hi, double dlen, float *len) {
   int nd = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if(nd < hi) len[nd] = (float)dlen;
@@ -224,8 +268,13 @@ struct WalkArgs {
   const double *x, *y, *z, *m;
   const int *nextp, *first, *count, *bucket, *sibling, *nextnode;
   const double *mass, *sx, *sy, *sz;
+  const double *quad;   // [6][nnodes] or null: monopole only
+  const float4 *quadf;  // [nnodes][2] FP32 copy (FP32 walk)
+  int nnodes;
   const float *len;
   double theta, h, G;
+  int plummer;          // Plummer softening of length eps instead of the spline of length h
+  double eps;
   double *ax, *ay, *az, *pot;
   unsigned long long *nint;
 };
@@ -275,6 +324,37 @@ __device__ __forceinline__ void interact32(float mass, float r2, float h, float 
   }
 }
 
+// Plummer-softened monopole: potential -m / sqrt(r^2 + eps^2)
+__device__ __forceinline__ void plummer64(double mass, double r2, double eps, double &fac, double &wp) {
+  const double s2 = r2 + eps * eps;
+  const double sinv = 1.0 / sqrt(s2);
+  fac = mass * sinv * sinv * sinv;
+  wp = -mass * sinv;
+}
+
+__device__ __forceinline__ void plummer32(float mass, float r2, float eps, float &fac, float &wp) {
+  const float sinv = __builtin_amdgcn_rsqf(r2 + eps * eps);
+  fac = mass * sinv * sinv * sinv;
+  wp = -mass * sinv;
+}
+
+// Quadrupole term of a node at offset d = (centre of mass - target), distance^2
+// s2 (softened for Plummer): potential -(1/2) d.Q.d / s^5, acceleration
+// -Q.d / s^5 + (5/2) (d.Q.d) d / s^7.
+template <typename T>
+__device__ __forceinline__ void quad_term(const T q[6], T dx, T dy, T dz, T s2, T sinv, T &ax, T &ay, T &az, T &pot) {
+  const T qx = q[0] * dx + q[3] * dy + q[4] * dz;
+  const T qy = q[3] * dx + q[1] * dy + q[5] * dz;
+  const T qz = q[4] * dx + q[5] * dy + q[2] * dz;
+  const T dqd = dx * qx + dy * qy + dz * qz;
+  const T s5 = sinv * sinv * sinv * sinv * sinv;
+  const T c = (T)2.5 * dqd * s5 / s2;
+  ax += -qx * s5 + c * dx;
+  ay += -qy * s5 + c * dy;
+  az += -qz * s5 + c * dz;
+  pot += (T)-0.5 * dqd * s5;
+}
+
 // forcetree_walk.c:58-192 for one target per lane (mode 0, one task)
 __global__ void __launch_bounds__(256) k_walk64(WalkArgs A) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,7 +387,10 @@ __global__ void __launch_bounds__(256) k_walk64(WalkArgs A) {
             double ex = A.x[p] - px, ey = A.y[p] - py, ez = A.z[p] - pz;
             double e2 = ex * ex + ey * ey + ez * ez;
             if(A.m[p]) {
-              interact64(A.m[p], e2, A.h, fac, wp);
+              if(A.plummer)
+                plummer64(A.m[p], e2, A.eps, fac, wp);
+              else
+                interact64(A.m[p], e2, A.h, fac, wp);
               acc_x += ex * fac;
               acc_y += ey * fac;
               acc_z += ez * fac;
@@ -322,9 +405,19 @@ __global__ void __launch_bounds__(256) k_walk64(WalkArgs A) {
         continue;
       }
       no = A.sibling[q];
+      if(A.quad && mass) {
+        const size_t nn = A.nnodes;
+        const double qq[6] = {A.quad[q], A.quad[nn + q], A.quad[2 * nn + q],
+                              A.quad[3 * nn + q], A.quad[4 * nn + q], A.quad[5 * nn + q]};
+        const double s2 = A.plummer ? r2 + A.eps * A.eps : r2;
+        if(A.plummer || s2 >= A.h * A.h) quad_term(qq, dx, dy, dz, s2, 1.0 / sqrt(s2), acc_x, acc_y, acc_z, potv);
+      }
     }
     if(mass) {
-      interact64(mass, r2, A.h, fac, wp);
+      if(A.plummer)
+        plummer64(mass, r2, A.eps, fac, wp);
+      else
+        interact64(mass, r2, A.h, fac, wp);
       acc_x += dx * fac;
       acc_y += dy * fac;
       acc_z += dz * fac;
@@ -347,7 +440,7 @@ __global__ void __launch_bounds__(256) k_walk32(WalkArgs A) {
   if(t >= A.ntarget) return;
   const double px = A.tx[t], py = A.ty[t], pz = A.tz[t];
   float acc_x = 0, acc_y = 0, acc_z = 0, potv = 0, fac, wp;
-  const float h = (float)A.h, th2 = (float)(A.theta * A.theta);
+  const float h = (float)A.h, th2 = (float)(A.theta * A.theta), eps = (float)A.eps;
   unsigned int nint = 0;
   int no = A.nsrc;
   while(no >= 0) {
@@ -373,7 +466,10 @@ __global__ void __launch_bounds__(256) k_walk32(WalkArgs A) {
             float ex = (float)(A.x[p] - px), ey = (float)(A.y[p] - py), ez = (float)(A.z[p] - pz);
             float e2 = ex * ex + ey * ey + ez * ez;
             if(A.m[p] != 0) {
-              interact32((float)A.m[p], e2, h, fac, wp);
+              if(A.plummer)
+                plummer32((float)A.m[p], e2, eps, fac, wp);
+              else
+                interact32((float)A.m[p], e2, h, fac, wp);
               acc_x += ex * fac;
               acc_y += ey * fac;
               acc_z += ez * fac;
@@ -388,9 +484,19 @@ __global__ void __launch_bounds__(256) k_walk32(WalkArgs A) {
         continue;
       }
       no = A.sibling[q];
+      if(A.quadf && mass != 0) {
+        const float4 qa = A.quadf[2 * q], qb = A.quadf[2 * q + 1];
+        const float qq[6] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y};
+        const float s2 = A.plummer ? r2 + eps * eps : r2;
+        if(A.plummer || s2 >= h * h)
+          quad_term(qq, dx, dy, dz, s2, __builtin_amdgcn_rsqf(s2), acc_x, acc_y, acc_z, potv);
+      }
     }
     if(mass != 0) {
-      interact32(mass, r2, h, fac, wp);
+      if(A.plummer)
+        plummer32(mass, r2, eps, fac, wp);
+      else
+        interact32(mass, r2, h, fac, wp);
       acc_x += dx * fac;
       acc_y += dy * fac;
       acc_z += dz * fac;
@@ -427,8 +533,8 @@ __global__ void k_project(int nb, const double *ax, const double *ay, const doub
 // tree's force error): one target per lane, sources staged through LDS.
 __global__ void __launch_bounds__(256) k_direct64(int nsrc, const double *sx, const double *sy, const double *sz,
                                                    const double *sm, int ntarget, const double *tx, const double *ty,
-                                                   const double *tz, double h, double G, double *ax, double *ay,
-                                                   double *az, double *pot) {
+                                                   const double *tz, double h, int plummer, double eps, double G,
+                                                   double *ax, double *ay, double *az, double *pot) {
   __shared__ double4 tile[256];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = t < ntarget;
@@ -443,7 +549,10 @@ __global__ void __launch_bounds__(256) k_direct64(int nsrc, const double *sx, co
       const double4 s = tile[k];
       if(s.w != 0) {
         const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
-        interact64(s.w, dx * dx + dy * dy + dz * dz, h, fac, wp);
+        if(plummer)
+          plummer64(s.w, dx * dx + dy * dy + dz * dz, eps, fac, wp);
+        else
+          interact64(s.w, dx * dx + dy * dy + dz * dz, h, fac, wp);
         acc_x += dx * fac;
         acc_y += dy * fac;
         acc_z += dz * fac;
@@ -596,14 +705,16 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
     TCHK(B.alloc(&T.sy, nnodes));
     TCHK(B.alloc(&T.sz, nnodes));
     TCHK(B.alloc(&T.len, nnodes));
+    if(precision & GALIC_TREE_QUADRUPOLE) TCHK(B.alloc(&T.quad, (size_t)6 * nnodes));
     TCHK(B.alloc(&T.sibling, nnodes));
     TCHK(B.alloc(&T.nextnode, nnodes));
     for(int d = ndepth - 1; d >= 0; d--) {
       int lo = T.level_lo[d], hi = T.level_lo[d + 1];
       if(hi <= lo) continue;
-      hipLaunchKernelGGL(k_moments, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, (const int *)T.child,
-                         (const int *)T.bucket, (const int *)T.first, (const int *)T.count, (const double *)T.x,
-                         (const double *)T.y, (const double *)T.z, (const double *)T.m, T.mass, T.sx, T.sy, T.sz);
+      hipLaunchKernelGGL(k_moments, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, nnodes,
+                         (const int *)T.child, (const int *)T.bucket, (const int *)T.first, (const int *)T.count,
+                         (const double *)T.x, (const double *)T.y, (const double *)T.z, (const double *)T.m, T.mass,
+                         T.sx, T.sy, T.sz, T.quad);
       // side length DomainLen / 2^d, held in float as NODE.len (allvars.h:1095)
       hipLaunchKernelGGL(k_len, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, ldexp(dlen, -d), T.len);
       TCHK(hipGetLastError());
@@ -638,8 +749,20 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
     W.sy = T.sy;
     W.sz = T.sz;
     W.len = T.len;
+    W.quad = T.quad;
+    W.quadf = nullptr;
+    if(T.quad && (precision & GALIC_TREE_FP32)) {
+      TCHK(B.alloc(&T.quadf, (size_t)2 * nnodes));
+      hipLaunchKernelGGL(k_quadf, dim3(nb256(nnodes)), dim3(256), 0, st, nnodes, (const double *)T.quad, T.quadf);
+      TCHK(hipGetLastError());
+      W.quadf = T.quadf;
+    }
+    W.nnodes = nnodes;
     W.theta = p->ErrTolTheta;
     W.h = p->ForceSoftening;
+    // ForceSoftening is 2.8 times the Softening parameter (init.c:78-84), the Plummer length
+    W.plummer = (precision & GALIC_TREE_PLUMMER) ? 1 : 0;
+    W.eps = p->ForceSoftening / 2.8;
     W.G = p->G;
     W.ax = dax;
     W.ay = day;
@@ -652,7 +775,7 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
     TCHK(hipEventSynchronize(e0));
     TCHK(hipEventElapsedTime(&build_ms, eb, e0));
     if(ntarget > 0) {
-      if(precision == 0)
+      if(!(precision & GALIC_TREE_FP32))
         hipLaunchKernelGGL(k_walk64, dim3(nb256(ntarget)), dim3(256), 0, st, W);
       else
         hipLaunchKernelGGL(k_walk32, dim3(nb256(ntarget)), dim3(256), 0, st, W);
@@ -713,7 +836,7 @@ int galic_tree_forces(const galic_params *p, int device, int nsrc, const double 
                       long long *ninteractions) {
   if(!p || nsrc < 0 || ntarget < 0 || (nsrc > 0 && (!spos || !smass)) || (ntarget > 0 && (!tpos || !acc || !pot)))
     return terr(GALIC_ERR_ARG, "bad tree arguments");
-  if(precision != 0 && precision != 1) return terr(GALIC_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+  if(precision & ~7) return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER");
   int rc = GALIC_OK;
   hipStream_t st = nullptr;
   DevBufs B;
@@ -751,7 +874,7 @@ int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *s
                     double *pot, double *dpotdr, double *dpotdz, int precision, long long *ninteractions) {
   if(!p || nsrc < 0 || (nsrc > 0 && (!spos || !smass)) || !pot || !dpotdr || !dpotdz || p->FG_Nbin < 2)
     return terr(GALIC_ERR_ARG, "bad table-fill arguments");
-  if(precision != 0 && precision != 1) return terr(GALIC_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+  if(precision & ~7) return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER");
   const int nb = p->FG_Nbin;
   const size_t ns = (size_t)GALIC_FG_SECTIONS * nb * nb;
   if(ns > 0x7fffffff) return terr(GALIC_ERR_ARG, "FG_Nbin too large");
@@ -809,8 +932,14 @@ done:
 
 int galic_direct_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
                         int ntarget, const double *tpos, double *acc, double *pot) {
+  return galic_direct_forces_mode(p, device, nsrc, spos, smass, ntarget, tpos, acc, pot, 0);
+}
+
+int galic_direct_forces_mode(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                             int ntarget, const double *tpos, double *acc, double *pot, int mode) {
   if(!p || nsrc < 0 || ntarget < 0 || (nsrc > 0 && (!spos || !smass)) || (ntarget > 0 && (!tpos || !acc || !pot)))
     return terr(GALIC_ERR_ARG, "bad direct-summation arguments");
+  if(mode != 0 && mode != GALIC_TREE_PLUMMER) return terr(GALIC_ERR_ARG, "mode: 0 or GALIC_TREE_PLUMMER");
   int rc = GALIC_OK;
   hipStream_t st = nullptr;
   DevBufs B;
@@ -827,7 +956,8 @@ int galic_direct_forces(const galic_params *p, int device, int nsrc, const doubl
   if(ntarget > 0) {
     hipLaunchKernelGGL(galic::tree::k_direct64, dim3(nb256(ntarget)), dim3(256), 0, st, nsrc, (const double *)px,
                        (const double *)py, (const double *)pz, (const double *)pm, ntarget, (const double *)tx,
-                       (const double *)ty, (const double *)tz, p->ForceSoftening, p->G, ax, ay, az, pt);
+                       (const double *)ty, (const double *)tz, p->ForceSoftening, mode == GALIC_TREE_PLUMMER ? 1 : 0,
+                       p->ForceSoftening / 2.8, p->G, ax, ay, az, pt);
     TCHK(hipGetLastError());
     TCHK(hipStreamSynchronize(st));
     h.resize(ntarget);

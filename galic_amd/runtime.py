@@ -221,15 +221,20 @@ def tree_fill(params, spos, smass, precision=0, device=0):
     return pot, dr, dz, int(nint[0])
 
 
-def direct_forces(params, spos, smass, tpos, device=0):
-    """FP64 direct summation with the tree's interaction: (acc [n][3], pot [n])."""
+# walk modes of the tree calls (include/galic_hip.h)
+TREE_FP32, TREE_QUADRUPOLE, TREE_PLUMMER = 1, 2, 4
+
+
+def direct_forces(params, spos, smass, tpos, device=0, plummer=False):
+    """FP64 direct summation with the tree's interaction (the spline, or Plummer): (acc [n][3], pot [n])."""
     spos = np.ascontiguousarray(spos, dtype=np.float64).reshape(-1, 3)
     smass = np.ascontiguousarray(smass, dtype=np.float64)
     tpos = np.ascontiguousarray(tpos, dtype=np.float64).reshape(-1, 3)
     acc = np.zeros_like(tpos)
     pot = np.zeros(len(tpos))
-    check(_abi.lib().galic_direct_forces(ctypes.byref(params), device, len(spos), dptr(spos), dptr(smass), len(tpos),
-                                         dptr(tpos), dptr(acc), dptr(pot)))
+    check(_abi.lib().galic_direct_forces_mode(ctypes.byref(params), device, len(spos), dptr(spos), dptr(smass),
+                                              len(tpos), dptr(tpos), dptr(acc), dptr(pot),
+                                              TREE_PLUMMER if plummer else 0))
     return acc, pot
 
 

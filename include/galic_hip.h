@@ -202,10 +202,17 @@ int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu);
 
 /* ---- Barnes-Hut tree fill of the force tables ---- */
 
+/* Walk modes, or-ed into the `precision` argument of the tree calls.  0 is the
+ * reference's walk: FP64, monopole moments (forcetree.c:619-697), Gadget-2
+ * spline softening of length ForceSoftening (forcetree_walk.c:158-191). */
+#define GALIC_TREE_FP32 1        /* FP32 interactions and sums per target, FP64 moments */
+#define GALIC_TREE_QUADRUPOLE 2  /* traceless quadrupole moments (FP64 sums) at accepted nodes; the reference has none */
+#define GALIC_TREE_PLUMMER 4     /* Plummer softening of length ForceSoftening / 2.8 (the Softening parameter) */
+
 /* Accelerations and potentials (times G) at ntarget points from nsrc sources
  * (forcetree.c:41-722, forcetree_walk.c:12-222, gravtree.c:455-463).
- * precision 0 = FP64 walk, 1 = FP32 walk with FP64 moments.  ninteractions
- * (optional) receives the interaction count. */
+ * precision: a combination of the GALIC_TREE_ modes (0: the reference's walk).
+ * ninteractions (optional) receives the interaction count. */
 int galic_tree_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
                       int ntarget, const double *tpos, double *acc, double *pot, int precision,
                       long long *ninteractions);
@@ -221,6 +228,9 @@ int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *s
  * error.  O(nsrc * ntarget). */
 int galic_direct_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
                         int ntarget, const double *tpos, double *acc, double *pot);
+/* The same with mode GALIC_TREE_PLUMMER for Plummer softening (0: the spline). */
+int galic_direct_forces_mode(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
+                             int ntarget, const double *tpos, double *acc, double *pot, int mode);
 
 /* Device times of the last galic_tree_forces / galic_tree_fill call of this
  * process: out[0] tree build ms, out[1] walk ms, out[2] interactions. */

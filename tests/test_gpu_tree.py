@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from galic_amd import model
-from galic_amd.runtime import direct_forces, tree_fill, tree_forces, tree_last_timings
+from galic_amd.runtime import (TREE_FP32, TREE_PLUMMER, TREE_QUADRUPOLE, direct_forces, tree_fill, tree_forces,
+                                tree_last_timings)
 
 pytestmark = pytest.mark.gpu
 
@@ -119,3 +120,68 @@ def test_direct_forces_match_oracle(oracle_mod):
     tree_forces(p, spos, smass, tpos, precision=1)
     tm = tree_last_timings()
     assert tm["interactions"] > 0 and tm["walk_ms"] > 0 and tm["build_ms"] > 0
+
+
+def plummer_direct_np(p, spos, smass, tpos):
+    """Plummer-softened direct summation in numpy (potential -G m / sqrt(r^2 + eps^2))."""
+    eps = p.ForceSoftening / 2.8
+    d = spos[None, :, :] - tpos[:, None, :]
+    s2 = (d * d).sum(-1) + eps * eps
+    inv = 1 / np.sqrt(s2)
+    acc = p.G * (smass[None, :, None] * d * inv[..., None] ** 3).sum(1)
+    pot = -p.G * (smass[None, :] * inv).sum(1)
+    return acc, pot
+
+
+def test_plummer_direct_sum_matches_numpy():
+    p, spos, smass = sources(800)
+    tpos = np.random.default_rng(8).normal(size=(60, 3)) * 3
+    acc_n, pot_n = plummer_direct_np(p, spos, smass, tpos)
+    acc_g, pot_g = direct_forces(p, spos, smass, tpos, plummer=True)
+    np.testing.assert_allclose(acc_g, acc_n, rtol=1e-12, atol=1e-14 * np.abs(acc_n).max())
+    np.testing.assert_allclose(pot_g, pot_n, rtol=1e-12)
+    # theta -> 0 opens every node: the Plummer walk equals the direct sum
+    p.ErrTolTheta = 1e-8
+    for mode in (TREE_PLUMMER, TREE_PLUMMER | TREE_QUADRUPOLE):
+        acc_t, pot_t, _ = tree_forces(p, spos, smass, tpos, precision=mode)
+        np.testing.assert_allclose(acc_t, acc_n, rtol=1e-11, atol=1e-13 * np.abs(acc_n).max())
+
+
+def test_quadrupole_of_a_distant_cluster():
+    """A compact cluster seen from afar: the monopole error falls as (s/D)^2, the
+    quadrupole error as (s/D)^3, so at s/D ~ 1/50 the quadrupole gains > 10x."""
+    p = model.make_params(model.config("composite-4e6"))
+    p.ForceSoftening = 1e-3
+    p.ErrTolTheta = 0.7
+    g = np.random.default_rng(9)
+    spos = g.normal(size=(400, 3)) * np.array([1.0, 0.5, 0.25])   # an anisotropic cluster: large quadrupole
+    smass = g.uniform(0.5, 1.5, 400)
+    u = g.normal(size=(200, 3))
+    tpos = 60 * u / np.linalg.norm(u, axis=1, keepdims=True)
+    acc_d, pot_d = direct_forces(p, spos, smass, tpos)
+    errs = {}
+    for mode in (0, TREE_QUADRUPOLE):
+        acc_t, pot_t, n = tree_forces(p, spos, smass, tpos, precision=mode)
+        assert n < 400 * 200   # the walk takes nodes, not every particle
+        errs[mode] = np.median(np.linalg.norm(acc_t - acc_d, axis=1) / np.linalg.norm(acc_d, axis=1))
+    print("distant cluster: monopole", errs[0], "quadrupole", errs[TREE_QUADRUPOLE])
+    assert errs[TREE_QUADRUPOLE] < errs[0] / 10
+
+
+def test_quadrupole_walks_on_the_composite():
+    """theta = 0.4 over 2e5 composite sources: quadrupole moments cut the force error
+    against direct summation, FP64 and FP32 walks agree, for both softenings."""
+    print("[gpu] quadrupole walks", flush=True)
+    p, spos, smass = sources(200000)
+    p.ErrTolTheta = 0.4
+    tpos = np.random.default_rng(10).normal(size=(2048, 3)) * 6
+    for soft in (0, TREE_PLUMMER):
+        acc_d, pot_d = direct_forces(p, spos, smass, tpos, plummer=bool(soft))
+        med = {}
+        for mode in (soft, soft | TREE_QUADRUPOLE, soft | TREE_QUADRUPOLE | TREE_FP32):
+            acc_t, pot_t, _ = tree_forces(p, spos, smass, tpos, precision=mode)
+            e = np.linalg.norm(acc_t - acc_d, axis=1) / np.linalg.norm(acc_d, axis=1)
+            med[mode] = np.median(e)
+        print("softening", soft, "median force errors", med)
+        assert med[soft | TREE_QUADRUPOLE] < med[soft] / 3
+        assert med[soft | TREE_QUADRUPOLE | TREE_FP32] < 2 * med[soft | TREE_QUADRUPOLE] + 1e-5
