@@ -63,6 +63,8 @@ EXPORTS = [
     "galic_timings", "galic_reset_timings", "galic_copy_probe", "galic_bind",
     "produce_orbit_response_field_mod", "produce_orbit_response_field",
     "forcegrid_get_potential", "forcegrid_get_escape_speed", "forcegrid_get_acceleration", "smooth_stack",
+    "forcegrid_get_cell", "densitygrid_get_cell", "energygrid_get_cell", "get_timestep",
+    "calc_stack_difference", "calc_stack_difference_mod",
 ]
 
 _P = ctypes.POINTER

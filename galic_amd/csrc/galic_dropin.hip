@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <math.h>
 #include <string.h>
 
 #include <vector>
@@ -13,6 +14,7 @@
 #include "../../include/galic_hip.h"
 #include "galic_ctx.h"
 #include "galic_kernels.h"
+#include "galic_handle.h"
 
 // defined in galic_api.hip
 struct galic_ctx;
@@ -141,6 +143,113 @@ double forcegrid_get_escape_speed(double *pos) {
 void forcegrid_get_acceleration(double *pos, double *acc) {
   galic_ctx *ctx = galic::bound_ctx();
   if(!ctx || galic_field_at(ctx, 1, pos, acc, nullptr) != GALIC_OK) dropin_fail("forcegrid_get_acceleration");
+}
+
+// Log-index cell finders: the force grid's nodes (grid.c:15-55) and the
+// cell-centred density and energy grids (grid.c:494-534, 896-938), on the host
+// in the reference's operation order.
+static void log_cell(const double *pos, double rmin, double fac, int nbin, double shift, int *iR, int *iz, double *fR,
+                     double *fz) {
+  double r = sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+  double z = fabs(pos[2]);
+  double br = (log(r / rmin + 1.0) / log(fac)) - shift;
+  double bz = (log(z / rmin + 1.0) / log(fac)) - shift;
+  if(br < 0) br = 0;
+  if(bz < 0) bz = 0;
+  int binr = (int)br, binz = (int)bz;
+  *fR = br - binr;
+  *fz = bz - binz;
+  if(binr >= nbin - 1) {
+    binr = nbin - 2;
+    *fR = 1;
+  }
+  if(binz >= nbin - 1) {
+    binz = nbin - 2;
+    *fz = 1;
+  }
+  *iR = binr;
+  *iz = binz;
+}
+
+static const galic_params &bound_params(const char *what) {
+  galic_ctx *ctx = galic::bound_ctx();
+  if(!ctx) dropin_fail(what);
+  return galic::ctx_state(ctx).p;
+}
+
+void forcegrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz) {
+  const galic_params &p = bound_params("forcegrid_get_cell (no context bound)");
+  log_cell(pos, p.FG_Rmin, p.FG_Fac, p.FG_Nbin, 0.0, iR, iz, fR, fz);
+}
+
+void densitygrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz) {
+  const galic_params &p = bound_params("densitygrid_get_cell (no context bound)");
+  log_cell(pos, p.DG_Rmin, p.DG_Fac, 1 << p.DG_MaxLevel, 0.5, iR, iz, fR, fz);
+}
+
+void energygrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz) {
+  const galic_params &p = bound_params("energygrid_get_cell (no context bound)");
+  log_cell(pos, p.EG_Rmin, p.EG_Fac, 1 << p.EG_MaxLevel, 0.5, iR, iz, fR, fz);
+}
+
+// orbit_response.c:16-26: DG_CellSize of the bound context's density grid
+double get_timestep(double *pos, double *vel, double *acc, int icell) {
+  (void)pos;
+  galic_ctx *ctx = galic::bound_ctx();
+  if(!ctx) dropin_fail("get_timestep (no context bound)");
+  const galic_params &p = galic::ctx_state(ctx).p;
+  if(icell < 0 || icell >= (int)ctx->cellsize.size()) dropin_fail("get_timestep (cell index)");
+  double v = sqrt(vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
+  double aa = sqrt(acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2]);
+  double torbit = p.V200 / aa;
+  double tcross = ctx->cellsize[icell] / v;
+  double a = p.TimeStepFactorOrbit * torbit, b = p.TimeStepFactorCellCross * tcross;
+  return a < b ? a : b;
+}
+
+// calc_stack_difference (grid.c:1086-1112) and calc_stack_difference_mod
+// (grid.c:1115-1151): the recursion stops at maxlevel or where ref1 < thresh
+// and sums the term of `flag` there.
+static double stack_term(const double *d1, const double *d2, int o, const double *ref1, const double *ref2,
+                         const double *dist, int flag, int mod, const galic_params &p) {
+  const double r20 = p.R200 / 20.0;
+  const double dcap = dist[o] < r20 ? dist[o] : r20;  // min() of grid.c
+  if(flag == 1 || (!mod && flag)) {
+    if(ref1[o] > 0 && ref2[o] > 0) {
+      const double q = d2[o] / ref2[o];
+      return fabs(d1[o] / ref1[o] - d2[o] / ref2[o]) / (q > p.LowerDispLimit ? q : p.LowerDispLimit);  // dmax()
+    }
+    return 0;
+  }
+  if(mod && flag == 2) {
+    if(ref2[o] > 0) return fabs(d1[o] - d2[o]) * (ref1[o] / ref2[o]) / dcap;
+    return 0;
+  }
+  return fabs(d1[o] - d2[o]) / dcap;
+}
+
+static double stack_difference(const double *d1, const double *d2, int l, int i, int j, int maxlevel,
+                               const double *ref1, const double *ref2, double thresh, const double *dist, int flag,
+                               int mod, const galic_params &p) {
+  const int o = GALIC_STACKOFFSET(l, i, j);
+  if(l >= maxlevel || ref1[o] < thresh) return stack_term(d1, d2, o, ref1, ref2, dist, flag, mod, p);
+  double sum = 0;
+  for(int ii = 2 * i; ii <= 2 * i + 1; ii++)
+    for(int jj = 2 * j; jj <= 2 * j + 1; jj++)
+      sum += stack_difference(d1, d2, l + 1, ii, jj, maxlevel, ref1, ref2, thresh, dist, flag, mod, p);
+  return sum;
+}
+
+double calc_stack_difference(double *d1, double *d2, int l, int i, int j, int maxlevel, double *ref1, double *ref2,
+                             double thresh, double *dist, int flag) {
+  const galic_params &p = bound_params("calc_stack_difference (no context bound)");
+  return stack_difference(d1, d2, l, i, j, maxlevel, ref1, ref2, thresh, dist, flag, 0, p);
+}
+
+double calc_stack_difference_mod(double *d1, double *d2, int l, int i, int j, int maxlevel, double *ref1, double *ref2,
+                                 double thresh, double *dist, int flag) {
+  const galic_params &p = bound_params("calc_stack_difference_mod (no context bound)");
+  return stack_difference(d1, d2, l, i, j, maxlevel, ref1, ref2, thresh, dist, flag, 1, p);
 }
 
 void smooth_stack(double *data, int maxlevel) {

@@ -268,6 +268,18 @@ double forcegrid_get_escape_speed(double *pos);
 void forcegrid_get_acceleration(double *pos, double *acc);
 /* grid.c:1247-1265 (proto.h:120) */
 void smooth_stack(double *data, int maxlevel);
+/* grid.c:15-55, 494-534, 896-938 (proto.h): the cell and fractions of pos on the
+ * force, density and energy grids of the bound context */
+void forcegrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz);
+void densitygrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz);
+void energygrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz);
+/* orbit_response.c:16-26 (proto.h): with the bound context's DG_CellSize */
+double get_timestep(double *pos, double *vel, double *acc, int icell);
+/* grid.c:1086-1112 and 1115-1151 (proto.h): R200 and LowerDispLimit of the bound context */
+double calc_stack_difference(double *d1, double *d2, int l, int i, int j, int maxlevel, double *ref1, double *ref2,
+                             double thresh, double *dist, int flag);
+double calc_stack_difference_mod(double *d1, double *d2, int l, int i, int j, int maxlevel, double *ref1,
+                                 double *ref2, double thresh, double *dist, int flag);
 
 #ifdef __cplusplus
 }

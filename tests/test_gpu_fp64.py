@@ -161,3 +161,60 @@ def test_dropin_orbit_signature(halo, oracle_mod):
     ok, worst = close(np.stack(bufs), ro["fields"])
     assert ok, worst
     g.close()
+
+
+def test_dropin_lookups_and_stack_differences(halodisk, oracle_mod):
+    """The remaining reference-signature entry points (grid.c:15-55, 494-534, 896-938,
+    1086-1151; orbit_response.c:16-26) on the bound context equal the oracle's restatements."""
+    import ctypes
+    w = halodisk
+    p = w.params
+    g = GalicGPU(w)
+    g.bind()
+    lib = g.lib
+    o = oracle_mod.Oracle(w, set_particles=False)
+    L = oracle_mod.lib()
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    rng = np.random.default_rng(12)
+    pts = np.concatenate([rng.normal(size=(300, 3)) * 10, rng.normal(size=(50, 3)) * 300, np.zeros((1, 3))])
+    for name in ("forcegrid_get_cell", "densitygrid_get_cell", "energygrid_get_cell"):
+        f = getattr(lib, name)
+        f.restype = None
+        f.argtypes = [dp, ip, ip, dp, dp]
+        ref = getattr(L, "or_" + name)
+        for x in pts:
+            x = np.ascontiguousarray(x)
+            a = [ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()]
+            b = [ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()]
+            f(x.ctypes.data_as(dp), *[ctypes.byref(v) for v in a])
+            ref(o.h, oracle_mod.d(x), *[ctypes.byref(v) for v in b])
+            assert [v.value for v in a] == [v.value for v in b], (name, x)
+    ts = lib.get_timestep
+    ts.restype = ctypes.c_double
+    ts.argtypes = [dp, dp, dp, ctypes.c_int]
+    for k in range(50):
+        x, v, a = [np.ascontiguousarray(rng.normal(size=3) * s) for s in (5, 100, 1000)]
+        icell = int(rng.integers(0, g.ngrid))
+        assert ts(x.ctypes.data_as(dp), v.ctypes.data_as(dp), a.ctypes.data_as(dp), icell) == \
+            L.or_get_timestep(o.h, oracle_mod.d(x), oracle_mod.d(v), oracle_mod.d(a), icell)
+    ns = g.nstack
+    Lv = p.DG_MaxLevel
+    d1, d2, r1, r2 = [np.ascontiguousarray(rng.uniform(0, 1, ns)) for _ in range(4)]
+    r1[rng.random(ns) < 0.1] = 0
+    dist = np.ascontiguousarray(rng.uniform(0.1, 50, ns))
+    for name, flags in (("calc_stack_difference", (0, 1)), ("calc_stack_difference_mod", (0, 1, 2))):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_double
+        f.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_double, dp,
+                      ctypes.c_int]
+        ref = getattr(L, "or_" + name)
+        for flag in flags:
+            for thresh in (0.2, 0.6):
+                args = [d1.ctypes.data_as(dp), d2.ctypes.data_as(dp), 0, 0, 0, Lv, r1.ctypes.data_as(dp),
+                        r2.ctypes.data_as(dp), thresh, dist.ctypes.data_as(dp), flag]
+                a = f(*args)
+                b = ref(o.h, oracle_mod.d(d1), oracle_mod.d(d2), 0, 0, 0, Lv, oracle_mod.d(r1), oracle_mod.d(r2),
+                        thresh, oracle_mod.d(dist), flag)
+                assert a == b and a > 0, (name, flag, thresh)
+    o.close()
+    g.close()
