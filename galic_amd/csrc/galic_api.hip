@@ -636,8 +636,8 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   *out = nullptr;
   if(p->DG_MaxLevel != p->EG_MaxLevel)
     return set_err(GALIC_ERR_ARG, "DG_MaxLevel (%d) must equal EG_MaxLevel (%d)", p->DG_MaxLevel, p->EG_MaxLevel);
-  if(p->DG_MaxLevel < 2 || p->DG_MaxLevel > 7)
-    return set_err(GALIC_ERR_ARG, "DG_MaxLevel %d outside the supported range 2..7", p->DG_MaxLevel);
+  if(p->DG_MaxLevel < 2 || p->DG_MaxLevel > 9)
+    return set_err(GALIC_ERR_ARG, "DG_MaxLevel %d outside the supported range 2..9", p->DG_MaxLevel);
   if(p->FG_Nbin < 2) return set_err(GALIC_ERR_ARG, "FG_Nbin must be at least 2");
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
@@ -767,6 +767,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv); dfree(c.psteps);
   dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_next);
   for(int t = 0; t < GALIC_NTYPES; t++) dfree(c.disp_tab[t]);
+  dfree(c.cs_scratch);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -1120,10 +1121,37 @@ int galic_rep(galic_ctx *ctx, int ntrial, const int *idx, const double *vtry, in
       E.fac[t][3] = c.p.fac_value_ps[t];
     }
     E.lowdisp = c.p.LowerDispLimit;
-    size_t shmem = sizeof(double) * ((c.Nstack - c.Ngrid) + 8 * 256);
+    // the coarse levels of the merit pyramid live in LDS up to level 7; above, each
+    // workgroup of a launch of at most kCsBlocks takes a slice of HBM scratch
+    const size_t ncoarse = (size_t)(c.Nstack - c.Ngrid);
+    size_t lds_max = 65536;
+    {
+      int v = 0;
+      if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess && v > 0)
+        lds_max = (size_t)v;
+    }
+    const size_t red_bytes = sizeof(double) * 8 * 256;
+    const bool cs_in_lds = sizeof(double) * ncoarse + red_bytes <= lds_max;
+    const int kCsBlocks = 2048;
+    E.cs_global = nullptr;
+    E.cs_stride = ncoarse;
+    if(!cs_in_lds) {
+      if(c.cs_scratch_len < (size_t)kCsBlocks * ncoarse) {
+        dfree(c.cs_scratch);
+        CHK(dalloc(&c.cs_scratch, (size_t)kCsBlocks * ncoarse));
+        c.cs_scratch_len = (size_t)kCsBlocks * ncoarse;
+      }
+      E.cs_global = c.cs_scratch;
+    }
+    const size_t shmem = cs_in_lds ? sizeof(double) * ncoarse + red_bytes : red_bytes;
+    const int per_launch = cs_in_lds ? std::max(nt, 1) : kCsBlocks;
     HIPCHK(hipEventRecord(c.ev0, c.stream));
-    hipLaunchKernelGGL(galic::k_eval_apply64, dim3(nt), dim3(256), shmem, c.stream, c.M, E);
-    HIPCHK(hipGetLastError());
+    for(int b0 = 0; b0 < nt; b0 += per_launch) {
+      E.block_base = b0;
+      hipLaunchKernelGGL(galic::k_eval_apply64, dim3(std::min(per_launch, nt - b0)), dim3(256), shmem, c.stream, c.M,
+                         E);
+      HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipEventRecord(c.ev1, c.stream));
     HIPCHK(hipEventSynchronize(c.ev1));
     float ms = 0;

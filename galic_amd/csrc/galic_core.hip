@@ -285,11 +285,12 @@ __device__ double inst_delta(const DevModel &M, const double *S, const double *W
 // F_try - F_base of main.c:464-465 up to rounding.
 __global__ void __launch_bounds__(256) k_eval_apply64(DevModel M, EvalArgs A) {
   extern __shared__ double lds[];
-  const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int i = A.block_base + blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
   const int ncoarse = (Nstack - ngrid);
-  double *cs = lds;                  // coarse levels 0..L-1
-  double *red = lds + ncoarse;       // reduction scratch, 8*nt
+  // coarse levels 0..L-1 in LDS, or in this workgroup's HBM scratch where they do not fit (levels 8, 9)
+  double *cs = A.cs_global ? A.cs_global + (size_t)blockIdx.x * A.cs_stride : lds;
+  double *red = A.cs_global ? lds : lds + ncoarse;  // reduction scratch, 8*nt
   __shared__ int s_accept;
 
   const int p = A.tidx[i];

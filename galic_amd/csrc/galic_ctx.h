@@ -158,6 +158,8 @@ struct Ctx {
   double disp_beta[GALIC_NTYPES] = {0, 0, 0, 0};   // HaloBetaParameter / BulgeBetaParameter
   double disp_bound[GALIC_NTYPES] = {0, 0, 0, 0};  // bound of every dispersion the tables give
   float last_target_ms = 0;                          // kernel time of the last target sampling
+  double *cs_scratch = nullptr;  // FP64 merit: coarse pyramid levels in HBM for grids above level 7
+  size_t cs_scratch_len = 0;
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -52,6 +52,9 @@ struct EvalArgs {
   double MType[GALIC_NTYPES];
   double fac[GALIC_NTYPES][4];  // fac_value rs, ts, qs, ps
   double lowdisp;
+  int block_base;      // trial of the launch's first workgroup
+  double *cs_global;   // coarse-level scratch in HBM, cs_stride doubles per workgroup (grids too large for LDS), or null
+  size_t cs_stride;
 };
 
 struct FitArgs {

@@ -53,6 +53,12 @@ def halodisk():
     return workload("halo-disk-1e6", 3000)
 
 
+@pytest.fixture(scope="module")
+def halodisk9():
+    """512 x 512 response grids (level 9): the merit pyramid's coarse levels no longer fit in LDS."""
+    return workload("halo-disk-1e6", 1200, level=9)
+
+
 @pytest.mark.parametrize("wname", ["halo", "halodisk"])
 def test_orbit_responses_match_oracle(wname, request, oracle_mod):
     w = request.getfixturevalue(wname)
@@ -94,7 +100,7 @@ def test_field_lookup_matches_oracle(wname, request, oracle_mod):
     g.close()
 
 
-@pytest.mark.parametrize("wname", ["halo", "halodisk"])
+@pytest.mark.parametrize("wname", ["halo", "halodisk", "halodisk9"])
 def test_optimiser_history_matches_oracle(wname, request, oracle_mod):
     """calc_all_response_fields, then reps of trials: grids, accept flags and
     the Sall history agree with the oracle."""
