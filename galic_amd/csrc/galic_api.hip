@@ -768,6 +768,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_next);
   for(int t = 0; t < GALIC_NTYPES; t++) dfree(c.disp_tab[t]);
   dfree(c.cs_scratch);
+  dfree(c.g32_scratch);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -1521,7 +1522,22 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
            ngrids * sizeof(long long) * (size_t)c.Nbin * (c.Nbin + (ngrids == 2 ? galic::MERIT_ROW_PAD : 0)) +
            sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
   };
-  if(c.L > 7) return set_err(GALIC_ERR_ARG, "the FP32 path supports response grids up to 128 x 128 (level 7)");
+  // grids above 128^2 (levels 8, 9): the merit's two grids per workgroup live in HBM scratch,
+  // kGBlocks workgroups per launch; the LDS holds the partials, the block-sum pyramid and the tables
+  const bool gmem = c.L > 7;
+  const int kGBlocks = 512;
+  auto eval_lds_gmem = [&]() {
+    return 8 * (size_t)K * sizeof(double) + sizeof(long long) * (size_t)GALIC_NSTACK(c.L - 4) +
+           sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
+  };
+  if(gmem) {
+    const size_t need = (size_t)kGBlocks * 2 * c.Ngrid;
+    if(c.g32_scratch_len < need) {
+      dfree(c.g32_scratch);
+      CHK(dalloc(&c.g32_scratch, need));  // zero; the kernel leaves it zero
+      c.g32_scratch_len = need;
+    }
+  }
   bool packed = false;
   float eval_ms = 0;
   int s0 = 0;
@@ -1586,8 +1602,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       CHK(lr);
     }
     cap = group_cap();
-    const bool use_o = eval_lds(2) <= lds_max;
-    const size_t shmem = eval_lds(use_o ? 2 : 1) - 1024;
+    const bool use_o = gmem || eval_lds(2) <= lds_max;
+    const size_t shmem = (gmem ? eval_lds_gmem() : eval_lds(use_o ? 2 : 1)) - 1024;
     if(shmem + 1024 > lds_max)
       return set_err(GALIC_ERR_ARG, "the merit kernel needs %zu bytes of LDS, the device gives %zu", shmem, lds_max);
     // evaluate the reps of the group in turn, committing each completed rep
@@ -1642,11 +1658,20 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.lowdisp = c.p.LowerDispLimit;
       for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
       E.iscale = c.fp32_iscale;
+      E.gscratch = c.g32_scratch;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
-      if(use_o)
-        hipLaunchKernelGGL(galic::k_eval_log32<true>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
-      else
-        hipLaunchKernelGGL(galic::k_eval_log32<false>, dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+      if(gmem) {
+        for(int b0 = 0; b0 < b - a; b0 += kGBlocks) {
+          E.block_base = b0;
+          hipLaunchKernelGGL((galic::k_eval_log32<true, true>), dim3(std::min(kGBlocks, b - a - b0)), dim3(512), shmem,
+                             c.stream, c.M, E);
+          HIPCHK(hipGetLastError());
+        }
+      } else if(use_o) {
+        hipLaunchKernelGGL((galic::k_eval_log32<true, false>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+      } else {
+        hipLaunchKernelGGL((galic::k_eval_log32<false, false>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+      }
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c.ev1, c.stream));
       HIPCHK(hipEventSynchronize(c.ev1));

@@ -160,6 +160,8 @@ struct Ctx {
   float last_target_ms = 0;                          // kernel time of the last target sampling
   double *cs_scratch = nullptr;  // FP64 merit: coarse pyramid levels in HBM for grids above level 7
   size_t cs_scratch_len = 0;
+  long long *g32_scratch = nullptr;  // FP32 merit: two int64 grids per workgroup for grids above level 7
+  size_t g32_scratch_len = 0;
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -62,8 +62,10 @@ constexpr int MERIT_ROW_PAD = 2;
 struct EvalArgs32 {
   Particles P;
   int K;
-  int trial_base;       // block b evaluates trial trial_base + b
+  int trial_base;       // block b evaluates trial trial_base + b (b = block_base + blockIdx.x)
+  int block_base;
   const int *slot_pos;  // ... whose orbits sit at chunk position slot_pos[b] (null: b)
+  long long *gscratch;  // grids above 128^2: two int64 grids per workgroup of the launch, zero between uses
   const int *tidx;
   const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
   const double *stacks, *fit;
@@ -104,7 +106,7 @@ __global__ void k_energy32(DevModel M, int n, const double *ox, const double *oy
 int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigned int *key, unsigned int *key2,
                          int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
 __global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
-template <bool USE_O>
+template <bool USE_O, bool GMEM>
 __global__ void k_eval_log32(DevModel M, EvalArgs32 A);
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);

@@ -839,6 +839,27 @@ __device__ __forceinline__ double merit_top(const double2 *bwc, int L, double un
   return acc;
 }
 
+// Levels above the blocks for grids of more than 8 x 8 blocks (above 128^2
+// cells): level by level through the pyramid of block sums held in LDS (top,
+// stack layout of levels 0..lblk, block sums at level lblk); every thread takes
+// part and the function ends at a barrier.
+__device__ double merit_top_levels(const double2 *bw, int lblk, double unit, long long *top) {
+  double acc = 0;
+  for(int l = lblk - 1; l >= 0; l--) {
+    const int n = 1 << l;
+    for(int q = threadIdx.x; q < n * n; q += blockDim.x) {
+      const int i = q / n, j = q % n;
+      const long long s = top[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * j)] + top[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * j)] +
+                          top[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * j + 1)] +
+                          top[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * j + 1)];
+      top[GALIC_STACKOFFSET(l, i, j)] = s;
+      acc += term_at(bw[GALIC_STACKOFFSET(l, i, j)], (double)s * unit);
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -852,24 +873,32 @@ __device__ __forceinline__ double wave_sum(double v) {
 // do not fit in LDS (128^2 cells) both logs go into G for every trial.
 // LDS: part [K][8] double | topb [64] int64 | bwc [coarse nodes] double2 | G [nb rows] int64 |
 //      O [nb rows] int64 (USE_O) | tbl [K+1][log_M] int; a row of G and O holds rs cells
-template <bool USE_O>
+template <bool USE_O, bool GMEM>
 __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
   const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
   const int ncb = GALIC_NSTACK(L - 2);  // nodes of levels 0..L-2
+  int bside, nblk, lblk;
+  grid_blocks(L, bside, nblk, lblk);
   double *part = lds_d;
-  long long *topb = (long long *)(part + 8 * K);
-  double2 *bwc = (double2 *)(topb + 64);
-  long long *G = (long long *)(bwc + ncb);
-  const int rs = nb + (USE_O ? MERIT_ROW_PAD : 0), nlds = nb * rs;  // LDS row stride, cells of one LDS grid
+  long long *top = (long long *)(part + 8 * K);  // GMEM: the pyramid of block sums, levels 0..lblk
+  long long *topb = GMEM ? top + GALIC_STACKOFFSET(lblk, 0, 0) : top;
+  const int ntop = GMEM ? GALIC_NSTACK(lblk) : 64;
+  double2 *bwc_lds = (double2 *)(top + ntop);
+  // GMEM: the two grids in this workgroup's slice of HBM scratch (kept zero between uses:
+  // every cell a log adds to is cleared when it is read), the coarse (residual, weight)
+  // pairs read from memory; otherwise both in LDS
+  const int rs = GMEM ? nb : nb + (USE_O ? MERIT_ROW_PAD : 0), nlds = nb * rs;
+  long long *G = GMEM ? A.gscratch + (size_t)blockIdx.x * 2 * nlds : (long long *)(bwc_lds + ncb);
   long long *O = USE_O ? G + nlds : nullptr;
-  int *tbl = (int *)(G + (USE_O ? 2 : 1) * nlds);
+  int *tbl = GMEM ? (int *)bwc_lds : (int *)(G + (USE_O ? 2 : 1) * nlds);
   __shared__ double s_dF[64];
   __shared__ int s_best, s_n[65];
 
-  const int ti = A.trial_base + blockIdx.x;
-  const int j = A.slot_pos ? A.slot_pos[blockIdx.x] : (int)blockIdx.x;
+  const int bid = A.block_base + blockIdx.x;
+  const int ti = A.trial_base + bid;
+  const int j = A.slot_pos ? A.slot_pos[bid] : bid;
   const int p = A.tidx[ti];
   const int t = A.P.type[p];
   const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
@@ -891,12 +920,15 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int o = q / A.log_M, b = q % A.log_M;
     if(b * LOG_B < s_n[o]) tbl[q] = A.log_blocks[(o0 + o) * A.log_M + b];
   }
-  for(int q = tid; q < (USE_O ? 2 : 1) * nlds; q += nt) G[q] = 0;
+  if(!GMEM)
+    for(int q = tid; q < (USE_O ? 2 : 1) * nlds; q += nt) G[q] = 0;
   __syncthreads();
 
   LogRegs R;
   for(int f = 0; f < kFields; f++) {
-    for(int q = tid; q < ncb; q += nt) bwc[q] = bw[(size_t)f * Nstack + q];
+    const double2 *bwc = GMEM ? bw + (size_t)f * Nstack : bwc_lds;
+    if(!GMEM)
+      for(int q = tid; q < ncb; q += nt) bwc_lds[q] = bw[(size_t)f * Nstack + q];
     if(USE_O) {
       log_load(A, tbl, s_n[0], f, R);
       log_add(A, tbl, s_n[0], f, R, nb, L, rs, 1.0f, (unsigned long long *)O);
@@ -916,7 +948,10 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       __syncthreads();
       double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, rs, A.log_unit[f], G, O, box, topb);
       __syncthreads();
-      acc += merit_top(bwc, L, A.log_unit[f], topb);
+      if(GMEM)
+        acc += merit_top_levels(bwc, lblk, A.log_unit[f], top);
+      else
+        acc += merit_top(bwc, L, A.log_unit[f], topb);
       acc = wave_sum(coef[f] * acc);
       if(lane == 0) part[k * 8 + wave] += acc;
     }
@@ -1000,8 +1035,9 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   }
 }
 
-template __global__ void k_eval_log32<true>(DevModel M, EvalArgs32 A);
-template __global__ void k_eval_log32<false>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<true, false>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<false, false>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<true, true>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 

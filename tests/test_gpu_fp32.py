@@ -222,3 +222,42 @@ def test_fp32_level7_grids():
     g64.close()
     g32.close()
     assert np.mean((a32 > 0) == (a64 > 0)) > 0.97
+
+
+@pytest.mark.parametrize("level", [8, 9])
+def test_fp32_grids_above_level7(level):
+    """256^2 and 512^2 response grids: the merit kernel keeps its two grids in HBM
+    scratch.  Reps in one launch equal rep-by-rep bit for bit, and K=1 decisions
+    follow the FP64 path (itself checked against the oracle at level 9)."""
+    print(f"[gpu] fp32 level-{level} grids", flush=True)
+    cfg = model.config("halo-disk-1e6", level=level, max_steps=2000000)
+    wl = model.make_workload(cfg, n=2000, target_samples=200000)
+    perm = model.permutation(wl, 4)
+    out = []
+    for mode in ("iterate", "reps"):
+        g = GalicGPU(wl)
+        g.fp32_calc_all_response_fields()
+        if mode == "iterate":
+            acc = g.fp32_iterate(perm, k=3, rep_size=500, sigma_frac=0.1, seed=9)
+        else:
+            acc = np.concatenate([g.fp32_rep(perm[a:a + 500], k=3, sigma_frac=0.1, seed=9)
+                                  for a in range(0, len(perm), 500)])
+        out.append([acc] + [g.stack(t, wh) for t in wl.types for wh in range(10)] + [g.particles()[0]])
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    assert (out[0][0] > 0).mean() > 0.2
+    g64 = GalicGPU(wl)
+    g64.calc_all_response_fields()
+    g32 = GalicGPU(wl)
+    g32.fp32_calc_all_response_fields()
+    idx = perm[:800]
+    v = model.trial_velocities(wl, idx, 5, 0)
+    a64 = g64.rep(idx, v)[0]
+    a32 = g32.fp32_rep(idx, k=1, vtry=v)
+    s64 = [g64.fit(t)[7] for t in wl.types]
+    s32 = [g32.fit(t)[7] for t in wl.types]
+    g64.close()
+    g32.close()
+    assert np.mean((a32 > 0) == (a64 > 0)) > 0.97
+    np.testing.assert_allclose(s32, s64, rtol=2e-3)
