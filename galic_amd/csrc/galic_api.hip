@@ -1379,7 +1379,12 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
   galic::Ctx &c = ctx->c;
   if(!ctx->fp32_initial_done) return set_err(GALIC_ERR_STATE, "galic_fp32_calc_all_response_fields must run first");
   HIPCHK(hipSetDevice(c.device));
-  const int n = std::min(nsample, c.P.n);
+  // no more orbits than the response logs of one launch fit in half the memory the arena may
+  // take (the other half covers the blocks the lanes claim and leave unused)
+  const double arena_blocks = (double)slot_budget(c) / log_block_bytes();
+  const double per_orbit = (c.log_est_blocks + 1.0) * 1.25;
+  const int fit_n = (int)std::max(1024.0, std::min(2e9, 0.5 * arena_blocks / per_orbit));
+  const int n = std::min(std::min(nsample, c.P.n), fit_n);
   CHK(alloc_orbit_scratch(c, n));
   // a strided sample of the model's own orbits at their current velocities,
   // longest expected first as in a rep
@@ -1527,8 +1532,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   const bool gmem = c.L > 7;
   const int kGBlocks = 512;
   auto eval_lds_gmem = [&]() {
+    const size_t nblk = (size_t)1 << (c.L - 4);
     return 8 * (size_t)K * sizeof(double) + sizeof(long long) * (size_t)GALIC_NSTACK(c.L - 4) +
-           sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
+           2 * sizeof(unsigned int) * ((nblk * nblk + 31) / 32) + sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
   };
   if(gmem) {
     const size_t need = (size_t)kGBlocks * 2 * c.Ngrid;

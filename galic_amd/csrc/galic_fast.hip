@@ -721,6 +721,19 @@ __device__ __forceinline__ void log_load(const EvalArgs32 &A, const int *tbl, in
 // integer atomic, so the sums are exact and independent of the order.
 // quad = iz * nb + iR names the 2x2 cells (iz..iz+1, iR..iR+1), nb = 2^L; the
 // LDS grid holds cell (iz, iR) at iz * rs + iR.
+// Marks the 16 x 16 blocks of cells the 2x2 cells of quad fall in (grids above 128^2,
+// where the pyramid visits only marked blocks); nblk blocks per row.
+__device__ __forceinline__ void mark_blocks(unsigned int *mask, int quad, int nb, int L, int nblk) {
+  const int iz = quad >> L, iR = quad & (nb - 1);
+  const int z0 = iz >> 4, z1 = (iz + 1) >> 4, r0 = iR >> 4, r1 = (iR + 1) >> 4;
+  const int b[4] = {z0 * nblk + r0, z0 * nblk + r1, z1 * nblk + r0, z1 * nblk + r1};
+#pragma unroll
+  for(int k = 0; k < 4; k++) {
+    const unsigned int bit = 1u << (b[k] & 31);
+    if(!(mask[b[k] >> 5] & bit)) atomicOr(&mask[b[k] >> 5], bit);
+  }
+}
+
 __device__ __forceinline__ void add_record(unsigned long long *G, int quad, float4 x, float sc, int nb, int L, int rs) {
   unsigned long long *g = G + (quad >> L) * rs + (quad & (nb - 1));
   atomicAdd(g, (unsigned long long)__float2ll_rn(x.x * sc));
@@ -731,15 +744,20 @@ __device__ __forceinline__ void add_record(unsigned long long *G, int quad, floa
 
 // Adds sign * field f of a whole log into G: the records held in R, then the rest.
 __device__ __forceinline__ void log_add(const EvalArgs32 &A, const int *tbl, int n, int f, const LogRegs &R, int nb,
-                                        int L, int rs, float sign, unsigned long long *G) {
+                                        int L, int rs, float sign, unsigned long long *G, unsigned int *mask = nullptr,
+                                        int nblk = 0) {
   const float sc = sign * A.log_scale[f];
 #pragma unroll
   for(int u = 0; u < LOG_RPT; u++)
-    if(R.quad[u] >= 0) add_record(G, R.quad[u], R.x[u], sc, nb, L, rs);
+    if(R.quad[u] >= 0) {
+      add_record(G, R.quad[u], R.x[u], sc, nb, L, rs);
+      if(mask) mark_blocks(mask, R.quad[u], nb, L, nblk);
+    }
   for(int r = threadIdx.x + LOG_RPT * blockDim.x; r < n; r += blockDim.x) {
     const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
-    add_record(G, A.log_q[(size_t)blk * LOG_B + jj], A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb, L,
-               rs);
+    const int quad = A.log_q[(size_t)blk * LOG_B + jj];
+    add_record(G, quad, A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb, L, rs);
+    if(mask) mark_blocks(mask, quad, nb, L, nblk);
   }
 }
 
@@ -774,7 +792,9 @@ __device__ __forceinline__ void grid_blocks(int L, int &bside, int &nblk, int &l
 // Block phase: this lane's terms of D = G - O (D = G without O; G cleared as it
 // is read), the block sums to topb.  Returns the lane's partial (without coef).
 __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *bwc, int L, int rs, double unit,
-                                              long long *G, const long long *O, int4 box, long long *topb) {
+                                              long long *G, const long long *O, int4 box, long long *topb,
+                                              const unsigned int *maskO = nullptr,
+                                              const unsigned int *maskG = nullptr) {
   const int nb = 1 << L, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   int bside, nblk, lblk;
   grid_blocks(L, bside, nblk, lblk);
@@ -784,9 +804,10 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
   double acc = 0;
   for(int blk = wave; blk < nblk * nblk; blk += nw) {
     const int bz = blk / nblk, br = blk % nblk;
-    const bool hit = box.y >= box.x && box.w >= box.z &&
-                     !(box.y < br * bside || box.x > br * bside + bside - 1 || box.w < bz * bside ||
-                       box.z > bz * bside + bside - 1);
+    bool hit = box.y >= box.x && box.w >= box.z &&
+               !(box.y < br * bside || box.x > br * bside + bside - 1 || box.w < bz * bside ||
+                 box.z > bz * bside + bside - 1);
+    if(maskG) hit = hit && (((maskO[blk >> 5] | maskG[blk >> 5]) >> (blk & 31)) & 1u);  // blocks either log touched
     long long s = 0;
     if(hit) {
       const int pz = bz * (bside >> 1) + jy, pr = br * (bside >> 1) + jx;
@@ -892,7 +913,12 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   const int rs = GMEM ? nb : nb + (USE_O ? MERIT_ROW_PAD : 0), nlds = nb * rs;
   long long *G = GMEM ? A.gscratch + (size_t)blockIdx.x * 2 * nlds : (long long *)(bwc_lds + ncb);
   long long *O = USE_O ? G + nlds : nullptr;
-  int *tbl = GMEM ? (int *)bwc_lds : (int *)(G + (USE_O ? 2 : 1) * nlds);
+  // GMEM: one bit per 16 x 16 block of cells that the current orbit's log (maskO) or the
+  // trial's log (maskG) touched; the pyramid and the delta pass visit only those blocks
+  const int nmw = GMEM ? (nblk * nblk + 31) / 32 : 0;
+  unsigned int *maskO = GMEM ? (unsigned int *)bwc_lds : nullptr;
+  unsigned int *maskG = GMEM ? maskO + nmw : nullptr;
+  int *tbl = GMEM ? (int *)(maskG + nmw) : (int *)(G + (USE_O ? 2 : 1) * nlds);
   __shared__ double s_dF[64];
   __shared__ int s_best, s_n[65];
 
@@ -922,6 +948,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   }
   if(!GMEM)
     for(int q = tid; q < (USE_O ? 2 : 1) * nlds; q += nt) G[q] = 0;
+  for(int q = tid; q < 2 * nmw; q += nt) maskO[q] = 0;
   __syncthreads();
 
   LogRegs R;
@@ -931,7 +958,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       for(int q = tid; q < ncb; q += nt) bwc_lds[q] = bw[(size_t)f * Nstack + q];
     if(USE_O) {
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, L, rs, 1.0f, (unsigned long long *)O);
+      log_add(A, tbl, s_n[0], f, R, nb, L, rs, 1.0f, (unsigned long long *)O, maskO, nblk);
     }
     // each trial's first records are loaded before the previous trial's pyramid,
     // so their latency overlaps it
@@ -939,15 +966,16 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     for(int k = 0; k < K; k++) {
       const int4 bn = A.bbox[o0 + 1 + k];
       const int4 box = make_int4(min(bo.x, bn.x), max(bo.y, bn.y), min(bo.z, bn.z), max(bo.w, bn.w));
-      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, L, rs, 1.0f, (unsigned long long *)G);
+      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, L, rs, 1.0f, (unsigned long long *)G, maskG, nblk);
       if(!USE_O) {
         log_load(A, tbl, s_n[0], f, R);
         log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G);
       }
       if(k + 1 < K) log_load(A, tbl + (2 + k) * A.log_M, s_n[2 + k], f, R);
       __syncthreads();
-      double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, rs, A.log_unit[f], G, O, box, topb);
+      double acc = merit_block(bw + (size_t)f * Nstack, bwc, L, rs, A.log_unit[f], G, O, box, topb, maskO, maskG);
       __syncthreads();
+      for(int q = tid; q < nmw; q += nt) maskG[q] = 0;  // the next trial's marks come after merit_top_levels' barriers
       if(GMEM)
         acc += merit_top_levels(bwc, lblk, A.log_unit[f], top);
       else
@@ -956,6 +984,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       if(lane == 0) part[k * 8 + wave] += acc;
     }
     if(USE_O) clear_box(O, rs, bo);
+    for(int q = tid; q < nmw; q += nt) maskO[q] = 0;
     __syncthreads();
   }
   // instantaneous terms, one trial per thread; then the total per trial
@@ -1020,15 +1049,33 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
     for(int f = 0; f < kFields; f++) {
       log_load(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R);
-      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, L, rs, 1.0f, (unsigned long long *)G);
+      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, L, rs, 1.0f, (unsigned long long *)G, maskG,
+              nblk);
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G);
+      log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G, maskG, nblk);
       __syncthreads();
-      for(int q = tid; q < nr * nzz; q += nt) {
-        const int cell = (z0 + q / nr) * nb + (r0 + q % nr), lc = (z0 + q / nr) * rs + (r0 + q % nr);
-        const long long d = G[lc];
-        G[lc] = 0;
-        if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
+      if(GMEM) {
+        // the 256 cells of each marked block, one block per pass of the workgroup's threads
+        for(int blk = 0; blk < nblk * nblk; blk++) {
+          if(!((maskG[blk >> 5] >> (blk & 31)) & 1u)) continue;
+          const int bz = blk / nblk, br = blk % nblk;
+          for(int q = tid; q < 256; q += nt) {
+            const int iz = bz * 16 + (q >> 4), iR = br * 16 + (q & 15);
+            const int cell = iz * nb + iR, lc = iz * rs + iR;
+            const long long d = G[lc];
+            G[lc] = 0;
+            if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
+          }
+        }
+        __syncthreads();
+        for(int q = tid; q < nmw; q += nt) maskG[q] = 0;
+      } else {
+        for(int q = tid; q < nr * nzz; q += nt) {
+          const int cell = (z0 + q / nr) * nb + (r0 + q % nr), lc = (z0 + q / nr) * rs + (r0 + q % nr);
+          const long long d = G[lc];
+          G[lc] = 0;
+          if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
+        }
       }
       __syncthreads();
     }
