@@ -770,6 +770,46 @@ __device__ __forceinline__ void clear_box(long long *G, int rs, int4 b) {
 __device__ __forceinline__ int morton_x(int lane) { return (lane & 1) | ((lane >> 1) & 2) | ((lane >> 2) & 4); }
 __device__ __forceinline__ int morton_y(int lane) { return ((lane >> 1) & 1) | ((lane >> 2) & 2) | ((lane >> 3) & 4); }
 
+// Sums over groups of 4, 16 and 64 lanes, with lanes in Morton order the
+// children of one node.  Within a row of 16 lanes the data moves with DPP
+// (quad permutations, row rotations), across rows with readlane: no LDS
+// permute, whose round trip would sit on the pyramid's dependency chain.
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int kDppRowRor4 = 0x124;
+constexpr int kDppRowRor8 = 0x128;
+
+template <int CTRL>
+__device__ __forceinline__ long long dpp_i64(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned int)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, 0xF, 0xF, false);
+  return (long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  return __longlong_as_double(dpp_i64<CTRL>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ long long readlane_i64(long long v, int lane) {
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned int)v, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return (long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo);
+}
+// step 0: every lane gets its quad's sum; step 1: its row's (16 lanes); step 2:
+// the wave's (64 lanes).  The groups of step s are the sums of step s - 1.
+// All 64 lanes of the wave must be active.
+__device__ __forceinline__ long long group_sum_i64(long long s, int step) {
+  if(step == 0) {
+    s += dpp_i64<kDppQuadXor1>(s);
+    s += dpp_i64<kDppQuadXor2>(s);
+  } else if(step == 1) {
+    s += dpp_i64<kDppRowRor4>(s);
+    s += dpp_i64<kDppRowRor8>(s);
+  } else {
+    s = readlane_i64(s, 0) + readlane_i64(s, 16) + readlane_i64(s, 32) + readlane_i64(s, 48);
+  }
+  return s;
+}
+
 __device__ __forceinline__ double term_at(const double2 &v, double d) {
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
@@ -830,8 +870,7 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
         acc += term_at(w[4], (double)s * unit);
       }
       for(int l = L - 2, g = 1; l >= lblk; l--, g += 2) {
-        s += __shfl_xor(s, 1 << (g - 1), 64);
-        s += __shfl_xor(s, 1 << g, 64);
+        s = group_sum_i64(s, g >> 1);
         const int mask = (1 << (g + 1)) - 1, sh = L - 1 - l;
         if((lane & mask) == 0 && valid) acc += term_at(bwc[GALIC_STACKOFFSET(l, pz >> sh, pr >> sh)], (double)s * unit);
       }
@@ -852,8 +891,7 @@ __device__ __forceinline__ double merit_top(const double2 *bwc, int L, double un
   long long s = in ? topb[by * nblk + bx] : 0;
   double acc = 0;
   for(int l = lblk - 1, g = 1; l >= 0; l--, g += 2) {
-    s += __shfl_xor(s, 1 << (g - 1), 64);
-    s += __shfl_xor(s, 1 << g, 64);
+    s = group_sum_i64(s, g >> 1);
     const int mask = (1 << (g + 1)) - 1, sh = lblk - l;
     if((lane & mask) == 0 && in) acc += term_at(bwc[GALIC_STACKOFFSET(l, by >> sh, bx >> sh)], (double)s * unit);
   }
@@ -881,9 +919,15 @@ __device__ double merit_top_levels(const double2 *bw, int lblk, double unit, lon
   return acc;
 }
 
+// The sum over the wave's 64 lanes (every lane active), in every lane.
 __device__ __forceinline__ double wave_sum(double v) {
-  for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f64<kDppQuadXor1>(v);
+  v += dpp_f64<kDppQuadXor2>(v);
+  v += dpp_f64<kDppRowRor4>(v);
+  v += dpp_f64<kDppRowRor8>(v);
+  const long long b = __double_as_longlong(v);
+  return (__longlong_as_double(readlane_i64(b, 0)) + __longlong_as_double(readlane_i64(b, 16))) +
+         (__longlong_as_double(readlane_i64(b, 32)) + __longlong_as_double(readlane_i64(b, 48)));
 }
 
 // One particle per workgroup: the merit change of each of its K trials (the
