@@ -223,42 +223,76 @@ __device__ __forceinline__ void run32_flush(const Run32 &r, int nb, int ngrid, f
 // Arena blocks are claimed LOG_CHUNK at a time per lane and handed to the
 // lane's orbits in turn: a claim is an atomic with return on one counter that
 // every lane of the chip shares, and its round trip stalls the whole wave.
+// Records are written in pairs (indices 2m and 2m + 1, 32 contiguous bytes of
+// each field's column of the block): a finished record waits in registers for
+// the next one, so the memory system receives whole 32-byte sectors instead of
+// half sectors that it would merge with what memory holds.
 struct Log32 {
   int n;              // records written
   int blk;            // current arena block
   bool full;          // out of block-table entries or arena blocks
+  bool held;          // rec holds a finished record not yet written
   unsigned int next;  // the lane's claimed blocks [next, end)
   unsigned int end;
+  Run32 rec;
 };
+
+// Claims the arena block that record n starts, if it starts one.
+__device__ __forceinline__ void log32_claim(Log32 &lg, int o, const OrbitArgs32 &A) {
+  if(lg.full || (lg.n & (LOG_B - 1)) != 0) return;
+  const int b = lg.n / LOG_B;
+  unsigned int nb = 0xffffffffu;
+  if(b < A.log_M) {
+    if(lg.next == lg.end) {
+      lg.next = atomicAdd(A.log_next, (unsigned int)LOG_CHUNK);
+      lg.end = lg.next + LOG_CHUNK;
+    }
+    nb = lg.next++;
+  }
+  if(nb >= A.log_cap) {
+    lg.full = true;
+  } else {
+    lg.blk = (int)nb;
+    A.log_blocks[(size_t)o * A.log_M + b] = (int)nb;
+  }
+}
 
 __device__ __forceinline__ void log32_flush(const Run32 &r, Log32 &lg, int o, const OrbitArgs32 &A) {
   if(r.quad < 0) return;
-  const int j = lg.n & (LOG_B - 1);
-  if(!lg.full && j == 0) {
-    const int b = lg.n / LOG_B;
-    unsigned int nb = 0xffffffffu;
-    if(b < A.log_M) {
-      if(lg.next == lg.end) {
-        lg.next = atomicAdd(A.log_next, (unsigned int)LOG_CHUNK);
-        lg.end = lg.next + LOG_CHUNK;
-      }
-      nb = lg.next++;
-    }
-    if(nb >= A.log_cap) {
-      lg.full = true;
-    } else {
-      lg.blk = (int)nb;
-      A.log_blocks[(size_t)o * A.log_M + b] = (int)nb;
+  if(!lg.held) {
+    lg.rec = r;
+    lg.held = true;
+    return;
+  }
+  log32_claim(lg, o, A);  // n is even: the pair shares a block
+  if(!lg.full) {
+    const int j = lg.n & (LOG_B - 1);
+    *(int2 *)(A.log_q + (size_t)lg.blk * LOG_B + j) = make_int2(lg.rec.quad, r.quad);
+    float4 *v = A.log_v + (size_t)lg.blk * kFields * LOG_B + j;
+#pragma unroll
+    for(int f = 0; f < kFields; f++) {
+      v[f * LOG_B] = make_float4(lg.rec.w[f][0], lg.rec.w[f][1], lg.rec.w[f][2], lg.rec.w[f][3]);
+      v[f * LOG_B + 1] = make_float4(r.w[f][0], r.w[f][1], r.w[f][2], r.w[f][3]);
     }
   }
+  lg.n += 2;
+  lg.held = false;
+}
+
+// At the orbit's end: the record still held, alone.
+__device__ __forceinline__ void log32_finish(Log32 &lg, int o, const OrbitArgs32 &A) {
+  if(!lg.held) return;
+  log32_claim(lg, o, A);
   if(!lg.full) {
-    A.log_q[(size_t)lg.blk * LOG_B + j] = r.quad;
+    const int j = lg.n & (LOG_B - 1);
+    A.log_q[(size_t)lg.blk * LOG_B + j] = lg.rec.quad;
     float4 *v = A.log_v + (size_t)lg.blk * kFields * LOG_B + j;
 #pragma unroll
     for(int f = 0; f < kFields; f++)
-      v[f * LOG_B] = make_float4(r.w[f][0], r.w[f][1], r.w[f][2], r.w[f][3]);
+      v[f * LOG_B] = make_float4(lg.rec.w[f][0], lg.rec.w[f][1], lg.rec.w[f][2], lg.rec.w[f][3]);
   }
   lg.n++;
+  lg.held = false;
 }
 
 // Trapezoid weight h of one point of the orbit (orbit_response.c:283-319),
@@ -343,6 +377,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   lg.n = 0;
   lg.blk = 0;
   lg.full = false;
+  lg.held = false;
   lg.next = lg.end = 0;
   auto flush = [&](const Run32 &r) {
     if(LOG)
@@ -426,6 +461,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
       flush(run);
       const int o = L.o;
       if(LOG) {
+        log32_finish(lg, o, A);
         A.log_n[o] = lg.full ? -1 : lg.n;
         if(lg.full) atomicAdd(&A.stats[3], 1ull);
         lg.n = 0;
