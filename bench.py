@@ -39,7 +39,7 @@ from galic_amd.runtime import (TREE_FP32, TREE_PLUMMER, TREE_QUADRUPOLE, GalicGP
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
 # HBM bytes per leapfrog step of k_orbit32_log in the iteration's launches, from the FETCH_SIZE and
 # WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_halo-disk-1e6.txt)
-HBM_BYTES_PER_STEP = 42.8
+HBM_BYTES_PER_STEP = 22.8
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 rate
 PEAK_HBM = 8000.0     # GB/s
 

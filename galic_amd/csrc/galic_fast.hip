@@ -703,9 +703,6 @@ __device__ double inst_delta_q(const DevModel &M, const double *S, const int *cu
   return total;
 }
 
-// One workgroup per particle: the change of the merit for each of its K trials
-// (FP32 slots, FP64 sums on the fixed cuts), the best trial taken when it lowers
-// the merit, deltas added as fixed point, and all K+1 slots cleared.
 // Node terms of the orbit-averaged merit, refreshed after every commit: b =
 // response - target and w = the node's weight, 1/min(dist, R200/20) for the
 // mass term and (Tm/E_x)/min(dist, R200/20) for an S term on the cut nodes, 0
