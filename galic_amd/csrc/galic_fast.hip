@@ -847,14 +847,15 @@ __device__ __forceinline__ double term_at(const double2 &v, double d) {
   return (v.y != 0 && d != 0) ? (fabs(v.x + d) - fabs(v.x)) * v.y : 0.0;
 }
 
-// The part of the merit pyramid one lane evaluates, for grids of at most 128^2
-// cells.  The grid is cut into blocks of 16 x 16 fine cells (the whole grid
-// below L = 4); a wave takes blocks wave, wave + nwaves, ...  In a block each
-// lane owns a 2x2 cell node, lanes in Morton order, so that groups of 4, 16 and
-// 64 lanes are the node's parent, grandparent and the block node, summed with
-// shuffles.  Wave 0 then forms the levels above the blocks from their sums.
-// The (residual, weight) pairs of fine cells and level L-1 nodes come from
-// memory (bw), those of the coarser levels from LDS (bwc, indexed by node).
+// The part of the merit pyramid one lane evaluates.  The grid is cut into
+// blocks of 16 x 16 fine cells (the whole grid below L = 4); a wave takes blocks
+// wave, wave + nwaves, ...  In a block each lane owns a 2x2 cell node, lanes in
+// Morton order, so that groups of 4, 16 and 64 lanes are the node's parent,
+// grandparent and the block node, summed with group_sum_i64.  The levels above
+// the blocks are formed from the block sums afterwards (merit_top up to 8 x 8
+// blocks, merit_top_levels above).  The (residual, weight) pairs of fine cells
+// and level L-1 nodes come from memory (bw), those of the coarser levels from
+// bwc (LDS, indexed by node, up to 128^2 cells; memory above).
 __device__ __forceinline__ void grid_blocks(int L, int &bside, int &nblk, int &lblk) {
   const int LB = L < 4 ? L : 4;
   bside = 1 << LB;
