@@ -1532,9 +1532,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   const bool gmem = c.L > 7;
   const int kGBlocks = 512;
   auto eval_lds_gmem = [&]() {
-    const size_t nblk = (size_t)1 << (c.L - 4);
     return 8 * (size_t)K * sizeof(double) + sizeof(long long) * (size_t)GALIC_NSTACK(c.L - 4) +
-           2 * sizeof(unsigned int) * ((nblk * nblk + 31) / 32) + sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
+           2 * sizeof(unsigned int) * (((size_t)c.Ngrid / 4 + 31) / 32) + sizeof(int) * (size_t)(K + 1) * c.log_M +
+           1024;
   };
   if(gmem) {
     const size_t need = (size_t)kGBlocks * 2 * c.Ngrid;

@@ -754,12 +754,13 @@ __device__ __forceinline__ void log_load(const EvalArgs32 &A, const int *tbl, in
 // integer atomic, so the sums are exact and independent of the order.
 // quad = iz * nb + iR names the 2x2 cells (iz..iz+1, iR..iR+1), nb = 2^L; the
 // LDS grid holds cell (iz, iR) at iz * rs + iR.
-// Marks the 16 x 16 blocks of cells the 2x2 cells of quad fall in (grids above 128^2,
-// where the pyramid visits only marked blocks); nblk blocks per row.
-__device__ __forceinline__ void mark_blocks(unsigned int *mask, int quad, int nb, int L, int nblk) {
-  const int iz = quad >> L, iR = quad & (nb - 1);
-  const int z0 = iz >> 4, z1 = (iz + 1) >> 4, r0 = iR >> 4, r1 = (iR + 1) >> 4;
-  const int b[4] = {z0 * nblk + r0, z0 * nblk + r1, z1 * nblk + r0, z1 * nblk + r1};
+// Marks the level L-1 nodes (2 x 2 cells) that the 2x2 cells of quad fall in
+// (grids above 128^2, where the pyramid and the delta pass read only the cells
+// of marked nodes); nb / 2 nodes per row.
+__device__ __forceinline__ void mark_nodes(unsigned int *mask, int quad, int nb, int L) {
+  const int iz = quad >> L, iR = quad & (nb - 1), nbp = nb >> 1;
+  const int z0 = iz >> 1, z1 = (iz + 1) >> 1, r0 = iR >> 1, r1 = (iR + 1) >> 1;
+  const int b[4] = {z0 * nbp + r0, z0 * nbp + r1, z1 * nbp + r0, z1 * nbp + r1};
 #pragma unroll
   for(int k = 0; k < 4; k++) {
     const unsigned int bit = 1u << (b[k] & 31);
@@ -777,21 +778,35 @@ __device__ __forceinline__ void add_record(unsigned long long *G, int quad, floa
 
 // Adds sign * field f of a whole log into G: the records held in R, then the rest.
 __device__ __forceinline__ void log_add(const EvalArgs32 &A, const int *tbl, int n, int f, const LogRegs &R, int nb,
-                                        int L, int rs, float sign, unsigned long long *G, unsigned int *mask = nullptr,
-                                        int nblk = 0) {
+                                        int L, int rs, float sign, unsigned long long *G, unsigned int *mask = nullptr) {
   const float sc = sign * A.log_scale[f];
 #pragma unroll
   for(int u = 0; u < LOG_RPT; u++)
     if(R.quad[u] >= 0) {
       add_record(G, R.quad[u], R.x[u], sc, nb, L, rs);
-      if(mask) mark_blocks(mask, R.quad[u], nb, L, nblk);
+      if(mask) mark_nodes(mask, R.quad[u], nb, L);
     }
   for(int r = threadIdx.x + LOG_RPT * blockDim.x; r < n; r += blockDim.x) {
     const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
     const int quad = A.log_q[(size_t)blk * LOG_B + jj];
     add_record(G, quad, A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb, L, rs);
-    if(mask) mark_blocks(mask, quad, nb, L, nblk);
+    if(mask) mark_nodes(mask, quad, nb, L);
   }
+}
+
+// Zeroes the four cells of every level L-1 node marked in mask (row stride nb;
+// nmw mask words, nb / 2 nodes per row).
+__device__ __forceinline__ void clear_marked(long long *G, int nb, const unsigned int *mask, int nmw) {
+  const int nbp = nb >> 1;
+  for(int q = threadIdx.x; q < nmw; q += blockDim.x)
+    for(unsigned int bits = mask[q]; bits; bits &= bits - 1) {
+      const int nd = q * 32 + __builtin_ctz(bits), iz = 2 * (nd / nbp), iR = 2 * (nd % nbp);
+      long long *g = G + iz * nb + iR;
+      g[0] = 0;
+      g[1] = 0;
+      g[nb] = 0;
+      g[nb + 1] = 0;
+    }
 }
 
 // Zeroes G (row stride rs) over the box of fine cells.
@@ -881,11 +896,28 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
     bool hit = box.y >= box.x && box.w >= box.z &&
                !(box.y < br * bside || box.x > br * bside + bside - 1 || box.w < bz * bside ||
                  box.z > bz * bside + bside - 1);
-    if(maskG) hit = hit && (((maskO[blk >> 5] | maskG[blk >> 5]) >> (blk & 31)) & 1u);  // blocks either log touched
     long long s = 0;
+    const int pz = bz * (bside >> 1) + jy, pr = br * (bside >> 1) + jx;
+    // with masks: only the lanes whose node either log touched read memory (the
+    // other nodes' cells are zero in both grids), and a block without one is skipped
+    bool mine = valid;
+    if(maskG) {
+      const int nd = pz * (nb >> 1) + pr;
+      mine = valid && ((((maskO[nd >> 5] | maskG[nd >> 5]) >> (nd & 31)) & 1u) != 0);
+      hit = hit && __ballot(mine) != 0;
+    }
     if(hit) {
-      const int pz = bz * (bside >> 1) + jy, pr = br * (bside >> 1) + jx;
-      if(valid) {
+      // with masks the grids and all pairs are in memory: the pairs of the block's
+      // coarse levels (levels L-2 .. L-4, used by the group leaders) are loaded with
+      // the fine ones, so a block costs one round trip to memory
+      double2 wc[3];
+      if(maskG)
+#pragma unroll
+        for(int u = 0; u < 3; u++) {
+          const int l = L - 2 - u, sh = L - 1 - l;
+          if(l >= lblk) wc[u] = bwc[GALIC_STACKOFFSET(l, pz >> sh, pr >> sh)];
+        }
+      if(mine) {
         int cell[4];
         double2 w[5];
 #pragma unroll
@@ -903,10 +935,14 @@ __device__ __forceinline__ double merit_block(const double2 *bw, const double2 *
         }
         acc += term_at(w[4], (double)s * unit);
       }
-      for(int l = L - 2, g = 1; l >= lblk; l--, g += 2) {
-        s = group_sum_i64(s, g >> 1);
+#pragma unroll
+      for(int u = 0; u < 3; u++) {
+        const int l = L - 2 - u, g = 2 * u + 1;
+        if(l < lblk) break;
+        s = group_sum_i64(s, u);
         const int mask = (1 << (g + 1)) - 1, sh = L - 1 - l;
-        if((lane & mask) == 0 && valid) acc += term_at(bwc[GALIC_STACKOFFSET(l, pz >> sh, pr >> sh)], (double)s * unit);
+        if((lane & mask) == 0 && valid)
+          acc += term_at(maskG ? wc[u] : bwc[GALIC_STACKOFFSET(l, pz >> sh, pr >> sh)], (double)s * unit);
       }
     }
     if(lane == 0) topb[blk] = s;  // 0 for blocks the box misses
@@ -991,9 +1027,9 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   const int rs = GMEM ? nb : nb + (USE_O ? MERIT_ROW_PAD : 0), nlds = nb * rs;
   long long *G = GMEM ? A.gscratch + (size_t)blockIdx.x * 2 * nlds : (long long *)(bwc_lds + ncb);
   long long *O = USE_O ? G + nlds : nullptr;
-  // GMEM: one bit per 16 x 16 block of cells that the current orbit's log (maskO) or the
-  // trial's log (maskG) touched; the pyramid and the delta pass visit only those blocks
-  const int nmw = GMEM ? (nblk * nblk + 31) / 32 : 0;
+  // GMEM: one bit per level L-1 node (2 x 2 cells) that the current orbit's log (maskO) or
+  // the trial's log (maskG) touched; the pyramid and the delta pass read only those nodes' cells
+  const int nmw = GMEM ? (ngrid / 4 + 31) / 32 : 0;
   unsigned int *maskO = GMEM ? (unsigned int *)bwc_lds : nullptr;
   unsigned int *maskG = GMEM ? maskO + nmw : nullptr;
   int *tbl = GMEM ? (int *)(maskG + nmw) : (int *)(G + (USE_O ? 2 : 1) * nlds);
@@ -1036,7 +1072,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       for(int q = tid; q < ncb; q += nt) bwc_lds[q] = bw[(size_t)f * Nstack + q];
     if(USE_O) {
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, L, rs, 1.0f, (unsigned long long *)O, maskO, nblk);
+      log_add(A, tbl, s_n[0], f, R, nb, L, rs, 1.0f, (unsigned long long *)O, maskO);
     }
     // each trial's first records are loaded before the previous trial's pyramid,
     // so their latency overlaps it
@@ -1044,7 +1080,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     for(int k = 0; k < K; k++) {
       const int4 bn = A.bbox[o0 + 1 + k];
       const int4 box = make_int4(min(bo.x, bn.x), max(bo.y, bn.y), min(bo.z, bn.z), max(bo.w, bn.w));
-      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, L, rs, 1.0f, (unsigned long long *)G, maskG, nblk);
+      log_add(A, tbl + (1 + k) * A.log_M, s_n[1 + k], f, R, nb, L, rs, 1.0f, (unsigned long long *)G, maskG);
       if(!USE_O) {
         log_load(A, tbl, s_n[0], f, R);
         log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G);
@@ -1061,7 +1097,11 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       acc = wave_sum(coef[f] * acc);
       if(lane == 0) part[k * 8 + wave] += acc;
     }
-    if(USE_O) clear_box(O, rs, bo);
+    if(GMEM)
+      clear_marked(O, nb, maskO, nmw);
+    else if(USE_O)
+      clear_box(O, rs, bo);
+    __syncthreads();
     for(int q = tid; q < nmw; q += nt) maskO[q] = 0;
     __syncthreads();
   }
@@ -1127,22 +1167,23 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     const int nr = (r1 < r0) ? 0 : r1 - r0 + 1, nzz = (z1 < z0) ? 0 : z1 - z0 + 1;
     for(int f = 0; f < kFields; f++) {
       log_load(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R);
-      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, L, rs, 1.0f, (unsigned long long *)G, maskG,
-              nblk);
+      log_add(A, tbl + (1 + best) * A.log_M, s_n[1 + best], f, R, nb, L, rs, 1.0f, (unsigned long long *)G, maskG);
       log_load(A, tbl, s_n[0], f, R);
-      log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G, maskG, nblk);
+      log_add(A, tbl, s_n[0], f, R, nb, L, rs, -1.0f, (unsigned long long *)G, maskG);
       __syncthreads();
       if(GMEM) {
-        // the 256 cells of each marked block, one block per pass of the workgroup's threads
-        for(int blk = 0; blk < nblk * nblk; blk++) {
-          if(!((maskG[blk >> 5] >> (blk & 31)) & 1u)) continue;
-          const int bz = blk / nblk, br = blk % nblk;
-          for(int q = tid; q < 256; q += nt) {
-            const int iz = bz * 16 + (q >> 4), iR = br * 16 + (q & 15);
-            const int cell = iz * nb + iR, lc = iz * rs + iR;
-            const long long d = G[lc];
-            G[lc] = 0;
-            if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
+        // the four cells of each marked node, a thread per mask word
+        const int nbp = nb >> 1;
+        for(int q = tid; q < nmw; q += nt) {
+          for(unsigned int bits = maskG[q]; bits; bits &= bits - 1) {
+            const int nd = q * 32 + __builtin_ctz(bits), iz = 2 * (nd / nbp), iR = 2 * (nd % nbp);
+#pragma unroll
+            for(int c = 0; c < 4; c++) {
+              const int cell = (iz + (c >> 1)) * nb + iR + (c & 1);
+              const long long d = G[cell];
+              G[cell] = 0;
+              if(d) atomic_q(&lad[(size_t)f * ngrid + cell], (double)d * A.log_to_loc[f]);
+            }
           }
         }
         __syncthreads();
