@@ -960,8 +960,11 @@ __device__ __forceinline__ double merit_top(const double2 *bwc, int L, double un
   const bool in = bx < nblk && by < nblk;
   long long s = in ? topb[by * nblk + bx] : 0;
   double acc = 0;
-  for(int l = lblk - 1, g = 1; l >= 0; l--, g += 2) {
-    s = group_sum_i64(s, g >> 1);
+#pragma unroll
+  for(int u = 0; u < 3; u++) {
+    const int l = lblk - 1 - u, g = 2 * u + 1;
+    if(l < 0) break;
+    s = group_sum_i64(s, u);
     const int mask = (1 << (g + 1)) - 1, sh = lblk - l;
     if((lane & mask) == 0 && in) acc += term_at(bwc[GALIC_STACKOFFSET(l, by >> sh, bx >> sh)], (double)s * unit);
   }
