@@ -1669,14 +1669,21 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       if(gmem) {
         for(int b0 = 0; b0 < b - a; b0 += kGBlocks) {
           E.block_base = b0;
-          hipLaunchKernelGGL((galic::k_eval_log32<true, true>), dim3(std::min(kGBlocks, b - a - b0)), dim3(512), shmem,
+          hipLaunchKernelGGL((galic::k_eval_log32<true, true, 0>), dim3(std::min(kGBlocks, b - a - b0)), dim3(512), shmem,
                              c.stream, c.M, E);
           HIPCHK(hipGetLastError());
         }
       } else if(use_o) {
-        hipLaunchKernelGGL((galic::k_eval_log32<true, false>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+        // the common grid levels with their index arithmetic fixed at compile time
+        if(c.L == 6)
+          hipLaunchKernelGGL((galic::k_eval_log32<true, false, 6>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+        else
+          hipLaunchKernelGGL((galic::k_eval_log32<true, false, 0>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       } else {
-        hipLaunchKernelGGL((galic::k_eval_log32<false, false>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+        if(c.L == 7)
+          hipLaunchKernelGGL((galic::k_eval_log32<false, false, 7>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+        else
+          hipLaunchKernelGGL((galic::k_eval_log32<false, false, 0>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       }
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c.ev1, c.stream));

@@ -106,7 +106,7 @@ __global__ void k_energy32(DevModel M, int n, const double *ox, const double *oy
 int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigned int *key, unsigned int *key2,
                          int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
 __global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
-template <bool USE_O, bool GMEM>
+template <bool USE_O, bool GMEM, int LF>
 __global__ void k_eval_log32(DevModel M, EvalArgs32 A);
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);

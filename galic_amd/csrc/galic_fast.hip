@@ -1011,11 +1011,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 // do not fit in LDS (128^2 cells) both logs go into G for every trial.
 // LDS: part [K][8] double | topb [64] int64 | bwc [coarse nodes] double2 | G [nb rows] int64 |
 //      O [nb rows] int64 (USE_O) | tbl [K+1][log_M] int; a row of G and O holds rs cells
-template <bool USE_O, bool GMEM>
+template <bool USE_O, bool GMEM, int LF>
 __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
-  const int L = M.DG_L, nb = M.DG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
+  // LF: the grid level fixed at compile time (0: M.DG_L), so that the index arithmetic of
+  // the pyramid folds
+  const int L = LF ? LF : M.DG_L, nb = 1 << L, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
   const int ncb = GALIC_NSTACK(L - 2);  // nodes of levels 0..L-2
   int bside, nblk, lblk;
   grid_blocks(L, bside, nblk, lblk);
@@ -1204,9 +1206,11 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   }
 }
 
-template __global__ void k_eval_log32<true, false>(DevModel M, EvalArgs32 A);
-template __global__ void k_eval_log32<false, false>(DevModel M, EvalArgs32 A);
-template __global__ void k_eval_log32<true, true>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<true, false, 0>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<true, false, 6>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<false, false, 0>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<false, false, 7>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<true, true, 0>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 
