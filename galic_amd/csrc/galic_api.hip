@@ -1523,7 +1523,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   // the merit kernel's LDS: per-wave partials, block sums, the coarse nodes' terms, one or two
   // int64 grids and the K+1 block tables; two grids where they fit
   auto eval_lds = [&](int ngrids) {
-    return (8 * (size_t)K + 64) * sizeof(double) + sizeof(double2) * (size_t)GALIC_NSTACK(std::max(c.L - 2, 0)) +
+    return (galic::MERIT_MAX_WAVES * (size_t)K + 64) * sizeof(double) +
+           sizeof(double2) * (size_t)GALIC_NSTACK(std::max(c.L - 2, 0)) +
            ngrids * sizeof(long long) * (size_t)c.Nbin * (c.Nbin + (ngrids == 2 ? galic::MERIT_ROW_PAD : 0)) +
            sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
   };
@@ -1532,7 +1533,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   const bool gmem = c.L > 7;
   const int kGBlocks = 512;
   auto eval_lds_gmem = [&]() {
-    return 8 * (size_t)K * sizeof(double) + sizeof(long long) * (size_t)GALIC_NSTACK(c.L - 4) +
+    return galic::MERIT_MAX_WAVES * (size_t)K * sizeof(double) + sizeof(long long) * (size_t)GALIC_NSTACK(c.L - 4) +
            2 * sizeof(unsigned int) * (((size_t)c.Ngrid / 4 + 31) / 32) + sizeof(int) * (size_t)(K + 1) * c.log_M +
            1024;
   };
@@ -1681,7 +1682,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
           hipLaunchKernelGGL((galic::k_eval_log32<true, false, 0>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       } else {
         if(c.L == 7)
-          hipLaunchKernelGGL((galic::k_eval_log32<false, false, 7>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
+          hipLaunchKernelGGL((galic::k_eval_log32<false, false, 7>), dim3(b - a), dim3(galic::merit_threads(7)), shmem,
+                             c.stream, c.M, E);
         else
           hipLaunchKernelGGL((galic::k_eval_log32<false, false, 0>), dim3(b - a), dim3(512), shmem, c.stream, c.M, E);
       }

@@ -1012,7 +1012,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // LDS: part [K][8] double | topb [64] int64 | bwc [coarse nodes] double2 | G [nb rows] int64 |
 //      O [nb rows] int64 (USE_O) | tbl [K+1][log_M] int; a row of G and O holds rs cells
 template <bool USE_O, bool GMEM, int LF>
-__global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
+__global__ void __launch_bounds__(merit_threads(LF)) k_eval_log32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
   // LF: the grid level fixed at compile time (0: M.DG_L), so that the index arithmetic of
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
   int bside, nblk, lblk;
   grid_blocks(L, bside, nblk, lblk);
   double *part = lds_d;
-  long long *top = (long long *)(part + 8 * K);  // GMEM: the pyramid of block sums, levels 0..lblk
+  long long *top = (long long *)(part + MERIT_MAX_WAVES * K);  // GMEM: the pyramid of block sums, levels 0..lblk
   long long *topb = GMEM ? top + GALIC_STACKOFFSET(lblk, 0, 0) : top;
   const int ntop = GMEM ? GALIC_NSTACK(lblk) : 64;
   double2 *bwc_lds = (double2 *)(top + ntop);
@@ -1059,7 +1059,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
 
   // block tables of the K+1 logs, clean grids, zero merit sums
   for(int q = tid; q <= K; q += nt) s_n[q] = A.log_n[o0 + q];
-  for(int q = tid; q < 8 * K; q += nt) part[q] = 0;
+  for(int q = tid; q < MERIT_MAX_WAVES * K; q += nt) part[q] = 0;
   __syncthreads();
   for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
     const int o = q / A.log_M, b = q % A.log_M;
@@ -1100,7 +1100,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
       else
         acc += merit_top(bwc, L, A.log_unit[f], topb);
       acc = wave_sum(coef[f] * acc);
-      if(lane == 0) part[k * 8 + wave] += acc;
+      if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
     }
     if(GMEM)
       clear_marked(O, nb, maskO, nmw);
@@ -1120,7 +1120,7 @@ __global__ void __launch_bounds__(512) k_eval_log32(DevModel M, EvalArgs32 A) {
     double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
     const double *fit = A.fit + (size_t)t * 16;
     double orbit_terms = 0;
-    for(int w = 0; w < 8; w++) orbit_terms += part[k * 8 + w];
+    for(int w = 0; w < MERIT_MAX_WAVES; w++) orbit_terms += part[k * MERIT_MAX_WAVES + w];
     s_dF[k] = orbit_terms + fit[5] / fit[6] * vinst;
   }
   __syncthreads();
