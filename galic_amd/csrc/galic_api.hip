@@ -1670,8 +1670,12 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       if(gmem) {
         for(int b0 = 0; b0 < b - a; b0 += kGBlocks) {
           E.block_base = b0;
-          hipLaunchKernelGGL((galic::k_eval_log32<true, true, 0>), dim3(std::min(kGBlocks, b - a - b0)), dim3(512), shmem,
-                             c.stream, c.M, E);
+          if(c.L == 9)
+            hipLaunchKernelGGL((galic::k_eval_log32<true, true, 9>), dim3(std::min(kGBlocks, b - a - b0)),
+                               dim3(galic::merit_threads(9)), shmem, c.stream, c.M, E);
+          else
+            hipLaunchKernelGGL((galic::k_eval_log32<true, true, 0>), dim3(std::min(kGBlocks, b - a - b0)), dim3(512),
+                               shmem, c.stream, c.M, E);
           HIPCHK(hipGetLastError());
         }
       } else if(use_o) {
