@@ -1673,6 +1673,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
           if(c.L == 9)
             hipLaunchKernelGGL((galic::k_eval_log32<true, true, 9>), dim3(std::min(kGBlocks, b - a - b0)),
                                dim3(galic::merit_threads(9)), shmem, c.stream, c.M, E);
+          else if(c.L == 8)
+            hipLaunchKernelGGL((galic::k_eval_log32<true, true, 8>), dim3(std::min(kGBlocks, b - a - b0)),
+                               dim3(galic::merit_threads(8)), shmem, c.stream, c.M, E);
           else
             hipLaunchKernelGGL((galic::k_eval_log32<true, true, 0>), dim3(std::min(kGBlocks, b - a - b0)), dim3(512),
                                shmem, c.stream, c.M, E);

@@ -59,11 +59,11 @@ constexpr int LOG_CHUNK = 8;
 // then fall in different LDS banks.
 constexpr int MERIT_ROW_PAD = 2;
 // Threads of a merit workgroup: 512; 1024 for 128^2 grids, where one workgroup
-// fills the LDS of a CU and brings all the waves the CU runs, and for 512^2 grids,
-// where more waves share one particle's long chain of blocks.  The per-wave merit
+// fills the LDS of a CU and brings all the waves the CU runs, and for 256^2 and
+// 512^2 grids, where more waves share one particle's long chain of blocks.  The per-wave merit
 // sums have room for MERIT_MAX_WAVES waves.
 constexpr int MERIT_MAX_WAVES = 16;
-__host__ __device__ constexpr int merit_threads(int level) { return level == 7 || level == 9 ? 1024 : 512; }
+__host__ __device__ constexpr int merit_threads(int level) { return level >= 7 ? 1024 : 512; }
 
 struct EvalArgs32 {
   Particles P;

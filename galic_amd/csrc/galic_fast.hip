@@ -1211,6 +1211,7 @@ template __global__ void k_eval_log32<true, false, 6>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_log32<false, false, 0>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_log32<false, false, 7>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_log32<true, true, 0>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_log32<true, true, 8>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_log32<true, true, 9>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
