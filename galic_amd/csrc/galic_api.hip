@@ -922,12 +922,40 @@ int galic_sample_targets(galic_ctx *ctx, int type, long long nsamples, unsigned 
     A.pos_out = dpts;
   }
   const int grid = orbit_grid((int)std::min<long long>(nsamples, 1 << 30), 256, 8);
+  // The maps are summed per workgroup in LDS where they fit in kTargetLds bytes (64^2
+  // grids: the density map, then the five energy maps two at a time), so the
+  // atomics on the small maps in memory are few; each launch draws the same points
+  // again (cheap beside the atomics), and only the first counts them and writes them out.
+  constexpr size_t kTargetLds = 65536;
+  const size_t per_map = ng * sizeof(long long), fit = kTargetLds / per_map;
+  struct Pass {
+    bool dg;
+    int f0, f1;
+  };
+  std::vector<Pass> passes;
+  if(fit == 0) {
+    passes.push_back({dgs_mass_target != nullptr, 0, egs_targets ? 5 : 0});
+  } else {
+    if(dgs_mass_target) passes.push_back({true, 0, 0});
+    if(egs_targets)
+      for(int f0 = 0; f0 < 5; f0 += (int)fit) passes.push_back({false, f0, std::min(5, f0 + (int)fit)});
+  }
   HIPCHK(hipEventRecord(c.ev0, c.stream));
-  if(galic::launch_sample_targets(A, grid, c.stream) != 0) {
-    dfree(maps);
-    dfree(counts);
-    dfree(dpts);
-    return set_err(GALIC_ERR_HIP, "target sampling launch failed");
+  for(size_t i = 0; i < passes.size(); i++) {
+    galic::TargetArgs P = A;
+    P.dg = passes[i].dg ? A.dg : nullptr;
+    P.f0 = passes[i].f0;
+    P.f1 = passes[i].f1;
+    P.lds = fit > 0 ? 1 : 0;
+    P.count = i == 0 ? 1 : 0;
+    P.pos_out = i == 0 ? A.pos_out : nullptr;
+    const size_t shmem = P.lds ? per_map * ((P.dg ? 1 : 0) + (size_t)(P.f1 - P.f0)) : 0;
+    if(galic::launch_sample_targets(P, grid, shmem, c.stream) != 0) {
+      dfree(maps);
+      dfree(counts);
+      dfree(dpts);
+      return set_err(GALIC_ERR_HIP, "target sampling launch failed");
+    }
   }
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipEventSynchronize(c.ev1));

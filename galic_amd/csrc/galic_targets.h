@@ -35,9 +35,13 @@ struct TargetArgs {
   double qeg[5];            // and of each eg map
   double *pos_out;          // [n][3] the sample points (0 for ids without an accepted draw), or null
   unsigned long long *counts;  // [0] points binned, [1] ids without an accepted draw
+  int count;                // add to counts (one launch of a sampling does)
+  int f0, f1;               // the eg maps this launch bins: f0 .. f1-1
+  int lds;                  // sum the launch's maps per workgroup in LDS first (dynamic shared memory
+                            // of 8 bytes per cell of each map: dg first, then eg maps f0 .. f1-1)
 };
 
 // Grid-stride launch over the sample ids on stream s.
-int launch_sample_targets(const TargetArgs &A, int grid, hipStream_t s);
+int launch_sample_targets(const TargetArgs &A, int grid, size_t shmem, hipStream_t s);
 
 }  // namespace galic

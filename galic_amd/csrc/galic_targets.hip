@@ -143,11 +143,20 @@ __device__ __forceinline__ void disp_rtp(const TargetArgs &A, const double pos[3
   }
 }
 
-__device__ __forceinline__ void add_q(long long *p, double v) {
-  atomicAdd((unsigned long long *)p, (unsigned long long)__double2ll_rn(v));
-}
+__device__ __forceinline__ unsigned long long to_q(double v) { return (unsigned long long)__double2ll_rn(v); }
 
+// The maps of one launch (A.dg and/or A.eg maps f0 .. f1-1), summed per
+// workgroup in LDS when A.lds and added to memory at the end, else added to
+// memory directly.  Integer sums: the result is the same bits either way.
 __global__ void __launch_bounds__(256) k_sample_targets(TargetArgs A) {
+  extern __shared__ unsigned long long smap[];
+  const int ngd = A.M.DG_Nbin * A.M.DG_Nbin, nge = A.M.EG_Nbin * A.M.EG_Nbin;
+  const int ndl = A.dg ? ngd : 0, nl = ndl + (A.eg ? (A.f1 - A.f0) * nge : 0);
+  unsigned long long *sdg = smap, *seg = smap + ndl;
+  if(A.lds) {
+    for(int q = threadIdx.x; q < nl; q += blockDim.x) smap[q] = 0;
+    __syncthreads();
+  }
   const long long stride = (long long)gridDim.x * blockDim.x;
   unsigned long long binned = 0, failed = 0;
   for(long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < A.n; id += stride) {
@@ -170,34 +179,55 @@ __global__ void __launch_bounds__(256) k_sample_targets(TargetArgs A) {
       double fR, fz;
       cell64(pos[0], pos[1], pos[2], A.M.DG_Rmin, A.M.DG_logFac, A.M.DG_Nbin, iR, iz, fR, fz);
       const int nb = A.M.DG_Nbin;
-      add_q(&A.dg[iz * nb + iR], A.qdg * ((1 - fR) * (1 - fz)));
-      add_q(&A.dg[iz * nb + (iR + 1)], A.qdg * ((fR) * (1 - fz)));
-      add_q(&A.dg[(iz + 1) * nb + iR], A.qdg * ((1 - fR) * (fz)));
-      add_q(&A.dg[(iz + 1) * nb + (iR + 1)], A.qdg * ((fR) * (fz)));
-    }
-    if(A.eg) {
-      int iR, iz;
-      double fR, fz;
-      cell64(pos[0], pos[1], pos[2], A.M.EG_Rmin, A.M.EG_logFac, A.M.EG_Nbin, iR, iz, fR, fz);
-      const int nb = A.M.EG_Nbin, ngrid = nb * nb;
-      double d[4];
-      disp_rtp(A, pos, d);
       const double w[4] = {(1 - fR) * (1 - fz), (fR) * (1 - fz), (1 - fR) * (fz), (fR) * (fz)};
       const int c[4] = {iz * nb + iR, iz * nb + (iR + 1), (iz + 1) * nb + iR, (iz + 1) * nb + (iR + 1)};
       for(int k = 0; k < 4; k++) {
-        add_q(&A.eg[c[k]], A.qeg[0] * w[k]);
-        for(int f = 0; f < 4; f++) add_q(&A.eg[(size_t)(1 + f) * ngrid + c[k]], A.qeg[1 + f] * (w[k] * d[f]));
+        const unsigned long long q = to_q(A.qdg * w[k]);
+        if(A.lds)
+          atomicAdd(&sdg[c[k]], q);
+        else
+          atomicAdd((unsigned long long *)&A.dg[c[k]], q);
       }
     }
+    if(A.eg && A.f1 > A.f0) {
+      int iR, iz;
+      double fR, fz;
+      cell64(pos[0], pos[1], pos[2], A.M.EG_Rmin, A.M.EG_logFac, A.M.EG_Nbin, iR, iz, fR, fz);
+      const int nb = A.M.EG_Nbin;
+      double d[4] = {0, 0, 0, 0};
+      if(A.f1 > 1) disp_rtp(A, pos, d);
+      const double w[4] = {(1 - fR) * (1 - fz), (fR) * (1 - fz), (1 - fR) * (fz), (fR) * (fz)};
+      const int c[4] = {iz * nb + iR, iz * nb + (iR + 1), (iz + 1) * nb + iR, (iz + 1) * nb + (iR + 1)};
+      for(int k = 0; k < 4; k++)
+        for(int f = A.f0; f < A.f1; f++) {
+          const unsigned long long q = to_q(A.qeg[f] * (f == 0 ? w[k] : w[k] * d[f - 1]));
+          if(A.lds)
+            atomicAdd(&seg[(size_t)(f - A.f0) * nge + c[k]], q);
+          else
+            atomicAdd((unsigned long long *)&A.eg[(size_t)f * nge + c[k]], q);
+        }
+    }
   }
-  if(binned) atomicAdd(&A.counts[0], binned);
-  if(failed) atomicAdd(&A.counts[1], failed);
+  if(A.lds) {
+    __syncthreads();
+    for(int q = threadIdx.x; q < nl; q += blockDim.x) {
+      const unsigned long long v = smap[q];
+      if(v == 0) continue;
+      unsigned long long *dst = q < ndl ? (unsigned long long *)&A.dg[q]
+                                        : (unsigned long long *)&A.eg[(size_t)A.f0 * nge + (q - ndl)];
+      atomicAdd(dst, v);
+    }
+  }
+  if(A.count) {
+    if(binned) atomicAdd(&A.counts[0], binned);
+    if(failed) atomicAdd(&A.counts[1], failed);
+  }
 }
 
 }  // namespace
 
-int launch_sample_targets(const TargetArgs &A, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_sample_targets, dim3(grid), dim3(256), 0, s, A);
+int launch_sample_targets(const TargetArgs &A, int grid, size_t shmem, hipStream_t s) {
+  hipLaunchKernelGGL(k_sample_targets, dim3(grid), dim3(256), shmem, s, A);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
