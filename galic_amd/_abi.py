@@ -59,7 +59,7 @@ EXPORTS = [
     "galic_write_fit_line", "galic_write_densfield", "galic_write_snapshot",
     "galic_set_particles", "galic_set_dispersions", "galic_get_particles", "galic_calc_all_response_fields", "galic_rep",
     "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_fp32_orbit_responses", "galic_field_at",
-    "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_fp32_set_fresh_trials", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_direct_forces_mode", "galic_tree_last_timings",
+    "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_fp32_set_fresh_trials", "galic_fp32_trial_merits", "galic_fp32_set_merit_form", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_direct_forces_mode", "galic_tree_last_timings",
     "galic_timings", "galic_reset_timings", "galic_copy_probe", "galic_bind",
     "produce_orbit_response_field_mod", "produce_orbit_response_field",
     "forcegrid_get_potential", "forcegrid_get_escape_speed", "forcegrid_get_acceleration", "smooth_stack",
@@ -127,6 +127,8 @@ def lib():
         "galic_fp32_autotune": [_vp, ctypes.c_int, _ip, _ip],
         "galic_fp32_set_launch": [_vp, ctypes.c_int, ctypes.c_int],
         "galic_fp32_set_fresh_trials": [_vp, ctypes.c_int],
+        "galic_fp32_trial_merits": [_vp, ctypes.c_int, ctypes.c_int, _dp],
+        "galic_fp32_set_merit_form": [_vp, ctypes.c_int],
         "galic_tree_forces": [_P(GalicParams), ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                               ctypes.c_int, _lp],
         "galic_tree_fill": [_P(GalicParams), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,

@@ -42,7 +42,7 @@ def hipcc():
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     cc = hipcc()
-    objs = []
+    objs, procs = [], []
     for unit, extra in UNITS.items():
         src = os.path.join(CSRC, unit)
         if not os.path.exists(src):
@@ -51,11 +51,17 @@ def build(verbose=False):
         cmd = [cc, *COMMON, *extra, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed on {unit}")
+        # the units compile concurrently (each is one hipcc process)
+        procs.append((unit, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
+    failed = []
+    for unit, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(unit)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

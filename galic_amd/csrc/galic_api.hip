@@ -754,7 +754,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
-  dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk);
+  dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk);
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
@@ -769,6 +769,9 @@ int galic_destroy(galic_ctx *ctx) {
   for(int t = 0; t < GALIC_NTYPES; t++) dfree(c.disp_tab[t]);
   dfree(c.cs_scratch);
   dfree(c.g32_scratch);
+  dfree(c.tile_list);
+  dfree(c.tile_aux);
+  dfree(c.lookup_buf);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -1505,9 +1508,13 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   size_t nk = (size_t)std::max(ntrial, 1) * K;
   if(nk > c.max_ktrials) {
     dfree(c.t_vk);
+    dfree(c.t_dFk);
     CHK(dalloc(&c.t_vk, 3 * nk));
+    CHK(dalloc(&c.t_dFk, nk));
     c.max_ktrials = nk;
   }
+  c.last_ntrial = ntrial;
+  c.last_k = K;
   double *vtx = c.t_vk, *vty = c.t_vk + nk, *vtz = c.t_vk + 2 * nk;
   if(ntrial > 0) {
     HIPCHK(hipMemcpy(c.t_idx, idx, sizeof(int) * ntrial, hipMemcpyHostToDevice));
@@ -1556,8 +1563,18 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
            ngrids * sizeof(long long) * (size_t)c.Nbin * (c.Nbin + (ngrids == 2 ? galic::MERIT_ROW_PAD : 0)) +
            sizeof(int) * (size_t)(K + 1) * c.log_M + 1024;
   };
-  // grids above 128^2 (levels 8, 9): the merit's two grids per workgroup live in HBM scratch,
-  // kGBlocks workgroups per launch; the LDS holds the partials, the block-sum pyramid and the tables
+  // The merit kernel's form.  Tiles (k_eval_tile32): the grid in LDS tiles of 32 x 32 cells,
+  // the default above 128^2 cells and wherever the LDS grids do not fit.  Otherwise one or two
+  // LDS grids of the whole response grid (k_eval_log32).  The HBM-scratch form of k_eval_log32
+  // (two int64 grids per workgroup in HBM) is kept for grids above 128^2 when tiles are
+  // switched off or K exceeds what the tile lists hold.
+  auto tile_lds = [&]() { return galic::tile_lds_bytes(c.L, K, c.log_M) + 1024; };
+  const bool tiles_ok = c.L >= 5 && K <= galic::TILE_MAX_K;
+  auto use_tiles = [&]() {
+    if(!tiles_ok || c.merit_form == 2) return false;
+    if(c.merit_form == 1 || c.L > 7) return tile_lds() <= lds_max;
+    return eval_lds(1) > lds_max && tile_lds() <= lds_max;
+  };
   const bool gmem = c.L > 7;
   const int kGBlocks = 512;
   auto eval_lds_gmem = [&]() {
@@ -1565,7 +1582,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
            2 * sizeof(unsigned int) * (((size_t)c.Ngrid / 4 + 31) / 32) + sizeof(int) * (size_t)(K + 1) * c.log_M +
            1024;
   };
-  if(gmem) {
+  if(gmem && !use_tiles()) {
     const size_t need = (size_t)kGBlocks * 2 * c.Ngrid;
     if(c.g32_scratch_len < need) {
       dfree(c.g32_scratch);
@@ -1637,10 +1654,47 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       CHK(lr);
     }
     cap = group_cap();
-    const bool use_o = gmem || eval_lds(2) <= lds_max;
-    const size_t shmem = (gmem ? eval_lds_gmem() : eval_lds(use_o ? 2 : 1)) - 1024;
+    // (the block tables may have grown with this launch: the form is chosen after it)
+    const bool tiles = use_tiles();
+    // without tiles, the HBM-scratch form above 128^2 cells and wherever one LDS grid does not fit
+    const bool gmem_now = !tiles && (gmem || eval_lds(1) > lds_max);
+    const bool use_o = gmem_now || eval_lds(2) <= lds_max;
+    const size_t shmem = (tiles ? tile_lds() : gmem_now ? eval_lds_gmem() : eval_lds(use_o ? 2 : 1)) - 1024;
     if(shmem + 1024 > lds_max)
       return set_err(GALIC_ERR_ARG, "the merit kernel needs %zu bytes of LDS, the device gives %zu", shmem, lds_max);
+    if(gmem_now && c.g32_scratch_len < (size_t)kGBlocks * 2 * c.Ngrid) {
+      dfree(c.g32_scratch);
+      CHK(dalloc(&c.g32_scratch, (size_t)kGBlocks * 2 * c.Ngrid));
+      c.g32_scratch_len = (size_t)kGBlocks * 2 * c.Ngrid;
+    }
+    // tile form: workgroups resident at once (each takes trials b, b + grid, ...), and list
+    // room for the largest particle of the group (a record is listed in at most four tiles)
+    int tile_grid = 0, tile_cap = 0;
+    if(tiles) {
+      int per_cu = 0, ncu = 0;
+      auto kern = c.L == 9 ? (const void *)galic::k_eval_tile32<9> : (const void *)galic::k_eval_tile32<0>;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 1024, shmem));
+      HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
+      if(per_cu < 1) return set_err(GALIC_ERR_ARG, "the tile merit kernel does not fit on a CU (%zu bytes of LDS)", shmem);
+      tile_grid = std::min(ng, per_cu * ncu);
+      if(!c.tile_aux) CHK(dalloc(&c.tile_aux, 2));
+      HIPCHK(hipMemsetAsync(c.tile_aux, 0, 2 * sizeof(unsigned int), c.stream));
+      hipLaunchKernelGGL(galic::k_log_totals, dim3(nblocks(ng, 256)), dim3(256), 0, c.stream, ng, K,
+                         (const int *)nullptr, (const int *)c.log_n, c.tile_aux);
+      HIPCHK(hipGetLastError());
+      unsigned int mx = 0;
+      HIPCHK(hipMemcpyAsync(&mx, c.tile_aux, sizeof(mx), hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+      const size_t want = 4 * (size_t)mx + 64;
+      if(want > 0x7fffffffu) return set_err(GALIC_ERR_NOMEM, "a particle's response logs hold %u records", mx);
+      tile_cap = (int)want;
+      if(c.tile_list_len < (size_t)tile_grid * tile_cap) {
+        dfree(c.tile_list);
+        c.tile_list_len = 0;
+        CHK(dalloc(&c.tile_list, (size_t)tile_grid * tile_cap));
+        c.tile_list_len = (size_t)tile_grid * tile_cap;
+      }
+    }
     // evaluate the reps of the group in turn, committing each completed rep
     for(int r = r0; r < nrep && r * rep_size < s1; r++) {
       const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
@@ -1683,6 +1737,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.locI_dq = c.locI_dq;
       E.accept = c.t_accept;
       E.dF = c.t_dF;
+      E.dFk = c.t_dFk;
       for(int t = 0; t < GALIC_NTYPES; t++) {
         E.MType[t] = c.p.MType[t];
         E.fac[t][0] = c.p.fac_value_rs[t];
@@ -1695,7 +1750,18 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.iscale = c.fp32_iscale;
       E.gscratch = c.g32_scratch;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
-      if(gmem) {
+      if(tiles) {
+        E.nblk = b - a;
+        E.tlist = c.tile_list;
+        E.tlist_cap = tile_cap;
+        E.tile_err = c.tile_aux + 1;
+        const int grid = std::min(b - a, tile_grid);
+        if(c.L == 9)
+          hipLaunchKernelGGL((galic::k_eval_tile32<9>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+        else
+          hipLaunchKernelGGL((galic::k_eval_tile32<0>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+        HIPCHK(hipGetLastError());
+      } else if(gmem_now) {
         for(int b0 = 0; b0 < b - a; b0 += kGBlocks) {
           E.block_base = b0;
           if(c.L == 9)
@@ -1729,6 +1795,11 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
       eval_ms += ms;
       c.acc_eval_ms += ms;
+      if(tiles) {
+        unsigned int terr = 0;
+        HIPCHK(hipMemcpy(&terr, c.tile_aux + 1, sizeof(terr), hipMemcpyDeviceToHost));
+        if(terr) return set_err(GALIC_ERR_STATE, "the tile lists of a particle outgrew their room");
+      }
       if(b == rb) {
         CHK(commit_q(c));
         CHK(global_fit(ctx));
@@ -1748,6 +1819,12 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
 
 }  // namespace
 
+int galic_fp32_set_merit_form(galic_ctx *ctx, int form) {
+  if(!ctx || form < 0 || form > 2) return set_err(GALIC_ERR_ARG, "merit form must be 0, 1 or 2");
+  ctx->c.merit_form = form;
+  return GALIC_OK;
+}
+
 int galic_fp32_set_fresh_trials(galic_ctx *ctx, int nfresh) {
   if(!ctx || nfresh < 0 || nfresh > 64) return set_err(GALIC_ERR_ARG, "nfresh must be 0..64");
   ctx->c.fp32_nfresh = nfresh;
@@ -1762,6 +1839,16 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
 int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, const double *vtry,
                        double sigma_frac, unsigned long long seed, int *accept) {
   return fp32_run(ctx, ntrial, idx, K, rep_size, vtry, sigma_frac, seed, accept);
+}
+
+int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk) {
+  if(!ctx || !dFk) return set_err(GALIC_ERR_ARG, "null argument");
+  galic::Ctx &c = ctx->c;
+  if(ntrial != c.last_ntrial || k != c.last_k || ntrial <= 0)
+    return set_err(GALIC_ERR_STATE, "the last FP32 call had %d trials of K=%d", c.last_ntrial, c.last_k);
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipMemcpy(dFk, c.t_dFk, sizeof(double) * (size_t)ntrial * k, hipMemcpyDeviceToHost));
+  return GALIC_OK;
 }
 
 int galic_timings(galic_ctx *ctx, double *out10) {

@@ -99,7 +99,9 @@ struct Ctx {
   double *qinv = nullptr;       // 1/scale of the 5 orbit-averaged then 4 instantaneous fields
   int *tovs_dev = nullptr;      // velocity structure per type, on the device
   double *t_vk = nullptr;       // K trial velocities per particle, SoA [3][max_ktrials]
+  double *t_dFk = nullptr;      // merit change of every trial of the last FP32 call [max_ktrials]
   size_t max_ktrials = 0;
+  int last_ntrial = 0, last_k = 0;  // trials and K of the last FP32 call
   double *fit = nullptr;        // [type][16] S, Sdisp_r,t,p,q, Srelfac, Srelfac_count, Sall, raw sums..., firstrun
   Particles P;
 
@@ -162,6 +164,12 @@ struct Ctx {
   size_t cs_scratch_len = 0;
   long long *g32_scratch = nullptr;  // FP32 merit: two int64 grids per workgroup for grids above level 7
   size_t g32_scratch_len = 0;
+  unsigned int *tile_list = nullptr;  // FP32 merit, tile form: the records' tile lists, per workgroup
+  size_t tile_list_len = 0;
+  unsigned int *tile_aux = nullptr;
+  double *lookup_buf = nullptr;  // galic_field_at: staging of positions, accelerations, potentials
+  size_t lookup_len = 0;         // points it holds   // [0] largest records of a particle's logs, [1] list overflow flag
+  int merit_form = 0;                 // 0 automatic, 1 tiles at every level, 2 LDS grids / HBM scratch (no tiles)
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

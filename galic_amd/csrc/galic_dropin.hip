@@ -9,6 +9,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/galic_hip.h"
@@ -41,6 +42,22 @@ __global__ void k_field_at(DevModel M, int n, const double *pos, double *acc, do
   if(pot) pot[i] = potential64(M, x, y, z);
 }
 
+// n lookups through device staging (positions dpos, accelerations dacc, potentials dpot):
+// copy in, one launch, copy out, on the context's stream.
+int field_lookup(Ctx &c, int n, const double *pos, double *acc, double *pot, double *dpos, double *dacc,
+                 double *dpot) {
+  if(hipMemcpyAsync(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+    return GALIC_ERR_HIP;
+  hipLaunchKernelGGL(k_field_at, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.M, n, (const double *)dpos, dacc,
+                     dpot);
+  if(hipGetLastError() != hipSuccess) return GALIC_ERR_HIP;
+  if(acc && hipMemcpyAsync(acc, dacc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+    return GALIC_ERR_HIP;
+  if(pot && hipMemcpyAsync(pot, dpot, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+    return GALIC_ERR_HIP;
+  return hipStreamSynchronize(c.stream) == hipSuccess ? GALIC_OK : GALIC_ERR_HIP;
+}
+
 // smooth_stack (grid.c:1247-1265) in one workgroup
 __global__ void k_smooth_one(double *data, int L) {
   for(int l = L - 1; l >= 0; l--) {
@@ -71,25 +88,17 @@ int galic_field_at(galic_ctx *ctx, int n, const double *pos, double *acc, double
   if(n == 0) return GALIC_OK;
   galic::Ctx &c = galic::ctx_state(ctx);
   if(hipSetDevice(c.device) != hipSuccess) return GALIC_ERR_HIP;
-  double *dpos = nullptr, *dacc = nullptr, *dpot = nullptr;
-  int rc = GALIC_OK;
-  if(hipMalloc(&dpos, sizeof(double) * 3 * n) != hipSuccess || hipMalloc(&dacc, sizeof(double) * 3 * n) != hipSuccess ||
-     hipMalloc(&dpot, sizeof(double) * n) != hipSuccess)
-    rc = GALIC_ERR_NOMEM;
-  if(rc == GALIC_OK && hipMemcpy(dpos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
-    rc = GALIC_ERR_HIP;
-  if(rc == GALIC_OK) {
-    hipLaunchKernelGGL(galic::k_field_at, dim3((n + 255) / 256), dim3(256), 0, c.stream, c.M, n, dpos, dacc, dpot);
-    if(hipGetLastError() != hipSuccess || hipStreamSynchronize(c.stream) != hipSuccess) rc = GALIC_ERR_HIP;
+  // the context's staging, grown to the largest call: a lookup in a loop allocates once
+  if((size_t)n > c.lookup_len) {
+    if(c.lookup_buf) (void)hipFree(c.lookup_buf);
+    c.lookup_buf = nullptr;
+    c.lookup_len = 0;
+    const size_t len = std::max<size_t>((size_t)n, 1024);
+    if(hipMalloc(&c.lookup_buf, sizeof(double) * 7 * len) != hipSuccess) return GALIC_ERR_NOMEM;
+    c.lookup_len = len;
   }
-  if(rc == GALIC_OK && acc && hipMemcpy(acc, dacc, sizeof(double) * 3 * n, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = GALIC_ERR_HIP;
-  if(rc == GALIC_OK && pot && hipMemcpy(pot, dpot, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = GALIC_ERR_HIP;
-  (void)hipFree(dpos);
-  (void)hipFree(dacc);
-  (void)hipFree(dpot);
-  return rc;
+  double *dpos = c.lookup_buf, *dacc = dpos + 3 * c.lookup_len, *dpot = dacc + 3 * c.lookup_len;
+  return galic::field_lookup(c, n, pos, acc, pot, dpos, dacc, dpot);
 }
 
 double produce_orbit_response_field_mod(double *pos, double *vel, int id, double *mfield, double *egyfield_r,
@@ -254,14 +263,24 @@ double calc_stack_difference_mod(double *d1, double *d2, int l, int i, int j, in
 
 void smooth_stack(double *data, int maxlevel) {
   if(maxlevel < 0 || maxlevel > 12) dropin_fail("smooth_stack (maxlevel)");
-  size_t n = GALIC_NSTACK(maxlevel);
-  double *d = nullptr;
-  if(hipMalloc(&d, sizeof(double) * n) != hipSuccess) dropin_fail("smooth_stack (alloc)");
+  // a device buffer per GPU, grown to the largest stack and kept: repeated calls do not allocate
+  static double *buf[64];
+  static size_t len[64];
+  int dev = 0;
+  if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dropin_fail("smooth_stack (device)");
+  const size_t n = GALIC_NSTACK(maxlevel);
+  if(n > len[dev]) {
+    if(buf[dev]) (void)hipFree(buf[dev]);
+    buf[dev] = nullptr;
+    len[dev] = 0;
+    if(hipMalloc(&buf[dev], sizeof(double) * n) != hipSuccess) dropin_fail("smooth_stack (alloc)");
+    len[dev] = n;
+  }
+  double *d = buf[dev];
   if(hipMemcpy(d, data, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) dropin_fail("smooth_stack");
   hipLaunchKernelGGL(galic::k_smooth_one, dim3(1), dim3(256), 0, 0, d, maxlevel);
   if(hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) dropin_fail("smooth_stack");
   if(hipMemcpy(data, d, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) dropin_fail("smooth_stack");
-  (void)hipFree(d);
 }
 
 }  // extern "C"

@@ -72,6 +72,10 @@ struct EvalArgs32 {
   int block_base;
   const int *slot_pos;  // ... whose orbits sit at chunk position slot_pos[b] (null: b)
   long long *gscratch;  // grids above 128^2: two int64 grids per workgroup of the launch, zero between uses
+  int nblk;             // tile form: trials of the launch (a workgroup takes trials b, b + gridDim.x, ...)
+  unsigned int *tlist;  // tile form: per workgroup, tlist_cap entries of the records' tile lists
+  int tlist_cap;
+  unsigned int *tile_err;  // tile form: set when a particle's lists exceed tlist_cap
   const int *tidx;
   const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
   const double *stacks, *fit;
@@ -91,6 +95,7 @@ struct EvalArgs32 {
   long long *locA_dq, *locI_dq;
   int *accept;
   double *dF;
+  double *dFk;  // merit change of trial k of trial ti at ti * K + k
   double MType[GALIC_NTYPES];
   double fac[GALIC_NTYPES][4];
   double lowdisp;
@@ -114,6 +119,26 @@ int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigne
 __global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
 template <bool USE_O, bool GMEM, int LF>
 __global__ void k_eval_log32(DevModel M, EvalArgs32 A);
+// Tile form of the FP32 merit: the response grid is cut into tiles of TILE x TILE cells,
+// each log's records are listed by the tiles their cells fall in, and the tiles the logs
+// touch are summed in LDS one at a time.  Rows of a tile grid are padded to TILE_RS cells.
+constexpr int TILE = 32;
+constexpr int TILE_RS = TILE + 2;
+constexpr int TILE_CELLS = TILE * TILE_RS;
+constexpr int TILE_MAX_K = 63;  // list entries hold the log index in 8 bits
+// LDS of k_eval_tile32 beyond its static arrays: per-wave partials [K][16] double, the
+// per-trial pyramid above the tiles [K][Nstack(L-5)] and block sums [2][K][4] int64, K + 2 tile
+// grids (two for the current orbit, alternating between tiles), tile counts, offsets and
+// the active list [3 * ntile + 1] int, the block tables [K+1][log_M] int.
+__host__ __device__ inline size_t tile_lds_bytes(int L, int K, int log_M) {
+  const size_t nst = GALIC_NSTACK(L - 5), ntile = (size_t)1 << (2 * (L - 5));
+  return (size_t)MERIT_MAX_WAVES * K * 8 + (size_t)K * nst * 8 + 8 * (size_t)K * 8 +
+         (size_t)(K + 2) * TILE_CELLS * 8 + (3 * ntile + 1) * 4 + (size_t)(K + 1) * log_M * 4;
+}
+template <int LF>
+__global__ void k_eval_tile32(DevModel M, EvalArgs32 A);
+__global__ void k_log_totals(int n, int K, const int *slot_pos, const int *log_n, unsigned int *maxsum);
+
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);
 __global__ void k_inst_init_q(DevModel M, Particles P, long long *locIq, long long *locMq, int ngrid, double iscale,

@@ -1003,6 +1003,74 @@ __device__ __forceinline__ double wave_sum(double v) {
          (__longlong_as_double(readlane_i64(b, 32)) + __longlong_as_double(readlane_i64(b, 48)));
 }
 
+// The instantaneous terms of each trial (one trial per thread, inst_delta_q), the total
+// merit change F_try - F_base per trial (part: the orbit-averaged terms per trial and wave)
+// and the choice: the trial of the smallest negative change is taken, with its
+// instantaneous deltas and the particle's new velocity and moments (main.c:464-527 for the
+// best of K).  Returns the chosen trial (-1: none) in every thread; ends at a barrier.
+__device__ int choose_trial(const DevModel &M, const EvalArgs32 &A, int ti, int p, int t, size_t o0,
+                            const double *part, double *s_dF, int *s_best) {
+  const int tid = threadIdx.x, nt = blockDim.x, K = A.K;
+  const int nb = M.EG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(M.DG_L);
+  const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
+  const double x = A.P.x[p], y = A.P.y[p], z = A.P.z[p], mass = A.P.mass[p];
+  const double dold[4] = {A.P.vr2[p], A.P.vt2[p], A.P.vp2[p], A.P.vq2[p]};
+  for(int k = tid; k < K; k += nt) {
+    const size_t q = (size_t)ti * K + k;
+    double dnew[4];
+    disp_components64(M, t, x, y, z, A.tvx[q], A.tvy[q], A.tvz[q], dnew[0], dnew[1], dnew[2], dnew[3]);
+    double cw[4];
+    int cc[4];
+    double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
+    const double *fit = A.fit + (size_t)t * 16;
+    double orbit_terms = 0;
+    for(int w = 0; w < MERIT_MAX_WAVES; w++) orbit_terms += part[k * MERIT_MAX_WAVES + w];
+    s_dF[k] = orbit_terms + fit[5] / fit[6] * vinst;
+    A.dFk[q] = s_dF[k];
+  }
+  __syncthreads();
+  if(tid == 0) {
+    int best = -1;
+    double bv = 0;
+    for(int k = 0; k < K; k++)
+      if(s_dF[k] < bv) {
+        bv = s_dF[k];
+        best = k;
+      }
+    *s_best = best;
+    A.accept[ti] = best + 1;
+    A.dF[ti] = (best >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
+    const size_t oc = (best >= 0) ? o0 + 1 + best : o0;
+    A.P.orbits[p] = A.norbit[oc];
+    A.psteps[p] = (int)min(A.osteps[oc], (long long)0x7fffffff);
+    if(best >= 0) {
+      const size_t q = (size_t)ti * K + best;
+      const double vt[3] = {A.tvx[q], A.tvy[q], A.tvz[q]};
+      double dnew[4];
+      disp_components64(M, t, x, y, z, vt[0], vt[1], vt[2], dnew[0], dnew[1], dnew[2], dnew[3]);
+      int iR, iz;
+      double fR, fz;
+      cell64(x, y, z, M.EG_Rmin, M.EG_logFac, M.EG_Nbin, iR, iz, fR, fz);
+      double cw[4] = {(1 - fR) * (1 - fz), fR * (1 - fz), (1 - fR) * fz, fR * fz};
+      int cc[4] = {iz * nb + iR, iz * nb + iR + 1, (iz + 1) * nb + iR, (iz + 1) * nb + iR + 1};
+      long long *lid = A.locI_dq + (size_t)t * 4 * ngrid;
+      for(int f = 0; f < 4; f++) {
+        double val = (dnew[f] - dold[f]) * mass;
+        for(int c = 0; c < 4; c++) atomic_q(&lid[(size_t)f * ngrid + cc[c]], A.iscale * cw[c] * val);
+      }
+      A.P.vx[p] = vt[0];
+      A.P.vy[p] = vt[1];
+      A.P.vz[p] = vt[2];
+      A.P.vr2[p] = dnew[0];
+      A.P.vt2[p] = dnew[1];
+      A.P.vp2[p] = dnew[2];
+      A.P.vq2[p] = dnew[3];
+    }
+  }
+  __syncthreads();
+  return *s_best;
+}
+
 // One particle per workgroup: the merit change of each of its K trials (the
 // orbit-averaged terms on the fixed density cut, the instantaneous terms on the
 // fixed cut of EGs_MassResponse), the best improving trial taken and its deltas
@@ -1110,60 +1178,7 @@ __global__ void __launch_bounds__(merit_threads(LF)) k_eval_log32(DevModel M, Ev
     for(int q = tid; q < nmw; q += nt) maskO[q] = 0;
     __syncthreads();
   }
-  // instantaneous terms, one trial per thread; then the total per trial
-  for(int k = tid; k < K; k += nt) {
-    const size_t q = (size_t)ti * K + k;
-    double dnew[4];
-    disp_components64(M, t, x, y, z, A.tvx[q], A.tvy[q], A.tvz[q], dnew[0], dnew[1], dnew[2], dnew[3]);
-    double cw[4];
-    int cc[4];
-    double vinst = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
-    const double *fit = A.fit + (size_t)t * 16;
-    double orbit_terms = 0;
-    for(int w = 0; w < MERIT_MAX_WAVES; w++) orbit_terms += part[k * MERIT_MAX_WAVES + w];
-    s_dF[k] = orbit_terms + fit[5] / fit[6] * vinst;
-  }
-  __syncthreads();
-  if(tid == 0) {
-    int best = -1;
-    double bv = 0;
-    for(int k = 0; k < K; k++)
-      if(s_dF[k] < bv) {
-        bv = s_dF[k];
-        best = k;
-      }
-    s_best = best;
-    A.accept[ti] = best + 1;
-    A.dF[ti] = (best >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
-    const size_t oc = (best >= 0) ? o0 + 1 + best : o0;
-    A.P.orbits[p] = A.norbit[oc];
-    A.psteps[p] = (int)min(A.osteps[oc], (long long)0x7fffffff);
-    if(best >= 0) {
-      const size_t q = (size_t)ti * K + best;
-      const double vt[3] = {A.tvx[q], A.tvy[q], A.tvz[q]};
-      double dnew[4];
-      disp_components64(M, t, x, y, z, vt[0], vt[1], vt[2], dnew[0], dnew[1], dnew[2], dnew[3]);
-      int iR, iz;
-      double fR, fz;
-      cell64(x, y, z, M.EG_Rmin, M.EG_logFac, M.EG_Nbin, iR, iz, fR, fz);
-      double cw[4] = {(1 - fR) * (1 - fz), fR * (1 - fz), (1 - fR) * fz, fR * fz};
-      int cc[4] = {iz * nb + iR, iz * nb + iR + 1, (iz + 1) * nb + iR, (iz + 1) * nb + iR + 1};
-      long long *lid = A.locI_dq + (size_t)t * 4 * ngrid;
-      for(int f = 0; f < 4; f++) {
-        double val = (dnew[f] - dold[f]) * mass;
-        for(int c = 0; c < 4; c++) atomic_q(&lid[(size_t)f * ngrid + cc[c]], A.iscale * cw[c] * val);
-      }
-      A.P.vx[p] = vt[0];
-      A.P.vy[p] = vt[1];
-      A.P.vz[p] = vt[2];
-      A.P.vr2[p] = dnew[0];
-      A.P.vt2[p] = dnew[1];
-      A.P.vp2[p] = dnew[2];
-      A.P.vq2[p] = dnew[3];
-    }
-  }
-  __syncthreads();
-  const int best = s_best;
+  const int best = choose_trial(M, A, ti, p, t, o0, part, s_dF, &s_best);
   if(best >= 0) {
     // the orbit-averaged deltas of the taken trial, exact in fixed point
     long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
@@ -1213,6 +1228,314 @@ template __global__ void k_eval_log32<false, false, 7>(DevModel M, EvalArgs32 A)
 template __global__ void k_eval_log32<true, true, 0>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_log32<true, true, 8>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_log32<true, true, 9>(DevModel M, EvalArgs32 A);
+
+// ----------------------------------------------------- FP32 merit in tiles --
+//
+// The same merit change as k_eval_log32, for grids whose two int64 copies do not fit in
+// LDS (256^2 and 512^2 cells, or 128^2 with long block tables).  Per particle:
+//   1. the records of its K+1 logs are listed by tile (a record whose 2x2 cells straddle a
+//      tile edge is listed in each tile it touches): a count per tile, offsets, then the
+//      entries (log << 24 | record) in this workgroup's slice of HBM;
+//   2. field by field, the tiles the logs touch are taken in turn: every listed record's
+//      cells inside the tile are added to its log's tile grid with LDS integer atomics (so
+//      D = G - O is exact), then each (16 x 16 block, trial) pair is one wave's pyramid from
+//      the cells up to the block node, with the terms of every node the difference reaches
+//      (cells, 2x2 nodes, 4x4, 8x8, the block); the block sums make the tile's node;
+//   3. the levels above the tiles, one wave per trial;
+//   4. the choice of the best trial and, if one is taken, its deltas tile by tile.
+// The orbit's own log O is summed into one of two grids that alternate between tiles, so the
+// grid of the previous tile is cleared while the current one is evaluated: two barriers per
+// tile.  Trial grids are cleared as they are read.
+
+// Tiles touched by the 2x2 cells of quad (cells (iz..iz+1, iR..iR+1)); ts tiles per side.
+template <typename Visit>
+__device__ __forceinline__ void tile_visit(int quad, int L, int ts, Visit visit) {
+  const int iz = quad >> L, iR = quad & ((1 << L) - 1);
+  const int z0 = iz / TILE, z1 = (iz + 1) / TILE, r0 = iR / TILE, r1 = (iR + 1) / TILE;
+  visit(z0 * ts + r0);
+  if(r1 != r0) visit(z0 * ts + r1);
+  if(z1 != z0) {
+    visit(z1 * ts + r0);
+    if(r1 != r0) visit(z1 * ts + r1);
+  }
+}
+
+// Offsets of the tiles' lists (exclusive scan of the counts, off[ntile] = total) and the
+// tiles with entries in increasing order; one wave.
+__device__ __forceinline__ void tile_scan(const int *cnt, int *off, int *act, int *nact, int ntile, int lane) {
+  const int per = (ntile + 63) / 64, a = min(lane * per, ntile), e = min(a + per, ntile);
+  int s = 0, na = 0;
+  for(int i = a; i < e; i++) {
+    s += cnt[i];
+    na += cnt[i] > 0;
+  }
+  int si = s, ni = na;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(si, d), z = __shfl_up(ni, d);
+    if(lane >= d) {
+      si += y;
+      ni += z;
+    }
+  }
+  int base = si - s, abase = ni - na;
+  for(int i = a; i < e; i++) {
+    off[i] = base;
+    base += cnt[i];
+    if(cnt[i] > 0) act[abase++] = i;
+  }
+  if(lane == 63) {
+    off[ntile] = si;
+    *nact = ni;
+  }
+}
+
+// Adds the cells of a record that fall inside the tile (tile-local first cell (lz, lr)).
+__device__ __forceinline__ void tile_add(unsigned long long *g, int lz, int lr, float4 x, float sc) {
+  const float v[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for(int c = 0; c < 4; c++) {
+    const int cz = lz + (c >> 1), cr = lr + (c & 1);
+    if((unsigned)cz < (unsigned)TILE && (unsigned)cr < (unsigned)TILE && v[c] != 0.f)
+      atomicAdd(&g[cz * TILE_RS + cr], (unsigned long long)__float2ll_rn(v[c] * sc));
+  }
+}
+
+// One wave's pyramid of one 16 x 16 block b (0..3, row-major) of tile (tz, tr) for one trial:
+// D = G - O on the block's cells (G cleared as read), the terms of the cells, the 2x2 nodes
+// (one per lane, lanes in Morton order) and of the 4x4, 8x8 and 16x16 nodes formed with
+// group_sum_i64.  The block node's sum goes to *bs (lane 0).  Returns the lane's terms.
+__device__ __forceinline__ double tile_block(const double2 *bwf, int L, double unit, long long *G, const long long *O,
+                                             int tz, int tr, int b, long long *bs) {
+  const int nb = 1 << L, lane = threadIdx.x & 63;
+  const int jx = morton_x(lane), jy = morton_y(lane);
+  const int lz = (b >> 1) * 16 + 2 * jy, lr = (b & 1) * 16 + 2 * jx;
+  const int gz = tz * TILE + lz, gr = tr * TILE + lr;
+  long long d[4], s = 0;
+  bool nz = false;
+#pragma unroll
+  for(int c = 0; c < 4; c++) {
+    const int q = (lz + (c >> 1)) * TILE_RS + lr + (c & 1);
+    d[c] = G[q] - O[q];
+    G[q] = 0;
+    s += d[c];
+    nz = nz || d[c] != 0;
+  }
+  double acc = 0;
+  if(__ballot(nz) == 0) {
+    if(lane == 0) *bs = 0;
+    return 0.0;
+  }
+  if(nz) {
+    const int offL = GALIC_STACKOFFSET(L, 0, 0);
+#pragma unroll
+    for(int c = 0; c < 4; c++)
+      if(d[c] != 0) acc += term_at(bwf[offL + (gz + (c >> 1)) * nb + gr + (c & 1)], (double)d[c] * unit);
+    if(s != 0) acc += term_at(bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)], (double)s * unit);
+  }
+#pragma unroll
+  for(int u = 0; u < 3; u++) {
+    s = group_sum_i64(s, u);
+    const int l = L - 2 - u, mask = (1 << (2 * u + 2)) - 1, sh = L - l;
+    if((lane & mask) == 0 && s != 0) acc += term_at(bwf[GALIC_STACKOFFSET(l, gz >> sh, gr >> sh)], (double)s * unit);
+  }
+  if(lane == 0) *bs = s;
+  return acc;
+}
+
+// Largest total record count of one particle's K+1 logs in a launch of n particles.
+__global__ void k_log_totals(int n, int K, const int *slot_pos, const int *log_n, unsigned int *maxsum) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if(b >= n) return;
+  const size_t o0 = (size_t)(slot_pos ? slot_pos[b] : b) * (K + 1);
+  unsigned int s = 0;
+  for(int o = 0; o <= K; o++) s += (unsigned int)max(log_n[o0 + o], 0);
+  atomicMax(maxsum, s);
+}
+
+template <int LF>
+__global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) {
+  extern __shared__ double lds_d[];
+  const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
+  const int L = LF ? LF : M.DG_L, nb = 1 << L, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
+  const int lt = L - 5, ts = 1 << lt, ntile = ts * ts, nst = GALIC_NSTACK(lt);
+  double *part = lds_d;                                          // [K][MERIT_MAX_WAVES]
+  long long *topst = (long long *)(part + MERIT_MAX_WAVES * K);   // [K][nst]: levels 0..lt per trial
+  long long *bsum = topst + (size_t)K * nst;                      // [2][K][4] block sums, alternating
+  long long *grids = bsum + 8 * K;                                // O (two, alternating), then G_1..G_K
+  int *tcount = (int *)(grids + (size_t)(K + 2) * TILE_CELLS);    // [ntile] counts, then cursors
+  int *toff = tcount + ntile;                                     // [ntile + 1]
+  int *tact = toff + ntile + 1;                                   // [ntile] tiles with entries
+  int *tbl = tact + ntile;                                        // [K+1][log_M]
+  __shared__ double s_dF[64];
+  __shared__ int s_n[TILE_MAX_K + 1], s_nact, s_best, s_over;
+  unsigned int *list = A.tlist + (size_t)blockIdx.x * A.tlist_cap;
+
+  for(int q = tid; q < (K + 2) * TILE_CELLS; q += nt) grids[q] = 0;
+  for(int bid = blockIdx.x; bid < A.nblk; bid += gridDim.x) {
+    const int ti = A.trial_base + bid;
+    const int j = A.slot_pos ? A.slot_pos[bid] : bid;
+    const int p = A.tidx[ti];
+    const int t = A.P.type[p];
+    const double2 *bw = A.merit_bw + (size_t)t * kFields * Nstack;
+    const size_t o0 = (size_t)j * (K + 1);
+    const double MT = A.MType[t];
+    // eval_fit_mod's combination (main.c:1215-1216): D/M + (f_rs V_rs + f_ts V_ts + f_qs V_qs + f_ps V_ps)/M/4
+    const double coef[kFields] = {1.0 / MT, A.fac[t][0] / MT / 4, A.fac[t][1] / MT / 4, A.fac[t][2] / MT / 4,
+                                  A.fac[t][3] / MT / 4};
+    __syncthreads();  // the previous particle's use of the shared arrays is over
+    for(int q = tid; q <= K; q += nt) s_n[q] = max(A.log_n[o0 + q], 0);
+    for(int q = tid; q < MERIT_MAX_WAVES * K; q += nt) part[q] = 0;
+    for(int q = tid; q < K * nst; q += nt) topst[q] = 0;
+    for(int q = tid; q < ntile; q += nt) tcount[q] = 0;
+    __syncthreads();
+    for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
+      const int o = q / A.log_M, b = q % A.log_M;
+      if(b * LOG_B < s_n[o]) tbl[q] = A.log_blocks[(o0 + o) * A.log_M + b];
+    }
+    __syncthreads();
+    // 1. the records' tile lists
+    for(int o = 0; o <= K; o++) {
+      const int n = s_n[o], *tb = tbl + o * A.log_M;
+      for(int r = tid; r < n; r += nt) {
+        const int quad = A.log_q[(size_t)tb[r / LOG_B] * LOG_B + (r & (LOG_B - 1))];
+        tile_visit(quad, L, ts, [&](int tile) { atomicAdd(&tcount[tile], 1); });
+      }
+    }
+    __syncthreads();
+    if(wave == 0) tile_scan(tcount, toff, tact, &s_nact, ntile, lane);
+    __syncthreads();
+    if(toff[ntile] > A.tlist_cap) {
+      // cannot happen when the caller sizes tlist_cap from k_log_totals; reported, not evaluated
+      if(tid == 0) {
+        atomicOr(A.tile_err, 1u);
+        A.accept[ti] = 0;
+        A.dF[ti] = 0;
+      }
+      continue;
+    }
+    for(int q = tid; q < ntile; q += nt) tcount[q] = 0;
+    __syncthreads();
+    for(int o = 0; o <= K; o++) {
+      const int n = s_n[o], *tb = tbl + o * A.log_M;
+      for(int r = tid; r < n; r += nt) {
+        const int quad = A.log_q[(size_t)tb[r / LOG_B] * LOG_B + (r & (LOG_B - 1))];
+        tile_visit(quad, L, ts, [&](int tile) {
+          list[toff[tile] + atomicAdd(&tcount[tile], 1)] = ((unsigned int)o << 24) | (unsigned int)r;
+        });
+      }
+    }
+    __syncthreads();
+    const int nact = s_nact;
+    // 2. field by field, tile by tile
+    for(int f = 0; f < kFields; f++) {
+      const float sc = A.log_scale[f];
+      const double unit = A.log_unit[f];
+      const double2 *bwf = bw + (size_t)f * Nstack;
+      for(int ai = 0; ai <= nact; ai++) {
+        const int cur = ai & 1;
+        if(ai < nact) {
+          const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          for(int e = toff[tile] + tid; e < toff[tile + 1]; e += nt) {
+            const unsigned int en = list[e];
+            const int o = (int)(en >> 24), r = (int)(en & 0xffffffu);
+            const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
+            const int quad = A.log_q[(size_t)blk * LOG_B + jj];
+            const float4 x = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+            long long *g = grids + (size_t)(o == 0 ? cur : o + 1) * TILE_CELLS;
+            tile_add((unsigned long long *)g, (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x, sc);
+          }
+        }
+        __syncthreads();
+        if(ai > 0) {
+          // the previous tile: its node's sums from the block sums, its O grid cleared
+          const int ptile = tact[ai - 1], pz = ptile / ts, pr = ptile % ts;
+          const long long *bs = bsum + (size_t)(1 - cur) * 4 * K;
+          for(int k = tid; k < K; k += nt)
+            topst[(size_t)k * nst + GALIC_STACKOFFSET(lt, pz, pr)] = bs[4 * k] + bs[4 * k + 1] + bs[4 * k + 2] + bs[4 * k + 3];
+          long long *Op = grids + (size_t)(1 - cur) * TILE_CELLS;
+          for(int q = tid; q < TILE * TILE; q += nt) Op[(q / TILE) * TILE_RS + q % TILE] = 0;
+        }
+        if(ai < nact) {
+          const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          const long long *O = grids + (size_t)cur * TILE_CELLS;
+          for(int it = wave; it < 4 * K; it += nw) {
+            const int k = it >> 2, b = it & 3;
+            double acc = tile_block(bwf, L, unit, grids + (size_t)(k + 2) * TILE_CELLS, O, tz, tr, b,
+                                    bsum + (size_t)cur * 4 * K + 4 * k + b);
+            acc = wave_sum(coef[f] * acc);
+            if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
+          }
+        }
+        __syncthreads();
+      }
+      // 3. the levels above the tiles, one wave per trial
+      for(int k = wave; k < K; k += nw) {
+        long long *tk = topst + (size_t)k * nst;
+        double acc = 0;
+        for(int l = lt; l >= 0; l--) {
+          const int n = 1 << l;
+          for(int q = lane; q < n * n; q += 64) {
+            const int i = q >> l, jn = q & (n - 1);
+            long long s;
+            if(l == lt) {
+              s = tk[GALIC_STACKOFFSET(l, i, jn)];
+            } else {
+              s = tk[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * jn)] + tk[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * jn + 1)] +
+                  tk[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * jn)] + tk[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * jn + 1)];
+              tk[GALIC_STACKOFFSET(l, i, jn)] = s;
+            }
+            if(s != 0) acc += term_at(bwf[GALIC_STACKOFFSET(l, i, jn)], (double)s * unit);
+          }
+          // the next level reads what other lanes of this wave wrote
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        acc = wave_sum(coef[f] * acc);
+        if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
+      }
+      __syncthreads();
+    }
+    // 4. the choice and the deltas of the taken trial, exact in fixed point
+    const int best = choose_trial(M, A, ti, p, t, o0, part, s_dF, &s_best);
+    if(best >= 0) {
+      long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
+      const unsigned int ob = (unsigned int)best + 1;
+      long long *G = grids;  // O's first grid, zero here
+      for(int f = 0; f < kFields; f++) {
+        const float sc = A.log_scale[f];
+        for(int ai = 0; ai < nact; ai++) {
+          const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          for(int e = toff[tile] + tid; e < toff[tile + 1]; e += nt) {
+            const unsigned int en = list[e], o = en >> 24;
+            if(o != 0 && o != ob) continue;
+            const int r = (int)(en & 0xffffffu);
+            const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
+            const int quad = A.log_q[(size_t)blk * LOG_B + jj];
+            const float4 x = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+            tile_add((unsigned long long *)G, (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x,
+                     o == 0 ? -sc : sc);
+          }
+          __syncthreads();
+          for(int q = tid; q < TILE * TILE; q += nt) {
+            const int lc = (q / TILE) * TILE_RS + q % TILE;
+            const long long d = G[lc];
+            if(d) {
+              G[lc] = 0;
+              atomic_q(&lad[(size_t)f * ngrid + (size_t)(tz * TILE + q / TILE) * nb + tr * TILE + q % TILE],
+                       (double)d * A.log_to_loc[f]);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+}
+
+template __global__ void k_eval_tile32<0>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_tile32<9>(DevModel M, EvalArgs32 A);
 
 // ---------------------------------------------------- fixed point commit --
 

@@ -84,6 +84,10 @@ CONFIGS = {
     # Batching form: halo + disk + bulge, N=4e6
     "composite-4e6": Config(name="composite-4e6", n=4000000, halo_beta=0.3, halo_tovs=1, disk_mass=4.0,
                             disk_frac=0.2, bulge_mass=1.0, bulge_frac=0.05, fg_nbin=256),
+    # Full v1.1 model: isotropic halo and bulge, disk of velocity structure 3 with sigma_R = sigma_z,
+    # N=2e7, 512^2 force tables, 512^2 response grids (level 9)
+    "composite-2e7": Config(name="composite-2e7", n=20000000, halo_tovs=0, disk_mass=4.0, disk_frac=0.2,
+                            disk_tovs=3, bulge_mass=1.0, bulge_frac=0.05, bulge_tovs=0, fg_nbin=512, level=9),
 }
 
 

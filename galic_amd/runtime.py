@@ -125,6 +125,12 @@ class GalicGPU:
                                           iptr(acc)))
         return acc
 
+    def fp32_trial_merits(self, ntrial, k):
+        """F_try - F_base of every trial of the last fp32_rep / fp32_iterate call, shape (ntrial, k)."""
+        out = np.zeros((ntrial, k))
+        check(self.lib.galic_fp32_trial_merits(self.h, int(ntrial), int(k), dptr(out)))
+        return out
+
     def fp32_autotune(self, nsample=65536):
         v = np.zeros(1, dtype=np.int32)
         b = np.zeros(1, dtype=np.int32)
@@ -133,6 +139,10 @@ class GalicGPU:
 
     def fp32_set_launch(self, variant, blocks_per_cu):
         check(self.lib.galic_fp32_set_launch(self.h, int(variant), int(blocks_per_cu)))
+
+    def fp32_set_merit_form(self, form):
+        """0 automatic, 1 LDS tiles at every level, 2 never tiles."""
+        check(self.lib.galic_fp32_set_merit_form(self.h, int(form)))
 
     def fp32_set_fresh_trials(self, nfresh):
         """The first nfresh device trials per particle are fresh draws (main.c:383-435)."""

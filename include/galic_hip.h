@@ -190,12 +190,24 @@ int galic_fp32_set_fresh_trials(galic_ctx *ctx, int nfresh);
 int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int k, int rep_size, const double *vtry,
                        double sigma_frac, unsigned long long seed, int *accept);
 
+/* The merit change F_try - F_base of every trial of the last galic_fp32_rep /
+ * galic_fp32_iterate call (main.c:464-465, one value per trial velocity):
+ * dFk[i*k + j] for trial j of the i-th particle of that call's idx[].  ntrial
+ * and k must be that call's. */
+int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk);
+
 /* Times the FP32 orbit kernel's persistent grid sizes (half, all and twice the
  * workgroups a CU holds at once) on nsample orbits of the model and keeps the
  * fastest for later reps; reports the choice (variant: the kernel variant, 0).
  * The launches give bit-identical results.  nsample should exceed the lanes the
  * GPU holds at once. */
 int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_per_cu);
+
+/* Form of the FP32 merit kernel: 0 automatic (LDS grids of the whole response grid where
+ * they fit, otherwise LDS tiles of 32 x 32 cells: the default above 128^2 cells), 1 tiles at
+ * every level, 2 never tiles (above 128^2 cells: two int64 grids per workgroup in HBM).  All
+ * forms give the same decisions and bit-identical grids; merit sums agree to FP64 rounding. */
+int galic_fp32_set_merit_form(galic_ctx *ctx, int form);
 
 /* Sets the FP32 orbit launch by hand (variant 0, 1..32 workgroups per CU). */
 int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu);

@@ -313,6 +313,11 @@ const double *or_stack(const oracle_model *m, int type, int which)
     case 8: return m->EGs_EgyResponseQS[type];
     case 9: return m->EGs_EgyResponsePS[type];
     case 10: return m->DGs_MassTarget[type];
+    case 11: return m->EGs_MassTarget[type];
+    case 12: return m->EGs_EgyTarget_r[type];
+    case 13: return m->EGs_EgyTarget_t[type];
+    case 14: return m->EGs_EgyTarget_p[type];
+    case 15: return m->EGs_EgyTarget_q[type];
   }
   return NULL;
 }

@@ -261,3 +261,169 @@ def test_fp32_grids_above_level7(level):
     g32.close()
     assert np.mean((a32 > 0) == (a64 > 0)) > 0.97
     np.testing.assert_allclose(s32, s64, rtol=2e-3)
+
+
+def composite_workload(n=3000, **kw):
+    cfg = model.config("composite-4e6", max_steps=2000000, **kw)
+    return model.make_workload(cfg, n=n, target_samples=200000)
+
+
+def k_trials(wl, idx, K, seed):
+    """K trial velocities per particle: one fresh draw of optimize()'s distribution
+    (main.c:383-435) and K-1 Gaussian perturbations of 10% of the target dispersions
+    in the local basis, all below MaxVelInUnitsVesc * Vesc; shape (n, K, 3)."""
+    from galic_amd.rng import gaussians
+    v = np.zeros((len(idx), K, 3))
+    v[:, 0] = model.trial_velocities(wl, idx, seed, 0)
+    p = wl.params
+    tovs = np.array([p.TypeOfVelocityStructure[t] for t in range(4)])[wl.ptype[idx]]
+    R, T, Q = model.local_basis(wl.pos[idx], False)
+    Rc, Tc, Qc = model.local_basis(wl.pos[idx], True)
+    cyl = (tovs == 2)[:, None]
+    R, T, Q = np.where(cyl, Rc, R), np.where(cyl, Tc, T), np.where(cyl, Qc, Q)
+    s = 0.1 * np.sqrt(wl.disp[idx, :3])
+    lim = p.MaxVelInUnitsVesc * wl.vesc[idx]
+    for k in range(1, K):
+        g = gaussians(seed, 500 + k, idx.astype(np.uint64), 0)
+        t = wl.vel[idx] + g[0][:, None] * s[:, :1] * R + g[1][:, None] * s[:, 1:2] * T + g[2][:, None] * s[:, 2:3] * Q
+        ok = np.linalg.norm(t, axis=1) < lim
+        v[:, k] = np.where(ok[:, None], t, wl.vel[idx])
+    return v
+
+
+def test_fp32_best_of_k_equals_k_single_trials():
+    """The fused K-trial launch (K=8) evaluates each trial exactly as a launch of that trial
+    alone would: per-trial merit changes bit-identical to K=1 launches from the same state,
+    and the particle takes the trial of the smallest negative merit change (main.c:464-465
+    applied to the best of K)."""
+    print("[gpu] fp32 best of K=8 vs single trials", flush=True)
+    wl = composite_workload()
+    idx = model.permutation(wl, 3)[:1000]
+    K = 8
+    V = k_trials(wl, idx, K, 21)
+    g = GalicGPU(wl)
+    g.fp32_calc_all_response_fields()
+    acc = g.fp32_rep(idx, k=K, vtry=V.reshape(-1, 3))
+    dfk = g.fp32_trial_merits(len(idx), K)
+    g.close()
+    best = np.where(dfk.min(axis=1) < 0, dfk.argmin(axis=1) + 1, 0)
+    np.testing.assert_array_equal(acc, best)
+    assert (acc > 0).mean() > 0.2 and len(set(acc[acc > 0])) == K
+    for k in (0, 3, 7):
+        g1 = GalicGPU(wl)
+        g1.fp32_calc_all_response_fields()
+        g1.fp32_rep(idx, k=1, vtry=np.ascontiguousarray(V[:, k]))
+        d1 = g1.fp32_trial_merits(len(idx), 1)[:, 0]
+        g1.close()
+        np.testing.assert_array_equal(d1, dfk[:, k])
+
+
+def test_fp32_best_of_k_matches_oracle(oracle_mod):
+    """Every trial of a K=8 launch against the oracle's merit change of that trial alone
+    (eval_trials, main.c:365-527, FP64 orbits on the oracle's own state): the FP32 path's
+    per-trial merit changes follow the oracle's to the FP32 orbit error, and the chosen trial
+    is the oracle's best wherever the oracle's best and second best differ by more than that
+    noise."""
+    print("[gpu] fp32 best of K=8 vs oracle", flush=True)
+    wl = composite_workload()
+    idx = model.permutation(wl, 3)[:600]
+    K = 8
+    V = k_trials(wl, idx, K, 22)
+    g = GalicGPU(wl)
+    g.fp32_calc_all_response_fields()
+    acc = g.fp32_rep(idx, k=K, vtry=V.reshape(-1, 3))
+    dfk = g.fp32_trial_merits(len(idx), K)
+    g.close()
+    o = oracle_mod.Oracle(wl)
+    o.calc_all_response_fields(8)
+    dfo = np.zeros_like(dfk)
+    for k in range(K):
+        fb, ft = o.eval_trials(idx, np.ascontiguousarray(V[:, k]), 8)
+        dfo[:, k] = ft - fb
+    o.close()
+    scale = np.abs(dfo).max(axis=1, keepdims=True)
+    rel = np.abs(dfk - dfo) / np.maximum(np.abs(dfo), 1e-3 * scale)
+    print(f"per-trial merit change vs oracle: median rel {np.median(rel):.2e}, p90 {np.quantile(rel, 0.9):.2e}")
+    assert np.median(rel) < 2e-2
+    best_o = np.where(dfo.min(axis=1) < 0, dfo.argmin(axis=1) + 1, 0)
+    srt = np.sort(dfo, axis=1)
+    margin = np.where(srt[:, 0] < 0, (srt[:, 1] - srt[:, 0]) / np.abs(srt[:, 0]), np.abs(srt[:, 0]) / scale[:, 0])
+    clear = margin > 0.2
+    agree = (acc == best_o)[clear].mean()
+    print(f"chosen trial equals the oracle's on {agree:.4f} of {clear.sum()} clear cases; "
+          f"all cases {(acc == best_o).mean():.4f}")
+    assert clear.sum() > 200 and agree > 0.97
+
+
+def run_forms(wl, forms, K, n1, rep, seed):
+    """One rep of n1 particles, then reps of `rep` over the rest, under each merit form."""
+    perm = model.permutation(wl, 6)
+    out = []
+    for form in forms:
+        g = GalicGPU(wl)
+        g.fp32_calc_all_response_fields()
+        g.fp32_set_merit_form(form)
+        acc = g.fp32_rep(perm[:n1], k=K, sigma_frac=0.1, seed=seed)
+        dfk = g.fp32_trial_merits(n1, K)
+        acc2 = g.fp32_iterate(perm[n1:], k=K, rep_size=rep, sigma_frac=0.1, seed=seed + 1)
+        out.append(dict(acc=np.concatenate([acc, acc2]), dfk=dfk,
+                        grids=[g.stack(t, wh) for t in wl.types for wh in range(10)],
+                        vel=g.particles()[0], fit=[g.fit(t) for t in wl.types]))
+        g.close()
+    return out
+
+
+def assert_same_run(a, b):
+    np.testing.assert_array_equal(a["acc"], b["acc"])
+    for x, y in zip(a["grids"], b["grids"]):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a["vel"], b["vel"])
+    scale = np.abs(b["dfk"]).max()
+    np.testing.assert_allclose(a["dfk"], b["dfk"], rtol=1e-11, atol=1e-13 * scale)
+    assert (a["acc"] > 0).mean() > 0.2
+
+
+@pytest.mark.parametrize("level", [6, 7])
+def test_fp32_tile_merit_equals_lds_merit(level):
+    """The tile form of the merit (the records of each log listed by 32 x 32-cell tile, each
+    tile summed in LDS) against the whole-grid LDS kernels on 64^2 and 128^2 grids: the same
+    decisions, bit-identical grids and velocities, per-trial merit changes equal up to the
+    order of the FP64 sums."""
+    print(f"[gpu] fp32 tile merit vs LDS merit, level {level}", flush=True)
+    wl = composite_workload(n=2500, level=level)
+    tiles, lds = run_forms(wl, (1, 0), K=6, n1=1000, rep=500, seed=13)
+    assert_same_run(tiles, lds)
+
+
+def test_fp32_tile_merit_equals_hbm_form_level9():
+    """On 512^2 grids (the full v1.1 model's form: isotropic halo and bulge, structure-3 disk,
+    512^2 force tables) the tile form against the form that keeps two int64 grids per
+    workgroup in HBM."""
+    print("[gpu] fp32 tile merit vs HBM form, level 9", flush=True)
+    wl = model.make_workload(model.config("composite-2e7", max_steps=2000000), n=1500, target_samples=200000)
+    tiles, hbm = run_forms(wl, (1, 2), K=4, n1=600, rep=300, seed=17)
+    assert_same_run(tiles, hbm)
+
+
+def test_fp32_level7_many_trials_run():
+    """On 128^2 grids one whole-grid LDS grid plus the block tables of K=24 trials exceed the
+    LDS of a workgroup; the automatic choice then takes a form that fits (the HBM-scratch
+    form: the tile grids of 26 logs do not fit either) instead of failing, and reps in one
+    launch equal reps one by one, bit for bit."""
+    print("[gpu] fp32 level 7 with K=24", flush=True)
+    wl = composite_workload(n=1200, level=7)
+    perm = model.permutation(wl, 6)
+    out = []
+    for mode in ("iterate", "reps"):
+        g = GalicGPU(wl)
+        g.fp32_calc_all_response_fields()
+        if mode == "iterate":
+            acc = g.fp32_iterate(perm, k=24, rep_size=400, sigma_frac=0.1, seed=19)
+        else:
+            acc = np.concatenate([g.fp32_rep(perm[a:a + 400], k=24, sigma_frac=0.1, seed=19)
+                                  for a in range(0, len(perm), 400)])
+        out.append([acc] + [g.stack(t, wh) for t in wl.types for wh in range(10)] + [g.particles()[0]])
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    assert (out[0][0] > 0).mean() > 0.2

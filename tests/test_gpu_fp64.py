@@ -59,7 +59,21 @@ def halodisk9():
     return workload("halo-disk-1e6", 1200, level=9)
 
 
-@pytest.mark.parametrize("wname", ["halo", "halodisk"])
+@pytest.fixture(scope="module")
+def composite():
+    """Halo (beta 0.3, structure 1) + disk (structure 3) + Hernquist bulge (structure 0): every
+    analytic component of the force (halo.c:63-84, bulge.c:64-85) and all three types."""
+    return workload("composite-4e6", 3000)
+
+
+@pytest.fixture(scope="module")
+def composite9():
+    """The full v1.1 model's form (isotropic halo and bulge, structure-3 disk) on 512^2 force
+    tables and 512^2 response grids."""
+    return workload("composite-2e7", 1500)
+
+
+@pytest.mark.parametrize("wname", ["halo", "halodisk", "composite"])
 def test_orbit_responses_match_oracle(wname, request, oracle_mod):
     w = request.getfixturevalue(wname)
     print(f"[gpu] orbit responses {wname} n=64", flush=True)
@@ -88,7 +102,7 @@ def test_orbit_responses_match_oracle(wname, request, oracle_mod):
     g.close()
 
 
-@pytest.mark.parametrize("wname", ["halo", "halodisk"])
+@pytest.mark.parametrize("wname", ["halo", "halodisk", "composite", "composite9"])
 def test_field_lookup_matches_oracle(wname, request, oracle_mod):
     w = request.getfixturevalue(wname)
     o = oracle_mod.Oracle(w, set_particles=False)
@@ -100,7 +114,7 @@ def test_field_lookup_matches_oracle(wname, request, oracle_mod):
     g.close()
 
 
-@pytest.mark.parametrize("wname", ["halo", "halodisk", "halodisk9"])
+@pytest.mark.parametrize("wname", ["halo", "halodisk", "halodisk9", "composite", "composite9"])
 def test_optimiser_history_matches_oracle(wname, request, oracle_mod):
     """calc_all_response_fields, then reps of trials: grids, accept flags and
     the Sall history agree with the oracle."""
@@ -166,6 +180,41 @@ def test_dropin_orbit_signature(halo, oracle_mod):
     assert norb.value == ro["orbits"]
     ok, worst = close(np.stack(bufs), ro["fields"])
     assert ok, worst
+    g.close()
+
+
+def test_dropin_plain_orbit_signature(composite, oracle_mod):
+    """produce_orbit_response_field (orbit_response.c:33-151, the plain variant: the mass field
+    only) with the reference signature, for particles of each type, against the oracle's plain
+    orbit."""
+    import ctypes
+    w = composite
+    g = GalicGPU(w)
+    g.bind()
+    fn = g.lib.produce_orbit_response_field
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn.restype = ctypes.c_double
+    fn.argtypes = [dp, dp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    o = oracle_mod.Oracle(w, set_particles=False)
+    L = oracle_mod.lib()
+    for t in w.types:
+        for i in np.nonzero(w.ptype == t)[0][:3]:
+            pos = np.ascontiguousarray(w.pos[i])
+            vel = np.ascontiguousarray(w.vel[i])
+            mf = np.zeros(g.ngrid)
+            norb = ctypes.c_int(0)
+            tg = fn(pos.ctypes.data_as(dp), vel.ctypes.data_as(dp), int(i), mf.ctypes.data_as(dp), w.mass[i],
+                    w.tint[i], ctypes.byref(norb))
+            mo = np.zeros(g.ngrid)
+            no = ctypes.c_int(0)
+            so = ctypes.c_longlong(0)
+            to = L.or_orbit_response_plain(o.h, oracle_mod.d(pos), oracle_mod.d(vel), mo.ctypes.data_as(dp),
+                                           w.mass[i], w.tint[i], ctypes.byref(no), ctypes.byref(so))
+            assert tg == pytest.approx(to, rel=1e-14), (t, i)
+            assert norb.value == no.value
+            assert np.abs(mf - mo).sum() <= RTOL * np.abs(mo).sum(), (t, i)
+            assert mf.sum() == pytest.approx(w.mass[i], rel=1e-12)
+    o.close()
     g.close()
 
 

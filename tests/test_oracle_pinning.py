@@ -443,3 +443,69 @@ def test_tree_monopole_accuracy(oracle_mod):
     err = np.linalg.norm(acc - acc2, axis=1) / np.linalg.norm(acc2, axis=1)
     assert np.median(err) < 5e-3
     assert nint < 64 * 3000   # fewer interactions than direct summation
+
+
+# ------------------------------------------------- first fit and the F_base terms --
+
+def composite_small(n=1500, **kw):
+    """Halo (beta model, structure 1) + disk (structure 3) + bulge (structure 0)."""
+    kw.setdefault("max_steps", 2000000)
+    return model.make_workload(model.config("composite-4e6", **kw), n=n, target_samples=60000)
+
+
+def test_first_fit_sall_is_twice_s(oracle_mod):
+    """At the first calc_global_fit, Srelfac = S / (sum Sdisp / count) (main.c:350-353), so the
+    logged Sall = S + sum(Srelfac * Sdisp) / count (main.c:1270) is exactly 2 S for every type,
+    whatever the velocity structure.  A dropped Srelfac factor, a wrong count or a misplaced
+    scaling of the Sdisp terms breaks it."""
+    w = composite_small()
+    o = oracle_mod.Oracle(w)
+    o.calc_all_response_fields(4)
+    for t in w.types:
+        f = o.fit(t)
+        S, Srelfac, count, Sall = f[0], f[5], f[6], f[7]
+        assert S > 0 and Srelfac > 0
+        assert count == (3.0 if w.params.TypeOfVelocityStructure[t] in (0, 1) else 4.0)
+        assert Sall == pytest.approx(2 * S, rel=1e-14, abs=0), (t, Sall, S)
+        # the scaled dispersion terms average to S
+        assert (f[1] + f[2] + f[3] + f[4]) / count == pytest.approx(S, rel=1e-14)
+
+
+def test_fbase_instantaneous_terms_equal_global_fit(oracle_mod):
+    """F_base of an unchanged velocity, with the orbit-averaged S weights set to zero, is the
+    density term plus Srelfac/count times the instantaneous terms of the current state
+    (main.c:1134-1216).  Those are the global fit's unscaled Sdisp values (main.c:327-346):
+    two code paths of the oracle (the 4-cell update of a zero difference, re-smoothed, against
+    the global stacks) must agree.  The V_q rules differ: the merit drops q for structure 0
+    or for the disk (main.c:1195), the fit for structures 0 and 1 (main.c:344-346), so the halo's
+    (structure 1) q term is evaluated separately from the global stacks."""
+    w = composite_small()
+    p = w.params
+    for t in range(4):
+        p.fac_value_rs[t] = p.fac_value_ts[t] = p.fac_value_qs[t] = p.fac_value_ps[t] = 0.0
+    o = oracle_mod.Oracle(w)
+    o.calc_all_response_fields(4)
+    lib = oracle_mod.lib()
+    D = oracle_mod.d
+    L = p.DG_MaxLevel
+    for t in w.types:
+        idx = np.nonzero(w.ptype == t)[0][:5].astype(np.int32)
+        fb, ft = o.eval_trials(idx, w.vel[idx], 2)
+        np.testing.assert_array_equal(fb, ft)
+        f = o.fit(t)
+        S, Srelfac, count = f[0], f[5], f[6]
+        dens = S * o.stack(t, 10)[0] / p.MType[t]
+        inst = (f[1] + f[2] + f[3]) / count   # scaled r, t, p terms
+        tovs = p.TypeOfVelocityStructure[t]
+        if tovs == 1:
+            # q of the halo: not in the fit, in the merit
+            th = p.MinParticlesPerBinForDispersionMeasurement * p.MType[t] / p.NType[t]
+            Iq, Eq, MR, MT = o.stack(t, 5), o.stack(t, 15), o.stack(t, 1), o.stack(t, 11)
+            vq = lib.or_calc_stack_difference(o.h, D(Iq), D(Eq), 0, 0, 0, L, D(MR), D(MT), th, None, 1)
+            assert vq > 0
+            inst += Srelfac * vq / count
+        want = dens + inst
+        np.testing.assert_allclose(fb, want, rtol=1e-12)
+        if tovs == 3:
+            # the fit counts the disk's q term, the merit does not: the q term is not zero here
+            assert f[4] > 0
