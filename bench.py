@@ -1,25 +1,30 @@
-"""Throughput of GALIC's equilibrium hot path on one MI355X.
+"""Throughput of GALIC's equilibrium hot path on one MI355X: the full v1.1 model.
 
-Workload: the precision-form model -- a beta=0.3 Hernquist halo (M=100,
-a=30 kpc) plus an exponential sech^2 disk (Md=4, Rd=3.5, z0=0.35 kpc),
-N=1e6 (80% halo, 20% disk), 256^2 force tables, 64^2 response grids.
+Workload: a Hernquist halo (M=100, a=30 kpc, isotropic) + an exponential sech^2
+disk (Md=4, Rd=3.5, z0=0.35 kpc, velocity structure 3 with sigma_R = sigma_z) +
+a Hernquist bulge (Mb=1, ab=0.7 kpc, isotropic), N=2e7 particles (75% halo, 20%
+disk, 5% bulge), 512^2 force tables filled by the GPU tree from 1e7 sampled
+sources (FP32 walk, quadrupole moments, Plummer softening 0.05 kpc) and filled
+again at the start of every iteration, 512^2 response grids (level 9), energy
+targets from 1e8 device-sampled points per component.
 
-A step is one optimisation iteration of the batched FP32 path over all N
-particles in reps of 1e4 (each rep evaluated against the grids at its start and
-committed before the next, main.c:93-100): every particle gets K=8 trial
-velocities drawn on the device -- one fresh draw from optimize()'s own trial
-distribution (main.c:383-435) and 7 Gaussian perturbations of 10% of its target
-dispersions -- its current orbit and the K trial orbits are integrated and
-binned, the merit change of each trial is evaluated, the best improving trial
-is taken, and each rep's deltas are committed and re-fitted.  orbits/s counts
+A step is a block of reps of the batched FP32 path (reps of 1e4 particles, each
+evaluated against the grids at its start and committed before the next,
+main.c:93-100): every particle gets K=8 trial velocities drawn on the device --
+one fresh draw from optimize()'s own distribution (main.c:383-435) and 7
+Gaussian perturbations of 10% of its target dispersions -- its current orbit and
+the K trial orbits are integrated and binned, the merit change of each trial is
+evaluated, the best improving trial is taken, and each rep is committed and
+re-fitted.  A run of steps covering all N particles is one iteration; the table
+fill at the start of each iteration is inside the timed region.  orbits/s counts
 every integrated and binned orbit (current + trials).
 
 Beside it the line reports the plain FP64 path on the same model with the
-reference's method (one fresh trial per particle, reps of 1e4), the final
-chi^2 (Sall) of the FP32 and FP64 paths run on identical trials, the chi^2
-reached by each method after the same number of iterations, the roofline of
-the FP32 orbit kernel, and the CPU oracle timed on the host's cores on a
-bounded sample.
+reference's method (one fresh trial per particle, reps of 1e4), the chi^2 (Sall)
+reached by both methods on a smaller realisation of the same model and the time
+the batched path takes to reach the plain path's final chi^2, the FP32 and FP64
+paths on identical trials, the roofline of the dominant kernel, and the CPU
+oracle timed on the host's cores on a bounded sample.
 """
 import argparse
 import json
@@ -37,49 +42,69 @@ from galic_amd.runtime import (TREE_FP32, TREE_PLUMMER, TREE_QUADRUPOLE, GalicGP
                                tree_fill, tree_forces, tree_last_timings)
 
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
-# HBM bytes per leapfrog step of k_orbit32_log in the iteration's launches, from the FETCH_SIZE and
-# WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_halo-disk-1e6.txt)
-HBM_BYTES_PER_STEP = 22.8
-PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 rate
+PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 vector rate
 PEAK_HBM = 8000.0     # GB/s
+TREE_MODE = TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER   # the force-field configuration's walk
+# HBM bytes per leapfrog step of k_orbit32_log in the full model's launches, from the FETCH_SIZE and
+# WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_composite-2e7.txt)
+HBM_BYTES_PER_STEP = 18.9
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fp32_iteration(g, perm, k, it, sigma, rep):
-    acc = g.fp32_iterate(perm, k=k, rep_size=rep, sigma_frac=sigma, seed=1000 + it)
-    return int((acc > 0).sum())
+def full_model(n, tree_n):
+    """The full v1.1 model: workload arrays on the host, every component sampled into the table."""
+    cfg = model.config("composite-2e7", n=n, halo_in_table=True)
+    w = model.make_workload(cfg, n=n, target_samples=200000)
+    p = w.params
+    nh, nd = int(0.75 * tree_n), int(0.2 * tree_n)
+    p.SampleForceNhalo, p.SampleForceNdisk, p.SampleForceNbulge = nh, nd, tree_n - nh - nd
+    return cfg, w
 
 
-def fp64_trials(w, perm, rep, it):
-    """Fresh trial velocities of one iteration of the plain method (main.c:407-435), per rep."""
-    return [(perm[a:a + rep], model.trial_velocities(w, perm[a:a + rep], 5 + it, r))
-            for r, a in enumerate(range(0, len(perm), rep))]
+def setup(w, cfg, target_points, targets=None):
+    """A context of the model: energy targets sampled on the device (or given), tree sources
+    sampled on the device (seed 3) and the tables filled from them."""
+    g = GalicGPU(w)
+    if targets is None:
+        targets = {}
+        for t in w.types:
+            beta, tabs = model.disp_tables(w.params, cfg, t)
+            g.set_disp_tables(t, beta, **tabs)
+            _, eg, _, _ = g.sample_targets(t, target_points, seed=1, stream=400 + t, density=False)
+            targets[t] = [w.targets[t][0]] + [eg[f] for f in range(5)]
+    for t, st in targets.items():
+        g.set_targets(t, st)
+    g.sample_tree_sources(3)
+    g.retabulate(TREE_MODE)
+    return g, targets
 
 
-def fp64_iteration(g, trials):
-    return np.concatenate([g.rep(idx, v)[0] for idx, v in trials])
+def fp64_trials(w, idx, seed, rep):
+    """Fresh trial velocities of the plain method (main.c:407-435)."""
+    return model.trial_velocities(w, idx, seed, rep)
 
 
-def cpu_baseline(cfg, seconds, nthreads):
-    """The oracle's optimize() (orbit pair + two merits + write-back, main.c:365-527),
-    rep after rep of 1000 trials until about `seconds` of CPU work.  The oracle holds a
-    20000-particle realisation of the same model, whose initial response pass gives
-    the global state the merits are evaluated on; per-trial work is that of the full
-    model (same tables, grids and orbit lengths)."""
+def cpu_baseline(seconds, nthreads, n):
+    """The oracle's optimize() (orbit pair + two merits + write-back, main.c:365-527) on an
+    n-particle realisation of the full model (same grids and tables), rep after rep of 200
+    trials until about `seconds` of CPU work; the initial response pass gives the state the
+    merits are evaluated on."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, used here only as the CPU baseline
-    ws = model.make_workload(cfg, n=20000, target_samples=200000)
+    cfg = model.config("composite-2e7")
+    ws = model.make_workload(cfg, n=n, target_samples=200000)
     o = oracle.Oracle(ws)
+    t0 = time.perf_counter()
     o.calc_all_response_fields(nthreads)
+    t_init = time.perf_counter() - t0
     perm = model.permutation(ws, 3)
-    ntr = 0
-    dt = 0.0
-    rep = 0
-    while dt < seconds and rep < 400:
-        idx = perm[(rep * 1000) % ws.n:(rep * 1000) % ws.n + 1000]
+    ntr, dt, rep = 0, 0.0, 0
+    while dt < seconds and rep < 1000:
+        a = (rep * 200) % ws.n
+        idx = perm[a:a + 200]
         v = model.trial_velocities(ws, idx, 77, rep)
         t0 = time.perf_counter()
         o.rep(idx, v, nthreads)
@@ -87,12 +112,12 @@ def cpu_baseline(cfg, seconds, nthreads):
         ntr += len(idx)
         rep += 1
     o.close()
-    return 2 * ntr / dt, dt, ntr
+    return 2 * ntr / dt, dt, ntr, t_init
 
 
 def tree_sources(p, n, seed):
-    """The composite's mass (halo 75%, disk 20%, bulge 5% of the particles) sampled
-    from its profiles, as forcedensitygrid_calculate's sources (grid.c:659-678)."""
+    """The composite's mass (halo 75%, disk 20%, bulge 5% of the particles) sampled from its
+    profiles, as forcedensitygrid_calculate's sources (grid.c:659-678)."""
     ids = np.arange(n, dtype=np.uint64)
     nd, nb = n // 5, n // 20
     nh = n - nd - nb
@@ -114,7 +139,6 @@ def tree_leg(n, nbin, ncheck):
     spos, smass = tree_sources(p, n, 3)
     log(f"[bench] tree: {n} sources sampled in {time.perf_counter() - t0:.1f}s")
     ntarget = 8 * nbin * nbin
-    # error check on random table nodes (section, z row, R column)
     rng = np.random.default_rng(3)
     s, k, j = rng.integers(0, 8, ncheck), rng.integers(0, nbin, ncheck), rng.integers(0, nbin, ncheck)
     r1 = p.FG_Rmin * (p.FG_Fac ** j - 1.0)
@@ -122,8 +146,7 @@ def tree_leg(n, nbin, ncheck):
     phi = 2 * np.pi / 8 * s
     tpos = np.stack([r1 * np.cos(phi), r1 * np.sin(phi), z1], axis=1)
     out = dict(sources=n, targets=ntarget, fg_nbin=nbin, theta=0.4, softening=cfg.softening)
-    for name, mode in (("quadrupole_plummer_fp32", TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER),
-                       ("reference_kernel_fp32", TREE_FP32)):
+    for name, mode in (("quadrupole_plummer_fp32", TREE_MODE), ("reference_kernel_fp32", TREE_FP32)):
         t0 = time.perf_counter()
         tree_fill(p, spos, smass, precision=mode)
         wall = time.perf_counter() - t0
@@ -143,73 +166,145 @@ def tree_leg(n, nbin, ncheck):
     return out
 
 
-def target_leg(cfg, n, ncpu):
-    """Sampled targets of the bench model (grid.c:536-617, 941-1076), as the reference builds them:
-    n fresh points per component for the density map and another n for the five energy-grid maps.
-    CPU baseline: the same kind of points drawn by the host sampler and binned by the oracle
-    (one thread), ncpu points per map."""
-    w = model.make_workload(cfg, n=2000, target_samples=20000)   # parameters and tables only
-    p = w.params
-    g = GalicGPU(w)
-    per_type = {}
-    for t in w.types:
-        beta, tabs = model.disp_tables(p, cfg, t)
-        g.set_disp_tables(t, beta, **tabs)
-        g.sample_targets(t, 1 << 20, seed=2)   # warm-up
-        t0 = time.perf_counter()
-        _, _, nd, ms_d = g.sample_targets(t, n, seed=1, stream=300 + t, energy=False)
-        _, _, ne, ms_e = g.sample_targets(t, n, seed=1, stream=400 + t, density=False)
-        wall = time.perf_counter() - t0
-        per_type[t] = dict(points=2 * n, binned=nd + ne, density_kernel_ms=ms_d, energy_kernel_ms=ms_e,
-                           wall_ms=wall * 1e3, points_per_s=2 * n / ((ms_d + ms_e) * 1e-3))
-        log(f"[bench] targets type {t}: 2 x {n:.3g} points, kernels {ms_d:.1f} + {ms_e:.1f} ms, wall {wall * 1e3:.0f} ms")
-    g.close()
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the checker, used here only as the CPU baseline
-    o = oracle.Oracle(w, set_particles=False)
-    cpu_s = 0.0
-    for t in w.types:
-        beta, tabs = model.disp_tables(p, cfg, t)
-        o.set_disp_tables(t, beta, **tabs)
-        ids = np.arange(ncpu, dtype=np.uint64)
-        t0 = time.perf_counter()
-        pos = model.sample_type(p, t, 1, 300 + t, ids)
-        o.bin_targets(t, pos, energy=False)
-        pos = model.sample_type(p, t, 1, 400 + t, ids)
-        o.bin_targets(t, pos, density=False)
-        cpu_s += time.perf_counter() - t0
-    o.close()
-    cpu_rate = 2 * ncpu * len(w.types) / cpu_s
-    gpu_rate = sum(v["points"] for v in per_type.values()) / sum(
-        (v["density_kernel_ms"] + v["energy_kernel_ms"]) * 1e-3 for v in per_type.values())
-    log(f"[bench] targets: GPU {gpu_rate:.4g} points/s, CPU {cpu_rate:.4g} points/s (1 thread)")
-    return dict(points_per_map=n, per_type=per_type, points_per_s=gpu_rate,
-                cpu=dict(points_per_s=cpu_rate, threads=1, kind="port",
-                         sample=f"{ncpu} points per map and type: host sampler (numpy) + oracle binning"),
-                gpu_over_cpu=gpu_rate / cpu_rate)
+def chi2_leg(n, iters64, k, sigma, fresh, rep, tree_n, target_points):
+    """Both methods on an n-particle realisation of the full model (same grids, tables and
+    targets): the plain FP64 path for iters64 iterations of one fresh trial per particle, the
+    batched FP32 path iteration by iteration until its Sall reaches the plain path's final
+    Sall for every type (at most 3 iters64 iterations); and one iteration of both paths on
+    identical trials (K=1), the precision of the FP32 path alone."""
+    cfg, w = full_model(n, tree_n)
+    perm = model.permutation(w, 1)
+    g64, targets = setup(w, cfg, target_points)
+    t0 = time.perf_counter()
+    g64.calc_all_response_fields()
+    hist64 = [{t: float(g64.fit(t)[7]) for t in w.types}]
+    wall64 = 0.0
+    for it in range(iters64):
+        trials = [(perm[a:a + rep], fp64_trials(w, perm[a:a + rep], 5 + it, r))
+                  for r, a in enumerate(range(0, w.n, rep))]
+        g64.retabulate(TREE_MODE)
+        t1 = time.perf_counter()
+        for idx, v in trials:
+            g64.rep(idx, v)
+        wall64 += time.perf_counter() - t1
+        hist64.append({t: float(g64.fit(t)[7]) for t in w.types})
+        log(f"[bench] chi2 leg: fp64 iteration {it} done ({time.perf_counter() - t0:.0f}s)")
+    g64.close()
+    g32, _ = setup(w, cfg, target_points, targets)
+    g32.fp32_calc_all_response_fields()
+    g32.fp32_set_fresh_trials(fresh)
+    hist32 = [{t: float(g32.fit(t)[7]) for t in w.types}]
+    goal = hist64[-1]
+    wall32, reach = 0.0, None
+    for it in range(3 * iters64):
+        t1 = time.perf_counter()
+        g32.retabulate(TREE_MODE)
+        g32.fp32_iterate(perm, k=k, rep_size=rep, sigma_frac=sigma, seed=1000 + it)
+        wall32 += time.perf_counter() - t1
+        hist32.append({t: float(g32.fit(t)[7]) for t in w.types})
+        if all(hist32[-1][t] <= goal[t] for t in w.types):
+            reach = it + 1
+            break
+    g32.close()
+    log(f"[bench] chi2 leg: fp64 {hist64[-1]} after {iters64} iterations ({wall64:.1f}s), fp32 {hist32[-1]} after "
+        f"{len(hist32) - 1} ({wall32:.1f}s)")
+    # identical trials (K=1) over one iteration
+    g64, _ = setup(w, cfg, target_points, targets)
+    g64.calc_all_response_fields()
+    g32, _ = setup(w, cfg, target_points, targets)
+    g32.fp32_calc_all_response_fields()
+    agree, drift = [], []
+    for r, a in enumerate(range(0, w.n, rep)):
+        idx = perm[a:a + rep]
+        v = fp64_trials(w, idx, 5, r)
+        a64 = g64.rep(idx, v)[0]
+        a32 = g32.fp32_rep(idx, k=1, vtry=v)
+        agree.append(round(float(np.mean((a32 > 0) == (a64 > 0))), 5))
+        drift.append({t: float(g32.fit(t)[7] / g64.fit(t)[7] - 1) for t in w.types})
+    same = {t: {"fp64": float(g64.fit(t)[7]), "fp32": float(g32.fit(t)[7])} for t in w.types}
+    g64.close()
+    g32.close()
+    worst = max(abs(c["fp32"] - c["fp64"]) / abs(c["fp64"]) for c in same.values())
+    final_rel = {t: (hist32[-1][t] / goal[t] - 1) for t in w.types}
+    return dict(
+        n_particles=w.n, rep=rep,
+        plain_fp64=dict(iterations=iters64, seconds=wall64, history=hist64),
+        batched_fp32=dict(iterations=len(hist32) - 1, seconds=wall32, history=hist32, k=k, fresh=fresh,
+                          sigma_frac=sigma),
+        time_to_plain_final_chi2=dict(batched_fp32_iterations=reach, batched_fp32_s=wall32 if reach else None,
+                                      plain_fp64_s=wall64, speedup=(wall64 / wall32) if reach else None,
+                                      batched_final_over_plain_final_minus_1=final_rel),
+        identical_trials=dict(per_type=same, max_rel_diff=worst, decision_agreement=agree[::max(1, len(agree) // 10)],
+                              rel_diff_per_rep=drift[::max(1, len(drift) // 10)], iterations=1))
+
+
+def control_leg(n, rep, reps):
+    """Sensitivity of the optimisation to a perturbation of the FP64 path itself, beside the FP32
+    path's difference: the precision-form model (beta=0.3 halo + disk, 64^2 grids) with n
+    particles; three runs on identical K=1 trials for `reps` reps of `rep`: the FP64 path, the
+    FP64 path whose initial velocities differ by one ulp, and the FP32 path.  Reported per rep:
+    decision agreement with the first run and Sall / Sall(first run) - 1 per type."""
+    cfg = model.config("halo-disk-1e6")
+    w = model.make_workload(cfg, n=n, target_samples=4000000)
+    perm = model.permutation(w, 1)
+    runs = {}
+    for name in ("fp64", "fp64_ulp", "fp32"):
+        g = GalicGPU(w)
+        if name == "fp64_ulp":
+            g.set_particles(w.pos, np.nextafter(w.vel, np.inf))
+        if name == "fp32":
+            g.fp32_calc_all_response_fields()
+        else:
+            g.calc_all_response_fields()
+        acc, sall = [], []
+        for r in range(reps):
+            idx = perm[(r * rep) % w.n:(r * rep) % w.n + rep]
+            v = model.trial_velocities(w, idx, 5, r)
+            a = g.fp32_rep(idx, k=1, vtry=v) if name == "fp32" else g.rep(idx, v)[0]
+            acc.append(a > 0)
+            sall.append({t: float(g.fit(t)[7]) for t in w.types})
+        g.close()
+        runs[name] = (acc, sall)
+        log(f"[bench] control: {name} done")
+    ref_acc, ref_sall = runs["fp64"]
+    out = dict(n_particles=w.n, rep=rep, reps=reps)
+    every = max(1, reps // 10)
+    for name in ("fp64_ulp", "fp32"):
+        acc, sall = runs[name]
+        agree = [float(np.mean(a == b)) for a, b in zip(acc, ref_acc)]
+        rel = [{t: s[t] / r[t] - 1 for t in w.types} for s, r in zip(sall, ref_sall)]
+        out[name] = dict(decision_agreement=agree[every - 1::every], sall_rel_diff=rel[every - 1::every],
+                         final_sall_rel_diff=rel[-1],
+                         max_abs_sall_rel_diff=max(abs(v) for d in rel for v in d.values()))
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=8, help="timed steps (reps blocks); 8 steps make one iteration")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1000000)
+    ap.add_argument("--n", type=int, default=20000000)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--sigma", type=float, default=0.1)
     ap.add_argument("--fresh", type=int, default=1,
                     help="of the K trials, how many are fresh draws from optimize()'s distribution")
     ap.add_argument("--rep", type=int, default=10000, help="trials per rep (both paths)")
-    ap.add_argument("--fp64-iters", type=int, default=-1,
-                    help="iterations of the plain FP64 path (default warmup+steps; 0: skip)")
+    ap.add_argument("--steps-per-iteration", type=int, default=8)
+    ap.add_argument("--tree-n", type=int, default=10000000, help="sources of the table fill")
+    ap.add_argument("--target-points", type=int, default=100000000, help="points per component for the targets")
+    ap.add_argument("--fp64-reps", type=int, default=3, help="reps of the plain FP64 path timed on the full model")
+    ap.add_argument("--chi2-n", type=int, default=200000, help="particles of the chi^2 comparison (0: skip)")
+    ap.add_argument("--chi2-iters", type=int, default=2, help="plain FP64 iterations of the chi^2 comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget in seconds (0: skip)")
-    ap.add_argument("--target-samples", type=int, default=4000000)
+    ap.add_argument("--cpu-n", type=int, default=3000, help="particles of the CPU baseline's realisation")
     ap.add_argument("--precision-sample", type=int, default=2000,
                     help="orbits integrated by both paths for the FP32 error figures (0: skip)")
-    ap.add_argument("--tree-n", type=int, default=10000000, help="sources of the table-fill leg (0: skip)")
-    ap.add_argument("--tree-nbin", type=int, default=512)
-    ap.add_argument("--target-points", type=int, default=100000000,
-                    help="points per map and component of the sampled-target leg (0: skip)")
+    ap.add_argument("--c4-tree-n", type=int, default=10000000, help="sources of the table-fill leg (0: skip)")
+    ap.add_argument("--control-n", type=int, default=0,
+                    help="particles of the FP64 perturbation control on the precision-form model (0: skip)")
+    ap.add_argument("--control-reps", type=int, default=100)
+    ap.add_argument("--only-control", action="store_true", help="run the control leg alone and print its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -218,127 +313,87 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.only_control:
+        print(json.dumps(control_leg(args.control_n or 1000000, args.rep, args.control_reps)), flush=True)
+        return
+
     t0 = time.perf_counter()
-    cfg = model.config("halo-disk-1e6")
-    w = model.make_workload(cfg, n=args.n, target_samples=args.target_samples)
+    cfg, w = full_model(args.n, args.tree_n)
     log(f"[bench] workload {cfg.name} n={w.n} built in {time.perf_counter() - t0:.1f}s")
     perm = model.permutation(w, 1)
+    spi = args.steps_per_iteration
+    bounds = np.linspace(0, w.n, spi + 1).astype(np.int64)
+    # a step covers whole reps
+    bounds = (bounds // args.rep) * args.rep
+    bounds[-1] = w.n
 
     # ---------------------------------------------------------- FP32 batched
-    g = GalicGPU(w)
-    g.fp32_calc_all_response_fields()
-    g.fp32_set_fresh_trials(args.fresh)
-    hist32 = [{t: float(g.fit(t)[7]) for t in w.types}]
-    variant, bpc = g.fp32_autotune(min(w.n, 524288))
-    log(f"[bench] fp32 initial pass done; autotuned orbit kernel: {bpc} workgroups/CU")
-    for it in range(args.warmup):
-        fp32_iteration(g, perm, args.k, it, args.sigma, args.rep)
-        hist32.append({t: float(g.fit(t)[7]) for t in w.types})
-        log(f"[bench] warmup {it} done")
-    g.reset_timings()
     t0 = time.perf_counter()
-    accepted = 0
-    for it in range(args.steps):
-        accepted += fp32_iteration(g, perm, args.k, args.warmup + it, args.sigma, args.rep)
-        hist32.append({t: float(g.fit(t)[7]) for t in w.types})
-        log(f"[bench] step {it} done")
+    g, targets = setup(w, cfg, args.target_points)
+    tm0 = tree_last_timings()
+    log(f"[bench] targets and table fill ready in {time.perf_counter() - t0:.1f}s (tree walk {tm0['walk_ms']:.0f} ms)")
+    t0 = time.perf_counter()
+    g.fp32_calc_all_response_fields()
+    init_s = time.perf_counter() - t0
+    ti = g.timings()
+    log(f"[bench] fp32 initial pass {init_s:.1f}s (orbit kernel {ti['orbit_ms']:.0f} ms)")
+    g.fp32_set_fresh_trials(args.fresh)
+    variant, bpc = g.fp32_autotune(min(w.n, 524288))
+    log(f"[bench] autotuned orbit kernel: {bpc} workgroups/CU")
+
+    def step(s, seed):
+        j = s % spi
+        retab_ms = 0.0
+        if j == 0:
+            t1 = time.perf_counter()
+            g.retabulate(TREE_MODE)
+            retab_ms = (time.perf_counter() - t1) * 1e3
+        idx = perm[bounds[j]:bounds[j + 1]]
+        acc = g.fp32_iterate(idx, k=args.k, rep_size=args.rep, sigma_frac=args.sigma, seed=seed)
+        return len(idx), int((acc > 0).sum()), retab_ms
+
+    for s in range(args.warmup):
+        step(s, 500 + s)
+        log(f"[bench] warmup {s} done")
+    g.reset_timings()
+    hist = [{t: float(g.fit(t)[7]) for t in w.types}]
+    t0 = time.perf_counter()
+    nparticles = accepted = 0
+    retab_ms = 0.0
+    for s in range(args.steps):
+        n1, a1, r1 = step(s, 1000 + s)
+        nparticles += n1
+        accepted += a1
+        retab_ms += r1
+        log(f"[bench] step {s} done ({n1} particles, {time.perf_counter() - t0:.1f}s)")
     wall = time.perf_counter() - t0
+    hist.append({t: float(g.fit(t)[7]) for t in w.types})
     tm = g.timings()
-    orbits = args.steps * w.n * (args.k + 1)
-    g.close()
+    orbits = nparticles * (args.k + 1)
     value = orbits / wall
     steps_total = tm["steps"]
     achieved = F_STEP * steps_total / (tm["orbit_ms"] * 1e-3) / 1e12
     breakdown = dict(orbit_kernel=tm["orbit_ms"] / args.steps, merit_kernel=tm["eval_ms"] / args.steps,
-                     fit=tm["fit_ms"] / args.steps,
-                     other=(wall * 1e3 - tm["orbit_ms"] - tm["eval_ms"] - tm["fit_ms"]) / args.steps)
+                     fit=tm["fit_ms"] / args.steps, table_fill=retab_ms / args.steps,
+                     other=(wall * 1e3 - tm["orbit_ms"] - tm["eval_ms"] - tm["fit_ms"] - retab_ms) / args.steps)
     log(f"[bench] fp32: {value:.4g} orbits/s, {steps_total / wall:.4g} steps/s, orbit kernel {tm['orbit_ms']:.1f} ms "
         f"of {wall * 1e3:.1f} ms, {achieved:.2f} TFLOP/s algorithmic; ms per step {breakdown}, "
         f"log relaunches {tm['log_relaunches']:.0f}")
 
-    # ------------------------------------------------------ FP64 plain path
-    fp64 = None
-    chi2 = None
-    iters64 = args.warmup + args.steps if args.fp64_iters < 0 else args.fp64_iters
-    if iters64 > 0:
-        trials = [fp64_trials(w, perm, args.rep, it) for it in range(iters64)]
-        g64 = GalicGPU(w)
-        g64.calc_all_response_fields()
-        hist64 = [{t: float(g64.fit(t)[7]) for t in w.types}]
-        g64.reset_timings()
-        wall64 = 0.0
-        for it in range(iters64):
-            t1 = time.perf_counter()
-            fp64_iteration(g64, trials[it])
-            wall64 += time.perf_counter() - t1
-            hist64.append({t: float(g64.fit(t)[7]) for t in w.types})
-            log(f"[bench] fp64 iteration {it} done")
-        del trials
-        tm64 = g64.timings()
-        g64.close()
-        fp64 = dict(orbits_per_s=2 * w.n * iters64 / wall64, ms_per_iteration=wall64 / iters64 * 1e3,
-                    steps_per_s=tm64["steps"] / wall64, orbit_kernel_ms=tm64["orbit_ms"], iterations=iters64)
-        log(f"[bench] fp64: {fp64['orbits_per_s']:.4g} orbits/s")
-
-        # identical trials on both paths (K=1): the precision of the FP32 path alone
-        g64 = GalicGPU(w)
-        g64.calc_all_response_fields()
-        g32 = GalicGPU(w)
-        g32.fp32_calc_all_response_fields()
-        init = {t: {"fp64": float(g64.fit(t)[7]), "fp32": float(g32.fit(t)[7])} for t in w.types}
-        agree, drift = [], []
-        for r, a in enumerate(range(0, w.n, args.rep)):
-            idx = perm[a:a + args.rep]
-            v = model.trial_velocities(w, idx, 5, r)
-            a64 = g64.rep(idx, v)[0]
-            a32 = g32.fp32_rep(idx, k=1, vtry=v)
-            agree.append(round(float(np.mean((a32 > 0) == (a64 > 0))), 5))
-            drift.append({t: float(g32.fit(t)[7] / g64.fit(t)[7] - 1) for t in w.types})
-        same = {t: {"fp64": float(g64.fit(t)[7]), "fp32": float(g32.fit(t)[7])} for t in w.types}
-        g64.close()
-        g32.close()
-        worst = max(abs(c["fp32"] - c["fp64"]) / abs(c["fp64"]) for c in same.values())
-        every = max(1, len(agree) // 10)
-        agree, drift = agree[every - 1::every], drift[every - 1::every]
-        log(f"[bench] identical trials: initial Sall {init}, final {same}, rel diff {worst:.2e}, "
-            f"decision agreement every {every} reps {agree}, Sall fp32/fp64 - 1 every {every} reps {drift}")
-        n_cmp = min(len(hist32), len(hist64)) - 1
-        # time to reach the plain path's final chi^2: iterations of the batched path
-        # until every type's Sall is at or below the plain path's after iters64
-        goal = hist64[-1]
-        reach = next((i for i in range(1, len(hist32)) if all(hist32[i][t] <= goal[t] for t in w.types)), None)
-        ms32 = wall / args.steps * 1e3
-        to_chi2 = dict(plain_fp64_iterations=iters64, plain_fp64_ms=fp64["ms_per_iteration"] * iters64,
-                       batched_fp32_iterations=reach, batched_fp32_ms=(reach * ms32) if reach else None,
-                       speedup=(fp64["ms_per_iteration"] * iters64 / (reach * ms32)) if reach else None)
-        # equal final chi^2: after the same iterations the batched path's Sall is within
-        # 1e-3 relative of the plain path's, or below it, for every type
-        equal = {t: hist32[n_cmp][t] <= hist64[n_cmp][t] * (1 + 1e-3) for t in w.types}
-        chi2 = dict(equal_final_chi2=all(equal.values()),
-                    identical_trials=dict(per_type=same, max_rel_diff=worst, initial=init, decision_agreement=agree,
-                                          rel_diff_per_rep=drift, sampled_every_reps=every, iterations=1,
-                                          rep=args.rep),
-                    methods=dict(iterations=n_cmp, batched_fp32=hist32[n_cmp], plain_fp64=hist64[n_cmp],
-                                 history_batched_fp32=hist32, history_plain_fp64=hist64),
-                    time_to_plain_final_chi2=to_chi2)
-        log(f"[bench] time to the plain path's final chi^2: {to_chi2}")
-
     # ------------------------------------------- FP32 orbits against FP64
     precision = None
     if args.precision_sample > 0:
-        g = GalicGPU(w)
         sel = np.random.default_rng(7).choice(w.n, args.precision_sample, replace=False)
         errs, d32, d64, dsteps = [], [], [], []
-        for a in range(0, len(sel), 500):
-            s = sel[a:a + 500]
-            orb = (w.pos[s], w.vel[s], w.mass[s], w.tint[s], w.ptype[s])
+        for a in range(0, len(sel), 250):
+            s_ = sel[a:a + 250]
+            orb = (w.pos[s_], w.vel[s_], w.mass[s_], w.tint[s_], w.ptype[s_])
             r64 = g.orbit_responses(*orb)
             r32 = g.orbit_responses(*orb, precision=32)
             errs.append(np.abs(r32["fields"] - r64["fields"]).sum(axis=(1, 2)) / np.abs(r64["fields"]).sum(axis=(1, 2)))
             d32.append(np.abs(r32["e1"] / r32["e0"] - 1))
             d64.append(np.abs(r64["e1"] / r64["e0"] - 1))
             dsteps.append(np.abs(r32["steps"] - r64["steps"]) / r64["steps"])
-        g.close()
         errs, d32, d64, dsteps = map(np.concatenate, (errs, d32, d64, dsteps))
         precision = dict(orbits=len(sel), response_l1_rel_err=dict(median=float(np.median(errs)),
                                                                   p99=float(np.quantile(errs, 0.99))),
@@ -347,22 +402,56 @@ def main():
                                            fp64_p99=float(np.quantile(d64, 0.99))),
                          step_count_rel_diff_median=float(np.median(dsteps)))
         log(f"[bench] fp32 vs fp64 orbits: {precision}")
+    g.close()
+
+    # ------------------------------------------------------ FP64 plain path
+    fp64 = None
+    if args.fp64_reps > 0:
+        g64, _ = setup(w, cfg, args.target_points, targets)
+        t0 = time.perf_counter()
+        g64.calc_all_response_fields()
+        init64 = time.perf_counter() - t0
+        g64.reset_timings()
+        wall64 = 0.0
+        ntr = 0
+        for r in range(args.fp64_reps):
+            idx = perm[r * args.rep:(r + 1) * args.rep]
+            v = fp64_trials(w, idx, 5, r)
+            t1 = time.perf_counter()
+            g64.rep(idx, v)
+            wall64 += time.perf_counter() - t1
+            ntr += len(idx)
+        tm64 = g64.timings()
+        g64.close()
+        fp64 = dict(orbits_per_s=2 * ntr / wall64, ms_per_rep=wall64 / args.fp64_reps * 1e3, reps=args.fp64_reps,
+                    steps_per_s=tm64["steps"] / wall64, orbit_kernel_ms=tm64["orbit_ms"], initial_pass_s=init64)
+        log(f"[bench] fp64: {fp64['orbits_per_s']:.4g} orbits/s over {args.fp64_reps} reps, initial pass {init64:.1f}s")
+    n_total = w.n
+    split = [int(w.params.NType[t]) for t in (1, 2, 3)]
+    del w
+
+    # --------------------------------------------- chi^2 of both methods
+    chi2 = None
+    if args.chi2_n > 0:
+        chi2 = chi2_leg(args.chi2_n, args.chi2_iters, args.k, args.sigma, args.fresh, args.rep,
+                        min(args.tree_n, 2000000), args.target_points // 10)
+        log(f"[bench] chi2: {chi2['time_to_plain_final_chi2']}")
+
+    control = control_leg(args.control_n, args.rep, args.control_reps) if args.control_n > 0 else None
 
     # ------------------------------------------------------------- CPU
     cpu = None
     if args.cpu_seconds > 0:
         nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        rate, dt, ntr = cpu_baseline(cfg, args.cpu_seconds, nthreads)
+        rate, dt, ntr, tinit = cpu_baseline(args.cpu_seconds, nthreads, args.cpu_n)
         cpu = {"value": rate, "unit": "orbits/s", "cores": nthreads, "kind": "port",
-               "sample": f"{ntr} trials in reps of 1000 (2 orbits + 2 merit evaluations each, the reference's "
-                         f"optimize()) on a 20000-particle realisation of this model, {nthreads} OpenMP threads, "
-                         f"{dt:.1f} s"}
+               "sample": f"{ntr} trials in reps of 200 (2 orbits + 2 merit evaluations each, the reference's "
+                         f"optimize()) on a {args.cpu_n}-particle realisation of the full model (512^2 grids and "
+                         f"tables), {nthreads} OpenMP threads, {dt:.1f} s (initial pass {tinit:.1f} s not counted)"}
         log(f"[bench] cpu: {rate:.4g} orbits/s on {nthreads} threads ({ntr} trials, {dt:.1f} s)")
 
     # ------------------------------------------------------ table fill
-    tree = tree_leg(args.tree_n, args.tree_nbin, 4096) if args.tree_n > 0 else None
-    # ------------------------------------------------- sampled targets
-    targets = target_leg(cfg, args.target_points, 1000000) if args.target_points > 0 else None
+    tree = tree_leg(args.c4_tree_n, 512, 4096) if args.c4_tree_n > 0 else None
 
     # ---------------------------------------------------- HBM calibration
     copy_ms = np.zeros(1)
@@ -383,31 +472,36 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "precision form: beta=0.3 Hernquist halo + exponential sech^2 disk",
-                   "n_particles": w.n, "halo_disk_split": [int(w.params.NType[1]), int(w.params.NType[2])],
-                   "force_table": [w.params.FG_Nbin, w.params.FG_Nbin], "response_grid": [1 << w.params.DG_MaxLevel] * 2,
+        "config": {"workload": "full v1.1 model: isotropic Hernquist halo + structure-3 exponential sech^2 disk + "
+                               "isotropic Hernquist bulge, tables re-filled by the GPU tree every iteration",
+                   "n_particles": n_total, "type_split": split,
+                   "force_table": [512, 512], "response_grid": [512, 512],
+                   "tree_sources": args.tree_n, "target_points_per_component": args.target_points,
                    "trials_per_particle": args.k, "fresh_trials": args.fresh, "trial_sigma_frac": args.sigma,
-                   "orbits_per_step": w.n * (args.k + 1), "rep": args.rep, "torbit_fac": cfg.torbit_fac,
-                   "autotuned": {"workgroups_per_cu": bpc}},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
+                   "rep": args.rep, "particles_per_step": int(nparticles / args.steps),
+                   "steps_per_iteration": spi, "orbits_per_step": int(orbits / args.steps),
+                   "torbit_fac": cfg.torbit_fac, "autotuned": {"workgroups_per_cu": bpc}},
+        "roofline": {"bound": "valu_fp32", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32,
                      "traffic": HBM_BYTES_PER_STEP * steps_total / max(1, tm["orbit_launches"]),
                      "kernel": "k_orbit32_log",
                      "note": f"roof: the FP32 vector ALU (no step is a contraction, MFMA unused); achieved = {F_STEP} "
                              "algorithmic flop/step x leapfrog steps / HIP-event time of the kernel; traffic = HBM "
-                             f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE, "
-                             "almost all of it response-log writes); waves spend most cycles waiting on memory "
-                             "(SQ_WAIT_ANY), not on the ALU"},
+                             f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE "
+                             "of the profile, almost all of it response-log writes)"},
         "steps_per_s": steps_total / wall,
         "ms_per_step_breakdown": breakdown,
-        "accept_frac": accepted / (args.steps * w.n),
+        "fp32_initial_pass_s": init_s,
+        "accept_frac": accepted / max(1, nparticles),
+        "sall_before_after_timed_steps": hist,
         "fp64_plain": fp64,
         "fp32_vs_fp64_speedup": (value / fp64["orbits_per_s"]) if fp64 else None,
         "chi2_check": chi2,
+        "perturbation_control": control,
         "fp32_orbit_precision": precision,
         "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
         "tree_fill": tree,
-        "target_fill": targets,
         "copy_probe_GBs": copy_gbs,
     }
     print(json.dumps(line), flush=True)

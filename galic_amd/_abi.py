@@ -65,6 +65,9 @@ EXPORTS = [
     "forcegrid_get_potential", "forcegrid_get_escape_speed", "forcegrid_get_acceleration", "smooth_stack",
     "forcegrid_get_cell", "densitygrid_get_cell", "energygrid_get_cell", "get_timestep",
     "calc_stack_difference", "calc_stack_difference_mod",
+    "galic_set_tree_sources", "galic_sample_tree_sources", "galic_retabulate", "galic_get_force_tables",
+    "forcedensitygrid_create", "forcedensitygrid_calculate", "galic_bind_tree_particles", "force_treebuild",
+    "force_update_node_recursive", "force_treeevaluate", "gravity",
 ]
 
 _P = ctypes.POINTER
@@ -140,6 +143,11 @@ def lib():
         "galic_reset_timings": [_vp],
         "galic_copy_probe": [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp],
         "galic_bind": [_vp],
+        "galic_set_tree_sources": [_vp, ctypes.c_int, _dp, _dp],
+        "galic_sample_tree_sources": [_vp, ctypes.c_uint],
+        "galic_retabulate": [_vp, ctypes.c_int, _lp],
+        "galic_get_force_tables": [_vp, _dp, _dp, _dp],
+        "galic_bind_tree_particles": [ctypes.c_int, _dp, _dp, _ip, _dp, _dp],
     }
     for name, args in sig.items():
         if not hasattr(L, name):

@@ -306,14 +306,24 @@ int global_fit(galic_ctx *ctx) {
   galic::Ctx &c = ctx->c;
   galic::FitArgs F = fit_args(ctx);
   HIPCHK(hipEventRecord(c.ev0, c.stream));
-  hipLaunchKernelGGL(galic::k_fit_stacks, dim3(30), dim3(256), 0, c.stream, F);
+  if(c.L >= 5) {
+    // grids of 32 x 32 cells and more: the stacks in tiles, then the levels above them
+    const int ntile = (c.Nbin / 32) * (c.Nbin / 32);
+    hipLaunchKernelGGL(galic::k_fit_tiles, dim3(ntile, 30), dim3(256), 0, c.stream, F);
+    hipLaunchKernelGGL(galic::k_fit_top, dim3(30), dim3(256), 0, c.stream, F);
+  } else {
+    hipLaunchKernelGGL(galic::k_fit_stacks, dim3(30), dim3(256), 0, c.stream, F);
+  }
   HIPCHK(hipGetLastError());
   if(!c.cuts_ready) {
     HIPCHK(hipStreamSynchronize(c.stream));
     CHK(build_weights(ctx, true));
     c.cuts_ready = true;
   }
-  hipLaunchKernelGGL(galic::k_fit_sums, dim3(3), dim3(256), 5 * 256 * sizeof(double), c.stream, F);
+  hipLaunchKernelGGL(galic::k_fit_partial, dim3(galic::kFitBlocks, 3), dim3(256), 0, c.stream, F, c.fit_part,
+                     galic::kFitBlocks);
+  hipLaunchKernelGGL(galic::k_fit_final, dim3(3), dim3(64), 0, c.stream, F, (const double *)c.fit_part,
+                     galic::kFitBlocks);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipEventSynchronize(c.ev1));
@@ -682,6 +692,7 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.qinv, 9));
   CHK(dalloc(&c.tovs_dev, GALIC_NTYPES));
   CHK(dalloc(&c.fit, (size_t)GALIC_NTYPES * 16));
+  CHK(dalloc(&c.fit_part, (size_t)3 * galic::kFitBlocks * 5));
   CHK(dalloc(&c.counter, 1));
   CHK(dalloc(&c.orbit_stats, 4));
   CHK(dalloc(&c.log_next, 1));
@@ -772,6 +783,8 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.tile_list);
   dfree(c.tile_aux);
   dfree(c.lookup_buf);
+  dfree(c.fit_part);
+  dfree(c.tsrc);
   if(c.sort_tmp) (void)hipFree(c.sort_tmp);
   if(c.slots) (void)hipFree(c.slots);
   if(c.ev0) (void)hipEventDestroy(c.ev0);
@@ -794,6 +807,111 @@ int galic_set_force_tables(galic_ctx *ctx, const double *pot, const double *dpot
   for(size_t i = 0; i < n; i++) f[i] = make_float2((float)dpotdr[i], (float)dpotdz[i]);
   HIPCHK(hipMemcpy(c.FG_Acc_f, f.data(), n * sizeof(float2), hipMemcpyHostToDevice));
   return GALIC_OK;
+}
+
+int galic_get_force_tables(galic_ctx *ctx, double *pot, double *dpotdr, double *dpotdz) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null context");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  const size_t n = c.FG_Ngrid;
+  if(pot) HIPCHK(hipMemcpy(pot, c.FG_Pot, n * sizeof(double), hipMemcpyDeviceToHost));
+  if(dpotdr) HIPCHK(hipMemcpy(dpotdr, c.FG_DPotDR, n * sizeof(double), hipMemcpyDeviceToHost));
+  if(dpotdz) HIPCHK(hipMemcpy(dpotdz, c.FG_DPotDz, n * sizeof(double), hipMemcpyDeviceToHost));
+  return GALIC_OK;
+}
+
+int galic_sample_tree_sources(galic_ctx *ctx, unsigned int seed) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null context");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  // grid.c:659-678: SampleForceN{halo,disk,bulge} fresh points of each sampled component,
+  // each of mass M / N (halo.c:17-40, disk.c:17-51, bulge.c:17-38)
+  const int counts[GALIC_NTYPES] = {0, c.p.SampleForceNhalo, c.p.SampleForceNdisk, c.p.SampleForceNbulge};
+  long long total = 0;
+  for(int t = 1; t < GALIC_NTYPES; t++)
+    if(c.p.MType[t] > 0 && counts[t] > 0) total += counts[t];
+  if(total < 1) return set_err(GALIC_ERR_STATE, "no component is sampled (SampleForceN of every component is 0)");
+  if(total > 0x7fffffff) return set_err(GALIC_ERR_ARG, "%lld sources", total);
+  std::vector<double> pos(3 * (size_t)total), mass(total);
+  double *dpts = nullptr;
+  unsigned long long *cnt = nullptr;
+  size_t at = 0;
+  int rc = GALIC_OK;
+  for(int t = 1; t < GALIC_NTYPES && rc == GALIC_OK; t++) {
+    if(!(c.p.MType[t] > 0) || counts[t] <= 0) continue;
+    const int n = counts[t];
+    CHK(dalloc(&dpts, 3 * (size_t)n));
+    CHK(dalloc(&cnt, 2));
+    galic::TargetArgs A;
+    memset(&A, 0, sizeof(A));
+    A.M = c.M;
+    A.type = t;
+    A.tovs = c.p.TypeOfVelocityStructure[t];
+    A.n = n;
+    A.seed = seed;
+    A.stream = 200 + t;
+    A.Rmax = c.p.Rmax;
+    A.Disk_H = c.p.Disk_H;
+    A.Disk_Z0 = c.p.Disk_Z0;
+    A.stretch = 1.0;
+    A.pos_out = dpts;
+    A.counts = cnt;
+    A.count = 1;
+    if(galic::launch_sample_targets(A, orbit_grid(n, 256, 8), 0, c.stream) != 0)
+      rc = set_err(GALIC_ERR_HIP, "source sampling launch failed");
+    unsigned long long hc[2] = {0, 0};
+    if(rc == GALIC_OK && (hipMemcpy(pos.data() + 3 * at, dpts, sizeof(double) * 3 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+                          hipMemcpy(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost) != hipSuccess))
+      rc = set_err(GALIC_ERR_HIP, "copying the sampled sources failed");
+    if(rc == GALIC_OK && hc[1] != 0) rc = set_err(GALIC_ERR_ARG, "%llu sources of type %d found no point", hc[1], t);
+    for(int i = 0; i < n; i++) mass[at + i] = c.p.MType[t] / n;
+    at += n;
+    dfree(dpts);
+    dfree(cnt);
+  }
+  if(rc != GALIC_OK) return rc;
+  return galic_set_tree_sources(ctx, (int)total, pos.data(), mass.data());
+}
+
+void forcedensitygrid_create(void) {
+  // grid.c:329-487 for the bound context: the density-grid geometry (DG_CellSize, DGs_Distance)
+  // from its parameters, which hold the Fac and Rmin of the grids (galic_grid_create)
+  galic_ctx *ctx = galic::bound_ctx();
+  if(!ctx) {
+    fprintf(stderr, "galic drop-in forcedensitygrid_create failed: no context bound\n");
+    exit(1);
+  }
+  galic::Ctx &c = ctx->c;
+  std::vector<double> cs(c.Ngrid), dist(c.Nstack);
+  bool ok = galic_density_geometry(&c.p, cs.data(), nullptr, dist.data()) == GALIC_OK &&
+            hipSetDevice(c.device) == hipSuccess &&
+            hipMemcpy(c.DG_CellSize, cs.data(), sizeof(double) * c.Ngrid, hipMemcpyHostToDevice) == hipSuccess;
+  std::vector<float> f(c.Ngrid);
+  for(int i = 0; i < c.Ngrid; i++) f[i] = (float)cs[i];
+  ok = ok && hipMemcpy(c.DG_CellSize_f, f.data(), sizeof(float) * c.Ngrid, hipMemcpyHostToDevice) == hipSuccess;
+  for(int t = 0; t < GALIC_NTYPES && ok; t++)
+    ok = hipMemcpy(c.stacks + ((size_t)t * galic::ST_COUNT + galic::ST_DIST) * c.Nstack, dist.data(),
+                   sizeof(double) * c.Nstack, hipMemcpyHostToDevice) == hipSuccess;
+  if(!ok) {
+    fprintf(stderr, "galic drop-in forcedensitygrid_create failed: %s\n", galic_last_error());
+    exit(1);
+  }
+  ctx->cellsize = cs;
+  ctx->distance = dist;
+}
+
+void forcedensitygrid_calculate(void) {
+  // grid.c:621-757 for the bound context: its tree sources (drawn on the device from the
+  // sampled components, seed 42 as init.c:25-27 seeds rank 0, if none were set), the
+  // reference's walk (FP64, monopole, spline softening), the tables installed in place
+  galic_ctx *ctx = galic::bound_ctx();
+  int rc = ctx ? GALIC_OK : set_err(GALIC_ERR_STATE, "no context bound");
+  if(rc == GALIC_OK && !ctx->c.tsrc) rc = galic_sample_tree_sources(ctx, 42);
+  if(rc == GALIC_OK) rc = galic_retabulate(ctx, 0, nullptr);
+  if(rc != GALIC_OK) {
+    fprintf(stderr, "galic drop-in forcedensitygrid_calculate failed: %s\n", galic_last_error());
+    exit(1);
+  }
 }
 
 int galic_set_vstream(galic_ctx *ctx, int type, const double *vstream) {

@@ -427,6 +427,141 @@ __global__ void k_fit_stacks(FitArgs A) {
   }
 }
 
+// calc_global_fit's stacks for grids of 32 x 32 cells and more, in tiles: workgroup (tile,
+// stack) copies the tile's 32 x 32 fine cells into the stack and forms the tile's nodes of
+// levels L-1 .. L-5 in LDS, each parent the sum of its four children in smooth_stack's order
+// (grid.c:1247-1265); k_fit_top then forms the levels above the tiles.  The same sums in the
+// same order as k_fit_stacks, spread over many workgroups.
+constexpr int FIT_TILE = 32;
+__global__ void __launch_bounds__(256) k_fit_tiles(FitArgs A) {
+  __shared__ double lv[2][FIT_TILE * FIT_TILE];
+  const int job = blockIdx.y;  // type*10 + k
+  const int t = job / 10 + 1, k = job % 10;
+  if(!A.present[t]) return;
+  const int L = A.L, nb = 1 << L, ngrid = nb * nb, Nstack = GALIC_NSTACK(L), ts = nb / FIT_TILE;
+  const int tz = blockIdx.x / ts, tr = blockIdx.x % ts;
+  double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
+  const int dst[10] = {ST_RM, ST_RSR, ST_RST, ST_RSQ, ST_RSP, ST_EMR, ST_IR, ST_IT, ST_IP, ST_IQ};
+  double *st = S + (size_t)dst[k] * Nstack;
+  const double *src;
+  if(k < 5)
+    src = A.locA + ((size_t)t * kFields + k) * ngrid;
+  else if(k == 5)
+    src = A.locM + (size_t)t * ngrid;
+  else
+    src = A.locI + ((size_t)t * 4 + (k - 6)) * ngrid;
+  const int off = GALIC_STACKOFFSET(L, 0, 0);
+  for(int q = threadIdx.x; q < FIT_TILE * FIT_TILE; q += blockDim.x) {
+    const int c = (tz * FIT_TILE + q / FIT_TILE) * nb + tr * FIT_TILE + q % FIT_TILE;
+    const double v = src[c];
+    st[off + c] = v;
+    lv[0][q] = v;
+  }
+  __syncthreads();
+  int cur = 0;
+  for(int u = 1; u <= 5; u++) {
+    const int n = FIT_TILE >> u, l = L - u;
+    const double *ch = lv[cur];
+    for(int q = threadIdx.x; q < n * n; q += blockDim.x) {
+      const int i = q / n, j = q % n, m = 2 * n;
+      const double v = ch[(2 * i) * m + 2 * j] + ch[(2 * i + 1) * m + 2 * j] + ch[(2 * i) * m + 2 * j + 1] +
+                       ch[(2 * i + 1) * m + 2 * j + 1];
+      lv[1 - cur][q] = v;
+      st[GALIC_STACKOFFSET(l, tz * n + i, tr * n + j)] = v;
+    }
+    cur = 1 - cur;
+    __syncthreads();
+  }
+}
+
+// The levels above the tiles of k_fit_tiles (L-6 .. 0), one workgroup per stack.
+__global__ void k_fit_top(FitArgs A) {
+  const int job = blockIdx.x;
+  const int t = job / 10 + 1, k = job % 10;
+  if(!A.present[t]) return;
+  const int L = A.L, Nstack = GALIC_NSTACK(L);
+  const int dst[10] = {ST_RM, ST_RSR, ST_RST, ST_RSQ, ST_RSP, ST_EMR, ST_IR, ST_IT, ST_IP, ST_IQ};
+  double *st = A.stacks + (size_t)t * ST_COUNT * Nstack + (size_t)dst[k] * Nstack;
+  for(int l = L - 6; l >= 0; l--) {
+    const int n = 1 << l;
+    for(int q = threadIdx.x; q < n * n; q += blockDim.x) {
+      const int i = q / n, j = q % n;
+      st[GALIC_STACKOFFSET(l, i, j)] = st[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * j)] +
+                                       st[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * j)] +
+                                       st[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * j + 1)] +
+                                       st[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * j + 1)];
+    }
+    __syncthreads();
+  }
+}
+
+// The cut sums of calc_global_fit spread over nblk workgroups per type: partial sums of
+// the five merit terms per workgroup, summed in workgroup order by k_fit_final.
+__global__ void __launch_bounds__(256) k_fit_partial(FitArgs A, double *part, int nblk) {
+  __shared__ double red[5 * 256];
+  const int t = blockIdx.y + 1, b = blockIdx.x;
+  if(!A.present[t]) return;
+  const int L = A.L, Nstack = GALIC_NSTACK(L);
+  const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
+  const double *W = A.weights + (size_t)t * W_COUNT * Nstack;
+  const double *den = W + W_A * Nstack, *cutB = W + W_B * Nstack;
+  const double *Rm = S + ST_RM * Nstack, *Tm = S + ST_MT * Nstack;
+  const double *MR = S + ST_EMR * Nstack, *MT = S + ST_EMT * Nstack;
+  const int stI[4] = {ST_IR, ST_IT, ST_IP, ST_IQ};
+  const int stE[4] = {ST_ER, ST_ET, ST_EP, ST_EQ};
+  double v[5] = {0, 0, 0, 0, 0};
+  for(int node = b * blockDim.x + threadIdx.x; node < Nstack; node += nblk * blockDim.x) {
+    if(den[node] != 0) v[0] += fabs(Rm[node] - Tm[node]) / den[node];
+    if(cutB[node] != 0 && MR[node] > 0 && MT[node] > 0)
+      for(int f = 0; f < 4; f++) {
+        const double *I = S + stI[f] * Nstack, *E = S + stE[f] * Nstack;
+        double tgt = E[node] / MT[node];
+        double dd = tgt > A.lowdisp ? tgt : A.lowdisp;
+        v[1 + f] += fabs(I[node] / MR[node] - tgt) / dd;
+      }
+  }
+  block_sum<5>(v, red);
+  if(threadIdx.x == 0)
+    for(int f = 0; f < 5; f++) part[((size_t)blockIdx.y * nblk + b) * 5 + f] = v[f];
+}
+
+// The bookkeeping of main.c:294-359 from the partial sums (one thread per type).
+__global__ void k_fit_final(FitArgs A, const double *part, int nblk) {
+  const int t = blockIdx.x + 1;
+  if(!A.present[t] || threadIdx.x != 0) return;
+  double v[5] = {0, 0, 0, 0, 0};
+  for(int b = 0; b < nblk; b++)
+    for(int f = 0; f < 5; f++) v[f] += part[((size_t)blockIdx.x * nblk + b) * 5 + f];
+  const int Nstack = GALIC_NSTACK(A.L);
+  const double *Tm = A.stacks + (size_t)t * ST_COUNT * Nstack + ST_MT * Nstack;
+  double *fit = A.fit + (size_t)t * 16;
+  double s = v[0] * (1.0 / Tm[0]);
+  double sr = v[1], st = v[2], sp = v[3], sq = v[4];
+  int tovs = A.tovs[t];
+  double cnt;
+  if(tovs == 0 || tovs == 1) {
+    cnt = 3.0;
+    sq = 0;
+  } else
+    cnt = 4.0;
+  if(fit[8] == 0) {
+    fit[5] = s / ((sr + st + sp + sq) / cnt);
+    fit[8] = 1;
+  }
+  double rf = fit[5];
+  sr *= rf;
+  st *= rf;
+  sp *= rf;
+  sq *= rf;
+  fit[0] = s;
+  fit[1] = sr;
+  fit[2] = st;
+  fit[3] = sp;
+  fit[4] = sq;
+  fit[6] = cnt;
+  fit[7] = s + (sr + st + sp + sq) / cnt;
+}
+
 // One workgroup per type: the cut sums of calc_global_fit and the Srelfac
 // bookkeeping (main.c:294-359).  fit[t][.] = S, Sdisp_r, Sdisp_t, Sdisp_p,
 // Sdisp_q, Srelfac, Srelfac_count, Sall, firstrun flag.

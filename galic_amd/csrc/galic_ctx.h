@@ -102,6 +102,7 @@ struct Ctx {
   double *t_dFk = nullptr;      // merit change of every trial of the last FP32 call [max_ktrials]
   size_t max_ktrials = 0;
   int last_ntrial = 0, last_k = 0;  // trials and K of the last FP32 call
+  double *fit_part = nullptr;   // partial cut sums of the global fit [type-1][kFitBlocks][5]
   double *fit = nullptr;        // [type][16] S, Sdisp_r,t,p,q, Srelfac, Srelfac_count, Sall, raw sums..., firstrun
   Particles P;
 
@@ -167,6 +168,9 @@ struct Ctx {
   unsigned int *tile_list = nullptr;  // FP32 merit, tile form: the records' tile lists, per workgroup
   size_t tile_list_len = 0;
   unsigned int *tile_aux = nullptr;
+  double *tsrc = nullptr;        // sources of the table fill (galic_set_tree_sources): [4][tsrc_n] x, y, z, mass
+  int tsrc_n = 0;
+  double tsrc_lo[3] = {0, 0, 0}, tsrc_hi[3] = {0, 0, 0};  // their extent
   double *lookup_buf = nullptr;  // galic_field_at: staging of positions, accelerations, potentials
   size_t lookup_len = 0;         // points it holds   // [0] largest records of a particle's logs, [1] list overflow flag
   int merit_form = 0;                 // 0 automatic, 1 tiles at every level, 2 LDS grids / HBM scratch (no tiles)

@@ -81,6 +81,11 @@ __global__ void k_eval_apply64(DevModel M, EvalArgs A);
 __global__ void k_commit(double *loc, double *delta, size_t n);
 __global__ void k_fit_stacks(FitArgs A);
 __global__ void k_fit_sums(FitArgs A);
+__global__ void k_fit_tiles(FitArgs A);
+__global__ void k_fit_top(FitArgs A);
+__global__ void k_fit_partial(FitArgs A, double *part, int nblk);
+__global__ void k_fit_final(FitArgs A, const double *part, int nblk);
+constexpr int kFitBlocks = 64;  // workgroups per type of the fit's cut sums
 struct OrbitIn {
   int *pidx;
   double *ox, *oy, *oz, *ovx, *ovy, *ovz, *omass, *otint;

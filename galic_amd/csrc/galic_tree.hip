@@ -32,9 +32,12 @@
 #include <vector>
 
 #include "../../include/galic_hip.h"
+#include "galic_ctx.h"
 #include "galic_kernels.h"
 
 namespace galic {
+galic_ctx *bound_ctx();
+Ctx &ctx_state(galic_ctx *ctx);
 namespace tree {
 
 constexpr int kMaxDepth = 21;
@@ -529,6 +532,24 @@ __global__ void k_project(int nb, const double *ax, const double *ay, const doub
   pot[i] = p / GALIC_FG_SECTIONS;
 }
 
+// Table-fill targets (grid.c:680-704): node (s, k, j) at (R_j cos phi_s, R_j sin phi_s, z_k)
+// with R_j = z_j = rj[j], the same products as the host-built targets.
+__global__ void k_fill_targets(int nb, const double *rj, const double *cs, const double *sn, double *tx, double *ty,
+                               double *tz) {
+  const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(n >= (size_t)GALIC_FG_SECTIONS * nb * nb) return;
+  const int s = (int)(n / ((size_t)nb * nb)), k = (int)((n / nb) % nb), j = (int)(n % nb);
+  tx[n] = rj[j] * cs[s];
+  ty[n] = rj[j] * sn[s];
+  tz[n] = rj[k];
+}
+
+// The FP32 orbit kernels' force table (DPotDR, DPotDz) from the FP64 tables.
+__global__ void k_acc_f(int n, const double *dr, const double *dz, float2 *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) out[i] = make_float2((float)dr[i], (float)dz[i]);
+}
+
 // Direct summation in FP64 with the walk's interaction (the check of the
 // tree's force error): one target per lane, sources staged through LDS.
 __global__ void __launch_bounds__(256) k_direct64(int nsrc, const double *sx, const double *sy, const double *sz,
@@ -614,22 +635,55 @@ struct DevBufs {
   }
 };
 
-// Build the tree of nsrc sources (device SoA inputs) and walk ntarget targets.
-int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const double *dpx, const double *dpy,
-                    const double *dpz, const double *dpm, int ntarget, const double *dtx, const double *dty,
-                    const double *dtz, double corner[3], double dlen, double *dax, double *day, double *daz,
-                    double *dpot, int precision, long long *ninteractions, float *walk_ms) {
+// Moments bottom-up (monopole, and the quadrupole where T.quad is allocated), node side
+// lengths DomainLen / 2^depth (NODE.len, allvars.h:1095, held in float), then the
+// nextnode/sibling threading top-down (forcetree.c:591-722).
+int tree_moments_dev(hipStream_t st, Tree &T, double dlen) {
+  const int nnodes = T.level_lo.back(), nsrc = T.nsrc;
+  const int ndepth = (int)T.level_lo.size() - 1;
+  for(int d = ndepth - 1; d >= 0; d--) {
+    int lo = T.level_lo[d], hi = T.level_lo[d + 1];
+    if(hi <= lo) continue;
+    hipLaunchKernelGGL(k_moments, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, nnodes,
+                       (const int *)T.child, (const int *)T.bucket, (const int *)T.first, (const int *)T.count,
+                       (const double *)T.x, (const double *)T.y, (const double *)T.z, (const double *)T.m, T.mass,
+                       T.sx, T.sy, T.sz, T.quad);
+    hipLaunchKernelGGL(k_len, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, ldexp(dlen, -d), T.len);
+    if(hipGetLastError() != hipSuccess) return terr(GALIC_ERR_HIP, "tree moments launch failed");
+  }
+  int minus1 = -1;
+  if(hipMemcpyAsync(T.sibling, &minus1, sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+    return terr(GALIC_ERR_HIP, "tree threading: copy failed");
+  for(int d = 0; d < ndepth; d++) {
+    int lo = T.level_lo[d], hi = T.level_lo[d + 1];
+    if(hi <= lo) continue;
+    hipLaunchKernelGGL(k_thread, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, (const int *)T.child,
+                       (const int *)T.bucket, T.sibling, T.nextnode, T.nextp);
+    if(hipGetLastError() != hipSuccess) return terr(GALIC_ERR_HIP, "tree threading launch failed");
+  }
+  if(T.quad && T.quadf) {
+    hipLaunchKernelGGL(k_quadf, dim3(nb256(nnodes)), dim3(256), 0, st, nnodes, (const double *)T.quad, T.quadf);
+    if(hipGetLastError() != hipSuccess) return terr(GALIC_ERR_HIP, "tree quadrupole copy failed");
+  }
+  return GALIC_OK;
+}
+
+// Builds the tree of nsrc sources (device SoA inputs, domain cube at corner with side dlen)
+// into T, its buffers owned by B.  sorted (optional) receives the device array giving, for each
+// sorted source, its index among the inputs.
+int tree_build_dev(const galic_params *p, hipStream_t st, int nsrc, const double *dpx, const double *dpy,
+                   const double *dpz, const double *dpm, const double corner[3], double dlen, int precision,
+                   DevBufs &B, Tree &T, int **sorted, float *build_ms) {
   int rc = GALIC_OK;
-  DevBufs B;
-  Tree T;
-  unsigned long long *keys_in = nullptr, *nint = nullptr;
+  unsigned long long *keys_in = nullptr;
   int *idx_in = nullptr, *idx = nullptr;
   void *tmp = nullptr;
   size_t tmp_bytes = 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr, eb = nullptr;
-  float build_ms = 0;
+  hipEvent_t eb = nullptr, ee = nullptr;
+  (void)p;
   T.nsrc = nsrc;
   TCHK(hipEventCreate(&eb));
+  TCHK(hipEventCreate(&ee));
   TCHK(hipEventRecord(eb, st));
   TCHK(B.alloc(&keys_in, nsrc));
   TCHK(B.alloc(&T.key, nsrc));
@@ -640,8 +694,6 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
   TCHK(B.alloc(&T.z, nsrc));
   TCHK(B.alloc(&T.m, nsrc));
   TCHK(B.alloc(&T.nextp, nsrc));
-  TCHK(B.alloc(&nint, 1));
-  TCHK(hipMemsetAsync(nint, 0, sizeof(unsigned long long), st));
   if(nsrc > 0) {
     hipLaunchKernelGGL(k_keys, dim3(nb256(nsrc)), dim3(256), 0, st, nsrc, dpx, dpy, dpz, corner[0], corner[1],
                        corner[2], 1.0 / dlen, keys_in, idx_in);
@@ -653,6 +705,7 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
                        T.y, T.z, T.m);
     TCHK(hipGetLastError());
   }
+  if(sorted) *sorted = idx;
   {
     // nodes: the unique insertion tree has fewer than 2 nsrc nodes unless
     // particles pair up at great depth; grow and rebuild if needed
@@ -699,103 +752,125 @@ int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const doubl
   }
   {
     const int nnodes = T.level_lo.back();
-    const int ndepth = (int)T.level_lo.size() - 1;
     TCHK(B.alloc(&T.mass, nnodes));
     TCHK(B.alloc(&T.sx, nnodes));
     TCHK(B.alloc(&T.sy, nnodes));
     TCHK(B.alloc(&T.sz, nnodes));
     TCHK(B.alloc(&T.len, nnodes));
-    if(precision & GALIC_TREE_QUADRUPOLE) TCHK(B.alloc(&T.quad, (size_t)6 * nnodes));
+    if(precision & GALIC_TREE_QUADRUPOLE) {
+      TCHK(B.alloc(&T.quad, (size_t)6 * nnodes));
+      if(precision & GALIC_TREE_FP32) TCHK(B.alloc(&T.quadf, (size_t)2 * nnodes));
+    }
     TCHK(B.alloc(&T.sibling, nnodes));
     TCHK(B.alloc(&T.nextnode, nnodes));
-    for(int d = ndepth - 1; d >= 0; d--) {
-      int lo = T.level_lo[d], hi = T.level_lo[d + 1];
-      if(hi <= lo) continue;
-      hipLaunchKernelGGL(k_moments, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, nnodes,
-                         (const int *)T.child, (const int *)T.bucket, (const int *)T.first, (const int *)T.count,
-                         (const double *)T.x, (const double *)T.y, (const double *)T.z, (const double *)T.m, T.mass,
-                         T.sx, T.sy, T.sz, T.quad);
-      // side length DomainLen / 2^d, held in float as NODE.len (allvars.h:1095)
-      hipLaunchKernelGGL(k_len, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, ldexp(dlen, -d), T.len);
-      TCHK(hipGetLastError());
-    }
-    int minus1 = -1;
-    TCHK(hipMemcpyAsync(T.sibling, &minus1, sizeof(int), hipMemcpyHostToDevice, st));
-    for(int d = 0; d < ndepth; d++) {
-      int lo = T.level_lo[d], hi = T.level_lo[d + 1];
-      if(hi <= lo) continue;
-      hipLaunchKernelGGL(k_thread, dim3(nb256(hi - lo)), dim3(256), 0, st, lo, hi, nsrc, (const int *)T.child,
-                         (const int *)T.bucket, T.sibling, T.nextnode, T.nextp);
-      TCHK(hipGetLastError());
-    }
-    WalkArgs W;
-    W.nsrc = nsrc;
-    W.ntarget = ntarget;
-    W.tx = dtx;
-    W.ty = dty;
-    W.tz = dtz;
-    W.x = T.x;
-    W.y = T.y;
-    W.z = T.z;
-    W.m = T.m;
-    W.nextp = T.nextp;
-    W.first = T.first;
-    W.count = T.count;
-    W.bucket = T.bucket;
-    W.sibling = T.sibling;
-    W.nextnode = T.nextnode;
-    W.mass = T.mass;
-    W.sx = T.sx;
-    W.sy = T.sy;
-    W.sz = T.sz;
-    W.len = T.len;
-    W.quad = T.quad;
-    W.quadf = nullptr;
-    if(T.quad && (precision & GALIC_TREE_FP32)) {
-      TCHK(B.alloc(&T.quadf, (size_t)2 * nnodes));
-      hipLaunchKernelGGL(k_quadf, dim3(nb256(nnodes)), dim3(256), 0, st, nnodes, (const double *)T.quad, T.quadf);
-      TCHK(hipGetLastError());
-      W.quadf = T.quadf;
-    }
-    W.nnodes = nnodes;
-    W.theta = p->ErrTolTheta;
-    W.h = p->ForceSoftening;
-    // ForceSoftening is 2.8 times the Softening parameter (init.c:78-84), the Plummer length
-    W.plummer = (precision & GALIC_TREE_PLUMMER) ? 1 : 0;
-    W.eps = p->ForceSoftening / 2.8;
-    W.G = p->G;
-    W.ax = dax;
-    W.ay = day;
-    W.az = daz;
-    W.pot = dpot;
-    W.nint = nint;
-    TCHK(hipEventCreate(&e0));
-    TCHK(hipEventCreate(&e1));
-    TCHK(hipEventRecord(e0, st));
-    TCHK(hipEventSynchronize(e0));
-    TCHK(hipEventElapsedTime(&build_ms, eb, e0));
-    if(ntarget > 0) {
-      if(!(precision & GALIC_TREE_FP32))
-        hipLaunchKernelGGL(k_walk64, dim3(nb256(ntarget)), dim3(256), 0, st, W);
-      else
-        hipLaunchKernelGGL(k_walk32, dim3(nb256(ntarget)), dim3(256), 0, st, W);
-      TCHK(hipGetLastError());
-    }
-    TCHK(hipEventRecord(e1, st));
-    TCHK(hipEventSynchronize(e1));
-    if(walk_ms) TCHK(hipEventElapsedTime(walk_ms, e0, e1));
+    if((rc = tree_moments_dev(st, T, dlen)) != GALIC_OK) goto done;
+  }
+  TCHK(hipEventRecord(ee, st));
+  TCHK(hipEventSynchronize(ee));
+  if(build_ms) TCHK(hipEventElapsedTime(build_ms, eb, ee));
+done:
+  if(eb) (void)hipEventDestroy(eb);
+  if(ee) (void)hipEventDestroy(ee);
+  return rc;
+}
+
+// Walk of ntarget targets (device SoA) through T; the results are scaled by G (gravity_tree
+// multiplies the walk's sums by G afterwards, gravtree.c:455-463).
+int tree_walk_dev(const galic_params *p, hipStream_t st, const Tree &T, int ntarget, const double *dtx,
+                  const double *dty, const double *dtz, double *dax, double *day, double *daz, double *dpot,
+                  int precision, double G, long long *ninteractions, float *walk_ms) {
+  int rc = GALIC_OK;
+  DevBufs B;
+  unsigned long long *nint = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  WalkArgs W;
+  TCHK(B.alloc(&nint, 1));
+  TCHK(hipMemsetAsync(nint, 0, sizeof(unsigned long long), st));
+  W.nsrc = T.nsrc;
+  W.ntarget = ntarget;
+  W.tx = dtx;
+  W.ty = dty;
+  W.tz = dtz;
+  W.x = T.x;
+  W.y = T.y;
+  W.z = T.z;
+  W.m = T.m;
+  W.nextp = T.nextp;
+  W.first = T.first;
+  W.count = T.count;
+  W.bucket = T.bucket;
+  W.sibling = T.sibling;
+  W.nextnode = T.nextnode;
+  W.mass = T.mass;
+  W.sx = T.sx;
+  W.sy = T.sy;
+  W.sz = T.sz;
+  W.len = T.len;
+  W.quad = (precision & GALIC_TREE_QUADRUPOLE) ? T.quad : nullptr;
+  W.quadf = (W.quad && (precision & GALIC_TREE_FP32)) ? T.quadf : nullptr;
+  if(W.quad && (precision & GALIC_TREE_FP32) && !W.quadf) {
+    rc = terr(GALIC_ERR_STATE, "the tree was built without the FP32 quadrupole copy");
+    goto done;
+  }
+  if((precision & GALIC_TREE_QUADRUPOLE) && !T.quad) {
+    rc = terr(GALIC_ERR_STATE, "the tree was built without quadrupole moments");
+    goto done;
+  }
+  W.nnodes = T.level_lo.back();
+  W.theta = p->ErrTolTheta;
+  W.h = p->ForceSoftening;
+  // ForceSoftening is 2.8 times the Softening parameter (init.c:78-84), the Plummer length
+  W.plummer = (precision & GALIC_TREE_PLUMMER) ? 1 : 0;
+  W.eps = p->ForceSoftening / 2.8;
+  W.G = G;
+  W.ax = dax;
+  W.ay = day;
+  W.az = daz;
+  W.pot = dpot;
+  W.nint = nint;
+  TCHK(hipEventCreate(&e0));
+  TCHK(hipEventCreate(&e1));
+  TCHK(hipEventRecord(e0, st));
+  if(ntarget > 0) {
+    if(!(precision & GALIC_TREE_FP32))
+      hipLaunchKernelGGL(k_walk64, dim3(nb256(ntarget)), dim3(256), 0, st, W);
+    else
+      hipLaunchKernelGGL(k_walk32, dim3(nb256(ntarget)), dim3(256), 0, st, W);
+    TCHK(hipGetLastError());
+  }
+  TCHK(hipEventRecord(e1, st));
+  TCHK(hipEventSynchronize(e1));
+  if(walk_ms) TCHK(hipEventElapsedTime(walk_ms, e0, e1));
+  {
     unsigned long long ni = 0;
     TCHK(hipMemcpy(&ni, nint, sizeof(ni), hipMemcpyDeviceToHost));
     if(ninteractions) *ninteractions = (long long)ni;
-    g_last_build_ms = build_ms;
-    g_last_walk_ms = walk_ms ? *walk_ms : 0;
-    g_last_interactions = (double)ni;
   }
 done:
   if(e0) (void)hipEventDestroy(e0);
   if(e1) (void)hipEventDestroy(e1);
-  if(eb) (void)hipEventDestroy(eb);
   return rc;
+}
+
+// Build the tree of nsrc sources (device SoA inputs) and walk ntarget targets.
+int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const double *dpx, const double *dpy,
+                    const double *dpz, const double *dpm, int ntarget, const double *dtx, const double *dty,
+                    const double *dtz, double corner[3], double dlen, double *dax, double *day, double *daz,
+                    double *dpot, int precision, long long *ninteractions, float *walk_ms) {
+  DevBufs B;
+  Tree T;
+  float build_ms = 0, wms = 0;
+  long long ni = 0;
+  int rc = tree_build_dev(p, st, nsrc, dpx, dpy, dpz, dpm, corner, dlen, precision, B, T, nullptr, &build_ms);
+  if(rc != GALIC_OK) return rc;
+  rc = tree_walk_dev(p, st, T, ntarget, dtx, dty, dtz, dax, day, daz, dpot, precision, p->G, &ni, &wms);
+  if(rc != GALIC_OK) return rc;
+  if(ninteractions) *ninteractions = ni;
+  if(walk_ms) *walk_ms = wms;
+  g_last_build_ms = build_ms;
+  g_last_walk_ms = wms;
+  g_last_interactions = (double)ni;
+  return GALIC_OK;
 }
 
 // domain_box.c:20-65 over sources and targets
@@ -827,9 +902,310 @@ int upload_soa(hipStream_t st, int n, const double *aos, double *dx, double *dy,
   return GALIC_OK;
 }
 
+// Node radii R_j = Rmin (Fac^j - 1) of the force grid (grid.c:680-704) and the section
+// angles' cosines and sines, as galic_tree_fill computes them.
+void fill_nodes(const galic_params *p, std::vector<double> &rj, std::vector<double> &cs, std::vector<double> &sn) {
+  const int nb = p->FG_Nbin;
+  rj.resize(nb);
+  cs.resize(GALIC_FG_SECTIONS);
+  sn.resize(GALIC_FG_SECTIONS);
+  for(int j = 0; j < nb; j++) rj[j] = p->FG_Rmin * (pow(p->FG_Fac, j) - 1.0);
+  for(int s = 0; s < GALIC_FG_SECTIONS; s++) {
+    double phi = 2 * M_PI / GALIC_FG_SECTIONS * s;
+    cs[s] = cos(phi);
+    sn[s] = sin(phi);
+  }
+}
+
+// ------------------------------------------------ forcetree drop-ins' state --
+
+// P[] of the forcetree drop-ins (allvars.h:844-868), caller-owned as the reference's P[]:
+// positions, masses, types, and the GravAccel and Potential the walks write.
+struct BoundParticles {
+  int n = 0;
+  double *pos = nullptr, *mass = nullptr, *acc = nullptr, *pot = nullptr;
+  int *type = nullptr;
+} g_P;
+
+// The tree of the last force_treebuild, kept on the device for force_treeevaluate and
+// force_update_node_recursive.
+struct BoundTree {
+  bool built = false;
+  int device = 0;
+  int npart = 0;                 // the build's npart: the root node's index (the reference's Tree_MaxPart)
+  hipStream_t st = nullptr;
+  DevBufs *B = nullptr;
+  Tree T;
+  int *sorted = nullptr;         // device: sorted source -> source index
+  std::vector<int> src_of;       // source index -> index in P[]
+  std::vector<int> targets;      // TargetList: the Type-5 particles (gravtree.c:115-119)
+  double corner[3] = {0, 0, 0}, len = 0;
+  galic_params p;
+  double *tbuf = nullptr;        // device: one target's position, acceleration and potential
+} g_T;
+
+void tree_release() {
+  if(g_T.B) {
+    (void)hipSetDevice(g_T.device);
+    delete g_T.B;
+  }
+  if(g_T.st) (void)hipStreamDestroy(g_T.st);
+  g_T = BoundTree();
+}
+
+void tree_dropin_fail(const char *what) {
+  // the reference terminates on errors in the tree code (allvars.h:129)
+  fprintf(stderr, "galic forcetree drop-in %s failed: %s\n", what, galic_last_error());
+  exit(1);
+}
+
+// Builds the bound tree over P[0..npart) without the Type-5 particles (forcetree.c:331);
+// the domain cube covers every one of them (domain_box.c:20-65).  Returns the node count.
+int tree_build_bound(int npart, int precision) {
+  galic_ctx *ctx = galic::bound_ctx();
+  if(!ctx) return terr(GALIC_ERR_STATE, "no context bound (galic_bind): the tree takes its parameters from it");
+  if(!g_P.pos || npart < 0 || npart > g_P.n)
+    return terr(GALIC_ERR_ARG, "force_treebuild: npart %d beyond the %d bound particles (galic_bind_tree_particles)",
+                npart, g_P.n);
+  tree_release();
+  const galic::Ctx &c = galic::ctx_state(ctx);
+  g_T.p = c.p;
+  g_T.device = c.device;
+  g_T.npart = npart;
+  int rc = GALIC_OK;
+  std::vector<double> sx, sy, sz, sm, tp;
+  std::vector<double> spos;
+  for(int i = 0; i < npart; i++) {
+    if(g_P.type[i] == 5) {
+      g_T.targets.push_back(i);
+      tp.insert(tp.end(), g_P.pos + 3 * (size_t)i, g_P.pos + 3 * (size_t)i + 3);
+    } else {
+      g_T.src_of.push_back(i);
+      spos.insert(spos.end(), g_P.pos + 3 * (size_t)i, g_P.pos + 3 * (size_t)i + 3);
+      sm.push_back(g_P.mass[i]);
+    }
+  }
+  const int nsrc = (int)g_T.src_of.size();
+  double *px, *py, *pz, *pm;
+  domain_extent(nsrc, spos.data(), (int)g_T.targets.size(), tp.data(), g_T.corner, &g_T.len);
+  g_T.B = new DevBufs();
+  TCHK(hipSetDevice(g_T.device));
+  TCHK(hipStreamCreate(&g_T.st));
+  TCHK(g_T.B->alloc(&px, nsrc));
+  TCHK(g_T.B->alloc(&py, nsrc));
+  TCHK(g_T.B->alloc(&pz, nsrc));
+  TCHK(g_T.B->alloc(&pm, nsrc));
+  TCHK(g_T.B->alloc(&g_T.tbuf, 8));
+  if((rc = upload_soa(g_T.st, nsrc, spos.data(), px, py, pz)) != GALIC_OK) goto done;
+  if(nsrc > 0) TCHK(hipMemcpy(pm, sm.data(), sizeof(double) * nsrc, hipMemcpyHostToDevice));
+  rc = tree_build_dev(&g_T.p, g_T.st, nsrc, px, py, pz, pm, g_T.corner, g_T.len, precision, *g_T.B, g_T.T,
+                      &g_T.sorted, nullptr);
+  if(rc == GALIC_OK) g_T.built = true;
+done:
+  if(rc != GALIC_OK) tree_release();
+  return rc == GALIC_OK ? g_T.T.level_lo.back() : rc;
+}
+
+// Walks n targets (positions pos, 3 each) of the bound tree; G scales the results.
+int tree_walk_bound(int n, const double *pos, double *acc, double *pot, double G, long long *nint) {
+  int rc = GALIC_OK;
+  DevBufs B;
+  double *tx, *ty, *tz, *ax, *ay, *az, *pt;
+  std::vector<double> h;
+  TCHK(hipSetDevice(g_T.device));
+  TCHK(B.alloc(&tx, n)); TCHK(B.alloc(&ty, n)); TCHK(B.alloc(&tz, n));
+  TCHK(B.alloc(&ax, n)); TCHK(B.alloc(&ay, n)); TCHK(B.alloc(&az, n)); TCHK(B.alloc(&pt, n));
+  if((rc = upload_soa(g_T.st, n, pos, tx, ty, tz)) != GALIC_OK) goto done;
+  if((rc = tree_walk_dev(&g_T.p, g_T.st, g_T.T, n, tx, ty, tz, ax, ay, az, pt, 0, G, nint, nullptr)) != GALIC_OK)
+    goto done;
+  h.resize(std::max(n, 1));
+  {
+    double *src[3] = {ax, ay, az};
+    for(int k = 0; k < 3; k++) {
+      if(n > 0) TCHK(hipMemcpy(h.data(), src[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+      for(int i = 0; i < n; i++) acc[3 * (size_t)i + k] = h[i];
+    }
+    if(n > 0) TCHK(hipMemcpy(pot, pt, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+done:
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
+
+int galic_bind_tree_particles(int n, double *pos, double *mass, int *type, double *grav_accel, double *potential) {
+  if(n < 0 || (n > 0 && (!pos || !mass || !type || !grav_accel || !potential)))
+    return terr(GALIC_ERR_ARG, "galic_bind_tree_particles: bad arguments");
+  tree_release();
+  g_P.n = n;
+  g_P.pos = pos;
+  g_P.mass = mass;
+  g_P.type = type;
+  g_P.acc = grav_accel;
+  g_P.pot = potential;
+  return GALIC_OK;
+}
+
+int force_treebuild(int npart, int optimized_domain_mapping) {
+  (void)optimized_domain_mapping;  // one GPU: no domain mapping to optimise
+  const int r = tree_build_bound(npart, 0);
+  if(r < 0) tree_dropin_fail("force_treebuild");
+  return r;
+}
+
+void force_update_node_recursive(int no, int sib, int father, int *last) {
+  if(!g_T.built) tree_dropin_fail("force_update_node_recursive (no tree built)");
+  if(no != g_T.npart || sib != -1 || father != -1) {
+    terr(GALIC_ERR_ARG, "node %d: the device tree numbers its own nodes; only the root (%d, sibling -1, father -1, "
+         "as forcetree.c:86 calls it) is addressed", no, g_T.npart);
+    tree_dropin_fail("force_update_node_recursive");
+  }
+  (void)hipSetDevice(g_T.device);
+  if(tree_moments_dev(g_T.st, g_T.T, g_T.len) != GALIC_OK || hipStreamSynchronize(g_T.st) != hipSuccess)
+    tree_dropin_fail("force_update_node_recursive");
+  if(last) {
+    // the walk order ends at the last source in key order (children are threaded in order)
+    int s = -1;
+    const int nsrc = g_T.T.nsrc;
+    if(nsrc > 0 && hipMemcpy(&s, g_T.sorted + (nsrc - 1), sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      tree_dropin_fail("force_update_node_recursive");
+    *last = (s >= 0) ? g_T.src_of[s] : -1;
+  }
+}
+
+int force_treeevaluate(int target, int mode, int thread_id) {
+  (void)thread_id;
+  if(!g_T.built) tree_dropin_fail("force_treeevaluate (no tree built)");
+  if(mode != 0) {
+    terr(GALIC_ERR_ARG, "mode %d: imported targets of other tasks have no counterpart on one GPU", mode);
+    tree_dropin_fail("force_treeevaluate");
+  }
+  if(target < 0 || target >= (int)g_T.targets.size()) {
+    terr(GALIC_ERR_ARG, "target %d beyond the %zu Type-5 particles", target, g_T.targets.size());
+    tree_dropin_fail("force_treeevaluate");
+  }
+  // TargetList[target] (forcetree_walk.c:28-41); the sums are left without G, as the walk
+  // leaves them (gravity_tree multiplies afterwards)
+  const int i = g_T.targets[target];
+  long long nint = 0;
+  if(tree_walk_bound(1, g_P.pos + 3 * (size_t)i, g_P.acc + 3 * (size_t)i, g_P.pot + i, 1.0, &nint) != GALIC_OK)
+    tree_dropin_fail("force_treeevaluate");
+  return (int)nint;
+}
+
+void gravity(void) {
+  // gravtree.c:35-50 and 88-467: build, walk every Type-5 particle, multiply by G, free
+  if(tree_build_bound(g_P.n, 0) < 0) tree_dropin_fail("gravity");
+  const int nt = (int)g_T.targets.size();
+  std::vector<double> tp(3 * (size_t)nt), acc(3 * (size_t)nt), pot(nt);
+  for(int k = 0; k < nt; k++)
+    for(int j = 0; j < 3; j++) tp[3 * (size_t)k + j] = g_P.pos[3 * (size_t)g_T.targets[k] + j];
+  if(tree_walk_bound(nt, tp.data(), acc.data(), pot.data(), g_T.p.G, nullptr) != GALIC_OK) tree_dropin_fail("gravity");
+  for(int k = 0; k < nt; k++) {
+    const int i = g_T.targets[k];
+    for(int j = 0; j < 3; j++) g_P.acc[3 * (size_t)i + j] = acc[3 * (size_t)k + j];
+    g_P.pot[i] = pot[k];
+  }
+  tree_release();
+}
+
+int galic_set_tree_sources(galic_ctx *ctx, int nsrc, const double *spos, const double *smass) {
+  if(!ctx || nsrc < 1 || !spos || !smass) return terr(GALIC_ERR_ARG, "galic_set_tree_sources: bad arguments");
+  galic::Ctx &c = galic::ctx_state(ctx);
+  int rc = GALIC_OK;
+  TCHK(hipSetDevice(c.device));
+  if(c.tsrc) (void)hipFree(c.tsrc);
+  c.tsrc = nullptr;
+  c.tsrc_n = 0;
+  TCHK(hipMalloc(&c.tsrc, sizeof(double) * 4 * (size_t)nsrc));
+  c.tsrc_n = nsrc;
+  for(int j = 0; j < 3; j++) {
+    c.tsrc_lo[j] = 1e30;
+    c.tsrc_hi[j] = -1e30;
+  }
+  for(int i = 0; i < nsrc; i++)
+    for(int j = 0; j < 3; j++) {
+      const double x = spos[3 * (size_t)i + j];
+      if(c.tsrc_lo[j] > x) c.tsrc_lo[j] = x;
+      if(c.tsrc_hi[j] < x) c.tsrc_hi[j] = x;
+    }
+  if((rc = upload_soa(c.stream, nsrc, spos, c.tsrc, c.tsrc + nsrc, c.tsrc + 2 * (size_t)nsrc)) != GALIC_OK) goto done;
+  TCHK(hipMemcpy(c.tsrc + 3 * (size_t)nsrc, smass, sizeof(double) * nsrc, hipMemcpyHostToDevice));
+done:
+  return rc;
+}
+
+int galic_retabulate(galic_ctx *ctx, int precision, long long *ninteractions) {
+  if(!ctx) return terr(GALIC_ERR_ARG, "null context");
+  if(precision & ~7) return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER");
+  galic::Ctx &c = galic::ctx_state(ctx);
+  if(!c.tsrc || c.tsrc_n < 1) return terr(GALIC_ERR_STATE, "no tree sources (galic_set_tree_sources)");
+  const galic_params *p = &c.p;
+  const int nb = p->FG_Nbin, nsrc = c.tsrc_n;
+  const size_t nn = (size_t)nb * nb, ns = GALIC_FG_SECTIONS * nn;
+  std::vector<double> rj, cs, sn;
+  fill_nodes(p, rj, cs, sn);
+  // the domain over sources and targets, as domain_extent() forms it for galic_tree_fill
+  double xmin[3], xmax[3], corner[3], len = 0;
+  for(int j = 0; j < 3; j++) {
+    xmin[j] = c.tsrc_lo[j];
+    xmax[j] = c.tsrc_hi[j];
+  }
+  for(int s = 0; s < GALIC_FG_SECTIONS; s++)
+    for(int j = 0; j < nb; j++) {
+      const double x = rj[j] * cs[s], y = rj[j] * sn[s];
+      xmin[0] = std::min(xmin[0], x);
+      xmax[0] = std::max(xmax[0], x);
+      xmin[1] = std::min(xmin[1], y);
+      xmax[1] = std::max(xmax[1], y);
+    }
+  for(int k = 0; k < nb; k++) {
+    xmin[2] = std::min(xmin[2], rj[k]);
+    xmax[2] = std::max(xmax[2], rj[k]);
+  }
+  for(int j = 0; j < 3; j++)
+    if(xmax[j] - xmin[j] > len) len = xmax[j] - xmin[j];
+  len *= 1.00001;
+  for(int j = 0; j < 3; j++) corner[j] = 0.5 * (xmin[j] + xmax[j]) - 0.5 * len;
+  int rc = GALIC_OK;
+  DevBufs B;
+  Tree T;
+  double *drj, *dcs, *dsn, *tx, *ty, *tz, *ax, *ay, *az, *pt;
+  long long ni = 0;
+  float bms = 0, wms = 0;
+  TCHK(hipSetDevice(c.device));
+  TCHK(B.alloc(&drj, nb)); TCHK(B.alloc(&dcs, GALIC_FG_SECTIONS)); TCHK(B.alloc(&dsn, GALIC_FG_SECTIONS));
+  TCHK(B.alloc(&tx, ns)); TCHK(B.alloc(&ty, ns)); TCHK(B.alloc(&tz, ns));
+  TCHK(B.alloc(&ax, ns)); TCHK(B.alloc(&ay, ns)); TCHK(B.alloc(&az, ns)); TCHK(B.alloc(&pt, ns));
+  TCHK(hipMemcpyAsync(drj, rj.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c.stream));
+  TCHK(hipMemcpyAsync(dcs, cs.data(), sizeof(double) * GALIC_FG_SECTIONS, hipMemcpyHostToDevice, c.stream));
+  TCHK(hipMemcpyAsync(dsn, sn.data(), sizeof(double) * GALIC_FG_SECTIONS, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_fill_targets, dim3(nb256(ns)), dim3(256), 0, c.stream, nb, (const double *)drj,
+                     (const double *)dcs, (const double *)dsn, tx, ty, tz);
+  TCHK(hipGetLastError());
+  if((rc = tree_build_dev(p, c.stream, nsrc, c.tsrc, c.tsrc + nsrc, c.tsrc + 2 * (size_t)nsrc,
+                          c.tsrc + 3 * (size_t)nsrc, corner, len, precision, B, T, nullptr, &bms)) != GALIC_OK)
+    goto done;
+  if((rc = tree_walk_dev(p, c.stream, T, (int)ns, tx, ty, tz, ax, ay, az, pt, precision, p->G, &ni, &wms)) !=
+     GALIC_OK)
+    goto done;
+  // installed as the context's tables: FG_DPotDR, FG_DPotDz, FG_Pot and the FP32 copy
+  hipLaunchKernelGGL(k_project, dim3(nb256(nn)), dim3(256), 0, c.stream, nb, (const double *)ax, (const double *)ay,
+                     (const double *)az, (const double *)pt, (const double *)dcs, (const double *)dsn, c.FG_DPotDR,
+                     c.FG_DPotDz, c.FG_Pot);
+  hipLaunchKernelGGL(k_acc_f, dim3(nb256(nn)), dim3(256), 0, c.stream, (int)nn, (const double *)c.FG_DPotDR,
+                     (const double *)c.FG_DPotDz, c.FG_Acc_f);
+  TCHK(hipGetLastError());
+  TCHK(hipStreamSynchronize(c.stream));
+  if(ninteractions) *ninteractions = ni;
+  g_last_build_ms = bms;
+  g_last_walk_ms = wms;
+  g_last_interactions = (double)ni;
+done:
+  return rc;
+}
 
 int galic_tree_forces(const galic_params *p, int device, int nsrc, const double *spos, const double *smass,
                       int ntarget, const double *tpos, double *acc, double *pot, int precision,

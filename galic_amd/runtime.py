@@ -34,6 +34,11 @@ class GalicGPU:
         self.set_particles(w.pos, w.vel)
         self.set_dispersions(w.disp)
 
+    def set_targets(self, t, stacks):
+        """DGs_MassTarget, EGs_MassTarget, EGs_EgyTarget_{r,t,p,q} of type t (6 stacks)."""
+        st = [np.ascontiguousarray(s, dtype=np.float64) for s in stacks]
+        check(self.lib.galic_set_targets(self.h, t, *[dptr(s) for s in st]))
+
     def set_dispersions(self, disp):
         disp = np.ascontiguousarray(disp, dtype=np.float64)
         check(self.lib.galic_set_dispersions(self.h, len(disp), dptr(disp)))
@@ -206,6 +211,28 @@ class GalicGPU:
 
     def bind(self):
         check(self.lib.galic_bind(self.h))
+
+    # -------------------------------------------------------- table fill --
+    def set_tree_sources(self, spos, smass):
+        """Sources of the context's table fill (forcedensitygrid_calculate's samples)."""
+        spos = np.ascontiguousarray(spos, dtype=np.float64).reshape(-1, 3)
+        smass = np.ascontiguousarray(smass, dtype=np.float64)
+        check(self.lib.galic_set_tree_sources(self.h, len(spos), dptr(spos), dptr(smass)))
+
+    def sample_tree_sources(self, seed=42):
+        check(self.lib.galic_sample_tree_sources(self.h, int(seed)))
+
+    def retabulate(self, precision=0):
+        """Re-fill the context's force tables from its tree sources on the GPU; returns the interactions."""
+        n = np.zeros(1, dtype=np.int64)
+        check(self.lib.galic_retabulate(self.h, int(precision), lptr(n)))
+        return int(n[0])
+
+    def force_tables(self):
+        n2 = self.w.params.FG_Nbin ** 2
+        pot, dr, dz = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+        check(self.lib.galic_get_force_tables(self.h, dptr(pot), dptr(dr), dptr(dz)))
+        return pot, dr, dz
 
 
 def tree_forces(params, spos, smass, tpos, precision=0, device=0):

@@ -248,6 +248,22 @@ int galic_direct_forces_mode(const galic_params *p, int device, int nsrc, const 
  * process: out[0] tree build ms, out[1] walk ms, out[2] interactions. */
 int galic_tree_last_timings(double *out3);
 
+/* Sources of the table fill for the context, kept on its GPU: nsrc positions (3 each) and
+ * masses, the fresh samples forcedensitygrid_calculate draws (grid.c:659-678). */
+int galic_set_tree_sources(galic_ctx *ctx, int nsrc, const double *spos, const double *smass);
+/* The sources drawn on the device: SampleForceNhalo / SampleForceNdisk / SampleForceNbulge
+ * points of each component with mass (halo.c:17-40, disk.c:17-51, bulge.c:17-38), each of
+ * mass MType / count, from Philox draws of (point id, round) under (seed, 200 + type). */
+int galic_sample_tree_sources(galic_ctx *ctx, unsigned int seed);
+/* forcedensitygrid_calculate's table fill (grid.c:621-757) from the context's tree sources,
+ * installed as the context's force tables (FG_Pot, FG_DPotDR, FG_DPotDz and the FP32 copy)
+ * without a host round trip: the per-iteration re-tabulation.  The targets and the domain are
+ * those of galic_tree_fill, so the tables equal its output on the same sources.
+ * precision: GALIC_TREE_ modes; ninteractions optional. */
+int galic_retabulate(galic_ctx *ctx, int precision, long long *ninteractions);
+/* Reads the context's force tables (any pointer may be NULL). */
+int galic_get_force_tables(galic_ctx *ctx, double *pot, double *dpotdr, double *dpotdz);
+
 /* ---- measurement ---- */
 
 /* Device time of the last launches, measured with HIP events on the context's
@@ -287,6 +303,36 @@ void densitygrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz)
 void energygrid_get_cell(double *pos, int *iR, int *iz, double *fR, double *fz);
 /* orbit_response.c:16-26 (proto.h): with the bound context's DG_CellSize */
 double get_timestep(double *pos, double *vel, double *acc, int icell);
+/* grid.c:329-487 (proto.h:104): the density-grid geometry (DG_CellSize, DGs_Distance) of the
+ * bound context recomputed from its parameters (galic_grid_create fills Rmax, Rmin, Fac) */
+void forcedensitygrid_create(void);
+/* grid.c:621-757 (proto.h:105): galic_retabulate of the bound context with the reference's
+ * walk (FP64, monopole, spline); sources from galic_set_tree_sources, else drawn by
+ * galic_sample_tree_sources with seed 42 */
+void forcedensitygrid_calculate(void);
+
+/* ---- forcetree module (forcetree.h:48-65, proto.h:195) ----
+ * P[] of these drop-ins is bound by the caller, as the reference's global particle array:
+ * n particles, positions (3 each), masses, types; Type-5 particles are the walk targets and
+ * are not inserted (forcetree.c:331, gravtree.c:115-119), and the walks write their
+ * GravAccel (3 each) and Potential.  The tree parameters (ErrTolTheta, ForceSoftening, G) and
+ * the GPU are the bound context's (galic_bind). */
+int galic_bind_tree_particles(int n, double *pos, double *mass, int *type, double *grav_accel, double *potential);
+/* forcetree.c:41: builds the tree of P[0..npart) on the GPU (the reference's construction:
+ * the unique insertion tree, FP64 monopole moments); returns the node count */
+int force_treebuild(int npart, int optimized_domain_mapping);
+/* forcetree.c:591: the moments and the walk threading of the tree recomputed from the root;
+ * *last receives the particle that ends the walk order.  The device tree numbers its nodes
+ * its own way: no must be the root, npart of the last force_treebuild (the reference's
+ * Tree_MaxPart), with sib = father = -1 as forcetree.c:86 calls it. */
+void force_update_node_recursive(int no, int sib, int father, int *last);
+/* forcetree_walk.c:12: the walk for TargetList[target] (the target-th Type-5 particle), its
+ * GravAccel and Potential written without the factor G; returns the interaction count.
+ * mode 0 only (mode 1 walks targets imported from other MPI tasks) */
+int force_treeevaluate(int target, int mode, int thread_id);
+/* gravtree.c:35-50, 88-467: tree of P[], every Type-5 particle walked, results times G */
+void gravity(void);
+
 /* grid.c:1086-1112 and 1115-1151 (proto.h): R200 and LowerDispLimit of the bound context */
 double calc_stack_difference(double *d1, double *d2, int l, int i, int j, int maxlevel, double *ref1, double *ref2,
                              double thresh, double *dist, int flag);

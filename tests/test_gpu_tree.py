@@ -185,3 +185,135 @@ def test_quadrupole_walks_on_the_composite():
         print("softening", soft, "median force errors", med)
         assert med[soft | TREE_QUADRUPOLE] < med[soft] / 3
         assert med[soft | TREE_QUADRUPOLE | TREE_FP32] < 2 * med[soft | TREE_QUADRUPOLE] + 1e-5
+
+
+# ----------------------------------------------- re-tabulation and drop-ins --
+
+def table_model(nbin=32, **kw):
+    cfg = model.config("composite-4e6", fg_nbin=nbin, max_steps=2000000, **kw)
+    return model.make_workload(cfg, n=1200, target_samples=50000)
+
+
+@pytest.mark.parametrize("mode", [0, TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER])
+def test_retabulate_equals_tree_fill(mode):
+    """galic_retabulate (the table fill on the device from the context's sources, installed in
+    place) builds the same targets and domain as galic_tree_fill: identical tables; and the
+    orbits integrated on them equal those after galic_set_force_tables of the same tables."""
+    from galic_amd.runtime import GalicGPU
+    print(f"[gpu] retabulate mode {mode}", flush=True)
+    w = table_model()
+    p, spos, smass = sources(30000)
+    pt, rt, zt, nt = tree_fill(w.params, spos, smass, precision=mode)
+    g = GalicGPU(w)
+    g.set_tree_sources(spos, smass)
+    n = g.retabulate(mode)
+    pg, rg, zg = g.force_tables()
+    assert n == nt
+    np.testing.assert_array_equal(pg, pt)
+    np.testing.assert_array_equal(rg, rt)
+    np.testing.assert_array_equal(zg, zt)
+    sel = np.arange(0, w.n, 40)
+    args = (w.pos[sel], w.vel[sel], w.mass[sel], w.tint[sel], w.ptype[sel])
+    a = g.orbit_responses(*args)
+    a32 = g.orbit_responses(*args, precision=32)
+    g.close()
+    g2 = GalicGPU(w)
+    import ctypes as ct
+    from galic_amd._abi import check, dptr
+    check(g2.lib.galic_set_force_tables(g2.h, dptr(pt), dptr(rt), dptr(zt)))
+    b = g2.orbit_responses(*args)
+    b32 = g2.orbit_responses(*args, precision=32)
+    g2.close()
+    for x, y in ((a, b), (a32, b32)):
+        np.testing.assert_array_equal(x["fields"], y["fields"])
+        np.testing.assert_array_equal(x["steps"], y["steps"])
+
+
+def test_forcedensitygrid_calculate_dropin(oracle_mod):
+    """forcedensitygrid_calculate() on the bound context (grid.c:621-757): SampleForceN points of
+    each sampled component drawn on the device (the samplers of halo.c, disk.c, bulge.c under
+    Philox keys (42, 200 + type)), the reference's walk, the tables installed; against the
+    oracle's table fill from the same points drawn by the oracle's samplers."""
+    from galic_amd.rng import uniforms
+    from galic_amd.runtime import GalicGPU
+    print("[gpu] forcedensitygrid_calculate", flush=True)
+    w = table_model()
+    p = w.params
+    p.SampleForceNhalo, p.SampleForceNdisk, p.SampleForceNbulge = 12000, 4000, 1000
+    g = GalicGPU(w)
+    g.bind()
+    fn = g.lib.forcedensitygrid_calculate
+    fn.restype = None
+    fn.argtypes = []
+    fn()
+    pg, rg, zg = g.force_tables()
+    g.close()
+    pts, ms = [], []
+    for t, n in ((1, 12000), (2, 4000), (3, 1000)):
+        pts.append(oracle_mod.sample_points(p, t, np.arange(n), uniforms, 42, 200 + t))
+        ms.append(np.full(n, p.MType[t] / n))
+    spos, smass = np.ascontiguousarray(np.concatenate(pts)), np.concatenate(ms)
+    P = ctypes.POINTER(ctypes.c_double)
+    nb2 = p.FG_Nbin ** 2
+    po, ro, zo = np.zeros(nb2), np.zeros(nb2), np.zeros(nb2)
+    oracle_mod.lib().or_table_fill(ctypes.byref(p), len(spos), oracle_mod.d(spos), oracle_mod.d(smass),
+                                   po.ctypes.data_as(P), ro.ctypes.data_as(P), zo.ctypes.data_as(P), 8)
+    np.testing.assert_allclose(pg, po, rtol=1e-10)
+    np.testing.assert_allclose(rg, ro, rtol=1e-9, atol=1e-11 * np.abs(ro).max())
+    np.testing.assert_allclose(zg, zo, rtol=1e-9, atol=1e-11 * np.abs(zo).max())
+
+
+def test_forcetree_dropins_match_oracle(oracle_mod):
+    """The forcetree module's entry points on a bound P[] (sources of types 1-3 and Type-5
+    targets): gravity() against the oracle's tree on the same sources and targets; then
+    force_treebuild, force_update_node_recursive from the root and force_treeevaluate target by
+    target give gravity()'s values without the factor G and the same interaction count."""
+    from galic_amd.runtime import GalicGPU
+    print("[gpu] forcetree drop-ins", flush=True)
+    w = table_model()
+    p, spos, smass = sources(20000)
+    rng = np.random.default_rng(8)
+    tpos = rng.normal(size=(600, 3)) * 8
+    n = len(spos) + len(tpos)
+    pos = np.ascontiguousarray(np.concatenate([spos[:7000], tpos[:300], spos[7000:], tpos[300:]]))
+    mass = np.concatenate([smass[:7000], np.zeros(300), smass[7000:], np.zeros(300)])
+    typ = np.concatenate([np.ones(7000), np.full(300, 5), np.full(len(spos) - 7000, 2), np.full(300, 5)]).astype(np.int32)
+    acc = np.zeros((n, 3))
+    pot = np.zeros(n)
+    g = GalicGPU(w)
+    g.bind()
+    lib = g.lib
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    assert lib.galic_bind_tree_particles(n, pos.ctypes.data_as(dp), mass.ctypes.data_as(dp), typ.ctypes.data_as(ip),
+                                         acc.ctypes.data_as(dp), pot.ctypes.data_as(dp)) == 0
+    lib.gravity.restype = None
+    lib.gravity.argtypes = []
+    lib.gravity()
+    is_t = typ == 5
+    acc_o, pot_o, n_o = oracle_forces(oracle_mod, w.params, np.ascontiguousarray(pos[~is_t]), mass[~is_t],
+                                      np.ascontiguousarray(pos[is_t]))
+    np.testing.assert_allclose(acc[is_t], acc_o, rtol=1e-10, atol=1e-12 * np.abs(acc_o).max())
+    np.testing.assert_allclose(pot[is_t], pot_o, rtol=1e-10)
+    assert (acc[~is_t] == 0).all()
+    acc_g, pot_g = acc[is_t].copy(), pot[is_t].copy()
+    acc[:] = 0
+    pot[:] = 0
+    lib.force_treebuild.restype = ctypes.c_int
+    lib.force_treebuild.argtypes = [ctypes.c_int, ctypes.c_int]
+    nodes = lib.force_treebuild(n, 1)
+    assert 0 < nodes < 2 * len(spos)
+    last = ctypes.c_int(-7)
+    lib.force_update_node_recursive.restype = None
+    lib.force_update_node_recursive.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
+    lib.force_update_node_recursive(n, -1, -1, ctypes.byref(last))
+    assert 0 <= last.value < n and typ[last.value] != 5
+    lib.force_treeevaluate.restype = ctypes.c_int
+    lib.force_treeevaluate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    total = 0
+    for i in range(int(is_t.sum())):
+        total += lib.force_treeevaluate(i, 0, 0)
+    assert total == n_o
+    G = w.params.G
+    np.testing.assert_array_equal(acc[is_t] * G, acc_g)
+    np.testing.assert_array_equal(pot[is_t] * G, pot_g)
+    g.close()
