@@ -58,7 +58,7 @@ EXPORTS = [
     "galic_set_disp_tables", "galic_sample_targets",
     "galic_write_fit_line", "galic_write_densfield", "galic_write_snapshot",
     "galic_set_particles", "galic_set_dispersions", "galic_get_particles", "galic_calc_all_response_fields", "galic_rep",
-    "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_fp32_orbit_responses", "galic_field_at",
+    "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_fp32_orbit_responses", "galic_field_at", "galic_field_at_static",
     "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_fp32_set_fresh_trials", "galic_fp32_trial_merits", "galic_fp32_set_merit_form", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_direct_forces_mode", "galic_tree_last_timings",
     "galic_timings", "galic_reset_timings", "galic_copy_probe", "galic_bind",
     "produce_orbit_response_field_mod", "produce_orbit_response_field",
@@ -123,6 +123,7 @@ def lib():
         "galic_orbit_responses": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _lp, _dp, _dp],
         "galic_fp32_orbit_responses": [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _ip, _lp, _dp, _dp],
         "galic_field_at": [_vp, ctypes.c_int, _dp, _dp, _dp],
+        "galic_field_at_static": [_vp, ctypes.c_int, _dp, _dp, _dp],
         "galic_fp32_calc_all_response_fields": [_vp],
         "galic_fp32_rep": [_vp, ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_ulonglong, _ip],
         "galic_fp32_iterate": [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,

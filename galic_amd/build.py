@@ -29,6 +29,7 @@ UNITS = {
     "galic_tree.hip": ["-ffp-contract=off"],
     "galic_targets.hip": ["-ffp-contract=off"],
     "galic_io.hip": ["-ffp-contract=off"],
+    "galic_noalloc.hip": ["-ffp-contract=off"],
 }
 
 

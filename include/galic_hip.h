@@ -163,6 +163,14 @@ int galic_fp32_orbit_responses(galic_ctx *ctx, int norbit, const double *pos, co
  * forcegrid_get_acceleration / forcegrid_get_potential (grid.c:59-197). */
 int galic_field_at(galic_ctx *ctx, int n, const double *pos, double *acc, double *pot);
 
+/* The lookups of galic_field_at without allocating device memory: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  The device staging is a static array of this library holding
+ * GALIC_FIELD_STATIC_MAX points, one copy per GPU kept until the process ends; more points
+ * are rejected with GALIC_ERR_ARG.  Calls that overlap on one GPU share the array and race. */
+#define GALIC_FIELD_STATIC_MAX 65536
+int galic_field_at_static(galic_ctx *ctx, int n, const double *pos, double *acc, double *pot);
+
 /* ---- FP32 batched path ---- */
 
 /* Initial response pass of the FP32 path: orbits in FP32, grids as 64-bit
