@@ -531,6 +531,49 @@ size_t slot_budget(galic::Ctx &c) {
   return std::max<size_t>((size_t)1 << 30, std::min(cap, have));
 }
 
+// The orbits of the initial pass (prepared at o_* [0, n)) through the logging kernel in
+// chunks the arena holds, each chunk's logs added to the Loc grids by k_log_to_loc.  The
+// first chunk is small: it sets the estimate of log blocks per orbit for the others.
+int fp32_initial_logs(galic::Ctx &c, int n) {
+  const size_t bb = log_block_bytes();
+  {
+    const size_t all = (size_t)(log_blocks_planned(c, n) * bb);
+    const size_t one = (size_t)(log_blocks_planned(c, std::min(n, 4096)) * bb);
+    CHK(ensure_arena(c, std::min(all, slot_budget(c)), one));
+  }
+  galic::QScale qs;
+  for(int f = 0; f < galic::kFields; f++) qs.q[f] = c.fp32_scale[f];
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+  int a = 0;
+  bool first = true;
+  while(a < n) {
+    const double blocks = (double)(c.slot_bytes / bb);
+    const double lanes = (double)orbit_grid(n - a, 256, c.fp32_bpc) * 256;
+    double fit = (blocks - 64 - lanes * galic::LOG_CHUNK) / ((c.log_est_blocks + 1.0) * 1.25);
+    if(first) fit = std::min(fit, 65536.0);
+    const int m = (int)std::max(1.0, std::min((double)(n - a), fit));
+    galic::OrbitArgs32 A = orbit_args32(c, m);
+    A.ox += a; A.oy += a; A.oz += a;
+    A.ovx += a; A.ovy += a; A.ovz += a;
+    A.omass += a; A.otint += a; A.otype += a;
+    A.tall += a; A.norbit += a; A.status += a; A.steps += a; A.bbox += a;
+    const int lr = launch_orbit32_log(c, A);
+    if(lr == kLogArenaFull) {
+      if(m <= 1) return set_err(GALIC_ERR_NOMEM, "the response log of one orbit does not fit in device memory");
+      continue;  // the estimate has doubled: plan the chunk again
+    }
+    CHK(lr);
+    hipLaunchKernelGGL(galic::k_log_to_loc, dim3(std::min(m, ncu * 16)), dim3(256), 0, c.stream, m,
+                       (const int *)(c.o_type + a), (const int *)c.log_blocks, (const int *)c.log_n, c.log_M,
+                       (const int *)c.log_q, (const float4 *)c.log_v, c.L, qs, c.locA_q);
+    HIPCHK(hipGetLastError());
+    a += m;
+    first = false;
+  }
+  return GALIC_OK;
+}
+
 }  // namespace
 
 namespace galic {
@@ -765,7 +808,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
-  dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk);
+  dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk); dfree(c.t_vinst);
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
@@ -782,6 +825,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.g32_scratch);
   dfree(c.tile_list);
   dfree(c.tile_aux);
+  dfree(c.merit_prof);
   dfree(c.lookup_buf);
   dfree(c.fit_part);
   dfree(c.tsrc);
@@ -1497,14 +1541,11 @@ int galic_fp32_calc_all_response_fields(galic_ctx *ctx) {
   CHK(build_weights(ctx, false));
   hipLaunchKernelGGL(galic::k_prepare_all, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.P, orbit_in(c));
   HIPCHK(hipGetLastError());
-  galic::OrbitArgs32 A = orbit_args32(c, n);
-  A.locq = c.locA_q;
-  const long long steps0 = c.acc_steps;
-  CHK(launch_orbit32(c, A));
-  // the response logs of later reps: fixed point bounds, and a first estimate of
-  // their length (about one flushed 2x2 run per 4 to 6 steps)
+  // the orbits into response logs, chunk by chunk as the arena holds them, each chunk's
+  // records then added to the fixed-point Loc grids (the logging kernel keeps its lanes
+  // integrating; atomics at every flush would stall them)
   log_setup(c);
-  c.log_est_blocks = 0.25 * (double)(c.acc_steps - steps0) / n / galic::LOG_B + 1.0;
+  CHK(fp32_initial_logs(c, n));
   hipLaunchKernelGGL(galic::k_store_orbits, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n, 0, c.o_norbit,
                      c.P.orbits);
   hipLaunchKernelGGL(galic::k_store_steps, dim3(nblocks(n, 256)), dim3(256), 0, c.stream, n,
@@ -1627,8 +1668,10 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   if(nk > c.max_ktrials) {
     dfree(c.t_vk);
     dfree(c.t_dFk);
+    dfree(c.t_vinst);
     CHK(dalloc(&c.t_vk, 3 * nk));
     CHK(dalloc(&c.t_dFk, nk));
+    CHK(dalloc(&c.t_vinst, nk));
     c.max_ktrials = nk;
   }
   c.last_ntrial = ntrial;
@@ -1686,13 +1729,21 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   // LDS grids of the whole response grid (k_eval_log32).  The HBM-scratch form of k_eval_log32
   // (two int64 grids per workgroup in HBM) is kept for grids above 128^2 when tiles are
   // switched off or K exceeds what the tile lists hold.
-  auto tile_lds = [&]() { return galic::tile_lds_bytes(c.L, K, c.log_M) + 1024; };
+  // with the tile's (residual, weight) pairs in LDS where they fit (K up to about 8 on 512^2 grids)
+  auto tile_lds_bw = [&](bool bwl) { return galic::tile_lds_bytes(c.L, K, c.log_M, bwl) + 1024; };
+  auto tile_lds = [&]() { return tile_lds_bw(false); };
   const bool tiles_ok = c.L >= 5 && K <= galic::TILE_MAX_K;
+  // (the block tables in LDS when they leave room, else read from memory)
+  auto sparse_tbl = [&]() { return galic::sparse_lds_bytes(c.L, K, c.log_M, true) + 1024 <= lds_max; };
+  auto sparse_lds = [&]() { return galic::sparse_lds_bytes(c.L, K, c.log_M, sparse_tbl()) + 1024; };
   auto use_tiles = [&]() {
     if(!tiles_ok || c.merit_form == 2) return false;
-    if(c.merit_form == 1 || c.L > 7) return tile_lds() <= lds_max;
+    if(c.merit_form == 1 || c.merit_form == 3 || c.L > 7) return tile_lds() <= lds_max;
     return eval_lds(1) > lds_max && tile_lds() <= lds_max;
   };
+  // tiles restricted to the nodes the logs touch, where their LDS fits (form 3; the dense
+  // tiles measured faster so far and are the automatic choice)
+  auto use_sparse = [&]() { return use_tiles() && c.merit_form == 3 && sparse_lds() <= lds_max; };
   const bool gmem = c.L > 7;
   const int kGBlocks = 512;
   auto eval_lds_gmem = [&]() {
@@ -1774,10 +1825,14 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     cap = group_cap();
     // (the block tables may have grown with this launch: the form is chosen after it)
     const bool tiles = use_tiles();
+    const bool sparse = use_sparse();
     // without tiles, the HBM-scratch form above 128^2 cells and wherever one LDS grid does not fit
     const bool gmem_now = !tiles && (gmem || eval_lds(1) > lds_max);
     const bool use_o = gmem_now || eval_lds(2) <= lds_max;
-    const size_t shmem = (tiles ? tile_lds() : gmem_now ? eval_lds_gmem() : eval_lds(use_o ? 2 : 1)) - 1024;
+    const bool bwl = tiles && tile_lds_bw(true) <= lds_max;
+    const size_t shmem =
+        (sparse ? sparse_lds() : tiles ? tile_lds_bw(bwl) : gmem_now ? eval_lds_gmem() : eval_lds(use_o ? 2 : 1)) -
+        1024;
     if(shmem + 1024 > lds_max)
       return set_err(GALIC_ERR_ARG, "the merit kernel needs %zu bytes of LDS, the device gives %zu", shmem, lds_max);
     if(gmem_now && c.g32_scratch_len < (size_t)kGBlocks * 2 * c.Ngrid) {
@@ -1790,7 +1845,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     int tile_grid = 0, tile_cap = 0;
     if(tiles) {
       int per_cu = 0, ncu = 0;
-      auto kern = c.L == 9 ? (const void *)galic::k_eval_tile32<9> : (const void *)galic::k_eval_tile32<0>;
+      auto kern = sparse ? (c.L == 9 ? (const void *)galic::k_eval_sparse32<9> : (const void *)galic::k_eval_sparse32<0>)
+                  : c.L == 9 ? (bwl ? (const void *)galic::k_eval_tile32<9, true> : (const void *)galic::k_eval_tile32<9, false>)
+                             : (bwl ? (const void *)galic::k_eval_tile32<0, true> : (const void *)galic::k_eval_tile32<0, false>);
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 1024, shmem));
       HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
       if(per_cu < 1) return set_err(GALIC_ERR_ARG, "the tile merit kernel does not fit on a CU (%zu bytes of LDS)", shmem);
@@ -1805,12 +1862,15 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       HIPCHK(hipStreamSynchronize(c.stream));
       const size_t want = 4 * (size_t)mx + 64;
       if(want > 0x7fffffffu) return set_err(GALIC_ERR_NOMEM, "a particle's response logs hold %u records", mx);
-      tile_cap = (int)want;
-      if(c.tile_list_len < (size_t)tile_grid * tile_cap) {
+      tile_cap = (int)((want + 3) & ~(size_t)3);
+      // the sparse form keeps a copy of each listed record (a word and five float4, 21 words);
+      // the dense form one word per entry
+      const size_t words = (size_t)tile_grid * tile_cap * (sparse ? 21 : 1);
+      if(c.tile_list_len < words) {
         dfree(c.tile_list);
         c.tile_list_len = 0;
-        CHK(dalloc(&c.tile_list, (size_t)tile_grid * tile_cap));
-        c.tile_list_len = (size_t)tile_grid * tile_cap;
+        CHK(dalloc(&c.tile_list, words));
+        c.tile_list_len = words;
       }
     }
     // evaluate the reps of the group in turn, committing each completed rep
@@ -1873,11 +1933,27 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
         E.tlist = c.tile_list;
         E.tlist_cap = tile_cap;
         E.tile_err = c.tile_aux + 1;
+        E.prof = c.merit_prof;
+        E.tbl_lds = sparse_tbl() ? 1 : 0;
         const int grid = std::min(b - a, tile_grid);
-        if(c.L == 9)
-          hipLaunchKernelGGL((galic::k_eval_tile32<9>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+        E.vinst = c.t_vinst;
+        if(sparse) {
+          // instantaneous terms before, the write-back of the choice after the merit kernel
+          hipLaunchKernelGGL(galic::k_trial_inst, dim3(nblocks((size_t)(b - a) * K, 256)), dim3(256), 0, c.stream, c.M,
+                             E, c.t_vinst);
+          if(c.L == 9)
+            hipLaunchKernelGGL((galic::k_eval_sparse32<9>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+          else
+            hipLaunchKernelGGL((galic::k_eval_sparse32<0>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+          hipLaunchKernelGGL(galic::k_apply_best, dim3(nblocks(b - a, 256)), dim3(256), 0, c.stream, c.M, E);
+        } else if(c.L == 9 && bwl)
+          hipLaunchKernelGGL((galic::k_eval_tile32<9, true>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+        else if(c.L == 9)
+          hipLaunchKernelGGL((galic::k_eval_tile32<9, false>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+        else if(bwl)
+          hipLaunchKernelGGL((galic::k_eval_tile32<0, true>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
         else
-          hipLaunchKernelGGL((galic::k_eval_tile32<0>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
+          hipLaunchKernelGGL((galic::k_eval_tile32<0, false>), dim3(grid), dim3(1024), shmem, c.stream, c.M, E);
         HIPCHK(hipGetLastError());
       } else if(gmem_now) {
         for(int b0 = 0; b0 < b - a; b0 += kGBlocks) {
@@ -1937,8 +2013,19 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
 
 }  // namespace
 
+int galic_fp32_merit_profile(galic_ctx *ctx, int on, unsigned long long *out16) {
+  if(!ctx) return set_err(GALIC_ERR_ARG, "null context");
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipSetDevice(c.device));
+  if(out16 && c.merit_prof) HIPCHK(hipMemcpy(out16, c.merit_prof, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  if(on && !c.merit_prof) CHK(dalloc(&c.merit_prof, 16));
+  if(on) HIPCHK(hipMemset(c.merit_prof, 0, 16 * sizeof(unsigned long long)));
+  if(!on) dfree(c.merit_prof);
+  return GALIC_OK;
+}
+
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form) {
-  if(!ctx || form < 0 || form > 2) return set_err(GALIC_ERR_ARG, "merit form must be 0, 1 or 2");
+  if(!ctx || form < 0 || form > 3) return set_err(GALIC_ERR_ARG, "merit form must be 0, 1, 2 or 3");
   ctx->c.merit_form = form;
   return GALIC_OK;
 }

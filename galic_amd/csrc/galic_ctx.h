@@ -100,6 +100,7 @@ struct Ctx {
   int *tovs_dev = nullptr;      // velocity structure per type, on the device
   double *t_vk = nullptr;       // K trial velocities per particle, SoA [3][max_ktrials]
   double *t_dFk = nullptr;      // merit change of every trial of the last FP32 call [max_ktrials]
+  double *t_vinst = nullptr;    // instantaneous merit change of every trial (sparse tile merit) [max_ktrials]
   size_t max_ktrials = 0;
   int last_ntrial = 0, last_k = 0;  // trials and K of the last FP32 call
   double *fit_part = nullptr;   // partial cut sums of the global fit [type-1][kFitBlocks][5]
@@ -173,6 +174,7 @@ struct Ctx {
   double tsrc_lo[3] = {0, 0, 0}, tsrc_hi[3] = {0, 0, 0};  // their extent
   double *lookup_buf = nullptr;  // galic_field_at: staging of positions, accelerations, potentials
   size_t lookup_len = 0;         // points it holds   // [0] largest records of a particle's logs, [1] list overflow flag
+  unsigned long long *merit_prof = nullptr;  // [16] phase times of the sparse tile merit (galic_fp32_merit_profile)
   int merit_form = 0;                 // 0 automatic, 1 tiles at every level, 2 LDS grids / HBM scratch (no tiles)
 
   // timing of the last launches (ms), measured with HIP events on ctx stream

@@ -76,6 +76,9 @@ struct EvalArgs32 {
   unsigned int *tlist;  // tile form: per workgroup, tlist_cap entries of the records' tile lists
   int tlist_cap;
   unsigned int *tile_err;  // tile form: set when a particle's lists exceed tlist_cap
+  unsigned long long *prof;  // sparse tile form, optional: time per phase and sizes, summed over particles
+  int tbl_lds;               // sparse tile form: the block tables staged in LDS (else read from log_blocks)
+  const double *vinst;       // sparse tile form: instantaneous merit change of each trial (k_trial_inst)
   const int *tidx;
   const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
   const double *stacks, *fit;
@@ -126,17 +129,44 @@ constexpr int TILE = 32;
 constexpr int TILE_RS = TILE + 2;
 constexpr int TILE_CELLS = TILE * TILE_RS;
 constexpr int TILE_MAX_K = 63;  // list entries hold the log index in 8 bits
+// Nodes of one tile's pyramid below its root: 32 x 32 cells, then 16 x 16, 8 x 8, 4 x 4 and
+// the 2 x 2 block nodes (levels L .. L-4).
+constexpr int TILE_NODES = 1024 + 256 + 64 + 16 + 4;
 // LDS of k_eval_tile32 beyond its static arrays: per-wave partials [K][16] double, the
 // per-trial pyramid above the tiles [K][Nstack(L-5)] and block sums [2][K][4] int64, K + 2 tile
-// grids (two for the current orbit, alternating between tiles), tile counts, offsets and
-// the active list [3 * ntile + 1] int, the block tables [K+1][log_M] int.
-__host__ __device__ inline size_t tile_lds_bytes(int L, int K, int log_M) {
+// grids (two for the current orbit, alternating between tiles), the logs touching each tile
+// [ntile] uint64, tile counts, offsets and the active list [3 * ntile + 1] int, the block
+// tables [K+1][log_M] int, and with BWL the tile's (residual, weight) pairs [TILE_NODES] double2.
+__host__ __device__ inline size_t tile_lds_bytes(int L, int K, int log_M, bool bwl) {
   const size_t nst = GALIC_NSTACK(L - 5), ntile = (size_t)1 << (2 * (L - 5));
   return (size_t)MERIT_MAX_WAVES * K * 8 + (size_t)K * nst * 8 + 8 * (size_t)K * 8 +
-         (size_t)(K + 2) * TILE_CELLS * 8 + (3 * ntile + 1) * 4 + (size_t)(K + 1) * log_M * 4;
+         (size_t)(K + 2) * TILE_CELLS * 8 + ntile * 8 + (3 * ntile + 1) * 4 + (size_t)(K + 1) * log_M * 4 +
+         (bwl ? (size_t)TILE_NODES * 16 + 16 : 0);
+}
+template <int LF, bool BWL>
+__global__ void k_eval_tile32(DevModel M, EvalArgs32 A);
+// LDS of k_eval_sparse32 beyond its static arrays: per-wave partials [K][16] double, the
+// per-trial pyramid above the tiles [K][Nstack(L-5)] and the 4x4-cell node sums [2][K][64]
+// int64, K + 2 tile grids, the logs touching each tile [ntile] uint64, tile counts, offsets
+// and the active list [3 * ntile + 1] int, the block tables [K+1][log_M] int (when they are
+// staged in LDS: a table grows with the longest orbit's log), the marked
+// nodes [2][8] uint32, the tile's pairs of cells and 2x2 nodes [1280] and of the coarser
+// nodes [2][84] double2.
+__host__ __device__ inline size_t sparse_lds_bytes(int L, int K, int log_M, bool tbl_lds) {
+  const size_t nst = GALIC_NSTACK(L - 5), ntile = (size_t)1 << (2 * (L - 5));
+  return (size_t)MERIT_MAX_WAVES * K * 8 + (size_t)K * nst * 8 + 128 * (size_t)K * 8 +
+         (size_t)(K + 2) * TILE_CELLS * 8 + ntile * 8 + (3 * ntile + 1) * 4 +
+         (tbl_lds ? (size_t)(K + 1) * log_M * 4 : 0) + 64 + 16 + (size_t)(1280 + 168) * 16;
 }
 template <int LF>
-__global__ void k_eval_tile32(DevModel M, EvalArgs32 A);
+__global__ void k_eval_sparse32(DevModel M, EvalArgs32 A);
+__global__ void k_trial_inst(DevModel M, EvalArgs32 A, double *vinst);
+__global__ void k_apply_best(DevModel M, EvalArgs32 A);
+struct QScale {
+  double q[kFields];
+};
+__global__ void k_log_to_loc(int n, const int *otype, const int *log_blocks, const int *log_n, int log_M,
+                             const int *log_q, const float4 *log_v, int L, QScale qs, long long *locq);
 __global__ void k_log_totals(int n, int K, const int *slot_pos, const int *log_n, unsigned int *maxsum);
 
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,

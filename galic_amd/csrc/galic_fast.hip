@@ -1301,12 +1301,34 @@ __device__ __forceinline__ void tile_add(unsigned long long *g, int lz, int lr, 
   }
 }
 
+// Index of a node of a tile's pyramid in the tile-local array of (residual, weight) pairs:
+// level L - u (u = 0 .. 4), tile-local row i and column j of that level.
+__device__ __forceinline__ int tile_node(int u, int i, int j) {
+  const int base[5] = {0, 1024, 1280, 1344, 1360};
+  return base[u] + i * (TILE >> u) + j;
+}
+
+// The tile's (residual, weight) pairs of levels L .. L-4 into LDS (all threads).
+__device__ __forceinline__ void tile_load_bw(const double2 *bwf, int L, int tz, int tr, double2 *tbw) {
+  for(int q = threadIdx.x; q < TILE_NODES; q += blockDim.x) {
+    int u = 0, r = q;
+    while(u < 4 && r >= (TILE >> u) * (TILE >> u)) {
+      r -= (TILE >> u) * (TILE >> u);
+      u++;
+    }
+    const int n = TILE >> u, i = r / n, j = r % n;
+    tbw[q] = bwf[GALIC_STACKOFFSET(L - u, tz * n + i, tr * n + j)];
+  }
+}
+
 // One wave's pyramid of one 16 x 16 block b (0..3, row-major) of tile (tz, tr) for one trial:
 // D = G - O on the block's cells (G cleared as read), the terms of the cells, the 2x2 nodes
 // (one per lane, lanes in Morton order) and of the 4x4, 8x8 and 16x16 nodes formed with
 // group_sum_i64.  The block node's sum goes to *bs (lane 0).  Returns the lane's terms.
-__device__ __forceinline__ double tile_block(const double2 *bwf, int L, double unit, long long *G, const long long *O,
-                                             int tz, int tr, int b, long long *bs) {
+// The (residual, weight) pairs come from the tile's LDS copy (tbw) or from memory (bwf).
+template <bool BWL>
+__device__ __forceinline__ double tile_block(const double2 *bwf, const double2 *tbw, int L, double unit, long long *G,
+                                             const long long *O, int tz, int tr, int b, long long *bs) {
   const int nb = 1 << L, lane = threadIdx.x & 63;
   const int jx = morton_x(lane), jy = morton_y(lane);
   const int lz = (b >> 1) * 16 + 2 * jy, lr = (b & 1) * 16 + 2 * jx;
@@ -1330,14 +1352,21 @@ __device__ __forceinline__ double tile_block(const double2 *bwf, int L, double u
     const int offL = GALIC_STACKOFFSET(L, 0, 0);
 #pragma unroll
     for(int c = 0; c < 4; c++)
-      if(d[c] != 0) acc += term_at(bwf[offL + (gz + (c >> 1)) * nb + gr + (c & 1)], (double)d[c] * unit);
-    if(s != 0) acc += term_at(bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)], (double)s * unit);
+      if(d[c] != 0)
+        acc += term_at(BWL ? tbw[tile_node(0, lz + (c >> 1), lr + (c & 1))]
+                           : bwf[offL + (gz + (c >> 1)) * nb + gr + (c & 1)],
+                       (double)d[c] * unit);
+    if(s != 0)
+      acc += term_at(BWL ? tbw[tile_node(1, lz >> 1, lr >> 1)] : bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)],
+                     (double)s * unit);
   }
 #pragma unroll
   for(int u = 0; u < 3; u++) {
     s = group_sum_i64(s, u);
     const int l = L - 2 - u, mask = (1 << (2 * u + 2)) - 1, sh = L - l;
-    if((lane & mask) == 0 && s != 0) acc += term_at(bwf[GALIC_STACKOFFSET(l, gz >> sh, gr >> sh)], (double)s * unit);
+    if((lane & mask) == 0 && s != 0)
+      acc += term_at(BWL ? tbw[tile_node(u + 2, lz >> sh, lr >> sh)] : bwf[GALIC_STACKOFFSET(l, gz >> sh, gr >> sh)],
+                     (double)s * unit);
   }
   if(lane == 0) *bs = s;
   return acc;
@@ -1353,7 +1382,7 @@ __global__ void k_log_totals(int n, int K, const int *slot_pos, const int *log_n
   atomicMax(maxsum, s);
 }
 
-template <int LF>
+template <int LF, bool BWL>
 __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) {
   extern __shared__ double lds_d[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
@@ -1363,10 +1392,14 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
   long long *topst = (long long *)(part + MERIT_MAX_WAVES * K);   // [K][nst]: levels 0..lt per trial
   long long *bsum = topst + (size_t)K * nst;                      // [2][K][4] block sums, alternating
   long long *grids = bsum + 8 * K;                                // O (two, alternating), then G_1..G_K
-  int *tcount = (int *)(grids + (size_t)(K + 2) * TILE_CELLS);    // [ntile] counts, then cursors
+  unsigned long long *tmask = (unsigned long long *)(grids + (size_t)(K + 2) * TILE_CELLS);  // [ntile] logs touching
+  int *tcount = (int *)(tmask + ntile);                           // [ntile] counts, then cursors
   int *toff = tcount + ntile;                                     // [ntile + 1]
   int *tact = toff + ntile + 1;                                   // [ntile] tiles with entries
   int *tbl = tact + ntile;                                        // [K+1][log_M]
+  // (offsets from the LDS base keep the pointers in the LDS address space: ds_ instructions)
+  const size_t tbw_off = (((size_t)((char *)(tbl + (K + 1) * A.log_M) - (char *)lds_d)) + 15) & ~(size_t)15;
+  double2 *tbw = (double2 *)((char *)lds_d + tbw_off);  // BWL: [TILE_NODES]
   __shared__ double s_dF[64];
   __shared__ int s_n[TILE_MAX_K + 1], s_nact, s_best, s_over;
   unsigned int *list = A.tlist + (size_t)blockIdx.x * A.tlist_cap;
@@ -1381,13 +1414,16 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
     const size_t o0 = (size_t)j * (K + 1);
     const double MT = A.MType[t];
     // eval_fit_mod's combination (main.c:1215-1216): D/M + (f_rs V_rs + f_ts V_ts + f_qs V_qs + f_ps V_ps)/M/4
-    const double coef[kFields] = {1.0 / MT, A.fac[t][0] / MT / 4, A.fac[t][1] / MT / 4, A.fac[t][2] / MT / 4,
-                                  A.fac[t][3] / MT / 4};
+    // (coef_of(f): the weight of field f's terms; a function, not an array indexed at run time)
+    auto coef_of = [&](int f) { return f == 0 ? 1.0 / MT : A.fac[t][f - 1] / MT / 4; };
     __syncthreads();  // the previous particle's use of the shared arrays is over
     for(int q = tid; q <= K; q += nt) s_n[q] = max(A.log_n[o0 + q], 0);
     for(int q = tid; q < MERIT_MAX_WAVES * K; q += nt) part[q] = 0;
     for(int q = tid; q < K * nst; q += nt) topst[q] = 0;
-    for(int q = tid; q < ntile; q += nt) tcount[q] = 0;
+    for(int q = tid; q < ntile; q += nt) {
+      tcount[q] = 0;
+      tmask[q] = 0;
+    }
     __syncthreads();
     for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
       const int o = q / A.log_M, b = q % A.log_M;
@@ -1399,7 +1435,10 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
       const int n = s_n[o], *tb = tbl + o * A.log_M;
       for(int r = tid; r < n; r += nt) {
         const int quad = A.log_q[(size_t)tb[r / LOG_B] * LOG_B + (r & (LOG_B - 1))];
-        tile_visit(quad, L, ts, [&](int tile) { atomicAdd(&tcount[tile], 1); });
+        tile_visit(quad, L, ts, [&](int tile) {
+          atomicAdd(&tcount[tile], 1);
+          if(!(tmask[tile] & (1ull << o))) atomicOr(&tmask[tile], 1ull << o);
+        });
       }
     }
     __syncthreads();
@@ -1427,7 +1466,49 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
     }
     __syncthreads();
     const int nact = s_nact;
-    // 2. field by field, tile by tile
+    // 2. field by field, tile by tile.  The entries (and the tile's pairs) of the next
+    // (field, tile) step are loaded into registers while the current tile's pyramid runs, so
+    // their latency is hidden; the step after the barrier adds them from registers.
+    constexpr int PF = 2;
+    int pf_o[PF], pf_quad[PF];
+    float4 pf_x[PF];
+    double2 pf_bw[2];
+    auto prefetch = [&](int step) {
+#pragma unroll
+      for(int u = 0; u < PF; u++) pf_o[u] = -1;
+      if(step >= kFields * nact) return;
+      const int f2 = step / nact, tile = tact[step % nact];
+#pragma unroll
+      for(int u = 0; u < PF; u++) {
+        const int e = toff[tile] + tid + u * nt;
+        if(e < toff[tile + 1]) {
+          const unsigned int en = list[e];
+          const int o = (int)(en >> 24), r = (int)(en & 0xffffffu);
+          const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
+          pf_o[u] = o;
+          pf_quad[u] = A.log_q[(size_t)blk * LOG_B + jj];
+          pf_x[u] = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
+        }
+      }
+      if(BWL) {
+        const double2 *bwn = bw + (size_t)f2 * Nstack;
+        const int tz = tile / ts, tr = tile % ts;
+#pragma unroll
+        for(int u = 0; u < 2; u++) {
+          const int q = tid + u * nt;
+          if(q < TILE_NODES) {
+            int lv = 0, r = q;
+            while(lv < 4 && r >= (TILE >> lv) * (TILE >> lv)) {
+              r -= (TILE >> lv) * (TILE >> lv);
+              lv++;
+            }
+            const int n = TILE >> lv;
+            pf_bw[u] = bwn[GALIC_STACKOFFSET(L - lv, tz * n + r / n, tr * n + r % n)];
+          }
+        }
+      }
+    };
+    prefetch(0);
     for(int f = 0; f < kFields; f++) {
       const float sc = A.log_scale[f];
       const double unit = A.log_unit[f];
@@ -1436,7 +1517,14 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
         const int cur = ai & 1;
         if(ai < nact) {
           const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
-          for(int e = toff[tile] + tid; e < toff[tile + 1]; e += nt) {
+#pragma unroll
+          for(int u = 0; u < PF; u++)
+            if(pf_o[u] >= 0) {
+              long long *g = grids + (size_t)(pf_o[u] == 0 ? cur : pf_o[u] + 1) * TILE_CELLS;
+              tile_add((unsigned long long *)g, (pf_quad[u] >> L) - tz * TILE, (pf_quad[u] & (nb - 1)) - tr * TILE,
+                       pf_x[u], sc);
+            }
+          for(int e = toff[tile] + tid + PF * nt; e < toff[tile + 1]; e += nt) {
             const unsigned int en = list[e];
             const int o = (int)(en >> 24), r = (int)(en & 0xffffffu);
             const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
@@ -1445,6 +1533,10 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
             long long *g = grids + (size_t)(o == 0 ? cur : o + 1) * TILE_CELLS;
             tile_add((unsigned long long *)g, (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x, sc);
           }
+          if(BWL)
+#pragma unroll
+            for(int u = 0; u < 2; u++)
+              if(tid + u * nt < TILE_NODES) tbw[tid + u * nt] = pf_bw[u];
         }
         __syncthreads();
         if(ai > 0) {
@@ -1458,12 +1550,19 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
         }
         if(ai < nact) {
           const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          prefetch(f * nact + ai + 1);
           const long long *O = grids + (size_t)cur * TILE_CELLS;
+          const unsigned long long tm = tmask[tile];
           for(int it = wave; it < 4 * K; it += nw) {
             const int k = it >> 2, b = it & 3;
-            double acc = tile_block(bwf, L, unit, grids + (size_t)(k + 2) * TILE_CELLS, O, tz, tr, b,
-                                    bsum + (size_t)cur * 4 * K + 4 * k + b);
-            acc = wave_sum(coef[f] * acc);
+            long long *bs = bsum + (size_t)cur * 4 * K + 4 * k + b;
+            if(!(tm & (1ull | (2ull << k)))) {
+              // neither the orbit nor the trial reaches this tile: D = 0 (the trial's grid is clean)
+              if(lane == 0) *bs = 0;
+              continue;
+            }
+            double acc = tile_block<BWL>(bwf, tbw, L, unit, grids + (size_t)(k + 2) * TILE_CELLS, O, tz, tr, b, bs);
+            acc = wave_sum(coef_of(f) * acc);
             if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
           }
         }
@@ -1492,7 +1591,7 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        acc = wave_sum(coef[f] * acc);
+        acc = wave_sum(coef_of(f) * acc);
         if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
       }
       __syncthreads();
@@ -1507,6 +1606,7 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
         const float sc = A.log_scale[f];
         for(int ai = 0; ai < nact; ai++) {
           const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          if(!(tmask[tile] & (1ull | (1ull << ob)))) continue;
           for(int e = toff[tile] + tid; e < toff[tile + 1]; e += nt) {
             const unsigned int en = list[e], o = en >> 24;
             if(o != 0 && o != ob) continue;
@@ -1534,8 +1634,541 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
   }
 }
 
-template __global__ void k_eval_tile32<0>(DevModel M, EvalArgs32 A);
-template __global__ void k_eval_tile32<9>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_tile32<0, false>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_tile32<0, true>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_tile32<9, false>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_tile32<9, true>(DevModel M, EvalArgs32 A);
+
+// ------------------------------------- trial choice around the tile merit --
+//
+// The tile merit kernels with the instantaneous terms and the write-back of choose_trial
+// split off into two small kernels, which keeps the FP64 moment code out of the merit
+// kernel's registers.
+
+// The instantaneous merit change of every trial of the launch (inst_delta_q), one thread
+// per trial: vinst[ti * K + k].
+__global__ void k_trial_inst(DevModel M, EvalArgs32 A, double *vinst) {
+  const int K = A.K;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if(q >= A.nblk * K) return;
+  const int ti = A.trial_base + q / K, k = q % K;
+  const int p = A.tidx[ti], t = A.P.type[p];
+  const int nb = M.EG_Nbin, ngrid = nb * nb, Nstack = GALIC_NSTACK(M.DG_L);
+  const double *S = A.stacks + (size_t)t * ST_COUNT * Nstack;
+  const double x = A.P.x[p], y = A.P.y[p], z = A.P.z[p], mass = A.P.mass[p];
+  const double dold[4] = {A.P.vr2[p], A.P.vt2[p], A.P.vp2[p], A.P.vq2[p]};
+  const size_t qq = (size_t)ti * K + k;
+  double dnew[4];
+  disp_components64(M, t, x, y, z, A.tvx[qq], A.tvy[qq], A.tvz[qq], dnew[0], dnew[1], dnew[2], dnew[3]);
+  double cw[4];
+  int cc[4];
+  vinst[qq] = inst_delta_q(M, S, A.cutB_of + (size_t)t * ngrid, t, x, y, z, mass, dnew, dold, A.lowdisp, cw, cc);
+}
+
+// The taken trial of each particle of the launch (A.accept = trial + 1, or 0), one thread per
+// particle: orbit count and step count of the kept orbit; for a taken trial its
+// instantaneous deltas and the particle's new velocity and moments (main.c:480-527).
+__global__ void k_apply_best(DevModel M, EvalArgs32 A) {
+  const int bid = blockIdx.x * blockDim.x + threadIdx.x;
+  if(bid >= A.nblk) return;
+  const int K = A.K, ti = A.trial_base + bid;
+  const int j = A.slot_pos ? A.slot_pos[bid] : bid;
+  const int p = A.tidx[ti], t = A.P.type[p];
+  const int best = A.accept[ti] - 1;
+  const size_t o0 = (size_t)j * (K + 1), oc = (best >= 0) ? o0 + 1 + best : o0;
+  A.P.orbits[p] = A.norbit[oc];
+  A.psteps[p] = (int)min(A.osteps[oc], (long long)0x7fffffff);
+  if(best < 0) return;
+  const int nb = M.EG_Nbin, ngrid = nb * nb;
+  const double x = A.P.x[p], y = A.P.y[p], z = A.P.z[p], mass = A.P.mass[p];
+  const double dold[4] = {A.P.vr2[p], A.P.vt2[p], A.P.vp2[p], A.P.vq2[p]};
+  const size_t q = (size_t)ti * K + best;
+  const double vt[3] = {A.tvx[q], A.tvy[q], A.tvz[q]};
+  double dnew[4];
+  disp_components64(M, t, x, y, z, vt[0], vt[1], vt[2], dnew[0], dnew[1], dnew[2], dnew[3]);
+  int iR, iz;
+  double fR, fz;
+  cell64(x, y, z, M.EG_Rmin, M.EG_logFac, M.EG_Nbin, iR, iz, fR, fz);
+  const double cw[4] = {(1 - fR) * (1 - fz), fR * (1 - fz), (1 - fR) * fz, fR * fz};
+  const int cc[4] = {iz * nb + iR, iz * nb + iR + 1, (iz + 1) * nb + iR, (iz + 1) * nb + iR + 1};
+  long long *lid = A.locI_dq + (size_t)t * 4 * ngrid;
+  for(int f = 0; f < 4; f++) {
+    const double v = (dnew[f] - dold[f]) * mass;
+    for(int c = 0; c < 4; c++) atomic_q(&lid[(size_t)f * ngrid + cc[c]], A.iscale * cw[c] * v);
+  }
+  A.P.vx[p] = vt[0];
+  A.P.vy[p] = vt[1];
+  A.P.vz[p] = vt[2];
+  A.P.vr2[p] = dnew[0];
+  A.P.vt2[p] = dnew[1];
+  A.P.vp2[p] = dnew[2];
+  A.P.vq2[p] = dnew[3];
+}
+
+// ------------------------------------------- FP32 merit in sparse tiles --
+//
+// k_eval_tile32 with the pyramid of each tile restricted to the 2x2 nodes the logs touch.
+// The adds of a tile mark the 2x2 nodes of the cells they reach (a bitmap of the tile's
+// 256 nodes); the pyramid then takes (node, trial) pairs of marked nodes only: a lane forms
+// D = G - O on the node's four cells, the terms of the cells and of the node, and adds the
+// node's sum into its 4 x 4-cell parent's sum (LDS integer atomics, exact); one wave per trial
+// then forms the tile's three coarser levels from the 8 x 8 parent sums with group_sum_i64.
+// That last step of a tile runs while the next tile's records are added (the orbit's grid,
+// the bitmap, the parent sums and the coarse pairs alternate between two copies), so a tile
+// costs two barriers.  Same sums as k_eval_tile32: the same decisions and deltas.
+
+// A workgroup barrier that waits for the wave's LDS operations only: loads from memory
+// issued before it stay in flight across it (their registers are waited on where used).
+// The phases it separates exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// The j-th set bit (0-based) of a 256-bit map held as eight words; -1 if there is none.
+__device__ __forceinline__ int select_bit(const unsigned int *w, int j) {
+#pragma unroll
+  for(int q = 0; q < 8; q++) {
+    const int c = __popc(w[q]);
+    if(j < c) {
+      unsigned int x = w[q];
+      for(int u = 0; u < j; u++) x &= x - 1;
+      return q * 32 + __ffs(x) - 1;
+    }
+    j -= c;
+  }
+  return -1;
+}
+
+// Adds the cells of a record that fall inside the tile and marks their 2x2 nodes.
+__device__ __forceinline__ void tile_add_mark(unsigned long long *g, unsigned int *bits, int lz, int lr, float4 x,
+                                              float sc) {
+  const float v[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+  for(int c = 0; c < 4; c++) {
+    const int cz = lz + (c >> 1), cr = lr + (c & 1);
+    if((unsigned)cz < (unsigned)TILE && (unsigned)cr < (unsigned)TILE && v[c] != 0.f) {
+      atomicAdd(&g[cz * TILE_RS + cr], (unsigned long long)__float2ll_rn(v[c] * sc));
+      const int nd = (cz >> 1) * (TILE / 2) + (cr >> 1);
+      const unsigned int bit = 1u << (nd & 31);
+      if(!(bits[nd >> 5] & bit)) atomicOr(&bits[nd >> 5], bit);
+    }
+  }
+}
+
+template <int LF>
+__global__ void __launch_bounds__(1024) k_eval_sparse32(DevModel M, EvalArgs32 A) {
+  extern __shared__ double lds_d[];
+  const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
+  const int L = LF ? LF : M.DG_L, nb = 1 << L, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
+  const int lt = L - 5, ts = 1 << lt, ntile = ts * ts, nst = GALIC_NSTACK(lt);
+  double *part = lds_d;                                           // [K][MERIT_MAX_WAVES]
+  long long *topst = (long long *)(part + MERIT_MAX_WAVES * K);    // [K][nst]: levels 0..lt per trial
+  long long *acc2 = topst + (size_t)K * nst;                       // [2][K][64] sums of the 4x4-cell nodes
+  long long *grids = acc2 + 128 * (size_t)K;                       // O (two, alternating), then G_1..G_K
+  unsigned long long *tmask = (unsigned long long *)(grids + (size_t)(K + 2) * TILE_CELLS);  // [ntile]
+  int *tcount = (int *)(tmask + ntile);                            // [ntile] counts, then cursors
+  int *toff = tcount + ntile;                                      // [ntile + 1]
+  int *tact = toff + ntile + 1;                                    // [ntile] tiles with entries
+  int *tbl_s = tact + ntile;                                       // [K+1][log_M] if staged
+  unsigned int *tbits = (unsigned int *)(tbl_s + (A.tbl_lds ? (K + 1) * A.log_M : 0));  // [2][8] marked nodes
+  // (offsets from the LDS base keep the pointers in the LDS address space: ds_ instructions)
+  const size_t tbw_off = (((size_t)((char *)(tbits + 16) - (char *)lds_d)) + 15) & ~(size_t)15;
+  double2 *tbw = (double2 *)((char *)lds_d + tbw_off);            // [1280] cells and 2x2 nodes
+  double2 *tbwc = tbw + 1280;                                      // [2][84] 4x4, 8x8, 16x16 nodes
+  __shared__ double s_dF[64];
+  __shared__ int s_n[TILE_MAX_K + 1], s_nact, s_best;
+  // this workgroup's copy of the records, sorted by tile: per entry the quad index with the
+  // log in bits 24.. (qo), then per field the entry's four values (val[f])
+  const size_t cap = (size_t)A.tlist_cap;
+  unsigned int *qo = A.tlist + (size_t)blockIdx.x * cap * 21;
+  float4 *val = (float4 *)(qo + cap);
+
+  for(int q = tid; q < (K + 2) * TILE_CELLS; q += nt) grids[q] = 0;
+  for(int q = tid; q < 256 * K; q += nt) acc2[q] = 0;
+  for(int q = tid; q < 16; q += nt) tbits[q] = 0;
+  // optional timing (A.prof): wall clock of the phases of each particle, summed by thread 0
+  unsigned long long tp0 = 0, tp1 = 0, tp2 = 0;
+  for(int bid = blockIdx.x; bid < A.nblk; bid += gridDim.x) {
+    if(A.prof && tid == 0) tp0 = wall_clock64();
+    const int ti = A.trial_base + bid;
+    const int j = A.slot_pos ? A.slot_pos[bid] : bid;
+    const int p = A.tidx[ti];
+    const int t = A.P.type[p];
+    const double2 *bw = A.merit_bw + (size_t)t * kFields * Nstack;
+    const size_t o0 = (size_t)j * (K + 1);
+    const double MT = A.MType[t];
+    // the block tables of the K+1 logs: in LDS or in memory (rows o0 .. o0+K of log_blocks)
+    const int *tbl = A.tbl_lds ? (const int *)tbl_s : A.log_blocks + o0 * A.log_M;
+    // eval_fit_mod's combination (main.c:1215-1216): D/M + (f_rs V_rs + f_ts V_ts + f_qs V_qs + f_ps V_ps)/M/4
+    auto coef_of = [&](int f) { return f == 0 ? 1.0 / MT : A.fac[t][f - 1] / MT / 4; };
+    __syncthreads();  // the previous particle's use of the shared arrays is over
+    for(int q = tid; q <= K; q += nt) s_n[q] = max(A.log_n[o0 + q], 0);
+    for(int q = tid; q < MERIT_MAX_WAVES * K; q += nt) part[q] = 0;
+    for(int q = tid; q < K * nst; q += nt) topst[q] = 0;
+    for(int q = tid; q < ntile; q += nt) {
+      tcount[q] = 0;
+      tmask[q] = 0;
+    }
+    __syncthreads();
+    if(A.tbl_lds)
+      for(int q = tid; q < (K + 1) * A.log_M; q += nt) {
+        const int o = q / A.log_M, b = q % A.log_M;
+        if(b * LOG_B < s_n[o]) tbl_s[q] = A.log_blocks[(o0 + o) * A.log_M + b];
+      }
+    __syncthreads();
+    // 1. the records' tile lists (as k_eval_tile32)
+    for(int o = 0; o <= K; o++) {
+      const int n = s_n[o], *tb = tbl + o * A.log_M;
+      for(int r = tid; r < n; r += nt) {
+        const int quad = A.log_q[(size_t)tb[r / LOG_B] * LOG_B + (r & (LOG_B - 1))];
+        tile_visit(quad, L, ts, [&](int tile) {
+          atomicAdd(&tcount[tile], 1);
+          if(!(tmask[tile] & (1ull << o))) atomicOr(&tmask[tile], 1ull << o);
+        });
+      }
+    }
+    __syncthreads();
+    if(wave == 0) tile_scan(tcount, toff, tact, &s_nact, ntile, lane);
+    __syncthreads();
+    if(toff[ntile] > A.tlist_cap) {
+      // cannot happen when the caller sizes tlist_cap from k_log_totals; reported, not evaluated
+      if(tid == 0) {
+        atomicOr(A.tile_err, 1u);
+        A.accept[ti] = 0;
+        A.dF[ti] = 0;
+      }
+      continue;
+    }
+    for(int q = tid; q < ntile; q += nt) tcount[q] = 0;
+    __syncthreads();
+    // the records copied into the tile order, all five fields at once: every later read of a
+    // tile's records is one contiguous load per field
+    for(int o = 0; o <= K; o++) {
+      const int n = s_n[o], *tb = tbl + o * A.log_M;
+      for(int r = tid; r < n; r += nt) {
+        const int blk = tb[r / LOG_B], jj = r & (LOG_B - 1);
+        const int quad = A.log_q[(size_t)blk * LOG_B + jj];
+        float4 x[kFields];
+#pragma unroll
+        for(int f = 0; f < kFields; f++) x[f] = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+        // the (one to four) tiles of the record's cells, as tile_visit takes them
+        const int iz = quad >> L, iR = quad & (nb - 1);
+        const int z0 = iz / TILE, z1 = (iz + 1) / TILE, r0 = iR / TILE, r1 = (iR + 1) / TILE;
+        const int tl[4] = {z0 * ts + r0, z0 * ts + r1, z1 * ts + r0, z1 * ts + r1};
+        const bool use[4] = {true, r1 != r0, z1 != z0, z1 != z0 && r1 != r0};
+#pragma unroll
+        for(int u = 0; u < 4; u++)
+          if(use[u]) {
+            const size_t e = toff[tl[u]] + atomicAdd(&tcount[tl[u]], 1);
+            qo[e] = ((unsigned int)o << 24) | (unsigned int)quad;
+#pragma unroll
+            for(int f = 0; f < kFields; f++) val[f * cap + e] = x[f];
+          }
+      }
+    }
+    __syncthreads();
+    const int nact = s_nact;
+    if(A.prof && tid == 0) {
+      tp1 = wall_clock64();
+      atomicAdd(&A.prof[0], tp1 - tp0);
+      atomicAdd(&A.prof[5], (unsigned long long)nact);
+      atomicAdd(&A.prof[6], (unsigned long long)toff[ntile]);
+      atomicAdd(&A.prof[7], 1ull);
+    }
+    // 2. field by field, tile by tile; the next step's entries and pairs are loaded into
+    // registers during the current tile's pyramid
+    constexpr int PF = 1;
+    struct Prefetch {
+      int o[PF], quad[PF];
+      float4 x[PF];
+      double2 bw[2];
+    };
+    auto prefetch = [&](int step) {
+      Prefetch r;
+#pragma unroll
+      for(int u = 0; u < PF; u++) r.o[u] = -1;
+      if(step >= kFields * nact) return r;
+      const int f2 = step / nact, tile = tact[step % nact];
+#pragma unroll
+      for(int u = 0; u < PF; u++) {
+        const int e = toff[tile] + tid + u * nt;
+        if(e < toff[tile + 1]) {
+          const unsigned int en = qo[e];
+          r.o[u] = (int)(en >> 24);
+          r.quad[u] = (int)(en & 0xffffffu);
+          r.x[u] = val[f2 * cap + e];
+        }
+      }
+      const double2 *bwn = bw + (size_t)f2 * Nstack;
+      const int tz = tile / ts, tr = tile % ts;
+#pragma unroll
+      for(int u = 0; u < 2; u++) {
+        const int q = tid + u * nt;
+        if(q < TILE_NODES) {
+          int lv = 0, rr = q;
+          while(lv < 4 && rr >= (TILE >> lv) * (TILE >> lv)) {
+            rr -= (TILE >> lv) * (TILE >> lv);
+            lv++;
+          }
+          const int n = TILE >> lv;
+          r.bw[u] = bwn[GALIC_STACKOFFSET(L - lv, tz * n + rr / n, tr * n + rr % n)];
+        }
+      }
+      return r;
+    };
+    Prefetch pf = prefetch(0);
+    for(int f = 0; f < kFields; f++) {
+      const float sc = A.log_scale[f];
+      const double unit = A.log_unit[f];
+      const double2 *bwf = bw + (size_t)f * Nstack;
+      for(int ai = 0; ai <= nact; ai++) {
+        const int cur = ai & 1, prv = 1 - cur;
+        unsigned long long ta = 0;
+        if(A.prof && tid == 0) ta = wall_clock64();
+        // phase A: this tile's records added and their nodes marked; the previous tile's
+        // coarse levels, its orbit grid cleared
+        if(ai < nact) {
+          const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          unsigned long long *O = (unsigned long long *)(grids + (size_t)cur * TILE_CELLS);
+          unsigned int *bits = tbits + 8 * cur;
+#pragma unroll
+          for(int u = 0; u < PF; u++)
+            if(pf.o[u] >= 0)
+              tile_add_mark(pf.o[u] == 0 ? O : (unsigned long long *)(grids + (size_t)(pf.o[u] + 1) * TILE_CELLS),
+                            bits, (pf.quad[u] >> L) - tz * TILE, (pf.quad[u] & (nb - 1)) - tr * TILE, pf.x[u], sc);
+          for(int e = toff[tile] + tid + PF * nt; e < toff[tile + 1]; e += nt) {
+            const unsigned int en = qo[e];
+            const int o = (int)(en >> 24), quad = (int)(en & 0xffffffu);
+            const float4 x = val[f * cap + e];
+            tile_add_mark(o == 0 ? O : (unsigned long long *)(grids + (size_t)(o + 1) * TILE_CELLS), bits,
+                          (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x, sc);
+          }
+#pragma unroll
+          for(int u = 0; u < 2; u++) {
+            const int q = tid + u * nt;
+            if(q < 1280)
+              tbw[q] = pf.bw[u];
+            else if(q < TILE_NODES)
+              tbwc[84 * cur + q - 1280] = pf.bw[u];
+          }
+        }
+        if(ai > 0) {
+          const int ptile = tact[ai - 1], pz = ptile / ts, pr = ptile % ts;
+          // coarse levels of the previous tile: wave k takes trial k, lanes the 8x8 nodes of
+          // 4x4 cells in Morton order
+          for(int k = wave; k < K; k += nw) {
+            long long *a2 = acc2 + ((size_t)prv * K + k) * 64;
+            const double2 *cw = tbwc + 84 * prv;
+            const int mx = morton_x(lane), my = morton_y(lane);
+            long long s = a2[my * 8 + mx];
+            a2[my * 8 + mx] = 0;
+            double acc = 0;
+            if(s != 0) acc += term_at(cw[my * 8 + mx], (double)s * unit);
+            s = group_sum_i64(s, 0);
+            if((lane & 3) == 0 && s != 0) acc += term_at(cw[64 + (my >> 1) * 4 + (mx >> 1)], (double)s * unit);
+            s = group_sum_i64(s, 1);
+            if((lane & 15) == 0 && s != 0) acc += term_at(cw[80 + (my >> 2) * 2 + (mx >> 2)], (double)s * unit);
+            s = group_sum_i64(s, 2);
+            if(lane == 0) topst[(size_t)k * nst + GALIC_STACKOFFSET(lt, pz, pr)] = s;
+            acc = wave_sum(coef_of(f) * acc);
+            if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
+          }
+          // the previous tile's orbit grid, at its marked nodes
+          const unsigned int *pb = tbits + 8 * prv;
+          long long *Op = grids + (size_t)prv * TILE_CELLS;
+          for(int q = tid; q < 256; q += nt)
+            if((pb[q >> 5] >> (q & 31)) & 1u) {
+              const int c0 = (q / 16) * 2 * TILE_RS + (q % 16) * 2;
+              Op[c0] = 0;
+              Op[c0 + 1] = 0;
+              Op[c0 + TILE_RS] = 0;
+              Op[c0 + TILE_RS + 1] = 0;
+            }
+        }
+        lds_barrier();
+        unsigned long long tb = 0;
+        if(A.prof && tid == 0) {
+          tb = wall_clock64();
+          atomicAdd(&A.prof[9], tb - ta);
+        }
+        // phase B: the pairs (marked node, trial) of this tile
+        if(ai > 0 && tid < 8) tbits[8 * prv + tid] = 0;  // the previous tile's map is read no more
+        if(ai < nact) {
+          const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          pf = prefetch(f * nact + ai + 1);
+          unsigned long long tpf = 0;
+          if(A.prof && tid == 0) {
+            tpf = wall_clock64();
+            atomicAdd(&A.prof[11], tpf - tb);
+          }
+          const long long *O = grids + (size_t)cur * TILE_CELLS;
+          const unsigned int *bits = tbits + 8 * cur;
+          int nmark = 0;
+#pragma unroll
+          for(int q = 0; q < 8; q++) nmark += __popc(bits[q]);
+          const int nch = (nmark + 63) >> 6;
+          const unsigned long long tm = tmask[tile];
+          for(int u = wave; u < K * nch; u += nw) {
+            const int k = u / nch, c = u % nch;
+            if(!(tm & (1ull | (2ull << k)))) continue;  // neither the orbit nor the trial reaches the tile
+            long long *G = grids + (size_t)(k + 2) * TILE_CELLS;
+            double acc = 0;
+            const int jn = c * 64 + lane;
+            if(jn < nmark) {
+              const int nd = select_bit(bits, jn), nz = nd / 16, nr = nd % 16;
+              const int c0 = 2 * nz * TILE_RS + 2 * nr;
+              const int cq[4] = {c0, c0 + 1, c0 + TILE_RS, c0 + TILE_RS + 1};
+              long long s = 0;
+#pragma unroll
+              for(int cc = 0; cc < 4; cc++) {
+                const long long d = G[cq[cc]] - O[cq[cc]];
+                G[cq[cc]] = 0;
+                s += d;
+                if(d != 0) acc += term_at(tbw[tile_node(0, 2 * nz + (cc >> 1), 2 * nr + (cc & 1))], (double)d * unit);
+              }
+              if(s != 0) {
+                acc += term_at(tbw[tile_node(1, nz, nr)], (double)s * unit);
+                atomicAdd((unsigned long long *)&acc2[((size_t)cur * K + k) * 64 + (nz >> 1) * 8 + (nr >> 1)],
+                          (unsigned long long)s);
+              }
+            }
+            acc = wave_sum(coef_of(f) * acc);
+            if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
+          }
+          if(A.prof && tid == 0) atomicAdd(&A.prof[12], wall_clock64() - tpf);
+        }
+        lds_barrier();
+        if(A.prof && tid == 0) atomicAdd(&A.prof[10], wall_clock64() - tb);
+      }
+      if(nact == 0 && tid < 16) tbits[tid] = 0;
+      // 3. the levels above the tiles, one wave per trial
+      for(int k = wave; k < K; k += nw) {
+        long long *tk = topst + (size_t)k * nst;
+        double acc = 0;
+        for(int l = lt; l >= 0; l--) {
+          const int n = 1 << l;
+          for(int q = lane; q < n * n; q += 64) {
+            const int i = q >> l, jn = q & (n - 1);
+            long long s;
+            if(l == lt) {
+              s = tk[GALIC_STACKOFFSET(l, i, jn)];
+            } else {
+              s = tk[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * jn)] + tk[GALIC_STACKOFFSET(l + 1, 2 * i, 2 * jn + 1)] +
+                  tk[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * jn)] + tk[GALIC_STACKOFFSET(l + 1, 2 * i + 1, 2 * jn + 1)];
+              tk[GALIC_STACKOFFSET(l, i, jn)] = s;
+            }
+            if(s != 0) acc += term_at(bwf[GALIC_STACKOFFSET(l, i, jn)], (double)s * unit);
+          }
+          // the next level reads what other lanes of this wave wrote
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        acc = wave_sum(coef_of(f) * acc);
+        if(lane == 0) part[k * MERIT_MAX_WAVES + wave] += acc;
+      }
+      __syncthreads();
+    }
+    // 4. the choice and the deltas of the taken trial, exact in fixed point
+    if(A.prof && tid == 0) {
+      tp2 = wall_clock64();
+      atomicAdd(&A.prof[1], tp2 - tp1);
+    }
+    // the total merit change of each trial, the best improving one taken (the instantaneous
+    // terms from k_trial_inst; the write-back follows in k_apply_best)
+    for(int k = tid; k < K; k += nt) {
+      const size_t q = (size_t)ti * K + k;
+      const double *fit = A.fit + (size_t)t * 16;
+      double orbit_terms = 0;
+      for(int w = 0; w < MERIT_MAX_WAVES; w++) orbit_terms += part[k * MERIT_MAX_WAVES + w];
+      s_dF[k] = orbit_terms + fit[5] / fit[6] * A.vinst[q];
+      A.dFk[q] = s_dF[k];
+    }
+    __syncthreads();
+    if(tid == 0) {
+      int bk = -1;
+      double bv = 0;
+      for(int k = 0; k < K; k++)
+        if(s_dF[k] < bv) {
+          bv = s_dF[k];
+          bk = k;
+        }
+      s_best = bk;
+      A.accept[ti] = bk + 1;
+      A.dF[ti] = (bk >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
+    }
+    __syncthreads();
+    const int best = s_best;
+    if(A.prof && tid == 0) {
+      tp1 = wall_clock64();
+      atomicAdd(&A.prof[2], tp1 - tp2);
+      if(best >= 0) atomicAdd(&A.prof[8], 1ull);
+    }
+    if(best >= 0) {
+      long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
+      const unsigned int ob = (unsigned int)best + 1;
+      long long *G = grids;  // O's first grid, zero here
+      for(int f = 0; f < kFields; f++) {
+        const float sc = A.log_scale[f];
+        for(int ai = 0; ai < nact; ai++) {
+          const int tile = tact[ai], tz = tile / ts, tr = tile % ts;
+          if(!(tmask[tile] & (1ull | (1ull << ob)))) continue;
+          for(int e = toff[tile] + tid; e < toff[tile + 1]; e += nt) {
+            const unsigned int en = qo[e], o = en >> 24;
+            if(o != 0 && o != ob) continue;
+            const int quad = (int)(en & 0xffffffu);
+            const float4 x = val[f * cap + e];
+            tile_add((unsigned long long *)G, (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x,
+                     o == 0 ? -sc : sc);
+          }
+          __syncthreads();
+          for(int q = tid; q < TILE * TILE; q += nt) {
+            const int lc = (q / TILE) * TILE_RS + q % TILE;
+            const long long d = G[lc];
+            if(d) {
+              G[lc] = 0;
+              atomic_q(&lad[(size_t)f * ngrid + (size_t)(tz * TILE + q / TILE) * nb + tr * TILE + q % TILE],
+                       (double)d * A.log_to_loc[f]);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+    if(A.prof && tid == 0) atomicAdd(&A.prof[3], wall_clock64() - tp1);
+  }
+}
+
+template __global__ void k_eval_sparse32<0>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_sparse32<9>(DevModel M, EvalArgs32 A);
+
+// ------------------------------------------------- initial pass from logs --
+
+// The initial response pass from response logs: every record of orbit o is added to the
+// fixed-point Loc grids of the orbit's type with the int64 values k_orbit32 adds for each
+// flushed run (qscale[f] times the FP32 record value, rounded), so the grids are the same
+// bits.  One workgroup per orbit at a time, a thread per record.
+__global__ void __launch_bounds__(256) k_log_to_loc(int n, const int *otype, const int *log_blocks, const int *log_n,
+                                                    int log_M, const int *log_q, const float4 *log_v, int L,
+                                                    QScale qs, long long *locq) {
+  const int nb = 1 << L, ngrid = nb * nb;
+  for(int o = blockIdx.x; o < n; o += gridDim.x) {
+    const int m = log_n[o];
+    long long *lq = locq + (size_t)otype[o] * kFields * ngrid;
+    for(int r = threadIdx.x; r < m; r += blockDim.x) {
+      const int blk = log_blocks[(size_t)o * log_M + r / LOG_B], jj = r & (LOG_B - 1);
+      const int quad = log_q[(size_t)blk * LOG_B + jj];
+      const int c[4] = {quad, quad + 1, quad + nb, quad + nb + 1};
+#pragma unroll
+      for(int f = 0; f < kFields; f++) {
+        const float4 x = log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+        const float v[4] = {x.x, x.y, x.z, x.w};
+        long long *s = lq + (size_t)f * ngrid;
+#pragma unroll
+        for(int k = 0; k < 4; k++)
+          if(v[k] != 0.f) atomic_q(&s[c[k]], qs.q[f] * (double)v[k]);
+      }
+    }
+  }
+}
 
 // ---------------------------------------------------- fixed point commit --
 

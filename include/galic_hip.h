@@ -212,10 +212,17 @@ int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk);
 int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_per_cu);
 
 /* Form of the FP32 merit kernel: 0 automatic (LDS grids of the whole response grid where
- * they fit, otherwise LDS tiles of 32 x 32 cells: the default above 128^2 cells), 1 tiles at
- * every level, 2 never tiles (above 128^2 cells: two int64 grids per workgroup in HBM).  All
- * forms give the same decisions and bit-identical grids; merit sums agree to FP64 rounding. */
+ * they fit, otherwise LDS tiles of 32 x 32 cells, else two int64 grids per workgroup in HBM),
+ * 1 dense tiles at every level, 2 never tiles, 3 tiles restricted to the 2x2 nodes the logs
+ * touch (where their LDS fits: K up to 8 on 512^2 grids).  All forms give the same decisions
+ * and bit-identical grids; merit sums agree to FP64 rounding. */
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form);
+
+/* Timing aid of the sparse tile merit: on != 0 starts (or restarts) summing, per phase of
+ * each particle, the wall-clock ticks (100 MHz) of the kernel's thread 0; out16 (optional)
+ * first receives the sums so far: [0] tile lists, [1] fields and tiles, [2] choice, [3]
+ * deltas of the taken trial, [5] tiles visited, [6] list entries, [7] particles, [8] taken. */
+int galic_fp32_merit_profile(galic_ctx *ctx, int on, unsigned long long *out16);
 
 /* Sets the FP32 orbit launch by hand (variant 0, 1..32 workgroups per CU). */
 int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu);

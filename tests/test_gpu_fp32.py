@@ -383,15 +383,15 @@ def assert_same_run(a, b):
     assert (a["acc"] > 0).mean() > 0.2
 
 
-@pytest.mark.parametrize("level", [6, 7])
-def test_fp32_tile_merit_equals_lds_merit(level):
+@pytest.mark.parametrize("level,form", [(6, 1), (7, 1), (6, 3), (7, 3)])
+def test_fp32_tile_merit_equals_lds_merit(level, form):
     """The tile form of the merit (the records of each log listed by 32 x 32-cell tile, each
     tile summed in LDS) against the whole-grid LDS kernels on 64^2 and 128^2 grids: the same
     decisions, bit-identical grids and velocities, per-trial merit changes equal up to the
     order of the FP64 sums."""
-    print(f"[gpu] fp32 tile merit vs LDS merit, level {level}", flush=True)
+    print(f"[gpu] fp32 tile merit (form {form}) vs LDS merit, level {level}", flush=True)
     wl = composite_workload(n=2500, level=level)
-    tiles, lds = run_forms(wl, (1, 0), K=6, n1=1000, rep=500, seed=13)
+    tiles, lds = run_forms(wl, (form, 0), K=6, n1=1000, rep=500, seed=13)
     assert_same_run(tiles, lds)
 
 
@@ -401,8 +401,9 @@ def test_fp32_tile_merit_equals_hbm_form_level9():
     workgroup in HBM."""
     print("[gpu] fp32 tile merit vs HBM form, level 9", flush=True)
     wl = model.make_workload(model.config("composite-2e7", max_steps=2000000), n=1500, target_samples=200000)
-    tiles, hbm = run_forms(wl, (1, 2), K=4, n1=600, rep=300, seed=17)
+    tiles, hbm, sparse = run_forms(wl, (1, 2, 3), K=4, n1=600, rep=300, seed=17)
     assert_same_run(tiles, hbm)
+    assert_same_run(sparse, hbm)
 
 
 def test_fp32_level7_many_trials_run():
