@@ -282,7 +282,7 @@ def control_leg(n, rep, reps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="timed steps (reps blocks); 8 steps make one iteration")
+    ap.add_argument("--steps", type=int, default=12, help="timed steps (blocks of reps)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=20000000)
     ap.add_argument("--k", type=int, default=8)
@@ -290,11 +290,15 @@ def main():
     ap.add_argument("--fresh", type=int, default=1,
                     help="of the K trials, how many are fresh draws from optimize()'s distribution")
     ap.add_argument("--rep", type=int, default=10000, help="trials per rep (both paths)")
-    ap.add_argument("--steps-per-iteration", type=int, default=8)
+    ap.add_argument("--steps-per-iteration", type=int, default=80,
+                    help="steps that make one iteration over all particles (the table fill opens each)")
     ap.add_argument("--tree-n", type=int, default=10000000, help="sources of the table fill")
     ap.add_argument("--target-points", type=int, default=100000000, help="points per component for the targets")
-    ap.add_argument("--fp64-reps", type=int, default=3, help="reps of the plain FP64 path timed on the full model")
-    ap.add_argument("--chi2-n", type=int, default=200000, help="particles of the chi^2 comparison (0: skip)")
+    ap.add_argument("--fp64-reps", type=int, default=3, help="reps of the plain FP64 path timed (0: skip)")
+    ap.add_argument("--fp64-n", type=int, default=2000000,
+                    help="particles of the model realisation the plain FP64 reps run on (a rep's cost does not "
+                         "depend on N: 1e4 trials against grids of the same size)")
+    ap.add_argument("--chi2-n", type=int, default=100000, help="particles of the chi^2 comparison (0: skip)")
     ap.add_argument("--chi2-iters", type=int, default=2, help="plain FP64 iterations of the chi^2 comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget in seconds (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=3000, help="particles of the CPU baseline's realisation")
@@ -405,8 +409,15 @@ def main():
     g.close()
 
     # ------------------------------------------------------ FP64 plain path
+    n_total = w.n
+    split = [int(w.params.NType[t]) for t in (1, 2, 3)]
     fp64 = None
     if args.fp64_reps > 0:
+        if args.fp64_n != w.n:
+            del w
+            cfg, w = full_model(args.fp64_n, args.tree_n)
+            perm = model.permutation(w, 1)
+            targets = None
         g64, _ = setup(w, cfg, args.target_points, targets)
         t0 = time.perf_counter()
         g64.calc_all_response_fields()
@@ -424,10 +435,9 @@ def main():
         tm64 = g64.timings()
         g64.close()
         fp64 = dict(orbits_per_s=2 * ntr / wall64, ms_per_rep=wall64 / args.fp64_reps * 1e3, reps=args.fp64_reps,
-                    steps_per_s=tm64["steps"] / wall64, orbit_kernel_ms=tm64["orbit_ms"], initial_pass_s=init64)
+                    steps_per_s=tm64["steps"] / wall64, orbit_kernel_ms=tm64["orbit_ms"], initial_pass_s=init64,
+                    n_particles=w.n, method="one fresh trial per particle (main.c:383-435), reps of 1e4")
         log(f"[bench] fp64: {fp64['orbits_per_s']:.4g} orbits/s over {args.fp64_reps} reps, initial pass {init64:.1f}s")
-    n_total = w.n
-    split = [int(w.params.NType[t]) for t in (1, 2, 3)]
     del w
 
     # --------------------------------------------- chi^2 of both methods

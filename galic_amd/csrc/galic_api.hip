@@ -543,6 +543,14 @@ int fp32_initial_logs(galic::Ctx &c, int n) {
   }
   galic::QScale qs;
   for(int f = 0; f < galic::kFields; f++) qs.q[f] = c.fp32_scale[f];
+  // copies of the fixed-point Loc grids (zero), summed into locA_q at the end
+  const size_t na = (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid;
+  long long *copies = nullptr;
+  struct FreeCopies {
+    long long *&p;
+    ~FreeCopies() { dfree(p); }
+  } free_copies{copies};
+  CHK(dalloc(&copies, (size_t)galic::kLocCopies * na));
   int ncu = 256;
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
   int a = 0;
@@ -566,11 +574,16 @@ int fp32_initial_logs(galic::Ctx &c, int n) {
     CHK(lr);
     hipLaunchKernelGGL(galic::k_log_to_loc, dim3(std::min(m, ncu * 16)), dim3(256), 0, c.stream, m,
                        (const int *)(c.o_type + a), (const int *)c.log_blocks, (const int *)c.log_n, c.log_M,
-                       (const int *)c.log_q, (const float4 *)c.log_v, c.L, qs, c.locA_q);
-    HIPCHK(hipGetLastError());
+                       (const int *)c.log_q, (const float4 *)c.log_v, c.L, qs, copies, galic::kLocCopies, na);
+    if(hipGetLastError() != hipSuccess) return set_err(GALIC_ERR_HIP, "k_log_to_loc launch failed");
     a += m;
     first = false;
   }
+  hipLaunchKernelGGL(galic::k_sum_copies, dim3(nblocks(na, 256)), dim3(256), 0, c.stream, c.locA_q, copies,
+                     galic::kLocCopies, na);
+  const hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(c.stream);
+  if(e != hipSuccess || e2 != hipSuccess) return set_err(GALIC_ERR_HIP, "summing the Loc copies failed");
   return GALIC_OK;
 }
 

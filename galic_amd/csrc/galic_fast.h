@@ -166,7 +166,10 @@ struct QScale {
   double q[kFields];
 };
 __global__ void k_log_to_loc(int n, const int *otype, const int *log_blocks, const int *log_n, int log_M,
-                             const int *log_q, const float4 *log_v, int L, QScale qs, long long *locq);
+                             const int *log_q, const float4 *log_v, int L, QScale qs, long long *locq, int ncopy,
+                             size_t copy_stride);
+__global__ void k_sum_copies(long long *dst, long long *src, int ncopy, size_t n);
+constexpr int kLocCopies = 16;  // copies of the Loc grids the initial pass adds into
 __global__ void k_log_totals(int n, int K, const int *slot_pos, const int *log_n, unsigned int *maxsum);
 
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,

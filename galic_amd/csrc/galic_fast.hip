@@ -2146,13 +2146,17 @@ template __global__ void k_eval_sparse32<9>(DevModel M, EvalArgs32 A);
 // fixed-point Loc grids of the orbit's type with the int64 values k_orbit32 adds for each
 // flushed run (qscale[f] times the FP32 record value, rounded), so the grids are the same
 // bits.  One workgroup per orbit at a time, a thread per record.
+// The workgroups add into ncopy copies of the grids (copy = workgroup mod ncopy, copy_stride
+// apart), so that the orbits crossing the same central cells contend less; k_sum_copies adds
+// the copies up (integer sums: the same bits in any order).
 __global__ void __launch_bounds__(256) k_log_to_loc(int n, const int *otype, const int *log_blocks, const int *log_n,
                                                     int log_M, const int *log_q, const float4 *log_v, int L,
-                                                    QScale qs, long long *locq) {
+                                                    QScale qs, long long *locq, int ncopy, size_t copy_stride) {
   const int nb = 1 << L, ngrid = nb * nb;
+  long long *base = locq + (size_t)(blockIdx.x % ncopy) * copy_stride;
   for(int o = blockIdx.x; o < n; o += gridDim.x) {
     const int m = log_n[o];
-    long long *lq = locq + (size_t)otype[o] * kFields * ngrid;
+    long long *lq = base + (size_t)otype[o] * kFields * ngrid;
     for(int r = threadIdx.x; r < m; r += blockDim.x) {
       const int blk = log_blocks[(size_t)o * log_M + r / LOG_B], jj = r & (LOG_B - 1);
       const int quad = log_q[(size_t)blk * LOG_B + jj];
@@ -2168,6 +2172,19 @@ __global__ void __launch_bounds__(256) k_log_to_loc(int n, const int *otype, con
       }
     }
   }
+}
+
+// dst[i] += the sum over the ncopy copies of src (n values each, copies n apart); the copies
+// are cleared.
+__global__ void k_sum_copies(long long *dst, long long *src, int ncopy, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  long long v = 0;
+  for(int c = 0; c < ncopy; c++) {
+    v += src[(size_t)c * n + i];
+    src[(size_t)c * n + i] = 0;
+  }
+  dst[i] += v;
 }
 
 // ---------------------------------------------------- fixed point commit --
