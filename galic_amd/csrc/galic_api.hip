@@ -467,11 +467,15 @@ int launch_orbit32_log(galic::Ctx &c, galic::OrbitArgs32 A) {
     }
     if(A.n > c.log_n_len) {
       dfree(c.log_n);
+      dfree(c.log_tot);
+      c.log_n_len = 0;
       CHK(dalloc(&c.log_n, (size_t)A.n));
+      CHK(dalloc(&c.log_tot, (size_t)A.n * galic::kFields));
       c.log_n_len = A.n;
     }
     A.log_blocks = c.log_blocks;
     A.log_n = c.log_n;
+    A.log_tot = c.log_tot;
     A.log_M = c.log_M;
     A.log_q = c.log_q = (int *)c.slots;
     A.log_v = c.log_v = (float4 *)((int *)c.slots + cap * galic::LOG_B);
@@ -832,7 +836,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.o_steps); dfree(c.o_bbox);
   dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
   dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv); dfree(c.psteps);
-  dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_next);
+  dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_tot); dfree(c.log_next);
   for(int t = 0; t < GALIC_NTYPES; t++) dfree(c.disp_tab[t]);
   dfree(c.cs_scratch);
   dfree(c.g32_scratch);
@@ -1757,6 +1761,11 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   // tiles restricted to the nodes the logs touch, where their LDS fits (form 3; the dense
   // tiles measured faster so far and are the automatic choice)
   auto use_sparse = [&]() { return use_tiles() && c.merit_form == 3 && sparse_lds() <= lds_max; };
+  // 16x16 tiles in the particle's own int32 fixed point (k_eval_t16): the automatic choice
+  // where it applies (form 4 asks for it)
+  const size_t t16_lds = galic::t16_lds_bytes(std::max(c.L, 4), K) + 1024;
+  const bool t16 = (c.merit_form == 0 || c.merit_form == 4) && c.L >= 4 && c.L <= galic::T16_MAX_LEVEL &&
+                   K <= galic::T16_MAX_K && t16_lds <= lds_max;
   const bool gmem = c.L > 7;
   const int kGBlocks = 512;
   auto eval_lds_gmem = [&]() {
@@ -1764,7 +1773,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
            2 * sizeof(unsigned int) * (((size_t)c.Ngrid / 4 + 31) / 32) + sizeof(int) * (size_t)(K + 1) * c.log_M +
            1024;
   };
-  if(gmem && !use_tiles()) {
+  if(gmem && !t16 && !use_tiles()) {
     const size_t need = (size_t)kGBlocks * 2 * c.Ngrid;
     if(c.g32_scratch_len < need) {
       dfree(c.g32_scratch);
@@ -1837,15 +1846,18 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     }
     cap = group_cap();
     // (the block tables may have grown with this launch: the form is chosen after it)
-    const bool tiles = use_tiles();
-    const bool sparse = use_sparse();
+    const bool tiles = !t16 && use_tiles();
+    const bool sparse = !t16 && use_sparse();
     // without tiles, the HBM-scratch form above 128^2 cells and wherever one LDS grid does not fit
-    const bool gmem_now = !tiles && (gmem || eval_lds(1) > lds_max);
+    const bool gmem_now = !t16 && !tiles && (gmem || eval_lds(1) > lds_max);
     const bool use_o = gmem_now || eval_lds(2) <= lds_max;
     const bool bwl = tiles && tile_lds_bw(true) <= lds_max;
-    const size_t shmem =
-        (sparse ? sparse_lds() : tiles ? tile_lds_bw(bwl) : gmem_now ? eval_lds_gmem() : eval_lds(use_o ? 2 : 1)) -
-        1024;
+    const size_t shmem = (t16      ? t16_lds
+                          : sparse ? sparse_lds()
+                          : tiles  ? tile_lds_bw(bwl)
+                          : gmem_now ? eval_lds_gmem()
+                                     : eval_lds(use_o ? 2 : 1)) -
+                         1024;
     if(shmem + 1024 > lds_max)
       return set_err(GALIC_ERR_ARG, "the merit kernel needs %zu bytes of LDS, the device gives %zu", shmem, lds_max);
     if(gmem_now && c.g32_scratch_len < (size_t)kGBlocks * 2 * c.Ngrid) {
@@ -1856,16 +1868,29 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
     // tile form: workgroups resident at once (each takes trials b, b + grid, ...), and list
     // room for the largest particle of the group (a record is listed in at most four tiles)
     int tile_grid = 0, tile_cap = 0;
-    if(tiles) {
+    const int t16_kr = K <= 8 ? 8 : 16;
+    const void *t16_kern =
+        c.L == 9 ? (t16_kr == 8 ? (const void *)galic::k_eval_t16<9, 8> : (const void *)galic::k_eval_t16<9, 16>)
+                 : (t16_kr == 8 ? (const void *)galic::k_eval_t16<0, 8> : (const void *)galic::k_eval_t16<0, 16>);
+    if(tiles || t16) {
       int per_cu = 0, ncu = 0;
-      auto kern = sparse ? (c.L == 9 ? (const void *)galic::k_eval_sparse32<9> : (const void *)galic::k_eval_sparse32<0>)
+      auto kern = t16 ? t16_kern
+                  : sparse ? (c.L == 9 ? (const void *)galic::k_eval_sparse32<9> : (const void *)galic::k_eval_sparse32<0>)
                   : c.L == 9 ? (bwl ? (const void *)galic::k_eval_tile32<9, true> : (const void *)galic::k_eval_tile32<9, false>)
                              : (bwl ? (const void *)galic::k_eval_tile32<0, true> : (const void *)galic::k_eval_tile32<0, false>);
-      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 1024, shmem));
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, t16 ? 64 * galic::kFields : 1024, shmem));
       HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
       if(per_cu < 1) return set_err(GALIC_ERR_ARG, "the tile merit kernel does not fit on a CU (%zu bytes of LDS)", shmem);
+      if(const char *ov = getenv("GALIC_MERIT_WG_PER_CU")) per_cu = std::max(1, atoi(ov));  // tuning override
       tile_grid = std::min(ng, per_cu * ncu);
-      if(!c.tile_aux) CHK(dalloc(&c.tile_aux, 2));
+      if(getenv("GALIC_VERBOSE")) {
+        hipFuncAttributes fa;
+        if(hipFuncGetAttributes(&fa, kern) == hipSuccess)
+          fprintf(stderr, "[galic] merit kernel: %d workgroups per CU x %d CUs, %zu B dynamic LDS, %d VGPRs, "
+                          "%zu B static LDS, %zu B scratch per lane\n",
+                  per_cu, ncu, shmem, fa.numRegs, fa.sharedSizeBytes, fa.localSizeBytes);
+      }
+      if(!c.tile_aux) CHK(dalloc(&c.tile_aux, 4));
       HIPCHK(hipMemsetAsync(c.tile_aux, 0, 2 * sizeof(unsigned int), c.stream));
       hipLaunchKernelGGL(galic::k_log_totals, dim3(nblocks(ng, 256)), dim3(256), 0, c.stream, ng, K,
                          (const int *)nullptr, (const int *)c.log_n, c.tile_aux);
@@ -1877,8 +1902,10 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       if(want > 0x7fffffffu) return set_err(GALIC_ERR_NOMEM, "a particle's response logs hold %u records", mx);
       tile_cap = (int)((want + 3) & ~(size_t)3);
       // the sparse form keeps a copy of each listed record (a word and five float4, 21 words);
-      // the dense form one word per entry
-      const size_t words = (size_t)tile_grid * tile_cap * (sparse ? 21 : 1);
+      // the dense form one word per entry, the 16x16 form two (the arena record index and the log)
+      if(t16 && (size_t)c.slot_bytes / log_block_bytes() * galic::LOG_B > 0xffffffffull)
+        return set_err(GALIC_ERR_ARG, "the log arena has more records than 32-bit list entries address");
+      const size_t words = (size_t)tile_grid * tile_cap * (t16 ? 2 : sparse ? 21 : 1);
       if(c.tile_list_len < words) {
         dfree(c.tile_list);
         c.tile_list_len = 0;
@@ -1941,7 +1968,37 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.iscale = c.fp32_iscale;
       E.gscratch = c.g32_scratch;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
-      if(tiles) {
+      if(t16) {
+        E.nblk = b - a;
+        E.tlist2 = (uint2 *)c.tile_list;
+        E.tlist_cap = tile_cap;
+        E.tile_err = c.tile_aux + 1;
+        E.log_tot = c.log_tot;
+        E.vinst = c.t_vinst;
+        E.prof = c.merit_prof;
+        if(galic::sort_trials_by_records(b - a, K, E.slot_pos, c.log_n, c.t_key, c.t_key2, c.t_ord, c.t_ord2,
+                                         &c.sort_tmp, &c.sort_tmp_bytes, c.stream) != 0)
+          return set_err(GALIC_ERR_HIP, "sorting the trials by record count failed");
+        E.work_order = c.t_ord2;
+        E.work_next = (int *)c.tile_aux + 2;
+        HIPCHK(hipMemsetAsync(E.work_next, 0, sizeof(int), c.stream));
+        const int grid = std::min(b - a, tile_grid);
+        // instantaneous terms before, the write-back of the choice after the merit kernel
+        hipLaunchKernelGGL(galic::k_trial_inst, dim3(nblocks((size_t)(b - a) * K, 256)), dim3(256), 0, c.stream, c.M, E,
+                           c.t_vinst);
+        const dim3 tb(64 * galic::kFields);
+        if(c.L == 9 && t16_kr == 8)
+          hipLaunchKernelGGL((galic::k_eval_t16<9, 8>), dim3(grid), tb, shmem, c.stream, c.M, E);
+        else if(c.L == 9)
+          hipLaunchKernelGGL((galic::k_eval_t16<9, 16>), dim3(grid), tb, shmem, c.stream, c.M, E);
+        else if(t16_kr == 8)
+          hipLaunchKernelGGL((galic::k_eval_t16<0, 8>), dim3(grid), tb, shmem, c.stream, c.M, E);
+        else
+          hipLaunchKernelGGL((galic::k_eval_t16<0, 16>), dim3(grid), tb, shmem, c.stream, c.M, E);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(galic::k_apply_best, dim3(nblocks(b - a, 256)), dim3(256), 0, c.stream, c.M, E);
+        HIPCHK(hipGetLastError());
+      } else if(tiles) {
         E.nblk = b - a;
         E.tlist = c.tile_list;
         E.tlist_cap = tile_cap;
@@ -2002,7 +2059,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
       eval_ms += ms;
       c.acc_eval_ms += ms;
-      if(tiles) {
+      if(tiles || t16) {
         unsigned int terr = 0;
         HIPCHK(hipMemcpy(&terr, c.tile_aux + 1, sizeof(terr), hipMemcpyDeviceToHost));
         if(terr) return set_err(GALIC_ERR_STATE, "the tile lists of a particle outgrew their room");
@@ -2038,7 +2095,7 @@ int galic_fp32_merit_profile(galic_ctx *ctx, int on, unsigned long long *out16) 
 }
 
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form) {
-  if(!ctx || form < 0 || form > 3) return set_err(GALIC_ERR_ARG, "merit form must be 0, 1, 2 or 3");
+  if(!ctx || form < 0 || form > 4) return set_err(GALIC_ERR_ARG, "merit form must be 0, 1, 2, 3 or 4");
   ctx->c.merit_form = form;
   return GALIC_OK;
 }

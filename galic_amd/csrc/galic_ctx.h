@@ -145,6 +145,7 @@ struct Ctx {
   int *log_blocks = nullptr;     // [orbit][log_M]
   size_t log_blocks_len = 0;
   int *log_n = nullptr;          // [orbit]
+  float *log_tot = nullptr;      // [orbit][5] sums of each orbit's record values
   int log_n_len = 0;
   int log_M = 64;                // block-table entries per orbit (doubled when an orbit needs more)
   unsigned int *log_next = nullptr;
@@ -175,7 +176,7 @@ struct Ctx {
   double *lookup_buf = nullptr;  // galic_field_at: staging of positions, accelerations, potentials
   size_t lookup_len = 0;         // points it holds   // [0] largest records of a particle's logs, [1] list overflow flag
   unsigned long long *merit_prof = nullptr;  // [16] phase times of the sparse tile merit (galic_fp32_merit_profile)
-  int merit_form = 0;                 // 0 automatic, 1 tiles at every level, 2 LDS grids / HBM scratch (no tiles)
+  int merit_form = 0;                 // 0 automatic, 1 32x32 tiles at every level, 2 LDS grids / HBM scratch (no tiles), 3 sparse 32x32 tiles, 4 16x16 tiles
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -42,6 +42,7 @@ struct OrbitArgs32 {
   // (quad index, 4 cells x 5 fields of fac * weight in FP32) to the orbit's blocks
   int *log_blocks;       // [orbit][log_M] arena block of each log block
   int *log_n;            // [orbit] records written (-1: the log overflowed)
+  float *log_tot;        // [orbit][5] sum of the orbit's record values per field
   int log_M;             // block-table entries per orbit
   int *log_q;            // arena: [block][LOG_B] quad index
   float4 *log_v;         // arena: [block][5][LOG_B] values
@@ -76,6 +77,10 @@ struct EvalArgs32 {
   unsigned int *tlist;  // tile form: per workgroup, tlist_cap entries of the records' tile lists
   int tlist_cap;
   unsigned int *tile_err;  // tile form: set when a particle's lists exceed tlist_cap
+  uint2 *tlist2;           // 16x16-tile form: per workgroup, tlist_cap entries (record index, log)
+  const float *log_tot;    // [orbit][5] sums of each log's values (k_orbit32_log)
+  const int *work_order;   // 16x16-tile form: trials in the order workgroups take them
+  int *work_next;          // ... from this counter (zero at launch)
   unsigned long long *prof;  // sparse tile form, optional: time per phase and sizes, summed over particles
   int tbl_lds;               // sparse tile form: the block tables staged in LDS (else read from log_blocks)
   const double *vinst;       // sparse tile form: instantaneous merit change of each trial (k_trial_inst)
@@ -119,6 +124,8 @@ __global__ void k_energy32(DevModel M, int n, const double *ox, const double *oy
                            const double *ovy, const double *ovz, const float *fin, double *e0, double *e1);
 int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigned int *key, unsigned int *key2,
                          int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
+int sort_trials_by_records(int n, int K, const int *slot_pos, const int *log_n, unsigned int *key, unsigned int *key2,
+                           int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
 __global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
 template <bool USE_O, bool GMEM, int LF>
 __global__ void k_eval_log32(DevModel M, EvalArgs32 A);
@@ -160,6 +167,25 @@ __host__ __device__ inline size_t sparse_lds_bytes(int L, int K, int log_M, bool
 }
 template <int LF>
 __global__ void k_eval_sparse32(DevModel M, EvalArgs32 A);
+// 16x16-tile form of the FP32 merit (k_eval_t16): int32 tile grids in the particle's own fixed
+// point, one wave per field, all trials of a tile per lane; at most T16_MAX_K trials.
+constexpr int T16 = 16;
+constexpr int T16_MAX_K = 16;
+constexpr int T16_MAX_LEVEL = 10;  // Morton codes of up to 64 x 64 tiles; open ancestors of up to 8 levels
+// int32 words at the front of its LDS: the K+1 grids of a tile's five fields, or the five
+// int64 32 x 32-cell grids of the delta pass, whichever is larger.
+__host__ __device__ inline int t16_grid_words(int K) {
+  const int a = (K + 1) * kFields * T16 * T16, b = 2 * kFields * 1024;
+  return a > b ? a : b;
+}
+// LDS of k_eval_t16 beyond its static arrays: the grids, then tile counts, offsets, the active
+// list and the logs reaching each tile (4 * ntile + 1 words).
+__host__ __device__ inline size_t t16_lds_bytes(int L, int K) {
+  const size_t ntile = (size_t)1 << (2 * (L - 4));
+  return 4 * ((size_t)t16_grid_words(K) + 4 * ntile + 1);
+}
+template <int LF, int KR>
+__global__ void k_eval_t16(DevModel M, EvalArgs32 A);
 __global__ void k_trial_inst(DevModel M, EvalArgs32 A, double *vinst);
 __global__ void k_apply_best(DevModel M, EvalArgs32 A);
 struct QScale {

@@ -234,6 +234,7 @@ struct Log32 {
   bool held;          // rec holds a finished record not yet written
   unsigned int next;  // the lane's claimed blocks [next, end)
   unsigned int end;
+  float tot[kFields];  // sums of the orbit's record values per field (all values are >= 0)
   Run32 rec;
 };
 
@@ -259,6 +260,8 @@ __device__ __forceinline__ void log32_claim(Log32 &lg, int o, const OrbitArgs32 
 
 __device__ __forceinline__ void log32_flush(const Run32 &r, Log32 &lg, int o, const OrbitArgs32 &A) {
   if(r.quad < 0) return;
+#pragma unroll
+  for(int f = 0; f < kFields; f++) lg.tot[f] += (r.w[f][0] + r.w[f][1]) + (r.w[f][2] + r.w[f][3]);
   if(!lg.held) {
     lg.rec = r;
     lg.held = true;
@@ -359,7 +362,10 @@ __device__ __forceinline__ bool take_orbit32(const OrbitArgs32 &A, int nb, Lane3
     A.steps[o] = 0;
     A.bbox[o] = make_int4(nb, -1, nb, -1);
     A.status[o] = GALIC_ERR_ARG;
-    if(A.log_n) A.log_n[o] = 0;
+    if(A.log_n) {
+      A.log_n[o] = 0;
+      for(int f = 0; f < kFields; f++) A.log_tot[(size_t)o * kFields + f] = 0.f;
+    }
     orbit_stats_add(A.stats, 0, GALIC_ERR_ARG, o);
   }
 }
@@ -379,6 +385,8 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   lg.full = false;
   lg.held = false;
   lg.next = lg.end = 0;
+#pragma unroll
+  for(int f = 0; f < kFields; f++) lg.tot[f] = 0.f;
   auto flush = [&](const Run32 &r) {
     if(LOG)
       log32_flush(r, lg, L.o, A);
@@ -464,6 +472,11 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
         log32_finish(lg, o, A);
         A.log_n[o] = lg.full ? -1 : lg.n;
         if(lg.full) atomicAdd(&A.stats[3], 1ull);
+#pragma unroll
+        for(int f = 0; f < kFields; f++) {
+          A.log_tot[(size_t)o * kFields + f] = lg.tot[f];
+          lg.tot[f] = 0.f;
+        }
         lg.n = 0;
         lg.full = false;
       }
@@ -629,6 +642,20 @@ __global__ void k_store_steps(int n, const long long *osteps, int *psteps) {
   if(o < n) psteps[o] = (int)min(osteps[o], (long long)0x7fffffff);
 }
 
+// Records of each trial's K+1 logs (the merit kernel's work for it).
+__global__ void k_record_keys(int n, int K, const int *slot_pos, const int *log_n, unsigned int *key, int *ord) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if(b >= n) return;
+  const size_t o0 = (size_t)(slot_pos ? slot_pos[b] : b) * (K + 1);
+  unsigned int s = 0;
+  for(int o = 0; o <= K; o++) s += (unsigned int)max(log_n[o0 + o], 0);
+  key[b] = s;
+  ord[b] = b;
+}
+
+int sort_desc(int n, unsigned int *key, unsigned int *key2, int *ord, int *ord2, void **tmp, size_t *tmp_bytes,
+              hipStream_t stream);
+
 // Trials in order of decreasing expected orbit length (stable, so the order is
 // reproducible): chunks then hold orbits of similar length and the longest
 // start first.
@@ -637,6 +664,22 @@ int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigne
   if(ntrial <= 0) return 0;
   hipLaunchKernelGGL(k_step_keys, dim3((ntrial + 255) / 256), dim3(256), 0, stream, ntrial, tidx, psteps, key, ord);
   if(hipGetLastError() != hipSuccess) return -1;
+  return sort_desc(ntrial, key, key2, ord, ord2, tmp, tmp_bytes, stream);
+}
+
+// The trials of a merit launch in order of decreasing record count (ord2): workgroups that take
+// them in this order from a shared counter finish together (longest first).
+int sort_trials_by_records(int n, int K, const int *slot_pos, const int *log_n, unsigned int *key, unsigned int *key2,
+                           int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream) {
+  if(n <= 0) return 0;
+  hipLaunchKernelGGL(k_record_keys, dim3((n + 255) / 256), dim3(256), 0, stream, n, K, slot_pos, log_n, key, ord);
+  if(hipGetLastError() != hipSuccess) return -1;
+  return sort_desc(n, key, key2, ord, ord2, tmp, tmp_bytes, stream);
+}
+
+// (key, ord) pairs sorted by decreasing key into (key2, ord2); stable.
+int sort_desc(int ntrial, unsigned int *key, unsigned int *key2, int *ord, int *ord2, void **tmp, size_t *tmp_bytes,
+              hipStream_t stream) {
   size_t need = 0;
   if(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, key, key2, ord, ord2, ntrial, 0, 32, stream) !=
      hipSuccess)
@@ -2139,6 +2182,514 @@ __global__ void __launch_bounds__(1024) k_eval_sparse32(DevModel M, EvalArgs32 A
 
 template __global__ void k_eval_sparse32<0>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_sparse32<9>(DevModel M, EvalArgs32 A);
+
+// ------------------------------------------------ FP32 merit, 16x16 tiles --
+//
+// The merit change of each trial -- the terms of every node of the pyramid that D = G_k - O
+// reaches, as k_eval_log32 -- organised for large grids:
+//  * Fixed point per particle: a record value v of field f counts round(v * 2^kp[f]) units,
+//    kp[f] = 30 - e with the largest total of the particle's K+1 orbits (k_orbit32_log's
+//    log_tot) below 2^e.  Every sum of one orbit's values then stays below 2^30 (+ one unit
+//    per record of rounding), so tile grids and node sums are exact int32.
+//  * Tiles of 16 x 16 cells, taken in Morton order.  The records of the K+1 logs are listed by
+//    tile (arena record index, log index; a record whose 2x2 cells straddle a tile edge is
+//    listed in each tile it reaches).  A tile's K+1 grids of all five fields sit in LDS at once,
+//    node-major: the four cells of a 2x2 node are one 16-byte word.
+//  * Wave f evaluates field f.  Lane j owns the tile's 2x2 node j (lanes in Morton order, so
+//    groups of 4 and 16 lanes are the 4x4 and 8x8 nodes and the wave is the tile); its
+//    (residual, weight) pairs stay in registers for all trials, and for each trial the lane forms
+//    D on its four cells, the node sum, the quad and row sums (DPP) and the tile sum, and adds
+//    the terms of every node it leads into a per-trial FP64 accumulator.
+//  * Levels above the tiles: a tile's ancestor at level l is complete once the walk reaches a
+//    tile with another ancestor at l.  Lane k keeps trial k's partial sums of the open
+//    ancestors (a stack of one sum per level) and closes them level by level.
+// Then the choice (the instantaneous terms come from k_trial_inst; the write-back follows in
+// k_apply_best) and the deltas of the taken trial: exact int64 sums at the log scale on
+// blocks of 32 x 32 cells (four consecutive tiles), the same values as k_eval_tile32.
+
+__device__ __forceinline__ int spread6(int x) {
+  x &= 63;
+  x = (x | (x << 4)) & 0x0F0F;
+  x = (x | (x << 2)) & 0x3333;
+  x = (x | (x << 1)) & 0x5555;
+  return x;
+}
+__device__ __forceinline__ int compact6(int x) {
+  x &= 0x5555;
+  x = (x | (x >> 1)) & 0x3333;
+  x = (x | (x >> 2)) & 0x0F0F;
+  x = (x | (x >> 4)) & 0x00FF;
+  return x;
+}
+// Morton code of tile (tz, tr) (tr in the even bits, as morton_x of a lane) and back.
+__device__ __forceinline__ int t16_code(int tz, int tr) { return spread6(tr) | (spread6(tz) << 1); }
+__device__ __forceinline__ int t16_z(int m) { return compact6(m >> 1); }
+__device__ __forceinline__ int t16_r(int m) { return compact6(m); }
+// Word of tile-local cell (cz, cr) in a node-major 16 x 16 grid.
+__device__ __forceinline__ int t16_cell(int cz, int cr) {
+  return ((spread6(cr >> 1) | (spread6(cz >> 1) << 1)) << 2) | ((cz & 1) << 1) | (cr & 1);
+}
+
+// Tiles (Morton codes) reached by the 2x2 cells of quad.
+template <typename Visit>
+__device__ __forceinline__ void t16_visit(int quad, int L, Visit visit) {
+  const int iz = quad >> L, iR = quad & ((1 << L) - 1);
+  const int z0 = iz >> 4, z1 = (iz + 1) >> 4, r0 = iR >> 4, r1 = (iR + 1) >> 4;
+  visit(t16_code(z0, r0));
+  if(r1 != r0) visit(t16_code(z0, r1));
+  if(z1 != z0) {
+    visit(t16_code(z1, r0));
+    if(r1 != r0) visit(t16_code(z1, r1));
+  }
+}
+
+// acc + w (|b + d| - |b|): a node's term with b and d in the particle's units (d = 0 adds 0).
+__device__ __forceinline__ double t16_term(double b, double w, double d, double acc) {
+  return fma(w, fabs(b + d) - fabs(b), acc);
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+constexpr int T16_LU = 8;  // records a thread takes at once in the list passes
+#ifndef T16_PREFETCH_VALUES
+#define T16_PREFETCH_VALUES 0
+#endif
+
+// Record index of the q-th record of the particle's logs (orbits' records in turn; s_off
+// the exclusive prefix of the K+1 counts) and its log.
+__device__ __forceinline__ unsigned int t16_record(const EvalArgs32 &A, const int *s_off, int K, size_t o0, int q,
+                                                   int &o) {
+  o = 0;
+  while(o < K && q >= s_off[o + 1]) o++;
+  const int r = q - s_off[o];
+  const int blk = A.log_blocks[(o0 + o) * A.log_M + r / LOG_B];
+  return (unsigned int)blk * LOG_B + (unsigned int)(r & (LOG_B - 1));
+}
+
+template <int LF, int KR>
+__global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalArgs32 A) {
+  extern __shared__ int lds_i[];
+  const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63;
+  const int L = LF ? LF : M.DG_L, nb = 1 << L, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
+  const int lt = L - 4, ntile = 1 << (2 * lt);
+  const int gwords = t16_grid_words(K);
+  int *grids = lds_i;                                      // [K+1][5][256] int32; commit: [5][32*32] int64
+  int *tcnt = grids + gwords;                              // [ntile] counts, then cursors
+  int *toff = tcnt + ntile;                                // [ntile + 1]
+  int *tact = toff + ntile + 1;                            // [ntile] tiles with entries, Morton order
+  unsigned int *tmask = (unsigned int *)(tact + ntile);    // [ntile] logs reaching the tile
+  __shared__ double s_part[kFields][KR];
+  __shared__ double s_dF[KR];
+  __shared__ int s_off[KR + 2], s_kp[kFields], s_nact, s_best, s_item;
+  __shared__ float s_sc[kFields];
+  uint2 *list = A.tlist2 + (size_t)blockIdx.x * A.tlist_cap;
+  const int jx = morton_x(lane), jy = morton_y(lane);
+
+  for(int q = tid; q < gwords; q += nt) grids[q] = 0;
+  const unsigned long long t_wg0 = A.prof ? wall_clock64() : 0;
+  while(true) {
+    // the next trial from the launch's counter, longest first
+    __syncthreads();  // the previous particle's use of the shared arrays is over
+    if(tid == 0) s_item = atomicAdd(A.work_next, 1);
+    __syncthreads();
+    if(s_item >= A.nblk) {
+      if(A.prof && tid == 0) {
+        const unsigned long long tw = wall_clock64() - t_wg0;
+        atomicAdd(&A.prof[13], tw);
+        atomicMax(&A.prof[14], tw);
+        atomicAdd(&A.prof[15], 1ull);
+      }
+      break;
+    }
+    const int bid = A.work_order[s_item];
+    const int ti = A.trial_base + bid;
+    const int j = A.slot_pos ? A.slot_pos[bid] : bid;
+    const int p = A.tidx[ti];
+    const int t = A.P.type[p];
+    const size_t o0 = (size_t)j * (K + 1);
+    unsigned long long tp0 = 0, tp1 = 0, tpa = 0, tpb = 0;
+    if(A.prof && tid == 0) tp0 = wall_clock64();
+    if(tid == 0) {
+      int s = 0;
+      for(int o = 0; o <= K; o++) {
+        s_off[o] = s;
+        s += max(A.log_n[o0 + o], 0);
+      }
+      s_off[K + 1] = s;
+    }
+    if(tid < kFields) {
+      float mx = 0.f;
+      for(int o = 0; o <= K; o++) mx = fmaxf(mx, A.log_tot[(o0 + o) * kFields + tid]);
+      int e = 0;
+      (void)frexpf(mx, &e);  // mx < 2^e
+      const int kp = min(126, 30 - e);
+      s_kp[tid] = kp;
+      s_sc[tid] = ldexpf(1.0f, kp);
+    }
+    for(int q = tid; q < ntile; q += nt) {
+      tcnt[q] = 0;
+      tmask[q] = 0;
+    }
+    __syncthreads();
+    const int R = s_off[K + 1];
+    // 1. the tile lists: counts, offsets, entries.  A thread takes T16_LU records at a time,
+    // their block-table and quad loads issued together.
+    auto records = [&](auto visit_record) {
+      for(int q0 = tid; q0 < R; q0 += nt * T16_LU) {
+        unsigned int rec[T16_LU];
+        int ob[T16_LU], quad[T16_LU];
+#pragma unroll
+        for(int u = 0; u < T16_LU; u++) {
+          const int q = q0 + u * nt;
+          ob[u] = -1;
+          rec[u] = 0;
+          if(q < R) rec[u] = t16_record(A, s_off, K, o0, q, ob[u]);
+        }
+#pragma unroll
+        for(int u = 0; u < T16_LU; u++) quad[u] = ob[u] >= 0 ? A.log_q[rec[u]] : 0;
+#pragma unroll
+        for(int u = 0; u < T16_LU; u++)
+          if(ob[u] >= 0) visit_record(rec[u], ob[u], quad[u]);
+      }
+    };
+    records([&](unsigned int rec, int o, int quad) {
+      t16_visit(quad, L, [&](int m) {
+        atomicAdd(&tcnt[m], 1);
+        if(!(tmask[m] & (1u << o))) atomicOr(&tmask[m], 1u << o);
+      });
+    });
+    __syncthreads();
+    if(wave == 0) tile_scan(tcnt, toff, tact, &s_nact, ntile, lane);
+    __syncthreads();
+    if(toff[ntile] > A.tlist_cap) {
+      // cannot happen when the caller sizes tlist_cap from k_log_totals; reported, not evaluated
+      if(tid == 0) {
+        atomicOr(A.tile_err, 1u);
+        A.accept[ti] = 0;
+        A.dF[ti] = 0;
+      }
+      continue;
+    }
+    for(int q = tid; q < ntile; q += nt) tcnt[q] = 0;
+    __syncthreads();
+    records([&](unsigned int rec, int o, int quad) {
+      t16_visit(quad, L, [&](int m) { list[toff[m] + atomicAdd(&tcnt[m], 1)] = make_uint2(rec, (unsigned int)o); });
+    });
+    __syncthreads();
+    const int nact = s_nact;
+    if(A.prof && tid == 0) {
+      tp1 = wall_clock64();
+      atomicAdd(&A.prof[0], tp1 - tp0);
+      atomicAdd(&A.prof[5], (unsigned long long)nact);
+      atomicAdd(&A.prof[6], (unsigned long long)toff[ntile]);
+      atomicAdd(&A.prof[7], 1ull);
+    }
+
+    // 2. the tiles in Morton order
+    const int f = wave;
+    const double2 *bwf = A.merit_bw + ((size_t)t * kFields + f) * Nstack;
+    const int kp = s_kp[f];
+    double acc[KR];
+#pragma unroll
+    for(int k = 0; k < KR; k++) acc[k] = 0.0;
+    double accs = 0.0;  // lane k: trial k's terms of the tile nodes and above
+    // lane 4q + j: the 4x4-node terms of trials j, j + 4, ...; lane 16r + k: trial k's 8x8 terms
+    double acc2[(KR + 3) / 4], acc3 = 0.0;
+#pragma unroll
+    for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = 0.0;
+    constexpr int NSTK = (LF ? LF : T16_MAX_LEVEL) - 4;
+    int stk[NSTK > 0 ? NSTK : 1];  // lane k: trial k's sums of the open ancestors, level l < lt
+#pragma unroll
+    for(int l = 0; l < (NSTK > 0 ? NSTK : 1); l++) stk[l] = 0;
+    double2 pc[4], p1, p2, p3, p4, up = make_double2(0.0, 0.0);
+    auto load_pairs = [&](int m) {
+      const int gz = t16_z(m) * T16 + 2 * jy, gr = t16_r(m) * T16 + 2 * jx;
+      pc[0] = bwf[GALIC_STACKOFFSET(L, gz, gr)];
+      pc[1] = bwf[GALIC_STACKOFFSET(L, gz, gr + 1)];
+      pc[2] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr)];
+      pc[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
+      p1 = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
+      p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
+      p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
+      p4 = bwf[GALIC_STACKOFFSET(L - 4, gz >> 4, gr >> 4)];
+    };
+    // closes trial k's open ancestors of the tile prev at levels from lt - 1 down to stop
+    // (with stop = 0: all of them); up holds their pairs (lane l: level l)
+    auto close_levels = [&](int prev, int m, bool all) {
+      bool open = true;
+#pragma unroll
+      for(int l = NSTK - 1; l >= 0; l--) {
+        if(l < lt && open) {
+          const int sh = 2 * (lt - l);
+          if(!all && (prev >> sh) == (m >> sh)) {
+            open = false;
+          } else {
+            const double b = ldexp(__shfl(up.x, l), kp), w = __shfl(up.y, l);
+            accs = t16_term(b, w, (double)stk[l], accs);
+            if(l > 0) stk[l > 0 ? l - 1 : 0] += stk[l];
+            stk[l] = 0;
+          }
+        }
+      }
+    };
+    // Phase A's data is fetched ahead: during tile ai's pyramid each thread loads the values
+    // of its first entry of tile ai+1 (whose list entry came during tile ai-1) and its first
+    // list entry of tile ai+2, so a tile's adds seldom wait on memory.
+    auto fetch_entry = [&](int ai2) {
+      uint2 en = make_uint2(0u, 0xffffffffu);
+      if(ai2 < nact) {
+        const int m2 = tact[ai2], e = toff[m2] + tid;
+        if(e < toff[m2 + 1]) en = list[e];
+      }
+      return en;
+    };
+    struct Vals {
+      int quad;
+      float4 x[kFields];
+    };
+    auto fetch_values = [&](uint2 en) {
+      Vals v;
+      v.quad = 0;
+      if(en.y != 0xffffffffu) {
+        const unsigned int rec = en.x;
+        const int blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
+        v.quad = A.log_q[rec];
+#pragma unroll
+        for(int f2 = 0; f2 < kFields; f2++) v.x[f2] = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
+      }
+      return v;
+    };
+    // the cells of one entry that fall in tile (tz, tr) into its log's grids
+    auto add_entry = [&](int o, const Vals &v, int tz, int tr) {
+      const int lz = (v.quad >> L) - tz * T16, lr = (v.quad & (nb - 1)) - tr * T16;
+      int *g = grids + o * kFields * 256;
+#pragma unroll
+      for(int c = 0; c < 4; c++) {
+        const int cz = lz + (c >> 1), cr = lr + (c & 1);
+        if((unsigned)cz < (unsigned)T16 && (unsigned)cr < (unsigned)T16) {
+          const int w = t16_cell(cz, cr);
+#pragma unroll
+          for(int f2 = 0; f2 < kFields; f2++) {
+            const float x = c == 0 ? v.x[f2].x : c == 1 ? v.x[f2].y : c == 2 ? v.x[f2].z : v.x[f2].w;
+            if(x != 0.f) atomicAdd(&g[f2 * 256 + w], __float2int_rn(x * s_sc[f2]));
+          }
+        }
+      }
+    };
+    uint2 pe_cur = fetch_entry(0);
+#if T16_PREFETCH_VALUES
+    Vals pv = fetch_values(pe_cur);
+#endif
+    uint2 pe = fetch_entry(1);
+    if(nact > 0) load_pairs(tact[0]);
+    int prev = -1;
+    for(int ai = 0; ai < nact; ai++) {
+      const int m = tact[ai], tz = t16_z(m), tr = t16_r(m);
+      if(A.prof && tid == 0) tpa = wall_clock64();
+      // phase A: the tile's entries into the K+1 grids of each field (the first one prefetched)
+#if T16_PREFETCH_VALUES
+      if(pe_cur.y != 0xffffffffu) add_entry((int)pe_cur.y, pv, tz, tr);
+#else
+      if(pe_cur.y != 0xffffffffu) add_entry((int)pe_cur.y, fetch_values(pe_cur), tz, tr);
+#endif
+      for(int e = toff[m] + tid + nt; e < toff[m + 1]; e += nt) {
+        const uint2 en = list[e];
+        add_entry((int)en.y, fetch_values(en), tz, tr);
+      }
+      __syncthreads();
+      if(A.prof && tid == 0) {
+        tpb = wall_clock64();
+        atomicAdd(&A.prof[9], tpb - tpa);
+      }
+      // phase B: wave f, field f (the next tiles' entries and values on their way)
+      pe_cur = pe;
+#if T16_PREFETCH_VALUES
+      pv = fetch_values(pe_cur);
+#endif
+      pe = fetch_entry(ai + 2);
+      if(prev >= 0) close_levels(prev, m, false);
+      if(lane < lt) {
+        const int a = m >> (2 * (lt - lane));
+        up = bwf[GALIC_STACKOFFSET(lane, t16_z(a), t16_r(a))];
+      }
+      {
+        // residuals in the particle's units
+#pragma unroll
+        for(int c = 0; c < 4; c++) pc[c].x = ldexp(pc[c].x, kp);
+        p1.x = ldexp(p1.x, kp);
+        int4 *og = (int4 *)(grids + f * 256) + lane;
+        const int4 o4 = *og;
+        *og = make_int4(0, 0, 0, 0);
+        const unsigned int tm = tmask[m];
+        // The coarser nodes' sums of trial k are handed to one lane each and their terms formed
+        // after the loop: the 4x4 node's (in all four lanes of its quad after the quad sum) to
+        // lane 4q + k % 4, register k / 4; the 8x8 node's (in all 16 lanes of its row) to lane
+        // 16r + k; the tile's to lane k.
+        int s2x[(KR + 3) / 4], s3x = 0, tsk = 0;
+#pragma unroll
+        for(int u = 0; u < (KR + 3) / 4; u++) s2x[u] = 0;
+#pragma unroll
+        for(int k = 0; k < KR; k++) {
+          // (a compiler barrier per trial: its grid is read where it is used, not all trials'
+          // words at once, which would hold 4 registers per trial)
+          asm volatile("" ::: "memory");
+          if(k < K && (tm & (1u | (2u << k)))) {
+            int4 *gg = (int4 *)(grids + ((k + 1) * kFields + f) * 256) + lane;
+            const int4 g4 = *gg;
+            *gg = make_int4(0, 0, 0, 0);
+            const int d0 = g4.x - o4.x, d1 = g4.y - o4.y, d2 = g4.z - o4.z, d3 = g4.w - o4.w;
+            int s = (d0 + d1) + (d2 + d3);
+            double a = acc[k];
+            a = t16_term(pc[0].x, pc[0].y, (double)d0, a);
+            a = t16_term(pc[1].x, pc[1].y, (double)d1, a);
+            a = t16_term(pc[2].x, pc[2].y, (double)d2, a);
+            a = t16_term(pc[3].x, pc[3].y, (double)d3, a);
+            acc[k] = t16_term(p1.x, p1.y, (double)s, a);
+            s += dpp_i32<kDppQuadXor1>(s);
+            s += dpp_i32<kDppQuadXor2>(s);
+            s2x[k >> 2] = (lane & 3) == (k & 3) ? s : s2x[k >> 2];
+            s += dpp_i32<kDppRowRor4>(s);
+            s += dpp_i32<kDppRowRor8>(s);
+            s3x = (lane & 15) == k ? s : s3x;
+            const int s4 = __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 16) +
+                           __builtin_amdgcn_readlane(s, 32) + __builtin_amdgcn_readlane(s, 48);
+            tsk = lane == k ? s4 : tsk;
+          }
+        }
+        // the coarser nodes' terms of the lane's trials (0 for trials beyond K or not evaluated)
+#pragma unroll
+        for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = t16_term(ldexp(p2.x, kp), p2.y, (double)s2x[u], acc2[u]);
+        acc3 = t16_term(ldexp(p3.x, kp), p3.y, (double)s3x, acc3);
+        accs = t16_term(ldexp(p4.x, kp), p4.y, (double)tsk, accs);
+#pragma unroll
+        for(int l = 0; l < NSTK; l++)
+          if(l == lt - 1) stk[l] += tsk;
+      }
+      prev = m;
+      if(ai + 1 < nact) load_pairs(tact[ai + 1]);
+      __syncthreads();
+      if(A.prof && tid == 0) atomicAdd(&A.prof[10], wall_clock64() - tpb);
+    }
+    if(prev >= 0) close_levels(prev, prev, true);
+    // per-trial totals of field f
+    {
+      const double MT = A.MType[t];
+      const double coef = f == 0 ? 1.0 / MT : (f == 1 ? A.fac[t][0] : f == 2 ? A.fac[t][1] : f == 3 ? A.fac[t][2]
+                                                                                                  : A.fac[t][3]) /
+                                                  MT / 4;
+      const double unit = ldexp(coef, -kp);
+#pragma unroll
+      for(int k = 0; k < KR; k++)
+        if(k < K) {
+          double v = acc[k];
+          if((lane & 3) == (k & 3)) v += acc2[k >> 2];
+          if((lane & 15) == k) v += acc3;
+          if(lane == k) v += accs;
+          v = wave_sum(v);
+          if(lane == 0) s_part[f][k] = v * unit;
+        }
+    }
+    __syncthreads();
+    // 3. the choice
+    for(int k = tid; k < K; k += nt) {
+      const size_t q = (size_t)ti * K + k;
+      const double *fit = A.fit + (size_t)t * 16;
+      double orbit_terms = 0;
+#pragma unroll
+      for(int f2 = 0; f2 < kFields; f2++) orbit_terms += s_part[f2][k];
+      s_dF[k] = orbit_terms + fit[5] / fit[6] * A.vinst[q];
+      A.dFk[q] = s_dF[k];
+    }
+    __syncthreads();
+    if(tid == 0) {
+      int bk = -1;
+      double bv = 0;
+      for(int k = 0; k < K; k++)
+        if(s_dF[k] < bv) {
+          bv = s_dF[k];
+          bk = k;
+        }
+      s_best = bk;
+      A.accept[ti] = bk + 1;
+      A.dF[ti] = (bk >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
+    }
+    __syncthreads();
+    const int best = s_best;
+    if(A.prof && tid == 0) {
+      const unsigned long long tp2 = wall_clock64();
+      atomicAdd(&A.prof[1], tp2 - tp1);
+      tp1 = tp2;
+      if(best >= 0) atomicAdd(&A.prof[8], 1ull);
+    }
+    // 4. the deltas of the taken trial, exact in int64 at the log scale, on blocks of four
+    // consecutive tiles (32 x 32 cells; the whole 16 x 16 grid when it is one tile)
+    if(best >= 0) {
+      long long *G = (long long *)grids;  // the int32 grids are all zero here
+      long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
+      const unsigned int ob = (unsigned int)best + 1;
+      const int gsh = lt > 0 ? 2 : 0, side = lt > 0 ? 32 : 16;
+      for(int ai = 0; ai < nact;) {
+        const int grp = tact[ai] >> gsh;
+        int aj = ai;
+        unsigned int any = 0;
+        while(aj < nact && (tact[aj] >> gsh) == grp) any |= tmask[tact[aj++]];
+        if(any & (1u | (1u << ob))) {
+          const int g0 = grp << gsh, gz0 = t16_z(g0) * T16, gr0 = t16_r(g0) * T16;
+          for(int a2 = ai; a2 < aj; a2++) {
+            const int m = tact[a2], tz = t16_z(m) * T16, tr = t16_r(m) * T16;
+            for(int e = toff[m] + tid; e < toff[m + 1]; e += nt) {
+              const uint2 en = list[e];
+              if(en.y != 0 && en.y != ob) continue;
+              const unsigned int rec = en.x;
+              const int quad = A.log_q[rec], blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
+              const int lz = (quad >> L) - tz, lr = (quad & (nb - 1)) - tr;
+#pragma unroll
+              for(int f2 = 0; f2 < kFields; f2++) {
+                const float4 x = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
+                const float sc = en.y == 0 ? -A.log_scale[f2] : A.log_scale[f2];
+#pragma unroll
+                for(int c = 0; c < 4; c++) {
+                  const int cz = lz + (c >> 1), cr = lr + (c & 1);
+                  const float v = c == 0 ? x.x : c == 1 ? x.y : c == 2 ? x.z : x.w;
+                  if((unsigned)cz < (unsigned)T16 && (unsigned)cr < (unsigned)T16 && v != 0.f)
+                    atomicAdd((unsigned long long *)&G[f2 * 1024 + (tz - gz0 + cz) * 32 + (tr - gr0 + cr)],
+                              (unsigned long long)__float2ll_rn(v * sc));
+                }
+              }
+            }
+          }
+          __syncthreads();
+          for(int q = tid; q < kFields * side * side; q += nt) {
+            const int f2 = q / (side * side), cz = (q / side) % side, cr = q % side;
+            long long *gp = &G[f2 * 1024 + cz * 32 + cr];
+            const long long d = *gp;
+            if(d) {
+              *gp = 0;
+              atomic_q(&lad[(size_t)f2 * ngrid + (size_t)(gz0 + cz) * nb + gr0 + cr], (double)d * A.log_to_loc[f2]);
+            }
+          }
+          __syncthreads();
+        }
+        ai = aj;
+      }
+    }
+    if(A.prof && tid == 0) {
+      const unsigned long long te = wall_clock64();
+      atomicAdd(&A.prof[3], te - tp1);
+      atomicMax(&A.prof[4], te - tp0);
+      atomicAdd(&A.prof[11], (te - tp0) * (te - tp0) / 100);
+      atomicMax(&A.prof[12], (unsigned long long)toff[ntile]);
+    }
+  }
+}
+
+template __global__ void k_eval_t16<0, 8>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_t16<9, 8>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_t16<0, 16>(DevModel M, EvalArgs32 A);
+template __global__ void k_eval_t16<9, 16>(DevModel M, EvalArgs32 A);
 
 // ------------------------------------------------- initial pass from logs --
 

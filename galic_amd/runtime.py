@@ -146,7 +146,8 @@ class GalicGPU:
         check(self.lib.galic_fp32_set_launch(self.h, int(variant), int(blocks_per_cu)))
 
     def fp32_set_merit_form(self, form):
-        """0 automatic, 1 dense LDS tiles at every level, 2 never tiles, 3 sparse LDS tiles at every level."""
+        """0 automatic (4 where it applies), 1 dense 32x32 LDS tiles at every level, 2 never tiles,
+        3 sparse 32x32 LDS tiles at every level, 4 16x16 tiles in each particle's int32 fixed point."""
         check(self.lib.galic_fp32_set_merit_form(self.h, int(form)))
 
     def fp32_set_fresh_trials(self, nfresh):

@@ -211,11 +211,15 @@ int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk);
  * GPU holds at once. */
 int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_per_cu);
 
-/* Form of the FP32 merit kernel: 0 automatic (LDS grids of the whole response grid where
- * they fit, otherwise LDS tiles of 32 x 32 cells, else two int64 grids per workgroup in HBM),
- * 1 dense tiles at every level, 2 never tiles, 3 tiles restricted to the 2x2 nodes the logs
- * touch (where their LDS fits: K up to 8 on 512^2 grids).  All forms give the same decisions
- * and bit-identical grids; merit sums agree to FP64 rounding. */
+/* Form of the FP32 merit kernel: 0 automatic (form 4 where it applies: grids of 16^2 to
+ * 1024^2 cells and K <= 16; otherwise LDS grids of the whole response grid where they fit,
+ * LDS tiles of 32 x 32 cells, else two int64 grids per workgroup in HBM), 1 dense 32 x 32
+ * tiles at every level, 2 never tiles, 3 32 x 32 tiles restricted to the 2x2 nodes the logs
+ * touch (where their LDS fits: K up to 8 on 512^2 grids), 4 tiles of 16 x 16 cells in each
+ * particle's own int32 fixed point (2^30 units for the largest total of its orbits).  Forms 1-3
+ * give the same decisions as form 2 and bit-identical grids, merit sums equal to FP64
+ * rounding; form 4's merit sums differ from them by its fixed point's rounding (about 2^-30 of
+ * an orbit's total per record), its deltas are the same int64 sums. */
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form);
 
 /* Timing aid of the sparse tile merit: on != 0 starts (or restarts) summing, per phase of
