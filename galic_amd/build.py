@@ -30,6 +30,7 @@ UNITS = {
     "galic_targets.hip": ["-ffp-contract=off"],
     "galic_io.hip": ["-ffp-contract=off"],
     "galic_noalloc.hip": ["-ffp-contract=off"],
+    "galic_commit.hip": ["-ffp-contract=off"],
 }
 
 

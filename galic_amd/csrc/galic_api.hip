@@ -538,6 +538,82 @@ size_t slot_budget(galic::Ctx &c) {
 // The orbits of the initial pass (prepared at o_* [0, n)) through the logging kernel in
 // chunks the arena holds, each chunk's logs added to the Loc grids by k_log_to_loc.  The
 // first chunk is small: it sets the estimate of log blocks per orbit for the others.
+// Adds logs to dst (fixed-point Loc grids [type][5][Ngrid]) through region buckets: the
+// entries of each (type, region) are counted on the device, their buckets laid out here, the
+// entries scattered, then summed region chunk by region chunk (galic_commit.hip).  A.n items,
+// with A.accept: each trial's taken orbit added and its current orbit taken away; without:
+// every orbit added.
+constexpr int kCommitChunk = 16384;  // bucket entries one workgroup sums
+int commit_logs(galic::Ctx &c, galic::CommitArgs A, long long *dst) {
+  if(A.n <= 0) return GALIC_OK;
+  const int gsh = std::min(c.L, galic::kCommitRegionLog2);
+  const int nreg = 1 << (c.L - gsh), ng = nreg * nreg, nk = GALIC_NTYPES * ng;
+  if(c.cm_nk < nk) {
+    dfree(c.cm_cnt);
+    dfree(c.cm_start);
+    c.cm_nk = 0;
+    CHK(dalloc(&c.cm_cnt, 2 * (size_t)nk));
+    CHK(dalloc(&c.cm_start, (size_t)nk));
+    c.cm_nk = nk;
+  }
+  A.L = c.L;
+  A.gsh = gsh;
+  A.log_blocks = c.log_blocks;
+  A.log_n = c.log_n;
+  A.log_M = c.log_M;
+  A.log_q = c.log_q;
+  A.log_v = c.log_v;
+  for(int f = 0; f < galic::kFields; f++) A.q[f] = c.fp32_scale[f];
+  A.gcnt = c.cm_cnt;
+  A.gcur = c.cm_cnt + nk;
+  HIPCHK(hipMemsetAsync(c.cm_cnt, 0, 2 * sizeof(unsigned int) * nk, c.stream));
+  int ncu = 256;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+  const int grid = std::min(A.n, ncu * 8);
+  const size_t sh = sizeof(unsigned int) * ng;
+  hipLaunchKernelGGL(galic::k_commit_count, dim3(grid), dim3(256), sh, c.stream, A);
+  HIPCHK(hipGetLastError());
+  std::vector<unsigned int> cnt(nk), start(nk);
+  HIPCHK(hipMemcpyAsync(cnt.data(), c.cm_cnt, sizeof(unsigned int) * nk, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  size_t total = 0;
+  std::vector<int4> chunks;
+  for(int k = 0; k < nk; k++) {
+    start[k] = (unsigned int)total;
+    for(size_t lo = 0; lo < cnt[k]; lo += kCommitChunk)
+      chunks.push_back(make_int4(k, (int)(total + lo), (int)(total + std::min<size_t>(cnt[k], lo + kCommitChunk)), 0));
+    total += cnt[k];
+    if(total > 0x7fffffffu) return set_err(GALIC_ERR_NOMEM, "a commit of %zu region entries exceeds 2^31", total);
+  }
+  if(total == 0) return GALIC_OK;
+  if(c.cm_buf_len < total) {
+    dfree(c.cm_buf);
+    c.cm_buf_len = 0;
+    const size_t want = total + total / 4;
+    CHK(dalloc(&c.cm_buf, want));
+    c.cm_buf_len = want;
+  }
+  if(c.cm_chunks_len < chunks.size()) {
+    dfree(c.cm_chunks);
+    c.cm_chunks_len = 0;
+    CHK(dalloc(&c.cm_chunks, chunks.size() * 2));
+    c.cm_chunks_len = chunks.size() * 2;
+  }
+  HIPCHK(hipMemcpyAsync(c.cm_start, start.data(), sizeof(unsigned int) * nk, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(c.cm_chunks, chunks.data(), sizeof(int4) * chunks.size(), hipMemcpyHostToDevice, c.stream));
+  A.gstart = c.cm_start;
+  A.gbuf = c.cm_buf;
+  hipLaunchKernelGGL(galic::k_commit_scatter, dim3(grid), dim3(256), sh, c.stream, A);
+  HIPCHK(hipGetLastError());
+  const size_t shr = sizeof(long long) * galic::kFields << (2 * gsh);
+  hipLaunchKernelGGL(galic::k_commit_regions, dim3((unsigned int)chunks.size()), dim3(256), shr, c.stream, A,
+                     (const int4 *)c.cm_chunks, dst);
+  HIPCHK(hipGetLastError());
+  // (the host vectors of the copies above must outlive them)
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return GALIC_OK;
+}
+
 int fp32_initial_logs(galic::Ctx &c, int n) {
   const size_t bb = log_block_bytes();
   {
@@ -545,18 +621,6 @@ int fp32_initial_logs(galic::Ctx &c, int n) {
     const size_t one = (size_t)(log_blocks_planned(c, std::min(n, 4096)) * bb);
     CHK(ensure_arena(c, std::min(all, slot_budget(c)), one));
   }
-  galic::QScale qs;
-  for(int f = 0; f < galic::kFields; f++) qs.q[f] = c.fp32_scale[f];
-  // copies of the fixed-point Loc grids (zero), summed into locA_q at the end
-  const size_t na = (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid;
-  long long *copies = nullptr;
-  struct FreeCopies {
-    long long *&p;
-    ~FreeCopies() { dfree(p); }
-  } free_copies{copies};
-  CHK(dalloc(&copies, (size_t)galic::kLocCopies * na));
-  int ncu = 256;
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
   int a = 0;
   bool first = true;
   while(a < n) {
@@ -576,18 +640,15 @@ int fp32_initial_logs(galic::Ctx &c, int n) {
       continue;  // the estimate has doubled: plan the chunk again
     }
     CHK(lr);
-    hipLaunchKernelGGL(galic::k_log_to_loc, dim3(std::min(m, ncu * 16)), dim3(256), 0, c.stream, m,
-                       (const int *)(c.o_type + a), (const int *)c.log_blocks, (const int *)c.log_n, c.log_M,
-                       (const int *)c.log_q, (const float4 *)c.log_v, c.L, qs, copies, galic::kLocCopies, na);
-    if(hipGetLastError() != hipSuccess) return set_err(GALIC_ERR_HIP, "k_log_to_loc launch failed");
+    // every record of the chunk's orbits into the Loc grids
+    galic::CommitArgs C;
+    memset(&C, 0, sizeof(C));
+    C.n = m;
+    C.otype = c.o_type + a;
+    CHK(commit_logs(c, C, c.locA_q));
     a += m;
     first = false;
   }
-  hipLaunchKernelGGL(galic::k_sum_copies, dim3(nblocks(na, 256)), dim3(256), 0, c.stream, c.locA_q, copies,
-                     galic::kLocCopies, na);
-  const hipError_t e = hipGetLastError();
-  const hipError_t e2 = hipStreamSynchronize(c.stream);
-  if(e != hipSuccess || e2 != hipSuccess) return set_err(GALIC_ERR_HIP, "summing the Loc copies failed");
   return GALIC_OK;
 }
 
@@ -842,6 +903,7 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.g32_scratch);
   dfree(c.tile_list);
   dfree(c.tile_aux);
+  dfree(c.cm_cnt); dfree(c.cm_start); dfree(c.cm_chunks); dfree(c.cm_buf);
   dfree(c.merit_prof);
   dfree(c.lookup_buf);
   dfree(c.fit_part);
@@ -1967,6 +2029,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       for(int f = 0; f < 5; f++) E.qscale[f] = c.fp32_scale[f];
       E.iscale = c.fp32_iscale;
       E.gscratch = c.g32_scratch;
+      E.no_deltas = 1;
       HIPCHK(hipEventRecord(c.ev0, c.stream));
       if(t16) {
         E.nblk = b - a;
@@ -2063,6 +2126,27 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
         unsigned int terr = 0;
         HIPCHK(hipMemcpy(&terr, c.tile_aux + 1, sizeof(terr), hipMemcpyDeviceToHost));
         if(terr) return set_err(GALIC_ERR_STATE, "the tile lists of a particle outgrew their room");
+      }
+      {
+        // the taken trials' orbits into the Loc deltas, the orbits they replace out (timed
+        // with the merit evaluation)
+        galic::CommitArgs C;
+        memset(&C, 0, sizeof(C));
+        C.n = b - a;
+        C.K = K;
+        C.trial_base = a;
+        C.slot_pos = E.slot_pos;
+        C.accept = c.t_accept;
+        C.tidx = c.t_idx;
+        C.ptype = c.P.type;
+        HIPCHK(hipEventRecord(c.ev0, c.stream));
+        CHK(commit_logs(c, C, c.locA_dq));
+        HIPCHK(hipEventRecord(c.ev1, c.stream));
+        HIPCHK(hipEventSynchronize(c.ev1));
+        float cms = 0;
+        HIPCHK(hipEventElapsedTime(&cms, c.ev0, c.ev1));
+        eval_ms += cms;
+        c.acc_eval_ms += cms;
       }
       if(b == rb) {
         CHK(commit_q(c));

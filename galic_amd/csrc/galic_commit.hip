@@ -1,0 +1,175 @@
+// galic_commit.hip -- adding response logs to the fixed-point Loc grids of the FP32 path.
+//
+// Every record of a log (quad, 4 cells x 5 fields of FP32 values) counts round(q[f] * v)
+// units at each of its cells, q[f] the Loc grid's fixed-point scale of field f, the rounding
+// done per record.  The Loc grids of the FP32 path are therefore always exactly the sum over
+// the particles of their current orbits' records, whatever the order of the adds: the initial
+// pass adds every orbit's records, and taking a trial adds its orbit's records and takes away
+// those of the orbit it replaces (the reference keeps Loc += response(new) - response(old),
+// main.c:503-513; here both responses are the logged FP32 ones).
+//
+// Few global atomics: the records are first bucketed by region of GS x GS cells (32 x 32, or
+// the whole grid below 32 x 32) -- k_commit_count counts the entries of each region (LDS
+// counts per particle, one global atomic per particle and region), the host turns the counts
+// into bucket offsets and chunks, k_commit_scatter writes (record, sign) entries into the
+// buckets -- and k_commit_regions sums a chunk of one region's entries in LDS (int64 atomics)
+// and adds each nonzero cell once to the destination grid.
+
+#include <hip/hip_runtime.h>
+
+#include "galic_fast.h"
+
+namespace galic {
+
+namespace {
+
+// Regions (index gz * nreg + gr, nreg regions per side) reached by the 2x2 cells of quad.
+template <typename Visit>
+__device__ __forceinline__ void region_visit(int quad, int L, int gsh, Visit visit) {
+  const int iz = quad >> L, iR = quad & ((1 << L) - 1), nreg = 1 << (L - gsh);
+  const int z0 = iz >> gsh, z1 = (iz + 1) >> gsh, r0 = iR >> gsh, r1 = (iR + 1) >> gsh;
+  visit(z0 * nreg + r0);
+  if(r1 != r0) visit(z0 * nreg + r1);
+  if(z1 != z0) {
+    visit(z1 * nreg + r0);
+    if(r1 != r0) visit(z1 * nreg + r1);
+  }
+}
+
+// The logs item b of a commit launch adds (sign 0) or takes away (sign 1): with accept, the
+// taken trial's orbit (+) and the particle's current orbit (-), nothing if no trial was taken;
+// without, orbit b (+).  Returns the number of logs (0..2) and the particle type.
+__device__ __forceinline__ int commit_logs(const CommitArgs &A, int b, size_t (&orb)[2], unsigned int (&sign)[2],
+                                           int &t) {
+  if(!A.accept) {
+    orb[0] = (size_t)b;
+    sign[0] = 0;
+    t = A.otype[b];
+    return 1;
+  }
+  const int ti = A.trial_base + b;
+  const int best = A.accept[ti] - 1;
+  t = A.ptype[A.tidx[ti]];
+  if(best < 0) return 0;
+  const size_t o0 = (size_t)(A.slot_pos ? A.slot_pos[b] : b) * (A.K + 1);
+  orb[0] = o0 + 1 + best;
+  sign[0] = 0;
+  orb[1] = o0;
+  sign[1] = 1;
+  return 2;
+}
+
+// Calls visit(record index, sign) for every record of the item's logs (all threads).
+template <typename Visit>
+__device__ __forceinline__ void commit_records(const CommitArgs &A, int nlog, const size_t (&orb)[2],
+                                               const unsigned int (&sign)[2], Visit visit) {
+  for(int l = 0; l < nlog; l++) {
+    const int n = max(A.log_n[orb[l]], 0);
+    const int *tb = A.log_blocks + orb[l] * A.log_M;
+    for(int r = threadIdx.x; r < n; r += blockDim.x) {
+      const unsigned int rec = (unsigned int)tb[r / LOG_B] * LOG_B + (unsigned int)(r & (LOG_B - 1));
+      visit(rec, sign[l]);
+    }
+  }
+}
+
+}  // namespace
+
+// Entries per (type, region): one workgroup per item, counts in LDS, then one global atomic
+// per region the item reaches.
+__global__ void __launch_bounds__(256) k_commit_count(CommitArgs A) {
+  extern __shared__ unsigned int cnt_s[];
+  const int nreg = 1 << (A.L - A.gsh), ng = nreg * nreg;
+  for(int b = blockIdx.x; b < A.n; b += gridDim.x) {
+    size_t orb[2];
+    unsigned int sign[2];
+    int t;
+    const int nlog = commit_logs(A, b, orb, sign, t);
+    if(nlog == 0) continue;
+    for(int g = threadIdx.x; g < ng; g += blockDim.x) cnt_s[g] = 0;
+    __syncthreads();
+    commit_records(A, nlog, orb, sign, [&](unsigned int rec, unsigned int) {
+      region_visit(A.log_q[rec], A.L, A.gsh, [&](int g) { atomicAdd(&cnt_s[g], 1u); });
+    });
+    __syncthreads();
+    for(int g = threadIdx.x; g < ng; g += blockDim.x)
+      if(cnt_s[g]) atomicAdd(&A.gcnt[(size_t)t * ng + g], cnt_s[g]);
+    __syncthreads();
+  }
+}
+
+// The entries into the buckets: per item, LDS counts, one global atomic per region to reserve
+// room in its bucket (gstart + cursor), then the entries at LDS cursors.
+__global__ void __launch_bounds__(256) k_commit_scatter(CommitArgs A) {
+  extern __shared__ unsigned int cnt_s[];
+  const int nreg = 1 << (A.L - A.gsh), ng = nreg * nreg;
+  for(int b = blockIdx.x; b < A.n; b += gridDim.x) {
+    size_t orb[2];
+    unsigned int sign[2];
+    int t;
+    const int nlog = commit_logs(A, b, orb, sign, t);
+    if(nlog == 0) continue;
+    for(int g = threadIdx.x; g < ng; g += blockDim.x) cnt_s[g] = 0;
+    __syncthreads();
+    commit_records(A, nlog, orb, sign, [&](unsigned int rec, unsigned int) {
+      region_visit(A.log_q[rec], A.L, A.gsh, [&](int g) { atomicAdd(&cnt_s[g], 1u); });
+    });
+    __syncthreads();
+    for(int g = threadIdx.x; g < ng; g += blockDim.x)
+      if(cnt_s[g]) {
+        const size_t k = (size_t)t * ng + g;
+        cnt_s[g] = A.gstart[k] + atomicAdd(&A.gcur[k], cnt_s[g]);
+      }
+    __syncthreads();
+    commit_records(A, nlog, orb, sign, [&](unsigned int rec, unsigned int s) {
+      region_visit(A.log_q[rec], A.L, A.gsh,
+                   [&](int g) { A.gbuf[atomicAdd(&cnt_s[g], 1u)] = make_uint2(rec, s); });
+    });
+    __syncthreads();
+  }
+}
+
+// One workgroup per chunk (type, region, entries [lo, hi) of its bucket): each entry's cells
+// inside the region, round(q[f] * v) with the entry's sign, into five int64 LDS grids of the
+// region; then every nonzero cell once into dst[type][f][cell] (a global atomic).
+__global__ void __launch_bounds__(256) k_commit_regions(CommitArgs A, const int4 *chunks, long long *dst) {
+  extern __shared__ long long reg_s[];  // [5][GS * GS]
+  const int gs = 1 << A.gsh, nreg = 1 << (A.L - A.gsh), nb = 1 << A.L, ngrid = nb * nb;
+  const int4 ch = chunks[blockIdx.x];  // (type * nreg^2 + region, lo, hi, 0)
+  const int t = ch.x / (nreg * nreg), g = ch.x % (nreg * nreg);
+  const int gz0 = (g / nreg) * gs, gr0 = (g % nreg) * gs;
+  const int cells = gs * gs;
+  for(int q = threadIdx.x; q < kFields * cells; q += blockDim.x) reg_s[q] = 0;
+  __syncthreads();
+  for(int e = ch.y + threadIdx.x; e < ch.z; e += blockDim.x) {
+    const uint2 en = A.gbuf[e];
+    const unsigned int rec = en.x;
+    const int quad = A.log_q[rec], blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
+    const int lz = (quad >> A.L) - gz0, lr = (quad & (nb - 1)) - gr0;
+#pragma unroll
+    for(int f = 0; f < kFields; f++) {
+      const float4 x = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+      const double q = en.y ? -A.q[f] : A.q[f];
+#pragma unroll
+      for(int c = 0; c < 4; c++) {
+        const int cz = lz + (c >> 1), cr = lr + (c & 1);
+        const float v = c == 0 ? x.x : c == 1 ? x.y : c == 2 ? x.z : x.w;
+        if((unsigned)cz < (unsigned)gs && (unsigned)cr < (unsigned)gs && v != 0.f)
+          atomicAdd((unsigned long long *)&reg_s[f * cells + cz * gs + cr],
+                    (unsigned long long)__double2ll_rn(q * (double)v));
+      }
+    }
+  }
+  __syncthreads();
+  long long *d = dst + (size_t)t * kFields * ngrid;
+  for(int q = threadIdx.x; q < kFields * cells; q += blockDim.x) {
+    const long long v = reg_s[q];
+    if(v) {
+      const int f = q / cells, cz = (q % cells) / gs, cr = q % gs;
+      atomicAdd((unsigned long long *)&d[(size_t)f * ngrid + (size_t)(gz0 + cz) * nb + gr0 + cr],
+                (unsigned long long)v);
+    }
+  }
+}
+
+}  // namespace galic

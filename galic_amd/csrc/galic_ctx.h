@@ -146,6 +146,14 @@ struct Ctx {
   size_t log_blocks_len = 0;
   int *log_n = nullptr;          // [orbit]
   float *log_tot = nullptr;      // [orbit][5] sums of each orbit's record values
+  // region commit of logs into the Loc grids (galic_commit.hip)
+  unsigned int *cm_cnt = nullptr;    // [2][type][region] counts, cursors
+  unsigned int *cm_start = nullptr;  // [type][region] bucket starts
+  int cm_nk = 0;                     // types x regions of the above
+  int4 *cm_chunks = nullptr;         // (type * regions + region, lo, hi, 0)
+  size_t cm_chunks_len = 0;
+  uint2 *cm_buf = nullptr;           // bucket entries
+  size_t cm_buf_len = 0;
   int log_n_len = 0;
   int log_M = 64;                // block-table entries per orbit (doubled when an orbit needs more)
   unsigned int *log_next = nullptr;

@@ -79,6 +79,7 @@ struct EvalArgs32 {
   unsigned int *tile_err;  // tile form: set when a particle's lists exceed tlist_cap
   uint2 *tlist2;           // 16x16-tile form: per workgroup, tlist_cap entries (record index, log)
   const float *log_tot;    // [orbit][5] sums of each log's values (k_orbit32_log)
+  int no_deltas;           // the taken trial's deltas are left to the region commit (k_commit_*)
   const int *work_order;   // 16x16-tile form: trials in the order workgroups take them
   int *work_next;          // ... from this counter (zero at launch)
   unsigned long long *prof;  // sparse tile form, optional: time per phase and sizes, summed over particles
@@ -188,15 +189,31 @@ template <int LF, int KR>
 __global__ void k_eval_t16(DevModel M, EvalArgs32 A);
 __global__ void k_trial_inst(DevModel M, EvalArgs32 A, double *vinst);
 __global__ void k_apply_best(DevModel M, EvalArgs32 A);
-struct QScale {
-  double q[kFields];
-};
-__global__ void k_log_to_loc(int n, const int *otype, const int *log_blocks, const int *log_n, int log_M,
-                             const int *log_q, const float4 *log_v, int L, QScale qs, long long *locq, int ncopy,
-                             size_t copy_stride);
-__global__ void k_sum_copies(long long *dst, long long *src, int ncopy, size_t n);
-constexpr int kLocCopies = 16;  // copies of the Loc grids the initial pass adds into
 __global__ void k_log_totals(int n, int K, const int *slot_pos, const int *log_n, unsigned int *maxsum);
+
+// Logs into the fixed-point Loc grids through region buckets (galic_commit.hip).
+struct CommitArgs {
+  int n;                    // items of the launch: trials (with accept) or orbits (without)
+  int K, trial_base;        // with accept: trial b is trial_base + b, its orbits at slot_pos[b] * (K + 1)
+  const int *slot_pos;
+  const int *accept;        // taken trial + 1 per trial; null: item b is orbit b, added whole
+  const int *tidx, *ptype;  // with accept: particle of each trial, type of each particle
+  const int *otype;         // without: type of each orbit
+  const int *log_blocks, *log_n;
+  int log_M;
+  const int *log_q;
+  const float4 *log_v;
+  int L, gsh;               // grid level; log2 of a region's side (min(L, 5))
+  unsigned int *gcnt;       // [type][region] entries (k_commit_count adds)
+  unsigned int *gcur;       // [type][region] cursors (zero before k_commit_scatter)
+  const unsigned int *gstart;  // [type][region] first entry of the bucket
+  uint2 *gbuf;              // entries (arena record index, sign: 0 add, 1 take away)
+  double q[kFields];        // Loc fixed-point units per response unit
+};
+constexpr int kCommitRegionLog2 = 5;
+__global__ void k_commit_count(CommitArgs A);
+__global__ void k_commit_scatter(CommitArgs A);
+__global__ void k_commit_regions(CommitArgs A, const int4 *chunks, long long *dst);
 
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);

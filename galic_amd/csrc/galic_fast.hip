@@ -1222,7 +1222,7 @@ __global__ void __launch_bounds__(merit_threads(LF)) k_eval_log32(DevModel M, Ev
     __syncthreads();
   }
   const int best = choose_trial(M, A, ti, p, t, o0, part, s_dF, &s_best);
-  if(best >= 0) {
+  if(best >= 0 && !A.no_deltas) {
     // the orbit-averaged deltas of the taken trial, exact in fixed point
     long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
     const int4 bn = A.bbox[o0 + 1 + best];
@@ -1641,7 +1641,7 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
     }
     // 4. the choice and the deltas of the taken trial, exact in fixed point
     const int best = choose_trial(M, A, ti, p, t, o0, part, s_dF, &s_best);
-    if(best >= 0) {
+    if(best >= 0 && !A.no_deltas) {
       long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
       const unsigned int ob = (unsigned int)best + 1;
       long long *G = grids;  // O's first grid, zero here
@@ -2145,7 +2145,7 @@ __global__ void __launch_bounds__(1024) k_eval_sparse32(DevModel M, EvalArgs32 A
       atomicAdd(&A.prof[2], tp1 - tp2);
       if(best >= 0) atomicAdd(&A.prof[8], 1ull);
     }
-    if(best >= 0) {
+    if(best >= 0 && !A.no_deltas) {
       long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
       const unsigned int ob = (unsigned int)best + 1;
       long long *G = grids;  // O's first grid, zero here
@@ -2626,7 +2626,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     }
     // 4. the deltas of the taken trial, exact in int64 at the log scale, on blocks of four
     // consecutive tiles (32 x 32 cells; the whole 16 x 16 grid when it is one tile)
-    if(best >= 0) {
+    if(best >= 0 && !A.no_deltas) {
       long long *G = (long long *)grids;  // the int32 grids are all zero here
       long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
       const unsigned int ob = (unsigned int)best + 1;
@@ -2690,53 +2690,6 @@ template __global__ void k_eval_t16<0, 8>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_t16<9, 8>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_t16<0, 16>(DevModel M, EvalArgs32 A);
 template __global__ void k_eval_t16<9, 16>(DevModel M, EvalArgs32 A);
-
-// ------------------------------------------------- initial pass from logs --
-
-// The initial response pass from response logs: every record of orbit o is added to the
-// fixed-point Loc grids of the orbit's type with the int64 values k_orbit32 adds for each
-// flushed run (qscale[f] times the FP32 record value, rounded), so the grids are the same
-// bits.  One workgroup per orbit at a time, a thread per record.
-// The workgroups add into ncopy copies of the grids (copy = workgroup mod ncopy, copy_stride
-// apart), so that the orbits crossing the same central cells contend less; k_sum_copies adds
-// the copies up (integer sums: the same bits in any order).
-__global__ void __launch_bounds__(256) k_log_to_loc(int n, const int *otype, const int *log_blocks, const int *log_n,
-                                                    int log_M, const int *log_q, const float4 *log_v, int L,
-                                                    QScale qs, long long *locq, int ncopy, size_t copy_stride) {
-  const int nb = 1 << L, ngrid = nb * nb;
-  long long *base = locq + (size_t)(blockIdx.x % ncopy) * copy_stride;
-  for(int o = blockIdx.x; o < n; o += gridDim.x) {
-    const int m = log_n[o];
-    long long *lq = base + (size_t)otype[o] * kFields * ngrid;
-    for(int r = threadIdx.x; r < m; r += blockDim.x) {
-      const int blk = log_blocks[(size_t)o * log_M + r / LOG_B], jj = r & (LOG_B - 1);
-      const int quad = log_q[(size_t)blk * LOG_B + jj];
-      const int c[4] = {quad, quad + 1, quad + nb, quad + nb + 1};
-#pragma unroll
-      for(int f = 0; f < kFields; f++) {
-        const float4 x = log_v[((size_t)blk * kFields + f) * LOG_B + jj];
-        const float v[4] = {x.x, x.y, x.z, x.w};
-        long long *s = lq + (size_t)f * ngrid;
-#pragma unroll
-        for(int k = 0; k < 4; k++)
-          if(v[k] != 0.f) atomic_q(&s[c[k]], qs.q[f] * (double)v[k]);
-      }
-    }
-  }
-}
-
-// dst[i] += the sum over the ncopy copies of src (n values each, copies n apart); the copies
-// are cleared.
-__global__ void k_sum_copies(long long *dst, long long *src, int ncopy, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  long long v = 0;
-  for(int c = 0; c < ncopy; c++) {
-    v += src[(size_t)c * n + i];
-    src[(size_t)c * n + i] = 0;
-  }
-  dst[i] += v;
-}
 
 // ---------------------------------------------------- fixed point commit --
 

@@ -455,6 +455,39 @@ def test_fp32_t16_merit_matches_int64_forms(level, ref):
         np.testing.assert_array_equal(a["vel"], b["vel"])
 
 
+@pytest.mark.parametrize("level,form", [(7, 2), (7, 4), (9, 4)])
+def test_fp32_loc_grids_equal_fresh_initial_pass(level, form):
+    """The FP32 path's orbit-averaged Loc grids are always the sum over the particles of their
+    current orbits' logged records, each record rounded into the fixed point on its own: after
+    reps of taken trials (each taken orbit added and the orbit it replaces taken away, through
+    the region commit), an initial pass from the particles' current velocities on a fresh
+    context gives the same bits (main.c:503-513 keeps Loc = the sum of the current responses)."""
+    print(f"[gpu] fp32 Loc grids after reps vs a fresh initial pass, level {level}, merit form {form}", flush=True)
+    if level == 9:
+        wl = model.make_workload(model.config("composite-2e7", max_steps=2000000), n=1500, target_samples=200000)
+        K, rep = 4, 300
+    else:
+        wl = composite_workload(n=2500, level=level)
+        K, rep = 6, 500
+    perm = model.permutation(wl, 6)
+    g = GalicGPU(wl)
+    g.fp32_calc_all_response_fields()
+    g.fp32_set_merit_form(form)
+    acc = g.fp32_iterate(perm, k=K, rep_size=rep, sigma_frac=0.1, seed=31)
+    vel = g.particles()[0]
+    grids = [g.stack(t, wh) for t in wl.types for wh in (0, 6, 7, 8, 9)]
+    g.close()
+    assert (acc > 0).mean() > 0.2
+    assert not np.array_equal(vel, wl.vel)
+    g2 = GalicGPU(wl)
+    g2.set_particles(wl.pos, vel)
+    g2.fp32_calc_all_response_fields()
+    fresh = [g2.stack(t, wh) for t in wl.types for wh in (0, 6, 7, 8, 9)]
+    g2.close()
+    for x, y in zip(grids, fresh):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_fp32_tile_merit_equals_hbm_form_level9():
     """On 512^2 grids (the full v1.1 model's form: isotropic halo and bulge, structure-3 disk,
     512^2 force tables) the tile form against the form that keeps two int64 grids per
