@@ -1825,7 +1825,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   auto use_sparse = [&]() { return use_tiles() && c.merit_form == 3 && sparse_lds() <= lds_max; };
   // 16x16 tiles in the particle's own int32 fixed point (k_eval_t16): the automatic choice
   // where it applies (form 4 asks for it)
-  const size_t t16_lds = galic::t16_lds_bytes(std::max(c.L, 4), K) + 1024;
+  const size_t t16_lds = galic::t16_lds_bytes(std::max(c.L, 4), K, K <= 8 ? 8 : 16) + 1024;
   const bool t16 = (c.merit_form == 0 || c.merit_form == 4) && c.L >= 4 && c.L <= galic::T16_MAX_LEVEL &&
                    K <= galic::T16_MAX_K && t16_lds <= lds_max;
   const bool gmem = c.L > 7;

@@ -173,17 +173,19 @@ __global__ void k_eval_sparse32(DevModel M, EvalArgs32 A);
 constexpr int T16 = 16;
 constexpr int T16_MAX_K = 16;
 constexpr int T16_MAX_LEVEL = 10;  // Morton codes of up to 64 x 64 tiles; open ancestors of up to 8 levels
-// int32 words at the front of its LDS: the K+1 grids of a tile's five fields, or the five
-// int64 32 x 32-cell grids of the delta pass, whichever is larger.
-__host__ __device__ inline int t16_grid_words(int K) {
-  const int a = (K + 1) * kFields * T16 * T16, b = 2 * kFields * 1024;
+// int32 words at the front of its LDS: the K grids D = G_k - O of a tile's five fields (the
+// tile counts share them while the lists are built).
+__host__ __device__ inline int t16_grid_words(int K, int L) {
+  const int a = K * kFields * T16 * T16, b = 1 << (2 * (L - 4));
   return a > b ? a : b;
 }
-// LDS of k_eval_t16 beyond its static arrays: the grids, then tile counts, offsets, the active
-// list and the logs reaching each tile (4 * ntile + 1 words).
-__host__ __device__ inline size_t t16_lds_bytes(int L, int K) {
-  const size_t ntile = (size_t)1 << (2 * (L - 4));
-  return 4 * ((size_t)t16_grid_words(K) + 4 * ntile + 1);
+// words of the per-tile log masks: two tiles a word for up to 15 trials (KR the trial capacity)
+__host__ __device__ inline int t16_mask_words(int ntile, int KR) { return KR <= 15 ? (ntile + 1) / 2 : ntile; }
+// LDS of k_eval_t16 beyond its static arrays: the grids, the list offsets [ntile + 1], the
+// active tiles [ntile] (16-bit) and the masks.
+__host__ __device__ inline size_t t16_lds_bytes(int L, int K, int KR) {
+  const int ntile = 1 << (2 * (L - 4));
+  return 4 * ((size_t)t16_grid_words(K, L) + ntile + 1 + (ntile + 1) / 2 + t16_mask_words(ntile, KR));
 }
 template <int LF, int KR>
 __global__ void k_eval_t16(DevModel M, EvalArgs32 A);

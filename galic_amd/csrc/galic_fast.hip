@@ -2253,10 +2253,52 @@ __device__ __forceinline__ int dpp_i32(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
 
+// The logs reaching each tile: one bit per log (bit 0 the current orbit, bit k+1 trial k), 16
+// bits per tile for up to 15 trials, 32 above.
+template <int KR>
+__device__ __forceinline__ unsigned int t16_mask_get(const unsigned int *w, int m) {
+  if(KR <= 15) return (w[m >> 1] >> ((m & 1) * 16)) & 0xffffu;
+  return w[m];
+}
+template <int KR>
+__device__ __forceinline__ void t16_mask_or(unsigned int *w, int m, unsigned int bits) {
+  if(KR <= 15)
+    atomicOr(&w[m >> 1], bits << ((m & 1) * 16));
+  else
+    atomicOr(&w[m], bits);
+}
+
+// Offsets of the tiles' lists (exclusive scan of the counts, off[ntile] = total) and the tiles
+// with entries in increasing order (16-bit indices); one wave.
+__device__ __forceinline__ void t16_scan(const int *cnt, int *off, unsigned short *act, int *nact, int ntile, int lane) {
+  const int per = (ntile + 63) / 64, a = min(lane * per, ntile), e = min(a + per, ntile);
+  int s = 0, na = 0;
+  for(int i = a; i < e; i++) {
+    s += cnt[i];
+    na += cnt[i] > 0;
+  }
+  int si = s, ni = na;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(si, d), z = __shfl_up(ni, d);
+    if(lane >= d) {
+      si += y;
+      ni += z;
+    }
+  }
+  int base = si - s, abase = ni - na;
+  for(int i = a; i < e; i++) {
+    off[i] = base;
+    base += cnt[i];
+    if(cnt[i] > 0) act[abase++] = (unsigned short)i;
+  }
+  if(lane == 63) {
+    off[ntile] = si;
+    *nact = ni;
+  }
+}
+
 constexpr int T16_LU = 8;  // records a thread takes at once in the list passes
-#ifndef T16_PREFETCH_VALUES
-#define T16_PREFETCH_VALUES 0
-#endif
 
 // Record index of the q-th record of the particle's logs (orbits' records in turn; s_off
 // the exclusive prefix of the K+1 counts) and its log.
@@ -2273,17 +2315,21 @@ template <int LF, int KR>
 __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalArgs32 A) {
   extern __shared__ int lds_i[];
   const int tid = threadIdx.x, nt = blockDim.x, K = A.K, wave = tid >> 6, lane = tid & 63;
-  const int L = LF ? LF : M.DG_L, nb = 1 << L, ngrid = nb * nb, Nstack = GALIC_NSTACK(L);
+  const int L = LF ? LF : M.DG_L, nb = 1 << L, Nstack = GALIC_NSTACK(L);
   const int lt = L - 4, ntile = 1 << (2 * lt);
-  const int gwords = t16_grid_words(K);
-  int *grids = lds_i;                                      // [K+1][5][256] int32; commit: [5][32*32] int64
-  int *tcnt = grids + gwords;                              // [ntile] counts, then cursors
-  int *toff = tcnt + ntile;                                // [ntile + 1]
-  int *tact = toff + ntile + 1;                            // [ntile] tiles with entries, Morton order
-  unsigned int *tmask = (unsigned int *)(tact + ntile);    // [ntile] logs reaching the tile
+  const int gwords = t16_grid_words(K, L);
+  int *grids = lds_i;                                  // [K][5][256] int32: D = G_k - O; counts while listing
+  int *tcnt = grids;                                   // [ntile] (list building only)
+  int *toff = grids + gwords;                          // [ntile + 1]
+  unsigned short *tact = (unsigned short *)(toff + ntile + 1);  // [ntile] tiles with entries, Morton order
+  unsigned int *tmask = (unsigned int *)(toff + ntile + 1 + (ntile + 1) / 2);  // logs reaching each tile
+  const int nmask = t16_mask_words(ntile, KR);
   __shared__ double s_part[kFields][KR];
   __shared__ double s_dF[KR];
   __shared__ int s_off[KR + 2], s_kp[kFields], s_nact, s_best, s_item;
+  constexpr int NSTK = (LF ? LF : T16_MAX_LEVEL) - 4;
+  // trial k's sums of the open ancestors of field f's walk at level l < lt: s_stk[f][l][k]
+  __shared__ int s_stk[kFields][NSTK > 0 ? NSTK : 1][KR];
   __shared__ float s_sc[kFields];
   uint2 *list = A.tlist2 + (size_t)blockIdx.x * A.tlist_cap;
   const int jx = morton_x(lane), jy = morton_y(lane);
@@ -2329,10 +2375,8 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       s_kp[tid] = kp;
       s_sc[tid] = ldexpf(1.0f, kp);
     }
-    for(int q = tid; q < ntile; q += nt) {
-      tcnt[q] = 0;
-      tmask[q] = 0;
-    }
+    for(int q = tid; q < ntile; q += nt) tcnt[q] = 0;
+    for(int q = tid; q < nmask; q += nt) tmask[q] = 0;
     __syncthreads();
     const int R = s_off[K + 1];
     // 1. the tile lists: counts, offsets, entries.  A thread takes T16_LU records at a time,
@@ -2358,14 +2402,15 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     records([&](unsigned int rec, int o, int quad) {
       t16_visit(quad, L, [&](int m) {
         atomicAdd(&tcnt[m], 1);
-        if(!(tmask[m] & (1u << o))) atomicOr(&tmask[m], 1u << o);
+        if(!(t16_mask_get<KR>(tmask, m) & (1u << o))) t16_mask_or<KR>(tmask, m, 1u << o);
       });
     });
     __syncthreads();
-    if(wave == 0) tile_scan(tcnt, toff, tact, &s_nact, ntile, lane);
+    if(wave == 0) t16_scan(tcnt, toff, tact, &s_nact, ntile, lane);
     __syncthreads();
     if(toff[ntile] > A.tlist_cap) {
       // cannot happen when the caller sizes tlist_cap from k_log_totals; reported, not evaluated
+      for(int q = tid; q < ntile; q += nt) tcnt[q] = 0;
       if(tid == 0) {
         atomicOr(A.tile_err, 1u);
         A.accept[ti] = 0;
@@ -2379,6 +2424,8 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       t16_visit(quad, L, [&](int m) { list[toff[m] + atomicAdd(&tcnt[m], 1)] = make_uint2(rec, (unsigned int)o); });
     });
     __syncthreads();
+    // the counts shared the grids' words: zero again for the tiles
+    for(int q = tid; q < ntile; q += nt) tcnt[q] = 0;
     const int nact = s_nact;
     if(A.prof && tid == 0) {
       tp1 = wall_clock64();
@@ -2400,11 +2447,10 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     double acc2[(KR + 3) / 4], acc3 = 0.0;
 #pragma unroll
     for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = 0.0;
-    constexpr int NSTK = (LF ? LF : T16_MAX_LEVEL) - 4;
-    int stk[NSTK > 0 ? NSTK : 1];  // lane k: trial k's sums of the open ancestors, level l < lt
-#pragma unroll
-    for(int l = 0; l < (NSTK > 0 ? NSTK : 1); l++) stk[l] = 0;
-    double2 pc[4], p1, p2, p3, p4, up = make_double2(0.0, 0.0);
+    int *stk = &s_stk[f][0][0];  // [l][k], read and written by lane k of wave f only
+    if(lane < KR)
+      for(int l = 0; l < NSTK; l++) stk[l * KR + lane] = 0;
+    double2 pc[4], p1, up = make_double2(0.0, 0.0);
     auto load_pairs = [&](int m) {
       const int gz = t16_z(m) * T16 + 2 * jy, gr = t16_r(m) * T16 + 2 * jx;
       pc[0] = bwf[GALIC_STACKOFFSET(L, gz, gr)];
@@ -2412,12 +2458,9 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       pc[2] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr)];
       pc[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
       p1 = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
-      p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
-      p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
-      p4 = bwf[GALIC_STACKOFFSET(L - 4, gz >> 4, gr >> 4)];
     };
-    // closes trial k's open ancestors of the tile prev at levels from lt - 1 down to stop
-    // (with stop = 0: all of them); up holds their pairs (lane l: level l)
+    // closes trial k's open ancestors of the tile prev (all of them, or those m does not
+    // share) from level lt - 1 upwards; up holds their pairs (lane l: level l)
     auto close_levels = [&](int prev, int m, bool all) {
       bool open = true;
 #pragma unroll
@@ -2428,16 +2471,18 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
             open = false;
           } else {
             const double b = ldexp(__shfl(up.x, l), kp), w = __shfl(up.y, l);
-            accs = t16_term(b, w, (double)stk[l], accs);
-            if(l > 0) stk[l > 0 ? l - 1 : 0] += stk[l];
-            stk[l] = 0;
+            const int sl = lane < KR ? stk[l * KR + lane] : 0;
+            accs = t16_term(b, w, (double)sl, accs);
+            if(lane < KR) {
+              if(l > 0) stk[(l - 1) * KR + lane] += sl;
+              stk[l * KR + lane] = 0;
+            }
           }
         }
       }
     };
-    // Phase A's data is fetched ahead: during tile ai's pyramid each thread loads the values
-    // of its first entry of tile ai+1 (whose list entry came during tile ai-1) and its first
-    // list entry of tile ai+2, so a tile's adds seldom wait on memory.
+    // Phase A's list entries are fetched ahead: during tile ai's pyramid each thread loads its
+    // first entry of tile ai+2, so a tile's adds start with the entry in registers.
     auto fetch_entry = [&](int ai2) {
       uint2 en = make_uint2(0u, 0xffffffffu);
       if(ai2 < nact) {
@@ -2446,26 +2491,10 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       }
       return en;
     };
-    struct Vals {
-      int quad;
-      float4 x[kFields];
-    };
-    auto fetch_values = [&](uint2 en) {
-      Vals v;
-      v.quad = 0;
-      if(en.y != 0xffffffffu) {
-        const unsigned int rec = en.x;
-        const int blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
-        v.quad = A.log_q[rec];
-#pragma unroll
-        for(int f2 = 0; f2 < kFields; f2++) v.x[f2] = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
-      }
-      return v;
-    };
-    // the cells of one entry that fall in tile (tz, tr) into its log's grids
-    auto add_entry = [&](int o, const Vals &v, int tz, int tr) {
-      const int lz = (v.quad >> L) - tz * T16, lr = (v.quad & (nb - 1)) - tr * T16;
-      int *g = grids + o * kFields * 256;
+    // the cells of one entry inside tile (tz, tr): the trial's values into its grid, or the
+    // current orbit's values (log 0) out of every trial's grid
+    auto add_values = [&](int o, int quad, const float4 (&x)[kFields], int tz, int tr) {
+      const int lz = (quad >> L) - tz * T16, lr = (quad & (nb - 1)) - tr * T16;
 #pragma unroll
       for(int c = 0; c < 4; c++) {
         const int cz = lz + (c >> 1), cr = lr + (c & 1);
@@ -2473,42 +2502,44 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
           const int w = t16_cell(cz, cr);
 #pragma unroll
           for(int f2 = 0; f2 < kFields; f2++) {
-            const float x = c == 0 ? v.x[f2].x : c == 1 ? v.x[f2].y : c == 2 ? v.x[f2].z : v.x[f2].w;
-            if(x != 0.f) atomicAdd(&g[f2 * 256 + w], __float2int_rn(x * s_sc[f2]));
+            const float v = c == 0 ? x[f2].x : c == 1 ? x[f2].y : c == 2 ? x[f2].z : x[f2].w;
+            if(v != 0.f) {
+              const int u = __float2int_rn(v * s_sc[f2]);
+              if(o > 0) {
+                atomicAdd(&grids[((o - 1) * kFields + f2) * 256 + w], u);
+              } else {
+                for(int k = 0; k < K; k++) atomicAdd(&grids[(k * kFields + f2) * 256 + w], -u);
+              }
+            }
           }
         }
       }
     };
+    auto add_entry = [&](uint2 en, int tz, int tr) {
+      const unsigned int rec = en.x;
+      const int blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
+      float4 x[kFields];
+#pragma unroll
+      for(int f2 = 0; f2 < kFields; f2++) x[f2] = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
+      add_values((int)en.y, A.log_q[rec], x, tz, tr);
+    };
     uint2 pe_cur = fetch_entry(0);
-#if T16_PREFETCH_VALUES
-    Vals pv = fetch_values(pe_cur);
-#endif
     uint2 pe = fetch_entry(1);
     if(nact > 0) load_pairs(tact[0]);
     int prev = -1;
     for(int ai = 0; ai < nact; ai++) {
       const int m = tact[ai], tz = t16_z(m), tr = t16_r(m);
       if(A.prof && tid == 0) tpa = wall_clock64();
-      // phase A: the tile's entries into the K+1 grids of each field (the first one prefetched)
-#if T16_PREFETCH_VALUES
-      if(pe_cur.y != 0xffffffffu) add_entry((int)pe_cur.y, pv, tz, tr);
-#else
-      if(pe_cur.y != 0xffffffffu) add_entry((int)pe_cur.y, fetch_values(pe_cur), tz, tr);
-#endif
-      for(int e = toff[m] + tid + nt; e < toff[m + 1]; e += nt) {
-        const uint2 en = list[e];
-        add_entry((int)en.y, fetch_values(en), tz, tr);
-      }
+      // phase A: the tile's entries into the trials' grids (the first one fetched ahead)
+      if(pe_cur.y != 0xffffffffu) add_entry(pe_cur, tz, tr);
+      for(int e = toff[m] + tid + nt; e < toff[m + 1]; e += nt) add_entry(list[e], tz, tr);
       __syncthreads();
       if(A.prof && tid == 0) {
         tpb = wall_clock64();
         atomicAdd(&A.prof[9], tpb - tpa);
       }
-      // phase B: wave f, field f (the next tiles' entries and values on their way)
+      // phase B: wave f, field f (the next tiles' entries on their way)
       pe_cur = pe;
-#if T16_PREFETCH_VALUES
-      pv = fetch_values(pe_cur);
-#endif
       pe = fetch_entry(ai + 2);
       if(prev >= 0) close_levels(prev, m, false);
       if(lane < lt) {
@@ -2516,14 +2547,16 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         up = bwf[GALIC_STACKOFFSET(lane, t16_z(a), t16_r(a))];
       }
       {
+        // the 4x4, 8x8 and tile nodes' pairs, used after the trials (their loads overlap them)
+        const int gz = tz * T16 + 2 * jy, gr = tr * T16 + 2 * jx;
+        const double2 p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
+        const double2 p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
+        const double2 p4 = bwf[GALIC_STACKOFFSET(L - 4, tz, tr)];
         // residuals in the particle's units
 #pragma unroll
         for(int c = 0; c < 4; c++) pc[c].x = ldexp(pc[c].x, kp);
         p1.x = ldexp(p1.x, kp);
-        int4 *og = (int4 *)(grids + f * 256) + lane;
-        const int4 o4 = *og;
-        *og = make_int4(0, 0, 0, 0);
-        const unsigned int tm = tmask[m];
+        const unsigned int tm = t16_mask_get<KR>(tmask, m);
         // The coarser nodes' sums of trial k are handed to one lane each and their terms formed
         // after the loop: the 4x4 node's (in all four lanes of its quad after the quad sum) to
         // lane 4q + k % 4, register k / 4; the 8x8 node's (in all 16 lanes of its row) to lane
@@ -2531,32 +2564,35 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         int s2x[(KR + 3) / 4], s3x = 0, tsk = 0;
 #pragma unroll
         for(int u = 0; u < (KR + 3) / 4; u++) s2x[u] = 0;
+        int4 *g0 = (int4 *)(grids + f * 256) + lane;  // trial k's word: g0 + k * kFields * 64
+        int4 gn = *g0;
 #pragma unroll
         for(int k = 0; k < KR; k++) {
-          // (a compiler barrier per trial: its grid is read where it is used, not all trials'
-          // words at once, which would hold 4 registers per trial)
-          asm volatile("" ::: "memory");
-          if(k < K && (tm & (1u | (2u << k)))) {
-            int4 *gg = (int4 *)(grids + ((k + 1) * kFields + f) * 256) + lane;
-            const int4 g4 = *gg;
-            *gg = make_int4(0, 0, 0, 0);
-            const int d0 = g4.x - o4.x, d1 = g4.y - o4.y, d2 = g4.z - o4.z, d3 = g4.w - o4.w;
-            int s = (d0 + d1) + (d2 + d3);
-            double a = acc[k];
-            a = t16_term(pc[0].x, pc[0].y, (double)d0, a);
-            a = t16_term(pc[1].x, pc[1].y, (double)d1, a);
-            a = t16_term(pc[2].x, pc[2].y, (double)d2, a);
-            a = t16_term(pc[3].x, pc[3].y, (double)d3, a);
-            acc[k] = t16_term(p1.x, p1.y, (double)s, a);
-            s += dpp_i32<kDppQuadXor1>(s);
-            s += dpp_i32<kDppQuadXor2>(s);
-            s2x[k >> 2] = (lane & 3) == (k & 3) ? s : s2x[k >> 2];
-            s += dpp_i32<kDppRowRor4>(s);
-            s += dpp_i32<kDppRowRor8>(s);
-            s3x = (lane & 15) == k ? s : s3x;
-            const int s4 = __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 16) +
-                           __builtin_amdgcn_readlane(s, 32) + __builtin_amdgcn_readlane(s, 48);
-            tsk = lane == k ? s4 : tsk;
+          if(k < K) {
+            const int4 d = gn;
+            if(k + 1 < K) gn = g0[(k + 1) * kFields * 64];
+            // (a compiler barrier: trial k+1's word is read here, one trial ahead, not all
+            // trials' words at once, which would hold 4 registers per trial)
+            asm volatile("" ::: "memory");
+            if(tm & (1u | (2u << k))) {
+              g0[k * kFields * 64] = make_int4(0, 0, 0, 0);
+              int s = (d.x + d.y) + (d.z + d.w);
+              double a = acc[k];
+              a = t16_term(pc[0].x, pc[0].y, (double)d.x, a);
+              a = t16_term(pc[1].x, pc[1].y, (double)d.y, a);
+              a = t16_term(pc[2].x, pc[2].y, (double)d.z, a);
+              a = t16_term(pc[3].x, pc[3].y, (double)d.w, a);
+              acc[k] = t16_term(p1.x, p1.y, (double)s, a);
+              s += dpp_i32<kDppQuadXor1>(s);
+              s += dpp_i32<kDppQuadXor2>(s);
+              s2x[k >> 2] = (lane & 3) == (k & 3) ? s : s2x[k >> 2];
+              s += dpp_i32<kDppRowRor4>(s);
+              s += dpp_i32<kDppRowRor8>(s);
+              s3x = (lane & 15) == k ? s : s3x;
+              const int s4 = __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 16) +
+                             __builtin_amdgcn_readlane(s, 32) + __builtin_amdgcn_readlane(s, 48);
+              tsk = lane == k ? s4 : tsk;
+            }
           }
         }
         // the coarser nodes' terms of the lane's trials (0 for trials beyond K or not evaluated)
@@ -2564,9 +2600,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = t16_term(ldexp(p2.x, kp), p2.y, (double)s2x[u], acc2[u]);
         acc3 = t16_term(ldexp(p3.x, kp), p3.y, (double)s3x, acc3);
         accs = t16_term(ldexp(p4.x, kp), p4.y, (double)tsk, accs);
-#pragma unroll
-        for(int l = 0; l < NSTK; l++)
-          if(l == lt - 1) stk[l] += tsk;
+        if(lt > 0 && lane < KR) stk[(lt - 1) * KR + lane] += tsk;
       }
       prev = m;
       if(ai + 1 < nact) load_pairs(tact[ai + 1]);
@@ -2593,7 +2627,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         }
     }
     __syncthreads();
-    // 3. the choice
+    // 3. the choice (the taken trial's deltas are the region commit's, k_commit_*)
     for(int k = tid; k < K; k += nt) {
       const size_t q = (size_t)ti * K + k;
       const double *fit = A.fit + (size_t)t * 16;
@@ -2615,73 +2649,14 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       s_best = bk;
       A.accept[ti] = bk + 1;
       A.dF[ti] = (bk >= 0) ? bv : (K > 0 ? s_dF[0] : 0.0);
-    }
-    __syncthreads();
-    const int best = s_best;
-    if(A.prof && tid == 0) {
-      const unsigned long long tp2 = wall_clock64();
-      atomicAdd(&A.prof[1], tp2 - tp1);
-      tp1 = tp2;
-      if(best >= 0) atomicAdd(&A.prof[8], 1ull);
-    }
-    // 4. the deltas of the taken trial, exact in int64 at the log scale, on blocks of four
-    // consecutive tiles (32 x 32 cells; the whole 16 x 16 grid when it is one tile)
-    if(best >= 0 && !A.no_deltas) {
-      long long *G = (long long *)grids;  // the int32 grids are all zero here
-      long long *lad = A.locA_dq + (size_t)t * kFields * ngrid;
-      const unsigned int ob = (unsigned int)best + 1;
-      const int gsh = lt > 0 ? 2 : 0, side = lt > 0 ? 32 : 16;
-      for(int ai = 0; ai < nact;) {
-        const int grp = tact[ai] >> gsh;
-        int aj = ai;
-        unsigned int any = 0;
-        while(aj < nact && (tact[aj] >> gsh) == grp) any |= tmask[tact[aj++]];
-        if(any & (1u | (1u << ob))) {
-          const int g0 = grp << gsh, gz0 = t16_z(g0) * T16, gr0 = t16_r(g0) * T16;
-          for(int a2 = ai; a2 < aj; a2++) {
-            const int m = tact[a2], tz = t16_z(m) * T16, tr = t16_r(m) * T16;
-            for(int e = toff[m] + tid; e < toff[m + 1]; e += nt) {
-              const uint2 en = list[e];
-              if(en.y != 0 && en.y != ob) continue;
-              const unsigned int rec = en.x;
-              const int quad = A.log_q[rec], blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
-              const int lz = (quad >> L) - tz, lr = (quad & (nb - 1)) - tr;
-#pragma unroll
-              for(int f2 = 0; f2 < kFields; f2++) {
-                const float4 x = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
-                const float sc = en.y == 0 ? -A.log_scale[f2] : A.log_scale[f2];
-#pragma unroll
-                for(int c = 0; c < 4; c++) {
-                  const int cz = lz + (c >> 1), cr = lr + (c & 1);
-                  const float v = c == 0 ? x.x : c == 1 ? x.y : c == 2 ? x.z : x.w;
-                  if((unsigned)cz < (unsigned)T16 && (unsigned)cr < (unsigned)T16 && v != 0.f)
-                    atomicAdd((unsigned long long *)&G[f2 * 1024 + (tz - gz0 + cz) * 32 + (tr - gr0 + cr)],
-                              (unsigned long long)__float2ll_rn(v * sc));
-                }
-              }
-            }
-          }
-          __syncthreads();
-          for(int q = tid; q < kFields * side * side; q += nt) {
-            const int f2 = q / (side * side), cz = (q / side) % side, cr = q % side;
-            long long *gp = &G[f2 * 1024 + cz * 32 + cr];
-            const long long d = *gp;
-            if(d) {
-              *gp = 0;
-              atomic_q(&lad[(size_t)f2 * ngrid + (size_t)(gz0 + cz) * nb + gr0 + cr], (double)d * A.log_to_loc[f2]);
-            }
-          }
-          __syncthreads();
-        }
-        ai = aj;
+      if(A.prof) {
+        const unsigned long long te = wall_clock64();
+        atomicAdd(&A.prof[1], te - tp1);
+        if(bk >= 0) atomicAdd(&A.prof[8], 1ull);
+        atomicMax(&A.prof[4], te - tp0);
+        atomicAdd(&A.prof[11], (te - tp0) * (te - tp0) / 100);
+        atomicMax(&A.prof[12], (unsigned long long)toff[ntile]);
       }
-    }
-    if(A.prof && tid == 0) {
-      const unsigned long long te = wall_clock64();
-      atomicAdd(&A.prof[3], te - tp1);
-      atomicMax(&A.prof[4], te - tp0);
-      atomicAdd(&A.prof[11], (te - tp0) * (te - tp0) / 100);
-      atomicMax(&A.prof[12], (unsigned long long)toff[ntile]);
     }
   }
 }
