@@ -543,7 +543,8 @@ size_t slot_budget(galic::Ctx &c) {
 // entries scattered, then summed region chunk by region chunk (galic_commit.hip).  A.n items,
 // with A.accept: each trial's taken orbit added and its current orbit taken away; without:
 // every orbit added.
-constexpr int kCommitChunk = 16384;  // bucket entries one workgroup sums
+constexpr int kCommitChunk = 16384;        // bucket entries one workgroup sums
+constexpr double kCommitEntries = 1 << 28;  // bucket entries of one initial-pass commit (2 GiB)
 int commit_logs(galic::Ctx &c, galic::CommitArgs A, long long *dst) {
   if(A.n <= 0) return GALIC_OK;
   const int gsh = std::min(c.L, galic::kCommitRegionLog2);
@@ -640,12 +641,18 @@ int fp32_initial_logs(galic::Ctx &c, int n) {
       continue;  // the estimate has doubled: plan the chunk again
     }
     CHK(lr);
-    // every record of the chunk's orbits into the Loc grids
-    galic::CommitArgs C;
-    memset(&C, 0, sizeof(C));
-    C.n = m;
-    C.otype = c.o_type + a;
-    CHK(commit_logs(c, C, c.locA_q));
+    // every record of the chunk's orbits into the Loc grids, in runs of orbits whose bucket
+    // entries (about 1.3 per record) stay within kCommitEntries
+    const double per_orbit = std::max(1.0, c.log_est_blocks * galic::LOG_B * 1.3);
+    const int run = (int)std::max(1.0, std::min((double)m, (double)kCommitEntries / per_orbit));
+    for(int b0 = 0; b0 < m; b0 += run) {
+      galic::CommitArgs C;
+      memset(&C, 0, sizeof(C));
+      C.n = std::min(run, m - b0);
+      C.orbit_base = b0;
+      C.otype = c.o_type + a + b0;
+      CHK(commit_logs(c, C, c.locA_q));
+    }
     a += m;
     first = false;
   }

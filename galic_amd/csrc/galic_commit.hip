@@ -42,7 +42,7 @@ __device__ __forceinline__ void region_visit(int quad, int L, int gsh, Visit vis
 __device__ __forceinline__ int commit_logs(const CommitArgs &A, int b, size_t (&orb)[2], unsigned int (&sign)[2],
                                            int &t) {
   if(!A.accept) {
-    orb[0] = (size_t)b;
+    orb[0] = (size_t)A.orbit_base + b;
     sign[0] = 0;
     t = A.otype[b];
     return 1;

@@ -200,7 +200,8 @@ struct CommitArgs {
   const int *slot_pos;
   const int *accept;        // taken trial + 1 per trial; null: item b is orbit b, added whole
   const int *tidx, *ptype;  // with accept: particle of each trial, type of each particle
-  const int *otype;         // without: type of each orbit
+  const int *otype;         // without: type of each orbit (indexed by b)
+  int orbit_base;           // without: item b is orbit orbit_base + b of the logs
   const int *log_blocks, *log_n;
   int log_M;
   const int *log_q;
