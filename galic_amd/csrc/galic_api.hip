@@ -257,6 +257,10 @@ int build_weights(galic_ctx *ctx, bool with_cutB) {
   const int Ns = c.Nstack, L = c.L, nb = c.Nbin;
   ctx->weights.assign((size_t)GALIC_NTYPES * galic::W_COUNT * Ns, 0.0);
   ctx->cutB_of.assign((size_t)GALIC_NTYPES * c.Ngrid, 0);
+  // per 16 x 16-cell tile (Morton index), the levels of its cut-A nodes: bit u for level L-u
+  // inside the tile (u = 0..4), bit 5 when the tile lies inside a cut-A node above its level
+  const int lt = L - 4, ntile = lt >= 0 ? 1 << (2 * lt) : 0;
+  std::vector<unsigned char> tcut((size_t)GALIC_NTYPES * ntile, 0);
   std::vector<double> emr(Ns);
   for(int t = 1; t < GALIC_NTYPES; t++) {
     if(!type_present(c.p, t) || !c.targets_set[t]) continue;
@@ -265,6 +269,24 @@ int build_weights(galic_ctx *ctx, bool with_cutB) {
     double thr_d = c.p.MinParticlesPerBinForDensityMeasurement * c.p.MType[t] / c.p.NType[t];
     std::vector<char> cutA(Ns, 0);
     mark_cut(Tm.data(), thr_d, 0, 0, 0, L, cutA);
+    for(int tz = 0; tz < (1 << std::max(lt, 0)) && lt >= 0; tz++)
+      for(int tr = 0; tr < (1 << lt); tr++) {
+        unsigned char m = 0;
+        for(int u = 0; u <= 4; u++) {
+          const int n = 16 >> u, l = L - u;
+          for(int i = 0; i < n && !(m & (1 << u)); i++)
+            for(int j = 0; j < n; j++)
+              if(cutA[GALIC_STACKOFFSET(l, tz * n + i, tr * n + j)]) {
+                m |= (unsigned char)(1 << u);
+                break;
+              }
+        }
+        for(int l = 0; l < lt; l++)
+          if(cutA[GALIC_STACKOFFSET(l, tz >> (lt - l), tr >> (lt - l))]) m |= 32;
+        int code = 0;
+        for(int b = 0; b < 6; b++) code |= (((tr >> b) & 1) << (2 * b)) | (((tz >> b) & 1) << (2 * b + 1));
+        tcut[(size_t)t * ntile + code] = m;
+      }
     const double rlim = c.p.R200 / 20.0;
     for(int o = 0; o < Ns; o++) {
       if(!cutA[o]) continue;
@@ -297,6 +319,10 @@ int build_weights(galic_ctx *ctx, bool with_cutB) {
     }
   }
   HIPCHK(hipMemcpy(c.weights, ctx->weights.data(), sizeof(double) * ctx->weights.size(), hipMemcpyHostToDevice));
+  if(ntile > 0) {
+    if(!c.tcut) CHK(dalloc(&c.tcut, tcut.size()));
+    HIPCHK(hipMemcpy(c.tcut, tcut.data(), tcut.size(), hipMemcpyHostToDevice));
+  }
   if(with_cutB)
     HIPCHK(hipMemcpy(c.cutB_of, ctx->cutB_of.data(), sizeof(int) * ctx->cutB_of.size(), hipMemcpyHostToDevice));
   return GALIC_OK;
@@ -890,7 +916,7 @@ int galic_destroy(galic_ctx *ctx) {
   (void)hipStreamSynchronize(c.stream);
   dfree(c.FG_Pot); dfree(c.FG_DPotDR); dfree(c.FG_DPotDz); dfree(c.FG_Vstream);
   dfree(c.FG_Acc_f); dfree(c.FG_Vstream_f); dfree(c.DG_CellSize); dfree(c.DG_CellSize_f);
-  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw);
+  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw); dfree(c.tcut);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
   dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk); dfree(c.t_vinst);
@@ -2046,6 +2072,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
         E.log_tot = c.log_tot;
         E.vinst = c.t_vinst;
         E.prof = c.merit_prof;
+        E.tcut = c.tcut;
         if(galic::sort_trials_by_records(b - a, K, E.slot_pos, c.log_n, c.t_key, c.t_key2, c.t_ord, c.t_ord2,
                                          &c.sort_tmp, &c.sort_tmp_bytes, c.stream) != 0)
           return set_err(GALIC_ERR_HIP, "sorting the trials by record count failed");

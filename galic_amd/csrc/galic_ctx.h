@@ -86,6 +86,7 @@ struct Ctx {
   // grids
   double *stacks = nullptr;     // [type][ST_COUNT][Nstack]
   double *weights = nullptr;    // [type][W_COUNT][Nstack]
+  unsigned char *tcut = nullptr;  // [type][16x16 tile, Morton index] levels of the tile's cut-A nodes
   double *locA = nullptr;       // orbit-averaged Loc [type][5][Ngrid]
   double *locA_d = nullptr;     // deltas
   double *locI = nullptr;       // instantaneous Loc [type][4][Ngrid]

@@ -80,6 +80,8 @@ struct EvalArgs32 {
   uint2 *tlist2;           // 16x16-tile form: per workgroup, tlist_cap entries (record index, log)
   const float *log_tot;    // [orbit][5] sums of each log's values (k_orbit32_log)
   int no_deltas;           // the taken trial's deltas are left to the region commit (k_commit_*)
+  const unsigned char *tcut;  // 16x16-tile form: [type][tile] levels of the tile's cut-A nodes (bit u: L-u
+                              // inside the tile, bit 5: a cut-A node above the tile)
   const int *work_order;   // 16x16-tile form: trials in the order workgroups take them
   int *work_next;          // ... from this counter (zero at launch)
   unsigned long long *prof;  // sparse tile form, optional: time per phase and sizes, summed over particles
@@ -173,10 +175,10 @@ __global__ void k_eval_sparse32(DevModel M, EvalArgs32 A);
 constexpr int T16 = 16;
 constexpr int T16_MAX_K = 16;
 constexpr int T16_MAX_LEVEL = 10;  // Morton codes of up to 64 x 64 tiles; open ancestors of up to 8 levels
-// int32 words at the front of its LDS: the K grids D = G_k - O of a tile's five fields (the
-// tile counts share them while the lists are built).
+// int32 words at the front of its LDS: the K+1 grids (the current orbit's, then the trials')
+// of a tile's five fields (the tile counts share them while the lists are built).
 __host__ __device__ inline int t16_grid_words(int K, int L) {
-  const int a = K * kFields * T16 * T16, b = 1 << (2 * (L - 4));
+  const int a = (K + 1) * kFields * T16 * T16, b = 1 << (2 * (L - 4));
   return a > b ? a : b;
 }
 // words of the per-tile log masks: two tiles a word for up to 15 trials (KR the trial capacity)

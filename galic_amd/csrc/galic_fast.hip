@@ -2318,7 +2318,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
   const int L = LF ? LF : M.DG_L, nb = 1 << L, Nstack = GALIC_NSTACK(L);
   const int lt = L - 4, ntile = 1 << (2 * lt);
   const int gwords = t16_grid_words(K, L);
-  int *grids = lds_i;                                  // [K][5][256] int32: D = G_k - O; counts while listing
+  int *grids = lds_i;                                  // [K+1][5][256] int32: O, then G_k; counts while listing
   int *tcnt = grids;                                   // [ntile] (list building only)
   int *toff = grids + gwords;                          // [ntile + 1]
   unsigned short *tact = (unsigned short *)(toff + ntile + 1);  // [ntile] tiles with entries, Morton order
@@ -2491,8 +2491,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       }
       return en;
     };
-    // the cells of one entry inside tile (tz, tr): the trial's values into its grid, or the
-    // current orbit's values (log 0) out of every trial's grid
+    // the cells of one entry inside tile (tz, tr) into its log's grids
     auto add_values = [&](int o, int quad, const float4 (&x)[kFields], int tz, int tr) {
       const int lz = (quad >> L) - tz * T16, lr = (quad & (nb - 1)) - tr * T16;
 #pragma unroll
@@ -2503,14 +2502,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
 #pragma unroll
           for(int f2 = 0; f2 < kFields; f2++) {
             const float v = c == 0 ? x[f2].x : c == 1 ? x[f2].y : c == 2 ? x[f2].z : x[f2].w;
-            if(v != 0.f) {
-              const int u = __float2int_rn(v * s_sc[f2]);
-              if(o > 0) {
-                atomicAdd(&grids[((o - 1) * kFields + f2) * 256 + w], u);
-              } else {
-                for(int k = 0; k < K; k++) atomicAdd(&grids[(k * kFields + f2) * 256 + w], -u);
-              }
-            }
+            if(v != 0.f) atomicAdd(&grids[(o * kFields + f2) * 256 + w], __float2int_rn(v * s_sc[f2]));
           }
         }
       }
@@ -2530,7 +2522,8 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     for(int ai = 0; ai < nact; ai++) {
       const int m = tact[ai], tz = t16_z(m), tr = t16_r(m);
       if(A.prof && tid == 0) tpa = wall_clock64();
-      // phase A: the tile's entries into the trials' grids (the first one fetched ahead)
+      // phase A: the tile's entries into the logs' grids (the first one's list entry fetched
+      // ahead)
       if(pe_cur.y != 0xffffffffu) add_entry(pe_cur, tz, tr);
       for(int e = toff[m] + tid + nt; e < toff[m + 1]; e += nt) add_entry(list[e], tz, tr);
       __syncthreads();
@@ -2557,6 +2550,11 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         for(int c = 0; c < 4; c++) pc[c].x = ldexp(pc[c].x, kp);
         p1.x = ldexp(p1.x, kp);
         const unsigned int tm = t16_mask_get<KR>(tmask, m);
+        // the levels that hold cut-A nodes in this tile: only their terms can be nonzero (the
+        // weights vanish off the cut), and the node sums are formed only if a level above the
+        // cells needs them
+        const unsigned int cm = A.tcut[(size_t)t * ntile + m];
+        const bool cells = cm & 1u, node1 = cm & 2u, sums = cm & 0x3eu;
         // The coarser nodes' sums of trial k are handed to one lane each and their terms formed
         // after the loop: the 4x4 node's (in all four lanes of its quad after the quad sum) to
         // lane 4q + k % 4, register k / 4; the 8x8 node's (in all 16 lanes of its row) to lane
@@ -2564,43 +2562,53 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         int s2x[(KR + 3) / 4], s3x = 0, tsk = 0;
 #pragma unroll
         for(int u = 0; u < (KR + 3) / 4; u++) s2x[u] = 0;
-        int4 *g0 = (int4 *)(grids + f * 256) + lane;  // trial k's word: g0 + k * kFields * 64
+        int4 *og = (int4 *)(grids + f * 256) + lane;  // O's word; trial k's: og + (k + 1) * kFields * 64
+        const int4 o4 = *og;
+        *og = make_int4(0, 0, 0, 0);
+        int4 *g0 = og + kFields * 64;
         int4 gn = *g0;
 #pragma unroll
         for(int k = 0; k < KR; k++) {
           if(k < K) {
-            const int4 d = gn;
+            const int4 g4 = gn;
             if(k + 1 < K) gn = g0[(k + 1) * kFields * 64];
             // (a compiler barrier: trial k+1's word is read here, one trial ahead, not all
             // trials' words at once, which would hold 4 registers per trial)
             asm volatile("" ::: "memory");
             if(tm & (1u | (2u << k))) {
               g0[k * kFields * 64] = make_int4(0, 0, 0, 0);
-              int s = (d.x + d.y) + (d.z + d.w);
+              const int4 d = make_int4(g4.x - o4.x, g4.y - o4.y, g4.z - o4.z, g4.w - o4.w);
               double a = acc[k];
-              a = t16_term(pc[0].x, pc[0].y, (double)d.x, a);
-              a = t16_term(pc[1].x, pc[1].y, (double)d.y, a);
-              a = t16_term(pc[2].x, pc[2].y, (double)d.z, a);
-              a = t16_term(pc[3].x, pc[3].y, (double)d.w, a);
-              acc[k] = t16_term(p1.x, p1.y, (double)s, a);
-              s += dpp_i32<kDppQuadXor1>(s);
-              s += dpp_i32<kDppQuadXor2>(s);
-              s2x[k >> 2] = (lane & 3) == (k & 3) ? s : s2x[k >> 2];
-              s += dpp_i32<kDppRowRor4>(s);
-              s += dpp_i32<kDppRowRor8>(s);
-              s3x = (lane & 15) == k ? s : s3x;
-              const int s4 = __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 16) +
-                             __builtin_amdgcn_readlane(s, 32) + __builtin_amdgcn_readlane(s, 48);
-              tsk = lane == k ? s4 : tsk;
+              if(cells) {
+                a = t16_term(pc[0].x, pc[0].y, (double)d.x, a);
+                a = t16_term(pc[1].x, pc[1].y, (double)d.y, a);
+                a = t16_term(pc[2].x, pc[2].y, (double)d.z, a);
+                a = t16_term(pc[3].x, pc[3].y, (double)d.w, a);
+              }
+              if(sums) {
+                int s = (d.x + d.y) + (d.z + d.w);
+                if(node1) a = t16_term(p1.x, p1.y, (double)s, a);
+                s += dpp_i32<kDppQuadXor1>(s);
+                s += dpp_i32<kDppQuadXor2>(s);
+                s2x[k >> 2] = (lane & 3) == (k & 3) ? s : s2x[k >> 2];
+                s += dpp_i32<kDppRowRor4>(s);
+                s += dpp_i32<kDppRowRor8>(s);
+                s3x = (lane & 15) == k ? s : s3x;
+                const int s4 = __builtin_amdgcn_readlane(s, 0) + __builtin_amdgcn_readlane(s, 16) +
+                               __builtin_amdgcn_readlane(s, 32) + __builtin_amdgcn_readlane(s, 48);
+                tsk = lane == k ? s4 : tsk;
+              }
+              acc[k] = a;
             }
           }
         }
         // the coarser nodes' terms of the lane's trials (0 for trials beyond K or not evaluated)
+        if(cm & 4u)
 #pragma unroll
-        for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = t16_term(ldexp(p2.x, kp), p2.y, (double)s2x[u], acc2[u]);
-        acc3 = t16_term(ldexp(p3.x, kp), p3.y, (double)s3x, acc3);
-        accs = t16_term(ldexp(p4.x, kp), p4.y, (double)tsk, accs);
-        if(lt > 0 && lane < KR) stk[(lt - 1) * KR + lane] += tsk;
+          for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = t16_term(ldexp(p2.x, kp), p2.y, (double)s2x[u], acc2[u]);
+        if(cm & 8u) acc3 = t16_term(ldexp(p3.x, kp), p3.y, (double)s3x, acc3);
+        if(cm & 16u) accs = t16_term(ldexp(p4.x, kp), p4.y, (double)tsk, accs);
+        if((cm & 32u) && lt > 0 && lane < KR) stk[(lt - 1) * KR + lane] += tsk;
       }
       prev = m;
       if(ai + 1 < nact) load_pairs(tact[ai + 1]);
