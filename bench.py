@@ -45,9 +45,10 @@ F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_respo
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 vector rate
 PEAK_HBM = 8000.0     # GB/s
 TREE_MODE = TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER   # the force-field configuration's walk
-# HBM bytes per leapfrog step of k_orbit32_log in the full model's launches, from the FETCH_SIZE and
-# WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_composite-2e7.txt)
-HBM_BYTES_PER_STEP = 18.9
+# HBM bytes per leapfrog step of k_orbit32_log in the full model's timed launches, from the FETCH_SIZE
+# and WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_composite-2e7.txt:
+# 34.3 B fetched + 38.7 B written per step)
+HBM_BYTES_PER_STEP = 73.0
 
 
 def log(*a):
@@ -498,7 +499,7 @@ def main():
                      "note": f"roof: the FP32 vector ALU (no step is a contraction, MFMA unused); achieved = {F_STEP} "
                              "algorithmic flop/step x leapfrog steps / HIP-event time of the kernel; traffic = HBM "
                              f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE "
-                             "of the profile, almost all of it response-log writes)"},
+                             "of the profile: the response-log writes and the table lookups that miss L2)"},
         "steps_per_s": steps_total / wall,
         "ms_per_step_breakdown": breakdown,
         "fp32_initial_pass_s": init_s,
