@@ -131,15 +131,24 @@ int alloc_orbit_scratch(galic::Ctx &c, int n) {
   return GALIC_OK;
 }
 
-int alloc_trial_scratch(galic::Ctx &c, int n) {
-  if(n <= c.max_trials) return GALIC_OK;
-  dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
+// The sort buffers of a group of trials (part of the active orbit set).
+int alloc_sort_scratch(galic::Ctx &c, int n) {
+  if(n <= c.max_sort_trials) return GALIC_OK;
   dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv);
+  c.max_sort_trials = 0;
   CHK(dalloc(&c.t_inv, n));
   CHK(dalloc(&c.t_key, n));
   CHK(dalloc(&c.t_key2, n));
   CHK(dalloc(&c.t_ord, n));
   CHK(dalloc(&c.t_ord2, n));
+  c.max_sort_trials = n;
+  return GALIC_OK;
+}
+
+int alloc_trial_scratch(galic::Ctx &c, int n) {
+  CHK(alloc_sort_scratch(c, n));
+  if(n <= c.max_trials) return GALIC_OK;
+  dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
   CHK(dalloc(&c.t_idx, n));
   CHK(dalloc(&c.t_v, 3 * (size_t)n));
   CHK(dalloc(&c.t_accept, n));
@@ -160,6 +169,51 @@ int alloc_slots(galic::Ctx &c, int norbits, size_t bytes_per_orbit) {
   HIPCHK(hipMemset(c.slots, 0, need));
   c.slot_bytes = need;
   return GALIC_OK;
+}
+
+// Exchanges the active orbit set with the other one (see galic::OrbitSet).
+void swap_orbit_set(galic::Ctx &c) {
+#define GALIC_OS_SWAP(T, f) std::swap(c.f, c.alt.f);
+  GALIC_ORBIT_SET_FIELDS(GALIC_OS_SWAP)
+#undef GALIC_OS_SWAP
+  c.set_parity ^= 1;
+}
+
+// The other set's stream, events and counters (its arrays grow on demand when it is active).
+int ensure_alt_set(galic::Ctx &c) {
+  if(c.alt_ready) return GALIC_OK;
+  swap_orbit_set(c);
+  int rc = GALIC_OK;
+  c.log_M = 64;
+  if(hipStreamCreate(&c.stream) != hipSuccess || hipEventCreate(&c.ev0) != hipSuccess ||
+     hipEventCreate(&c.ev1) != hipSuccess)
+    rc = set_err(GALIC_ERR_HIP, "creating the second orbit stream failed");
+  if(rc == GALIC_OK) rc = dalloc(&c.counter, 1);
+  if(rc == GALIC_OK) rc = dalloc(&c.orbit_stats, 4);
+  if(rc == GALIC_OK) rc = dalloc(&c.log_next, 1);
+  swap_orbit_set(c);
+  if(rc == GALIC_OK) c.alt_ready = true;
+  return rc;
+}
+
+// Frees the active orbit set's memory, events and stream.
+void free_orbit_set(galic::Ctx &c) {
+  if(c.stream) (void)hipStreamSynchronize(c.stream);
+  dfree(c.o_pidx); dfree(c.o_type); dfree(c.o_x); dfree(c.o_y); dfree(c.o_z);
+  dfree(c.o_vx); dfree(c.o_vy); dfree(c.o_vz); dfree(c.o_mass); dfree(c.o_tint);
+  dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
+  dfree(c.o_steps); dfree(c.o_bbox); dfree(c.counter); dfree(c.orbit_stats);
+  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv);
+  dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_tot); dfree(c.log_next);
+  if(c.sort_tmp) (void)hipFree(c.sort_tmp);
+  c.sort_tmp = nullptr;
+  if(c.slots) (void)hipFree(c.slots);
+  c.slots = nullptr;
+  if(c.ev0) (void)hipEventDestroy(c.ev0);
+  if(c.ev1) (void)hipEventDestroy(c.ev1);
+  if(c.stream) (void)hipStreamDestroy(c.stream);
+  c.ev0 = c.ev1 = nullptr;
+  c.stream = nullptr;
 }
 
 galic::OrbitIn orbit_in(galic::Ctx &c) {
@@ -201,7 +255,8 @@ int orbit_grid(int n, int bs, int per_cu = 8) {
 // Steps and the first failure of an orbit launch (orbit_response.c:449-456 exits on the step cap).
 int finish_orbit_launch(galic::Ctx &c, int n) {
   unsigned long long st[3];
-  HIPCHK(hipMemcpy(st, c.orbit_stats, sizeof(st), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(st, c.orbit_stats, sizeof(st), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
   c.acc_steps += (long long)st[0];
   c.acc_orbits += n;
   if(st[1]) {
@@ -328,10 +383,10 @@ int build_weights(galic_ctx *ctx, bool with_cutB) {
   return GALIC_OK;
 }
 
-int global_fit(galic_ctx *ctx) {
+// The fit's kernels on c.stream (the first call also sets the cuts, which waits for the stream).
+int global_fit_launch(galic_ctx *ctx) {
   galic::Ctx &c = ctx->c;
   galic::FitArgs F = fit_args(ctx);
-  HIPCHK(hipEventRecord(c.ev0, c.stream));
   if(c.L >= 5) {
     // grids of 32 x 32 cells and more: the stacks in tiles, then the levels above them
     const int ntile = (c.Nbin / 32) * (c.Nbin / 32);
@@ -351,6 +406,13 @@ int global_fit(galic_ctx *ctx) {
   hipLaunchKernelGGL(galic::k_fit_final, dim3(3), dim3(64), 0, c.stream, F, (const double *)c.fit_part,
                      galic::kFitBlocks);
   HIPCHK(hipGetLastError());
+  return GALIC_OK;
+}
+
+int global_fit(galic_ctx *ctx) {
+  galic::Ctx &c = ctx->c;
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  CHK(global_fit_launch(ctx));
   HIPCHK(hipEventRecord(c.ev1, c.stream));
   HIPCHK(hipEventSynchronize(c.ev1));
   HIPCHK(hipEventElapsedTime(&c.last_fit_ms, c.ev0, c.ev1));
@@ -474,67 +536,87 @@ double log_blocks_planned(const galic::Ctx &c, int n) {
 
 constexpr int kLogArenaFull = 1;  // launch_orbit32_log: the arena ran out, plan a smaller launch
 
-// One launch of the logging orbit kernel over A.n orbits into the whole arena.
-// If an orbit needs more blocks than its table holds, the table grows and the
-// launch is repeated (the orbits and their logs do not depend on the launch, so
-// the result is the same).  If the arena runs out, the block estimate doubles and
-// kLogArenaFull asks the caller for a smaller launch.
+constexpr int kLogTableFull = 2;  // orbit32_log_end: an orbit outgrew its block table (log_M doubled)
+
+// One launch of the logging orbit kernel over A.n orbits into the active orbit set's arena, on
+// that set's stream, not waited on: the tables sized, the counters cleared, the launch bracketed
+// by the set's events.  orbit32_log_end waits for it.
+int orbit32_log_begin(galic::Ctx &c, galic::OrbitArgs32 &A) {
+  const size_t bb = log_block_bytes();
+  const size_t cap = c.slot_bytes / bb;
+  if(cap < 64) return set_err(GALIC_ERR_NOMEM, "no room for orbit response logs");
+  const size_t nt = (size_t)A.n * c.log_M;
+  if(nt > c.log_blocks_len) {
+    dfree(c.log_blocks);
+    c.log_blocks_len = 0;
+    CHK(dalloc(&c.log_blocks, nt));
+    c.log_blocks_len = nt;
+  }
+  if(A.n > c.log_n_len) {
+    dfree(c.log_n);
+    dfree(c.log_tot);
+    c.log_n_len = 0;
+    CHK(dalloc(&c.log_n, (size_t)A.n));
+    CHK(dalloc(&c.log_tot, (size_t)A.n * galic::kFields));
+    c.log_n_len = A.n;
+  }
+  A.log_blocks = c.log_blocks;
+  A.log_n = c.log_n;
+  A.log_tot = c.log_tot;
+  A.log_M = c.log_M;
+  A.log_q = c.log_q = (int *)c.slots;
+  A.log_v = c.log_v = (float4 *)((int *)c.slots + cap * galic::LOG_B);
+  A.log_next = c.log_next;
+  A.log_cap = (unsigned int)std::min<size_t>(cap, 0xf0000000u);  // claims past the cap stay below 2^32
+  HIPCHK(hipMemsetAsync(c.log_next, 0, sizeof(unsigned int), c.stream));
+  HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
+  HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 4 * sizeof(unsigned long long), c.stream));
+  HIPCHK(hipEventRecord(c.ev0, c.stream));
+  hipLaunchKernelGGL(galic::k_orbit32_log, dim3(orbit_grid(A.n, 256, c.fp32_bpc)), dim3(256), 0, c.stream,
+                     fast_model(c), A);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev1, c.stream));
+  return GALIC_OK;
+}
+
+// Waits for the launch of orbit32_log_begin.  If an orbit needed more blocks than its table
+// holds, the table grows and kLogTableFull asks for the same launch again (the orbits and their
+// logs do not depend on the launch, so the result is the same); if the arena ran out, the block
+// estimate doubles and kLogArenaFull asks the caller for a smaller launch.
+int orbit32_log_end(galic::Ctx &c, const galic::OrbitArgs32 &A) {
+  const size_t cap = c.slot_bytes / log_block_bytes();
+  HIPCHK(hipEventSynchronize(c.ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  unsigned long long st[4];
+  unsigned int used = 0;
+  HIPCHK(hipMemcpyAsync(st, c.orbit_stats, sizeof(st), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipMemcpyAsync(&used, c.log_next, sizeof(used), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if(st[3] == 0) {
+    c.last_orbit_ms = ms;
+    c.acc_orbit_ms += ms;
+    c.acc_orbit_launches++;
+    c.log_est_blocks = std::max(1.0, (double)std::min<size_t>(used, cap) / A.n);
+    return finish_orbit_launch(c, A.n);
+  }
+  c.acc_log_relaunches++;
+  if((size_t)used >= cap) {
+    c.log_est_blocks *= 2;
+    return kLogArenaFull;
+  }
+  c.log_M *= 2;  // an orbit needed more blocks than its table holds
+  return kLogTableFull;
+}
+
+// One launch of the logging orbit kernel, waited on (tables grown and the launch repeated as
+// needed); kLogArenaFull if the arena ran out.
 int launch_orbit32_log(galic::Ctx &c, galic::OrbitArgs32 A) {
   if(A.n <= 0) return GALIC_OK;
-  const size_t bb = log_block_bytes();
   for(int attempt = 0; attempt < 8; attempt++) {
-    const size_t cap = c.slot_bytes / bb;
-    if(cap < 64) return set_err(GALIC_ERR_NOMEM, "no room for orbit response logs");
-    const size_t nt = (size_t)A.n * c.log_M;
-    if(nt > c.log_blocks_len) {
-      dfree(c.log_blocks);
-      CHK(dalloc(&c.log_blocks, nt));
-      c.log_blocks_len = nt;
-    }
-    if(A.n > c.log_n_len) {
-      dfree(c.log_n);
-      dfree(c.log_tot);
-      c.log_n_len = 0;
-      CHK(dalloc(&c.log_n, (size_t)A.n));
-      CHK(dalloc(&c.log_tot, (size_t)A.n * galic::kFields));
-      c.log_n_len = A.n;
-    }
-    A.log_blocks = c.log_blocks;
-    A.log_n = c.log_n;
-    A.log_tot = c.log_tot;
-    A.log_M = c.log_M;
-    A.log_q = c.log_q = (int *)c.slots;
-    A.log_v = c.log_v = (float4 *)((int *)c.slots + cap * galic::LOG_B);
-    A.log_next = c.log_next;
-    A.log_cap = (unsigned int)std::min<size_t>(cap, 0xf0000000u);  // claims past the cap stay below 2^32
-    HIPCHK(hipMemsetAsync(c.log_next, 0, sizeof(unsigned int), c.stream));
-    HIPCHK(hipMemsetAsync(c.counter, 0, sizeof(unsigned int), c.stream));
-    HIPCHK(hipMemsetAsync(c.orbit_stats, 0, 4 * sizeof(unsigned long long), c.stream));
-    HIPCHK(hipEventRecord(c.ev0, c.stream));
-    hipLaunchKernelGGL(galic::k_orbit32_log, dim3(orbit_grid(A.n, 256, c.fp32_bpc)), dim3(256), 0, c.stream,
-                       fast_model(c), A);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c.ev1, c.stream));
-    HIPCHK(hipEventSynchronize(c.ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-    unsigned long long st[4];
-    unsigned int used = 0;
-    HIPCHK(hipMemcpy(st, c.orbit_stats, sizeof(st), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&used, c.log_next, sizeof(used), hipMemcpyDeviceToHost));
-    if(st[3] == 0) {
-      c.last_orbit_ms = ms;
-      c.acc_orbit_ms += ms;
-      c.acc_orbit_launches++;
-      c.log_est_blocks = std::max(1.0, (double)std::min<size_t>(used, cap) / A.n);
-      return finish_orbit_launch(c, A.n);
-    }
-    c.acc_log_relaunches++;
-    if((size_t)used >= cap) {
-      c.log_est_blocks *= 2;
-      return kLogArenaFull;
-    }
-    c.log_M *= 2;  // an orbit needed more blocks than its table holds
+    CHK(orbit32_log_begin(c, A));
+    const int r = orbit32_log_end(c, A);
+    if(r != kLogTableFull) return r;
   }
   return set_err(GALIC_ERR_NOMEM, "orbit response logs still overflow after growing the block tables");
 }
@@ -634,10 +716,77 @@ int commit_logs(galic::Ctx &c, galic::CommitArgs A, long long *dst) {
   HIPCHK(hipGetLastError());
   const size_t shr = sizeof(long long) * galic::kFields << (2 * gsh);
   hipLaunchKernelGGL(galic::k_commit_regions, dim3((unsigned int)chunks.size()), dim3(256), shr, c.stream, A,
-                     (const int4 *)c.cm_chunks, dst);
+                     (const int4 *)c.cm_chunks, dst, (const unsigned int *)nullptr);
   HIPCHK(hipGetLastError());
   // (the host vectors of the copies above must outlive them)
   HIPCHK(hipStreamSynchronize(c.stream));
+  return GALIC_OK;
+}
+
+// Room for a commit of at most cap_entries bucket entries laid out on the device
+// (commit_logs_launch); cap_entries <= 2^31 - 1.
+int ensure_commit_room(galic::Ctx &c, size_t cap_entries, size_t *cap_chunks) {
+  const int gsh = std::min(c.L, galic::kCommitRegionLog2);
+  const int nreg = 1 << (c.L - gsh), nk = GALIC_NTYPES * nreg * nreg;
+  if(c.cm_nk < nk) {
+    dfree(c.cm_cnt);
+    dfree(c.cm_start);
+    c.cm_nk = 0;
+    CHK(dalloc(&c.cm_cnt, 2 * (size_t)nk));
+    CHK(dalloc(&c.cm_start, (size_t)nk));
+    c.cm_nk = nk;
+  }
+  if(c.cm_buf_len < cap_entries) {
+    dfree(c.cm_buf);
+    c.cm_buf_len = 0;
+    CHK(dalloc(&c.cm_buf, cap_entries));
+    c.cm_buf_len = cap_entries;
+  }
+  *cap_chunks = (size_t)nk + cap_entries / kCommitChunk + 1;
+  if(c.cm_chunks_len < *cap_chunks) {
+    dfree(c.cm_chunks);
+    c.cm_chunks_len = 0;
+    CHK(dalloc(&c.cm_chunks, *cap_chunks));
+    c.cm_chunks_len = *cap_chunks;
+  }
+  if(!c.cm_aux) CHK(dalloc(&c.cm_aux, 2));
+  return GALIC_OK;
+}
+
+// commit_logs with the bucket layout made on the device (k_commit_layout), nothing waited for
+// on the host, so that it can be captured in a graph.  The room (ensure_commit_room) must hold
+// the commit's entries; if it does not, cm_aux[1] is set and nothing is committed.
+int commit_logs_launch(galic::Ctx &c, galic::CommitArgs A, long long *dst, size_t cap_entries, size_t cap_chunks,
+                       int ncu) {
+  if(A.n <= 0) return GALIC_OK;
+  const int gsh = std::min(c.L, galic::kCommitRegionLog2);
+  const int nreg = 1 << (c.L - gsh), ng = nreg * nreg, nk = GALIC_NTYPES * ng;
+  A.L = c.L;
+  A.gsh = gsh;
+  A.log_blocks = c.log_blocks;
+  A.log_n = c.log_n;
+  A.log_M = c.log_M;
+  A.log_q = c.log_q;
+  A.log_v = c.log_v;
+  for(int f = 0; f < galic::kFields; f++) A.q[f] = c.fp32_scale[f];
+  A.gcnt = c.cm_cnt;
+  A.gcur = c.cm_cnt + nk;
+  A.gstart = c.cm_start;
+  A.gbuf = c.cm_buf;
+  A.gerr = c.cm_aux + 1;
+  hipLaunchKernelGGL(galic::k_fill_u32, dim3(nblocks(2 * (size_t)nk, 256)), dim3(256), 0, c.stream, c.cm_cnt, 0u,
+                     2 * (size_t)nk);
+  const int grid = std::min(A.n, ncu * 8);
+  const size_t sh = sizeof(unsigned int) * ng;
+  hipLaunchKernelGGL(galic::k_commit_count, dim3(grid), dim3(256), sh, c.stream, A);
+  hipLaunchKernelGGL(galic::k_commit_layout, dim3(1), dim3(1024), 0, c.stream, (const unsigned int *)c.cm_cnt, nk,
+                     kCommitChunk, c.cm_start, c.cm_chunks, c.cm_aux, (unsigned int)cap_entries,
+                     (unsigned int)cap_chunks, c.cm_aux + 1);
+  hipLaunchKernelGGL(galic::k_commit_scatter, dim3(grid), dim3(256), sh, c.stream, A);
+  const size_t shr = sizeof(long long) * galic::kFields << (2 * gsh);
+  hipLaunchKernelGGL(galic::k_commit_regions, dim3((unsigned int)cap_chunks), dim3(256), shr, c.stream, A,
+                     (const int4 *)c.cm_chunks, dst, (const unsigned int *)c.cm_aux);
+  HIPCHK(hipGetLastError());
   return GALIC_OK;
 }
 
@@ -919,33 +1068,28 @@ int galic_destroy(galic_ctx *ctx) {
   dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw); dfree(c.tcut);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
-  dfree(c.fit); dfree(c.counter); dfree(c.orbit_stats); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk); dfree(c.t_vinst);
+  dfree(c.fit); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk); dfree(c.t_vinst);
   galic::Particles &P = c.P;
   dfree(P.x); dfree(P.y); dfree(P.z); dfree(P.vx); dfree(P.vy); dfree(P.vz);
   dfree(P.mass); dfree(P.tint); dfree(P.vesc); dfree(P.vr2); dfree(P.vt2); dfree(P.vp2); dfree(P.vq2);
   dfree(P.type); dfree(P.orbits); dfree(P.disp);
-  dfree(c.o_pidx); dfree(c.o_type); dfree(c.o_x); dfree(c.o_y); dfree(c.o_z);
-  dfree(c.o_vx); dfree(c.o_vy); dfree(c.o_vz); dfree(c.o_mass); dfree(c.o_tint);
-  dfree(c.o_tall); dfree(c.o_e0); dfree(c.o_e1); dfree(c.o_norbit); dfree(c.o_status);
-  dfree(c.o_steps); dfree(c.o_bbox);
-  dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best);
-  dfree(c.t_key); dfree(c.t_key2); dfree(c.t_ord); dfree(c.t_ord2); dfree(c.t_inv); dfree(c.psteps);
-  dfree(c.log_blocks); dfree(c.log_n); dfree(c.log_tot); dfree(c.log_next);
+  dfree(c.t_idx); dfree(c.t_v); dfree(c.t_accept); dfree(c.t_dF); dfree(c.t_best); dfree(c.psteps);
   for(int t = 0; t < GALIC_NTYPES; t++) dfree(c.disp_tab[t]);
   dfree(c.cs_scratch);
   dfree(c.g32_scratch);
   dfree(c.tile_list);
   dfree(c.tile_aux);
-  dfree(c.cm_cnt); dfree(c.cm_start); dfree(c.cm_chunks); dfree(c.cm_buf);
+  dfree(c.cm_cnt); dfree(c.cm_start); dfree(c.cm_chunks); dfree(c.cm_buf); dfree(c.cm_aux); dfree(c.cm_bound);
   dfree(c.merit_prof);
   dfree(c.lookup_buf);
   dfree(c.fit_part);
   dfree(c.tsrc);
-  if(c.sort_tmp) (void)hipFree(c.sort_tmp);
-  if(c.slots) (void)hipFree(c.slots);
-  if(c.ev0) (void)hipEventDestroy(c.ev0);
-  if(c.ev1) (void)hipEventDestroy(c.ev1);
-  if(c.stream) (void)hipStreamDestroy(c.stream);
+  if(c.alt_ready) {
+    swap_orbit_set(c);
+    free_orbit_set(c);
+    swap_orbit_set(c);
+  }
+  free_orbit_set(c);
   if(g_bound == ctx) g_bound = nullptr;
   delete ctx;
   return GALIC_OK;
@@ -1798,20 +1942,45 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       HIPCHK(hipMemcpy(c.t_vk, tv.data(), sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
     }
   }
-  // the log arena: as much of the free memory as the call could use, kept across calls
+  // the log arena: as much of the free memory as the call could use, kept across calls; split
+  // between the two orbit sets when the call's orbits need more than one launch, so the next
+  // group's orbits integrate while a group is judged
+  bool overlap = false;
   {
     const size_t bb = log_block_bytes();
     const size_t all = (size_t)(log_blocks_planned(c, std::max(ntrial, 1) * (K + 1)) * bb);
     const size_t one = (size_t)(log_blocks_planned(c, std::min(std::max(ntrial, 1), 1024) * (K + 1)) * bb);
-    CHK(ensure_arena(c, std::min(all, slot_budget(c)), one));
+    const size_t budget = slot_budget(c);
+    overlap = c.overlap == 1 || (c.overlap < 0 && all > budget / 2);
+    if(overlap) {
+      CHK(ensure_alt_set(c));
+      // each set's arena about half of what one arena could take (the one left by earlier calls
+      // shrinks first, so that the other has room)
+      const size_t half = std::max(one, (budget + c.alt.slot_bytes) / 2);
+      auto fit_half = [&]() -> int {
+        if(c.slot_bytes >= one && c.slot_bytes <= half + half / 8 && 2 * c.slot_bytes >= half) return GALIC_OK;
+        if(c.slots) (void)hipFree(c.slots);
+        c.slots = nullptr;
+        c.slot_bytes = 0;
+        return ensure_arena(c, half, one);
+      };
+      CHK(fit_half());
+      swap_orbit_set(c);
+      const int rc = fit_half();
+      swap_orbit_set(c);
+      CHK(rc);
+    } else {
+      CHK(ensure_arena(c, std::min(all, budget), one));
+    }
   }
-  // trials whose orbits' logs fit in the arena at the current estimate
+  // trials whose orbits' logs fit in the arena at the current estimate (and the caller's cap)
   auto group_cap = [&]() {
     const double blocks = (double)(c.slot_bytes / log_block_bytes());
     const double per_trial = (K + 1) * (c.log_est_blocks + 1.0) * 1.25;
     const double lanes = (double)orbit_grid(std::max(ntrial, 1) * (K + 1), 256, c.fp32_bpc) * 256;
-    return (int)std::max(1.0, std::min((double)std::max(ntrial, 1),
-                                       (blocks - 64 - lanes * galic::LOG_CHUNK) / per_trial));
+    double cap = std::min((double)std::max(ntrial, 1), (blocks - 64 - lanes * galic::LOG_CHUNK) / per_trial);
+    if(c.max_group_trials > 0) cap = std::min(cap, (double)c.max_group_trials);
+    return (int)std::max(1.0, cap);
   };
   int cap = group_cap();
   const int nrep = (ntrial + rep_size - 1) / rep_size;
@@ -1878,8 +2047,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
   }
   bool packed = false;
   float eval_ms = 0;
-  int s0 = 0;
-  while(s0 < ntrial) {
+  // the end of the group that starts at trial s0
+  auto plan = [&](int s0) -> int {
     int s1;
     const int r0 = s0 / rep_size;
     const int rep_end0 = std::min(ntrial, (r0 + 1) * rep_size);
@@ -1908,7 +2077,12 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
         s1 = e;
       }
     }
+    return s1;
+  };
+  // trial draws, orbit order and the orbits' launch of group [s0, s1) on the active orbit set
+  auto start = [&](int s0, int s1, galic::OrbitArgs32 &A) -> int {
     const int ng = s1 - s0;
+    CHK(alloc_sort_scratch(c, std::max(ntrial, 1)));
     if(!vtry) {
       // trials drawn from the particles' current velocities (unchanged by earlier reps
       // of the group: its particles are distinct)
@@ -1929,17 +2103,31 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
                        (const double *)(vty + (size_t)s0 * K), (const double *)(vtz + (size_t)s0 * K), c.P,
                        orbit_in(c));
     HIPCHK(hipGetLastError());
-    {
-      const int lr = launch_orbit32_log(c, orbit_args32(c, ng * (K + 1)));
-      if(lr == kLogArenaFull) {
-        // the logs were longer than planned: plan this group again, smaller
-        if(ng <= 1) return set_err(GALIC_ERR_NOMEM, "the response logs of one trial do not fit in device memory");
-        cap = group_cap();
+    A = orbit_args32(c, ng * (K + 1));
+    return orbit32_log_begin(c, A);
+  };
+  // waits for the active set's launch: a table that overflowed grows and the launch repeats; if
+  // the arena ran out, the group is planned again, smaller, and launched again
+  auto finish = [&](int s0, int &s1, galic::OrbitArgs32 &A) -> int {
+    for(int attempt = 0; attempt < 64; attempt++) {
+      const int r = orbit32_log_end(c, A);
+      if(r == GALIC_OK) return GALIC_OK;
+      if(r == kLogTableFull) {
+        CHK(orbit32_log_begin(c, A));
         continue;
       }
-      CHK(lr);
+      if(r != kLogArenaFull) return r;
+      if(s1 - s0 <= 1) return set_err(GALIC_ERR_NOMEM, "the response logs of one trial do not fit in device memory");
+      cap = group_cap();
+      s1 = plan(s0);
+      CHK(start(s0, s1, A));
     }
-    cap = group_cap();
+    return set_err(GALIC_ERR_NOMEM, "orbit response logs still overflow after growing the block tables");
+  };
+  // the reps of group [s0, s1), its orbits integrated: merit, choice, commit and fit per rep
+  auto judge = [&](int s0, int s1) -> int {
+    const int r0 = s0 / rep_size;
+    const int ng = s1 - s0;
     // (the block tables may have grown with this launch: the form is chosen after it)
     const bool tiles = !t16 && use_tiles();
     const bool sparse = !t16 && use_sparse();
@@ -2008,16 +2196,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
         c.tile_list_len = words;
       }
     }
-    // evaluate the reps of the group in turn, committing each completed rep
-    for(int r = r0; r < nrep && r * rep_size < s1; r++) {
-      const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
-      const int a = std::max(ra, s0), b = std::min(rb, s1);
-      if(!packed) {
-        hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream,
-                           (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.Nstack);
-        HIPCHK(hipGetLastError());
-        packed = true;
-      }
+    // the merit launch arguments of trials [a, ...) of the group
+    auto eval_args = [&](int a) {
       galic::EvalArgs32 E;
       memset(&E, 0, sizeof(E));
       E.P = c.P;
@@ -2063,38 +2243,168 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.iscale = c.fp32_iscale;
       E.gscratch = c.g32_scratch;
       E.no_deltas = 1;
-      HIPCHK(hipEventRecord(c.ev0, c.stream));
-      if(t16) {
-        E.nblk = b - a;
-        E.tlist2 = (uint2 *)c.tile_list;
-        E.tlist_cap = tile_cap;
-        E.tile_err = c.tile_aux + 1;
-        E.log_tot = c.log_tot;
-        E.vinst = c.t_vinst;
-        E.prof = c.merit_prof;
-        E.tcut = c.tcut;
+      return E;
+    };
+    // the 16x16-tile merit of trials [a, b) of the group and the write-back of each choice; the
+    // launch order by decreasing record count made by hipcub, or in one workgroup (device_order,
+    // which a graph can hold: hipcub's sort sets memory with memset nodes)
+    auto t16_rep = [&](galic::EvalArgs32 &E, int a, int b, bool device_order) -> int {
+      E.nblk = b - a;
+      E.tlist2 = (uint2 *)c.tile_list;
+      E.tlist_cap = tile_cap;
+      E.tile_err = c.tile_aux + 1;
+      E.log_tot = c.log_tot;
+      E.vinst = c.t_vinst;
+      E.prof = c.merit_prof;
+      E.tcut = c.tcut;
+      E.work_order = c.t_ord2;
+      E.work_next = (int *)c.tile_aux + 2;
+      if(device_order) {
+        int m = 1;
+        while(m < b - a) m <<= 1;
+        hipLaunchKernelGGL(galic::k_record_order, dim3(1), dim3(1024), sizeof(unsigned long long) * m, c.stream, b - a,
+                           K, E.slot_pos, (const int *)c.log_n, c.t_ord2);
+        hipLaunchKernelGGL(galic::k_fill_u32, dim3(1), dim3(64), 0, c.stream, c.tile_aux + 2, 0u, (size_t)1);
+      } else {
         if(galic::sort_trials_by_records(b - a, K, E.slot_pos, c.log_n, c.t_key, c.t_key2, c.t_ord, c.t_ord2,
                                          &c.sort_tmp, &c.sort_tmp_bytes, c.stream) != 0)
           return set_err(GALIC_ERR_HIP, "sorting the trials by record count failed");
-        E.work_order = c.t_ord2;
-        E.work_next = (int *)c.tile_aux + 2;
         HIPCHK(hipMemsetAsync(E.work_next, 0, sizeof(int), c.stream));
-        const int grid = std::min(b - a, tile_grid);
-        // instantaneous terms before, the write-back of the choice after the merit kernel
-        hipLaunchKernelGGL(galic::k_trial_inst, dim3(nblocks((size_t)(b - a) * K, 256)), dim3(256), 0, c.stream, c.M, E,
-                           c.t_vinst);
-        const dim3 tb(64 * galic::kFields);
-        if(c.L == 9 && t16_kr == 8)
-          hipLaunchKernelGGL((galic::k_eval_t16<9, 8>), dim3(grid), tb, shmem, c.stream, c.M, E);
-        else if(c.L == 9)
-          hipLaunchKernelGGL((galic::k_eval_t16<9, 16>), dim3(grid), tb, shmem, c.stream, c.M, E);
-        else if(t16_kr == 8)
-          hipLaunchKernelGGL((galic::k_eval_t16<0, 8>), dim3(grid), tb, shmem, c.stream, c.M, E);
-        else
-          hipLaunchKernelGGL((galic::k_eval_t16<0, 16>), dim3(grid), tb, shmem, c.stream, c.M, E);
+      }
+      const int grid = std::min(b - a, tile_grid);
+      // instantaneous terms before, the write-back of the choice after the merit kernel
+      hipLaunchKernelGGL(galic::k_trial_inst, dim3(nblocks((size_t)(b - a) * K, 256)), dim3(256), 0, c.stream, c.M, E,
+                         c.t_vinst);
+      const dim3 tb(64 * galic::kFields);
+      if(c.L == 9 && t16_kr == 8)
+        hipLaunchKernelGGL((galic::k_eval_t16<9, 8>), dim3(grid), tb, shmem, c.stream, c.M, E);
+      else if(c.L == 9)
+        hipLaunchKernelGGL((galic::k_eval_t16<9, 16>), dim3(grid), tb, shmem, c.stream, c.M, E);
+      else if(t16_kr == 8)
+        hipLaunchKernelGGL((galic::k_eval_t16<0, 8>), dim3(grid), tb, shmem, c.stream, c.M, E);
+      else
+        hipLaunchKernelGGL((galic::k_eval_t16<0, 16>), dim3(grid), tb, shmem, c.stream, c.M, E);
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(galic::k_apply_best, dim3(nblocks(b - a, 256)), dim3(256), 0, c.stream, c.M, E);
+      HIPCHK(hipGetLastError());
+      return GALIC_OK;
+    };
+    // the commit arguments of trials [a, b): the taken trials' orbits into the Loc deltas, the
+    // orbits they replace out
+    auto commit_args = [&](const galic::EvalArgs32 &E, int a, int b) {
+      galic::CommitArgs C;
+      memset(&C, 0, sizeof(C));
+      C.n = b - a;
+      C.K = K;
+      C.trial_base = a;
+      C.slot_pos = E.slot_pos;
+      C.accept = c.t_accept;
+      C.tidx = c.t_idx;
+      C.ptype = c.P.type;
+      return C;
+    };
+    // The reps of the group captured in one graph (the 16x16-tile form) and launched once:
+    // nothing is waited for between the kernels of the reps.  The commits' bucket room comes from
+    // a bound read back once per group (k_commit_bound), the rest of the host's work per rep
+    // (the bucket layout, the launch order) is done on the device.
+    int nrep_g = 0, max_n = 0;
+    for(int r = r0; r < nrep && r * rep_size < s1; r++) {
+      nrep_g++;
+      max_n = std::max(max_n, std::min({ntrial, (r + 1) * rep_size, s1}) - std::max(r * rep_size, s0));
+    }
+    int m_order = 1;
+    while(m_order < max_n) m_order <<= 1;
+    if(t16 && c.use_graph && c.cuts_ready && max_n <= galic::kOrderMax &&
+       sizeof(unsigned long long) * m_order <= lds_max) {
+      if(c.cm_bound_len < nrep_g) {
+        dfree(c.cm_bound);
+        c.cm_bound_len = 0;
+        CHK(dalloc(&c.cm_bound, (size_t)nrep_g));
+        c.cm_bound_len = nrep_g;
+      }
+      hipLaunchKernelGGL(galic::k_fill_u32, dim3(nblocks(2 * (size_t)nrep_g, 256)), dim3(256), 0, c.stream,
+                         (unsigned int *)c.cm_bound, 0u, 2 * (size_t)nrep_g);
+      hipLaunchKernelGGL(galic::k_commit_bound, dim3(nblocks(ng, 256)), dim3(256), 0, c.stream, ng, K, s0, rep_size,
+                         (const int *)c.t_inv, (const int *)c.log_n, c.cm_bound);
+      HIPCHK(hipGetLastError());
+      std::vector<unsigned long long> bound(nrep_g);
+      HIPCHK(hipMemcpyAsync(bound.data(), c.cm_bound, sizeof(unsigned long long) * nrep_g, hipMemcpyDeviceToHost,
+                            c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+      size_t room = 1;
+      for(unsigned long long v : bound) room = std::max(room, (size_t)v);
+      if(room <= 0x7fffffffu) {
+        size_t cap_chunks = 0;
+        CHK(ensure_commit_room(c, room, &cap_chunks));
+        int ncu = 256;
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device);
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        int rc = GALIC_OK;
+        hipLaunchKernelGGL(galic::k_fill_u32, dim3(1), dim3(64), 0, c.stream, c.cm_aux, 0u, (size_t)2);
+        for(int r = r0; rc == GALIC_OK && r < nrep && r * rep_size < s1; r++) {
+          const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
+          const int a = std::max(ra, s0), b = std::min(rb, s1);
+          if(!packed) {
+            hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream,
+                               (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.Nstack);
+            packed = true;
+          }
+          galic::EvalArgs32 E = eval_args(a);
+          rc = t16_rep(E, a, b, true);
+          if(rc == GALIC_OK) rc = commit_logs_launch(c, commit_args(E, a, b), c.locA_dq, room, cap_chunks, ncu);
+          if(rc == GALIC_OK && b == rb) {
+            rc = commit_q(c);
+            if(rc == GALIC_OK) rc = global_fit_launch(ctx);
+            packed = false;
+          }
+        }
+        const hipError_t ec = hipStreamEndCapture(c.stream, &graph);
+        if(rc != GALIC_OK || ec != hipSuccess) {
+          if(graph) (void)hipGraphDestroy(graph);
+          return rc != GALIC_OK ? rc
+                                : set_err(GALIC_ERR_HIP, "capturing the reps of a group failed: %s",
+                                          hipGetErrorString(ec));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if(ei != hipSuccess)
+          return set_err(GALIC_ERR_HIP, "instantiating the reps' graph failed: %s", hipGetErrorString(ei));
+        hipError_t el = hipEventRecord(c.ev0, c.stream);
+        if(el == hipSuccess) el = hipGraphLaunch(exec, c.stream);
+        if(el == hipSuccess) el = hipEventRecord(c.ev1, c.stream);
+        if(el == hipSuccess) el = hipEventSynchronize(c.ev1);
+        (void)hipGraphExecDestroy(exec);
+        if(el != hipSuccess) return set_err(GALIC_ERR_HIP, "the reps' graph failed: %s", hipGetErrorString(el));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        eval_ms += ms;
+        c.acc_eval_ms += ms;
+        c.acc_graph_launches++;
+        unsigned int terr = 0, cerr = 0;
+        HIPCHK(hipMemcpyAsync(&terr, c.tile_aux + 1, sizeof(terr), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipMemcpyAsync(&cerr, c.cm_aux + 1, sizeof(cerr), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        if(terr) return set_err(GALIC_ERR_STATE, "the tile lists of a particle outgrew their room");
+        if(cerr) return set_err(GALIC_ERR_STATE, "a commit outgrew its bound on bucket entries");
+        return GALIC_OK;
+      }
+    }
+    // evaluate the reps of the group in turn, committing each completed rep
+    for(int r = r0; r < nrep && r * rep_size < s1; r++) {
+      const int ra = r * rep_size, rb = std::min(ntrial, ra + rep_size);
+      const int a = std::max(ra, s0), b = std::min(rb, s1);
+      if(!packed) {
+        hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream,
+                           (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.Nstack);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(galic::k_apply_best, dim3(nblocks(b - a, 256)), dim3(256), 0, c.stream, c.M, E);
-        HIPCHK(hipGetLastError());
+        packed = true;
+      }
+      galic::EvalArgs32 E = eval_args(a);
+      HIPCHK(hipEventRecord(c.ev0, c.stream));
+      if(t16) {
+        CHK(t16_rep(E, a, b, false));
       } else if(tiles) {
         E.nblk = b - a;
         E.tlist = c.tile_list;
@@ -2158,21 +2468,14 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       c.acc_eval_ms += ms;
       if(tiles || t16) {
         unsigned int terr = 0;
-        HIPCHK(hipMemcpy(&terr, c.tile_aux + 1, sizeof(terr), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(&terr, c.tile_aux + 1, sizeof(terr), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
         if(terr) return set_err(GALIC_ERR_STATE, "the tile lists of a particle outgrew their room");
       }
       {
         // the taken trials' orbits into the Loc deltas, the orbits they replace out (timed
         // with the merit evaluation)
-        galic::CommitArgs C;
-        memset(&C, 0, sizeof(C));
-        C.n = b - a;
-        C.K = K;
-        C.trial_base = a;
-        C.slot_pos = E.slot_pos;
-        C.accept = c.t_accept;
-        C.tidx = c.t_idx;
-        C.ptype = c.P.type;
+        galic::CommitArgs C = commit_args(E, a, b);
         HIPCHK(hipEventRecord(c.ev0, c.stream));
         CHK(commit_logs(c, C, c.locA_dq));
         HIPCHK(hipEventRecord(c.ev1, c.stream));
@@ -2188,8 +2491,52 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
         packed = false;
       }
     }
-    s0 = s1;
+    return GALIC_OK;
+  };
+  // Group after group.  With two orbit sets, the next group's orbits integrate on the other set
+  // while a group is judged, when the two groups' particles are distinct: the next group's trial
+  // draws and orbits then depend on nothing the judging changes, so the results are the bits of
+  // one group after the other.
+  std::vector<char> in_group(overlap ? c.P.n : 0, 0);
+  galic::OrbitArgs32 A_cur, A_next;
+  int s0 = 0, s1 = 0;
+  if(ntrial > 0) {
+    s1 = plan(0);
+    CHK(start(0, s1, A_cur));
   }
+  while(s0 < ntrial) {
+    CHK(finish(s0, s1, A_cur));
+    cap = group_cap();
+    int n0 = s1, n1 = s1;
+    bool ahead = false;
+    if(overlap && n0 < ntrial) {
+      for(int i = s0; i < s1; i++) in_group[idx[i]] = 1;
+      n1 = plan(n0);
+      bool distinct = true;
+      for(int i = n0; i < n1 && distinct; i++) distinct = !in_group[idx[i]];
+      for(int i = s0; i < s1; i++) in_group[idx[i]] = 0;
+      if(distinct) {
+        swap_orbit_set(c);
+        const int rc = start(n0, n1, A_next);
+        swap_orbit_set(c);
+        CHK(rc);
+        ahead = true;
+      }
+    }
+    CHK(judge(s0, s1));
+    if(n0 < ntrial) {
+      if(ahead) {
+        swap_orbit_set(c);
+        A_cur = A_next;
+      } else {
+        n1 = plan(n0);
+        CHK(start(n0, n1, A_cur));
+      }
+    }
+    s0 = n0;
+    s1 = n1;
+  }
+  if(c.set_parity) swap_orbit_set(c);  // the first set active again
   if(ntrial == 0) {
     CHK(commit_q(c));
     CHK(global_fit(ctx));
@@ -2215,6 +2562,15 @@ int galic_fp32_merit_profile(galic_ctx *ctx, int on, unsigned long long *out16) 
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form) {
   if(!ctx || form < 0 || form > 4) return set_err(GALIC_ERR_ARG, "merit form must be 0, 1, 2, 3 or 4");
   ctx->c.merit_form = form;
+  return GALIC_OK;
+}
+
+int galic_fp32_set_schedule(galic_ctx *ctx, int max_group_trials, int overlap, int graph) {
+  if(!ctx || max_group_trials < 0 || overlap < -1 || overlap > 1 || graph < 0 || graph > 1)
+    return set_err(GALIC_ERR_ARG, "max_group_trials must be >= 0, overlap -1, 0 or 1 and graph 0 or 1");
+  ctx->c.max_group_trials = max_group_trials;
+  ctx->c.overlap = overlap;
+  ctx->c.use_graph = graph;
   return GALIC_OK;
 }
 
@@ -2244,19 +2600,20 @@ int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk) {
   return GALIC_OK;
 }
 
-int galic_timings(galic_ctx *ctx, double *out10) {
-  if(!ctx || !out10) return set_err(GALIC_ERR_ARG, "null argument");
+int galic_timings(galic_ctx *ctx, double *out11) {
+  if(!ctx || !out11) return set_err(GALIC_ERR_ARG, "null argument");
   galic::Ctx &c = ctx->c;
-  out10[0] = c.last_orbit_ms;
-  out10[1] = c.last_eval_ms;
-  out10[2] = c.last_fit_ms;
-  out10[3] = c.acc_orbit_ms;
-  out10[4] = (double)c.acc_orbit_launches;
-  out10[5] = (double)c.acc_orbits;
-  out10[6] = (double)c.acc_steps;
-  out10[7] = c.acc_eval_ms;
-  out10[8] = c.acc_fit_ms;
-  out10[9] = (double)c.acc_log_relaunches;
+  out11[0] = c.last_orbit_ms;
+  out11[1] = c.last_eval_ms;
+  out11[2] = c.last_fit_ms;
+  out11[3] = c.acc_orbit_ms;
+  out11[4] = (double)c.acc_orbit_launches;
+  out11[5] = (double)c.acc_orbits;
+  out11[6] = (double)c.acc_steps;
+  out11[7] = c.acc_eval_ms;
+  out11[8] = c.acc_fit_ms;
+  out11[9] = (double)c.acc_log_relaunches;
+  out11[10] = (double)c.acc_graph_launches;
   return GALIC_OK;
 }
 
@@ -2270,6 +2627,7 @@ int galic_reset_timings(galic_ctx *ctx) {
   c.acc_eval_ms = 0;
   c.acc_fit_ms = 0;
   c.acc_log_relaunches = 0;
+  c.acc_graph_launches = 0;
   return GALIC_OK;
 }
 

@@ -10,8 +10,9 @@
 //
 // Few global atomics: the records are first bucketed by region of GS x GS cells (32 x 32, or
 // the whole grid below 32 x 32) -- k_commit_count counts the entries of each region (LDS
-// counts per particle, one global atomic per particle and region), the host turns the counts
-// into bucket offsets and chunks, k_commit_scatter writes (record, sign) entries into the
+// counts per particle, one global atomic per particle and region), the host (or k_commit_layout,
+// where the commit is captured in a graph) turns the counts into bucket offsets and chunks,
+// k_commit_scatter writes (record, sign) entries into the
 // buckets -- and k_commit_regions sums a chunk of one region's entries in LDS (int64 atomics)
 // and adds each nonzero cell once to the destination grid.
 
@@ -102,6 +103,7 @@ __global__ void __launch_bounds__(256) k_commit_count(CommitArgs A) {
 // room in its bucket (gstart + cursor), then the entries at LDS cursors.
 __global__ void __launch_bounds__(256) k_commit_scatter(CommitArgs A) {
   extern __shared__ unsigned int cnt_s[];
+  if(A.gerr && *A.gerr) return;
   const int nreg = 1 << (A.L - A.gsh), ng = nreg * nreg;
   for(int b = blockIdx.x; b < A.n; b += gridDim.x) {
     size_t orb[2];
@@ -132,8 +134,12 @@ __global__ void __launch_bounds__(256) k_commit_scatter(CommitArgs A) {
 // One workgroup per chunk (type, region, entries [lo, hi) of its bucket): each entry's cells
 // inside the region, round(q[f] * v) with the entry's sign, into five int64 LDS grids of the
 // region; then every nonzero cell once into dst[type][f][cell] (a global atomic).
-__global__ void __launch_bounds__(256) k_commit_regions(CommitArgs A, const int4 *chunks, long long *dst) {
+// (nchunk: the number of chunks when the grid was sized for at most that many; null when the
+// grid is exactly one workgroup per chunk)
+__global__ void __launch_bounds__(256) k_commit_regions(CommitArgs A, const int4 *chunks, long long *dst,
+                                                        const unsigned int *nchunk) {
   extern __shared__ long long reg_s[];  // [5][GS * GS]
+  if(nchunk && blockIdx.x >= *nchunk) return;
   const int gs = 1 << A.gsh, nreg = 1 << (A.L - A.gsh), nb = 1 << A.L, ngrid = nb * nb;
   const int4 ch = chunks[blockIdx.x];  // (type * nreg^2 + region, lo, hi, 0)
   const int t = ch.x / (nreg * nreg), g = ch.x % (nreg * nreg);
@@ -170,6 +176,69 @@ __global__ void __launch_bounds__(256) k_commit_regions(CommitArgs A, const int4
                 (unsigned long long)v);
     }
   }
+}
+
+// The bucket layout of k_commit_regions on the device (one workgroup of 1024): gstart[k] the
+// first entry of bucket k (the buckets in order), and chunks of at most `chunk` entries of one
+// bucket each, nchunk of them.  If the entries or chunks exceed the room (cap_entries,
+// cap_chunks), nothing is laid out: nchunk = 0 and *err = 1.
+__global__ void __launch_bounds__(1024) k_commit_layout(const unsigned int *gcnt, int nk, int chunk,
+                                                        unsigned int *gstart, int4 *chunks, unsigned int *nchunk,
+                                                        unsigned int cap_entries, unsigned int cap_chunks,
+                                                        unsigned int *err) {
+  __shared__ unsigned long long s_e[1024];
+  __shared__ unsigned int s_c[1024];
+  const int tid = threadIdx.x, per = (nk + 1023) / 1024;
+  const int lo = min(nk, tid * per), hi = min(nk, lo + per);
+  unsigned long long e = 0;
+  unsigned int c = 0;
+  for(int k = lo; k < hi; k++) {
+    e += gcnt[k];
+    c += (gcnt[k] + chunk - 1) / chunk;
+  }
+  s_e[tid] = e;
+  s_c[tid] = c;
+  __syncthreads();
+  for(int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long ve = tid >= off ? s_e[tid - off] : 0;
+    const unsigned int vc = tid >= off ? s_c[tid - off] : 0;
+    __syncthreads();
+    s_e[tid] += ve;
+    s_c[tid] += vc;
+    __syncthreads();
+  }
+  const unsigned long long tot_e = s_e[1023];
+  const unsigned int tot_c = s_c[1023];
+  if(tot_e > cap_entries || tot_c > cap_chunks) {
+    if(tid == 0) {
+      *nchunk = 0;
+      *err = 1;
+    }
+    return;
+  }
+  unsigned int e0 = (unsigned int)(s_e[tid] - e), c0 = s_c[tid] - c;
+  for(int k = lo; k < hi; k++) {
+    const unsigned int n = gcnt[k];
+    gstart[k] = e0;
+    for(unsigned int l = 0; l < n; l += chunk)
+      chunks[c0++] = make_int4(k, (int)(e0 + l), (int)(e0 + min(n, l + chunk)), 0);
+    e0 += n;
+  }
+  if(tid == 0) *nchunk = tot_c;
+}
+
+// Room a commit of trials [0, n) of a group needs, per rep: a trial adds at most its largest
+// trial orbit's records and takes away its current orbit's, each record in at most 4 region
+// buckets.  Trial b is trial first + b, of rep (first + b) / rep_size - first / rep_size.
+__global__ void k_commit_bound(int n, int K, int first, int rep_size, const int *slot_pos, const int *log_n,
+                               unsigned long long *rep_bound) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if(b >= n) return;
+  const size_t o0 = (size_t)slot_pos[b] * (K + 1);
+  int mx = 0;
+  for(int k = 1; k <= K; k++) mx = max(mx, log_n[o0 + k]);
+  const unsigned long long e = 4ull * ((unsigned long long)mx + (unsigned long long)max(log_n[o0], 0));
+  atomicAdd(&rep_bound[(first + b) / rep_size - first / rep_size], e);
 }
 
 }  // namespace galic

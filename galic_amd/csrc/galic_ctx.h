@@ -66,6 +66,27 @@ struct Particles {
   double *disp = nullptr;  // target vr2, vt2, vp2, vq2 per particle (set_particles.c:158-161)
 };
 
+// The per-launch state of the FP32 path's orbit integration: the stream and events it runs
+// on, the log arena, the per-orbit inputs and outputs, the response logs' tables and counters,
+// and the sort buffers of a group of trials.  A context holds two sets, the active one in its
+// own fields and the other in Ctx::alt: one group's orbits integrate on one set's stream while
+// the previous group's reps are judged with the other's (swap_orbit_set in galic_api.hip).
+#define GALIC_ORBIT_SET_FIELDS(X) \
+  X(hipStream_t, stream) X(hipEvent_t, ev0) X(hipEvent_t, ev1) X(void *, slots) X(size_t, slot_bytes) \
+  X(int, max_slot_orbits) X(int, max_orbits) X(int *, o_pidx) X(int *, o_type) X(double *, o_x) X(double *, o_y) \
+  X(double *, o_z) X(double *, o_vx) X(double *, o_vy) X(double *, o_vz) X(double *, o_mass) X(double *, o_tint) \
+  X(double *, o_tall) X(double *, o_e0) X(double *, o_e1) X(int *, o_norbit) X(int *, o_status) \
+  X(long long *, o_steps) X(int4 *, o_bbox) X(unsigned int *, counter) X(unsigned long long *, orbit_stats) \
+  X(int, max_sort_trials) X(unsigned int *, t_key) X(unsigned int *, t_key2) X(int *, t_ord) X(int *, t_ord2) \
+  X(int *, t_inv) X(void *, sort_tmp) X(size_t, sort_tmp_bytes) X(int *, log_blocks) X(size_t, log_blocks_len) \
+  X(int *, log_n) X(float *, log_tot) X(int, log_n_len) X(int, log_M) X(unsigned int *, log_next) X(int *, log_q) \
+  X(float4 *, log_v)
+struct OrbitSet {
+#define GALIC_OS_FIELD(T, f) T f{};
+  GALIC_ORBIT_SET_FIELDS(GALIC_OS_FIELD)
+#undef GALIC_OS_FIELD
+};
+
 struct Ctx {
   galic_params p;
   int device = 0;
@@ -138,6 +159,7 @@ struct Ctx {
   unsigned int *t_key = nullptr, *t_key2 = nullptr;  // trial sort keys (expected orbit steps)
   int *t_ord = nullptr, *t_ord2 = nullptr;           // trial order, longest expected orbit first
   int *t_inv = nullptr;                              // chunk position of each trial
+  int max_sort_trials = 0;                           // capacity of the five arrays above
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int *psteps = nullptr;  // per particle: steps of its current orbit (FP32 path's length prediction)
@@ -155,6 +177,12 @@ struct Ctx {
   size_t cm_chunks_len = 0;
   uint2 *cm_buf = nullptr;           // bucket entries
   size_t cm_buf_len = 0;
+  // reps of a group captured in one hipGraph (the layout above made on the device)
+  unsigned int *cm_aux = nullptr;          // [0] chunks laid out, [1] room exceeded
+  unsigned long long *cm_bound = nullptr;  // [rep of the group] entries a commit may need
+  int cm_bound_len = 0;
+  int use_graph = 1;                       // capture a group's reps (merit, choice, commit, fit)
+  long long acc_graph_launches = 0;
   int log_n_len = 0;
   int log_M = 64;                // block-table entries per orbit (doubled when an orbit needs more)
   unsigned int *log_next = nullptr;
@@ -186,6 +214,12 @@ struct Ctx {
   size_t lookup_len = 0;         // points it holds   // [0] largest records of a particle's logs, [1] list overflow flag
   unsigned long long *merit_prof = nullptr;  // [16] phase times of the sparse tile merit (galic_fp32_merit_profile)
   int merit_form = 0;                 // 0 automatic, 1 32x32 tiles at every level, 2 LDS grids / HBM scratch (no tiles), 3 sparse 32x32 tiles, 4 16x16 tiles
+  OrbitSet alt;                       // the other orbit set (GALIC_ORBIT_SET_FIELDS); allocated on first use
+  bool alt_ready = false;
+  int set_parity = 0;                 // 1 while the sets are swapped
+  int overlap = -1;                   // FP32 reps: integrate the next group's orbits while judging a group
+                                      // (-1 when a call needs more than one launch, 0 never, 1 always)
+  int max_group_trials = 0;           // FP32 reps: at most this many trials per orbit launch (0: the arena's)
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

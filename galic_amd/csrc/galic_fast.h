@@ -214,11 +214,22 @@ struct CommitArgs {
   const unsigned int *gstart;  // [type][region] first entry of the bucket
   uint2 *gbuf;              // entries (arena record index, sign: 0 add, 1 take away)
   double q[kFields];        // Loc fixed-point units per response unit
+  const unsigned int *gerr; // set: the layout did not fit its room, k_commit_scatter writes nothing
 };
 constexpr int kCommitRegionLog2 = 5;
 __global__ void k_commit_count(CommitArgs A);
 __global__ void k_commit_scatter(CommitArgs A);
-__global__ void k_commit_regions(CommitArgs A, const int4 *chunks, long long *dst);
+__global__ void k_commit_regions(CommitArgs A, const int4 *chunks, long long *dst, const unsigned int *nchunk);
+__global__ void k_commit_layout(const unsigned int *gcnt, int nk, int chunk, unsigned int *gstart, int4 *chunks,
+                                unsigned int *nchunk, unsigned int cap_entries, unsigned int cap_chunks,
+                                unsigned int *err);
+__global__ void k_commit_bound(int n, int K, int first, int rep_size, const int *slot_pos, const int *log_n,
+                               unsigned long long *rep_bound);
+__global__ void k_fill_u32(unsigned int *p, unsigned int v, size_t n);
+// Trials [0, n) of a merit launch in order of decreasing record count (ties by index), one
+// workgroup of 1024, n <= kOrderMax
+constexpr int kOrderMax = 16384;
+__global__ void k_record_order(int n, int K, const int *slot_pos, const int *log_n, int *ord);
 
 __global__ void k_commit_q(long long *locq, long long *dq, double *loc, size_t n_per_field, int nfield,
                            const double *inv_scale, size_t n);

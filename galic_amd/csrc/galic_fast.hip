@@ -656,6 +656,44 @@ __global__ void k_record_keys(int n, int K, const int *slot_pos, const int *log_
 int sort_desc(int n, unsigned int *key, unsigned int *key2, int *ord, int *ord2, void **tmp, size_t *tmp_bytes,
               hipStream_t stream);
 
+__global__ void k_fill_u32(unsigned int *p, unsigned int v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) p[i] = v;
+}
+
+// The order of k_record_keys + sort_desc in one workgroup: (inverted key, index) pairs in LDS
+// (dynamic, 8 bytes per element of the power of two above n), bitonic sort ascending.
+__global__ void __launch_bounds__(1024) k_record_order(int n, int K, const int *slot_pos, const int *log_n, int *ord) {
+  extern __shared__ unsigned long long ko[];
+  int m = 1;
+  while(m < n) m <<= 1;
+  for(int b = threadIdx.x; b < m; b += blockDim.x) {
+    unsigned long long v = ~0ull;
+    if(b < n) {
+      const size_t o0 = (size_t)slot_pos[b] * (K + 1);
+      unsigned int s = 0;
+      for(int o = 0; o <= K; o++) s += (unsigned int)max(log_n[o0 + o], 0);
+      v = ((unsigned long long)(~s) << 32) | (unsigned int)b;
+    }
+    ko[b] = v;
+  }
+  __syncthreads();
+  for(int size = 2; size <= m; size <<= 1)
+    for(int stride = size >> 1; stride > 0; stride >>= 1) {
+      for(int i = threadIdx.x; i < m / 2; i += blockDim.x) {
+        const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const unsigned long long a = ko[lo], c = ko[hi];
+        if((a > c) == up) {
+          ko[lo] = c;
+          ko[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  for(int b = threadIdx.x; b < n; b += blockDim.x) ord[b] = (int)(unsigned int)ko[b];
+}
+
 // Trials in order of decreasing expected orbit length (stable, so the order is
 // reproducible): chunks then hold orbits of similar length and the longest
 // start first.

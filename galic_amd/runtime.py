@@ -150,6 +150,13 @@ class GalicGPU:
         3 sparse 32x32 LDS tiles at every level, 4 16x16 tiles in each particle's int32 fixed point."""
         check(self.lib.galic_fp32_set_merit_form(self.h, int(form)))
 
+    def fp32_set_schedule(self, max_group_trials=0, overlap=-1, graph=1):
+        """Group the FP32 reps' orbit launches at most max_group_trials trials each (0: as many as
+        the log arena holds); overlap 1 integrates the next group while a group is judged, 0 never,
+        -1 when a call needs more than one launch; graph 1 captures a group's reps in one hipGraph
+        (16x16-tile merit), 0 launches them rep by rep.  The results do not depend on the schedule."""
+        check(self.lib.galic_fp32_set_schedule(self.h, int(max_group_trials), int(overlap), int(graph)))
+
     def fp32_set_fresh_trials(self, nfresh):
         """The first nfresh device trials per particle are fresh draws (main.c:383-435)."""
         check(self.lib.galic_fp32_set_fresh_trials(self.h, int(nfresh)))
@@ -201,10 +208,10 @@ class GalicGPU:
         return acc, pot
 
     def timings(self):
-        out = np.zeros(10)
+        out = np.zeros(11)
         check(self.lib.galic_timings(self.h, dptr(out)))
         keys = ["last_orbit_ms", "last_eval_ms", "last_fit_ms", "orbit_ms", "orbit_launches", "orbits", "steps",
-                "eval_ms", "fit_ms", "log_relaunches"]
+                "eval_ms", "fit_ms", "log_relaunches", "graph_launches"]
         return dict(zip(keys, out.tolist()))
 
     def reset_timings(self):

@@ -222,6 +222,17 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
  * an orbit's total per record), its deltas are the same int64 sums. */
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form);
 
+/* Schedule of the FP32 reps (no counterpart in the reference, whose reps are one loop).
+ * The trials of a call are integrated in groups of whole reps, one orbit launch per group;
+ * max_group_trials > 0 caps a group at that many trials (0: as many as the log arena holds).
+ * overlap = 1: the next group's orbits are integrated on a second stream while a group's reps
+ * are judged (when the two groups' particles are distinct), 0: one group after the other,
+ * -1 (default): overlap when a call's orbits need more than one launch.  graph = 1 (default):
+ * with the 16x16-tile merit, the reps of a group (merit, choice, commit, fit) are captured in
+ * one hipGraph and launched once, 0: launched rep by rep with the host between them.  The
+ * results are the same bits whatever the schedule. */
+int galic_fp32_set_schedule(galic_ctx *ctx, int max_group_trials, int overlap, int graph);
+
 /* Timing aid of the sparse tile merit: on != 0 starts (or restarts) summing, per phase of
  * each particle, the wall-clock ticks (100 MHz) of the kernel's thread 0; out16 (optional)
  * first receives the sums so far: [0] tile lists, [1] fields and tiles, [2] choice, [3]
@@ -290,8 +301,9 @@ int galic_get_force_tables(galic_ctx *ctx, double *pot, double *dpotdr, double *
  * out[2] fit ms, out[3] accumulated orbit-kernel ms, out[4] orbit-kernel
  * launches, out[5] orbits integrated, out[6] leapfrog steps integrated,
  * out[7] accumulated merit-kernel ms, out[8] accumulated fit ms, out[9] orbit
- * launches repeated to give the response logs more room. */
-int galic_timings(galic_ctx *ctx, double *out10);
+ * launches repeated to give the response logs more room, out[10] graphs of reps
+ * launched (their whole time, fit included, counts as merit time). */
+int galic_timings(galic_ctx *ctx, double *out11);
 /* A plain device copy of nbytes (reps times): the known-traffic kernel that
  * calibrates the HBM byte counters; ms_per_copy receives the mean duration. */
 int galic_copy_probe(int device, size_t nbytes, int reps, double *ms_per_copy);

@@ -120,6 +120,29 @@ def test_fp32_iterate_equals_rep_by_rep(w):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("group,overlap,graph", [(700, 1, 1), (1000, 1, 1), (1000, 0, 1), (0, 0, 1), (1000, 1, 0)])
+def test_fp32_schedule_changes_nothing(w, group, overlap, graph):
+    """Orbit launches of a few reps each, integrated on a second stream while
+    the previous group is judged (overlap 1) or one after the other (0), with
+    each group's reps captured in one hipGraph (graph 1) or not, give the same
+    bits as one launch for the whole iteration with its reps launched one by one."""
+    print(f"[gpu] fp32 schedule group={group} overlap={overlap} graph={graph}", flush=True)
+    perm = model.permutation(w, 2)
+    out = []
+    for sched in ((0, 0, 0), (group, overlap, graph)):
+        g = GalicGPU(w)
+        g.fp32_calc_all_response_fields()
+        g.fp32_set_schedule(*sched)
+        acc = g.fp32_iterate(perm, k=3, rep_size=700, sigma_frac=0.1, seed=5)
+        out.append([acc] + [g.stack(t, wh) for t in w.types for wh in range(10)] + [g.particles()[0]]
+                   + [g.fit(t) for t in w.types])
+        launches = g.timings()["graph_launches"]
+        assert launches == 0 if sched[2] == 0 else launches >= 1
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_fp32_history_tracks_fp64(w):
     """Same trials (K=1) on both paths for several reps."""
     print("[gpu] fp32 vs fp64 history", flush=True)
