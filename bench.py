@@ -38,13 +38,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from galic_amd import _abi, model  # noqa: E402
-from galic_amd.runtime import (TREE_FP32, TREE_PLUMMER, TREE_QUADRUPOLE, GalicGPU, direct_forces,  # noqa: E402
+from galic_amd.runtime import (TREE_FP32, TREE_GROUP, TREE_PLUMMER, TREE_QUADRUPOLE, GalicGPU, direct_forces,  # noqa: E402
                                tree_fill, tree_forces, tree_last_timings)
 
 F_STEP = 376          # algorithmic FP operations per leapfrog step (orbit_response.c:211-458 + lookups)
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X published FP32 vector rate
 PEAK_HBM = 8000.0     # GB/s
-TREE_MODE = TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER   # the force-field configuration's walk
+TREE_MODE_TARGET = TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER   # the force-field configuration's walk
+TREE_MODE = TREE_MODE_TARGET | TREE_GROUP   # ... with the targets' groups walking together
 # HBM bytes per leapfrog step of k_orbit32_log in the full model's timed launches, from the FETCH_SIZE
 # and WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_composite-2e7.txt:
 # 34.3 B fetched + 38.7 B written per step)
@@ -146,13 +147,27 @@ def tree_leg(n, nbin, ncheck):
     z1 = p.FG_Rmin * (p.FG_Fac ** k - 1.0)
     phi = 2 * np.pi / 8 * s
     tpos = np.stack([r1 * np.cos(phi), r1 * np.sin(phi), z1], axis=1)
+    # the group walk's error on the same nodes, each walked inside its own 8 x 8 block of its
+    # section, as the fill groups them
+    kk, jj = np.meshgrid(np.arange(8), np.arange(8), indexing="ij")
+    bk = (k // 8 * 8)[:, None] + kk.ravel()[None, :]
+    bj = (j // 8 * 8)[:, None] + jj.ravel()[None, :]
+    br = p.FG_Rmin * (p.FG_Fac ** bj - 1.0)
+    bz = p.FG_Rmin * (p.FG_Fac ** bk - 1.0)
+    bpos = np.stack([br * np.cos(phi)[:, None], br * np.sin(phi)[:, None], bz], axis=2).reshape(-1, 3)
+    lane = np.arange(ncheck) * 64 + (k % 8) * 8 + (j % 8)
     out = dict(sources=n, targets=ntarget, fg_nbin=nbin, theta=0.4, softening=cfg.softening)
-    for name, mode in (("quadrupole_plummer_fp32", TREE_MODE), ("reference_kernel_fp32", TREE_FP32)):
+    for name, mode in (("quadrupole_plummer_fp32", TREE_MODE_TARGET), ("reference_kernel_fp32", TREE_FP32),
+                       ("quadrupole_plummer_fp32_group", TREE_MODE_TARGET | TREE_GROUP)):
         t0 = time.perf_counter()
         tree_fill(p, spos, smass, precision=mode)
         wall = time.perf_counter() - t0
         tm = tree_last_timings()
-        acc32, pot32, _ = tree_forces(p, spos, smass, tpos, precision=mode)
+        if mode & TREE_GROUP:
+            acc32, pot32, _ = tree_forces(p, spos, smass, bpos, precision=mode)
+            acc32, pot32 = acc32[lane], pot32[lane]
+        else:
+            acc32, pot32, _ = tree_forces(p, spos, smass, tpos, precision=mode)
         accd, potd = direct_forces(p, spos, smass, tpos, plummer=bool(mode & TREE_PLUMMER))
         err = np.linalg.norm(acc32 - accd, axis=1) / np.linalg.norm(accd, axis=1)
         perr = np.abs(pot32 / potd - 1)
@@ -289,7 +304,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12, help="timed steps (blocks of reps)")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=20000000)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--sigma", type=float, default=0.1)
@@ -314,6 +329,10 @@ def main():
     ap.add_argument("--overlap", type=int, default=-1, choices=(-1, 0, 1),
                     help="integrate the next group's orbits while a group is judged (1), never (0), "
                          "or when a step needs more than one orbit launch (-1)")
+    ap.add_argument("--orbit-wg-per-cu", type=int, default=0, help="orbit kernel workgroups per CU (0: autotuned)")
+    ap.add_argument("--merit-wg-per-cu", type=int, default=0, help="merit kernel workgroups per CU (0: as many as fit)")
+    ap.add_argument("--tune-schedule", type=int, default=1, choices=(0, 1),
+                    help="tune the CU sharing of overlapped orbit launches and judging in the first warm-up step")
     ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
                     help="capture the reps of each orbit launch's group in one hipGraph (1) or launch them "
                          "rep by rep (0)")
@@ -357,6 +376,11 @@ def main():
     g.fp32_set_schedule(0, args.overlap, args.graph)
     variant, bpc = g.fp32_autotune(min(w.n, 524288))
     log(f"[bench] autotuned orbit kernel: {bpc} workgroups/CU")
+    if args.orbit_wg_per_cu:
+        g.fp32_set_launch(0, args.orbit_wg_per_cu)
+        bpc = args.orbit_wg_per_cu
+    g.fp32_set_merit_launch(args.merit_wg_per_cu)
+    sched = None
 
     def step(s, seed):
         j = s % spi
@@ -369,7 +393,17 @@ def main():
         acc = g.fp32_iterate(idx, k=args.k, rep_size=args.rep, sigma_frac=args.sigma, seed=seed)
         return len(idx), int((acc > 0).sum()), retab_ms
 
-    for s in range(args.warmup):
+    w0 = 0
+    if args.tune_schedule and args.warmup > 0:
+        # the warm-up steps (real steps of the optimisation, as any other, run as one call) also
+        # pick the sharing of the CUs between the orbit launches and the judging they overlap
+        w0 = min(args.warmup, spi)
+        idx = perm[bounds[0]:bounds[w0]]
+        g.retabulate(TREE_MODE)
+        _, ob, mb = g.fp32_autotune_schedule(idx, k=args.k, rep_size=args.rep, sigma_frac=args.sigma, seed=500)
+        sched = {"orbit_wg_per_cu_beside_judging": ob, "merit_wg_per_cu": mb or "all"}
+        log(f"[bench] schedule tuned over {w0} warm-up steps: {sched}")
+    for s in range(w0, args.warmup):
         step(s, 500 + s)
         log(f"[bench] warmup {s} done")
     g.reset_timings()
@@ -506,7 +540,7 @@ def main():
                    "trials_per_particle": args.k, "fresh_trials": args.fresh, "trial_sigma_frac": args.sigma,
                    "rep": args.rep, "particles_per_step": int(nparticles / args.steps),
                    "steps_per_iteration": spi, "orbits_per_step": int(orbits / args.steps),
-                   "torbit_fac": cfg.torbit_fac, "autotuned": {"workgroups_per_cu": bpc},
+                   "torbit_fac": cfg.torbit_fac, "autotuned": {"workgroups_per_cu": bpc, "schedule": sched},
                    "overlap": args.overlap, "graph": args.graph,
                    "graph_launches_per_step": tm["graph_launches"] / args.steps},
         "roofline": {"bound": "valu_fp32", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",

@@ -59,7 +59,7 @@ EXPORTS = [
     "galic_write_fit_line", "galic_write_densfield", "galic_write_snapshot",
     "galic_set_particles", "galic_set_dispersions", "galic_get_particles", "galic_calc_all_response_fields", "galic_rep",
     "galic_get_stack", "galic_get_fit", "galic_orbit_responses", "galic_fp32_orbit_responses", "galic_field_at", "galic_field_at_static",
-    "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_fp32_set_fresh_trials", "galic_fp32_trial_merits", "galic_fp32_set_merit_form", "galic_fp32_set_schedule", "galic_fp32_merit_profile", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_direct_forces_mode", "galic_tree_last_timings",
+    "galic_fp32_calc_all_response_fields", "galic_fp32_rep", "galic_fp32_iterate", "galic_fp32_autotune", "galic_fp32_set_launch", "galic_fp32_set_fresh_trials", "galic_fp32_trial_merits", "galic_fp32_set_merit_form", "galic_fp32_set_schedule", "galic_fp32_set_merit_launch", "galic_fp32_autotune_schedule", "galic_fp32_merit_profile", "galic_tree_forces", "galic_tree_fill", "galic_direct_forces", "galic_direct_forces_mode", "galic_tree_last_timings",
     "galic_timings", "galic_reset_timings", "galic_copy_probe", "galic_bind",
     "produce_orbit_response_field_mod", "produce_orbit_response_field",
     "forcegrid_get_potential", "forcegrid_get_escape_speed", "forcegrid_get_acceleration", "smooth_stack",
@@ -134,6 +134,9 @@ def lib():
         "galic_fp32_trial_merits": [_vp, ctypes.c_int, ctypes.c_int, _dp],
         "galic_fp32_set_merit_form": [_vp, ctypes.c_int],
         "galic_fp32_set_schedule": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "galic_fp32_set_merit_launch": [_vp, ctypes.c_int],
+        "galic_fp32_autotune_schedule": [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_ulonglong, _ip, _ip, _ip],
         "galic_fp32_merit_profile": [_vp, ctypes.c_int, _P(ctypes.c_ulonglong)],
         "galic_tree_forces": [_P(GalicParams), ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                               ctypes.c_int, _lp],

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -1973,6 +1974,14 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       CHK(ensure_arena(c, std::min(all, budget), one));
     }
   }
+  // the orbit launches beside a group's judging share the CUs with the merit kernel: their own
+  // workgroups per CU (set around those launches; restored on every return)
+  struct Restore {
+    int &ref;
+    int saved;
+    ~Restore() { ref = saved; }
+  } bpc_restore{c.fp32_bpc, c.fp32_bpc};
+  const int bpc_alone = c.fp32_bpc, bpc_beside = c.fp32_bpc_overlap > 0 ? c.fp32_bpc_overlap : c.fp32_bpc;
   // trials whose orbits' logs fit in the arena at the current estimate (and the caller's cap)
   auto group_cap = [&]() {
     const double blocks = (double)(c.slot_bytes / log_block_bytes());
@@ -2164,7 +2173,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, t16 ? 64 * galic::kFields : 1024, shmem));
       HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device));
       if(per_cu < 1) return set_err(GALIC_ERR_ARG, "the tile merit kernel does not fit on a CU (%zu bytes of LDS)", shmem);
-      if(const char *ov = getenv("GALIC_MERIT_WG_PER_CU")) per_cu = std::max(1, atoi(ov));  // tuning override
+      if(c.merit_wg_per_cu > 0) per_cu = std::min(per_cu, c.merit_wg_per_cu);
       tile_grid = std::min(ng, per_cu * ncu);
       if(getenv("GALIC_VERBOSE")) {
         hipFuncAttributes fa;
@@ -2517,7 +2526,9 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       for(int i = s0; i < s1; i++) in_group[idx[i]] = 0;
       if(distinct) {
         swap_orbit_set(c);
+        c.fp32_bpc = bpc_beside;
         const int rc = start(n0, n1, A_next);
+        c.fp32_bpc = bpc_alone;
         swap_orbit_set(c);
         CHK(rc);
         ahead = true;
@@ -2574,6 +2585,12 @@ int galic_fp32_set_schedule(galic_ctx *ctx, int max_group_trials, int overlap, i
   return GALIC_OK;
 }
 
+int galic_fp32_set_merit_launch(galic_ctx *ctx, int wg_per_cu) {
+  if(!ctx || wg_per_cu < 0 || wg_per_cu > 32) return set_err(GALIC_ERR_ARG, "wg_per_cu must be 0..32");
+  ctx->c.merit_wg_per_cu = wg_per_cu;
+  return GALIC_OK;
+}
+
 int galic_fp32_set_fresh_trials(galic_ctx *ctx, int nfresh) {
   if(!ctx || nfresh < 0 || nfresh > 64) return set_err(GALIC_ERR_ARG, "nfresh must be 0..64");
   ctx->c.fp32_nfresh = nfresh;
@@ -2588,6 +2605,48 @@ int galic_fp32_rep(galic_ctx *ctx, int ntrial, const int *idx, int K, const doub
 int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, const double *vtry,
                        double sigma_frac, unsigned long long seed, int *accept) {
   return fp32_run(ctx, ntrial, idx, K, rep_size, vtry, sigma_frac, seed, accept);
+}
+
+int galic_fp32_autotune_schedule(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size,
+                                 double sigma_frac, unsigned long long seed, int *accept, int *orbit_wg_per_cu,
+                                 int *merit_wg_per_cu) {
+  if(!ctx || !idx || ntrial < 1 || rep_size < 1) return set_err(GALIC_ERR_ARG, "bad schedule-tuning arguments");
+  galic::Ctx &c = ctx->c;
+  if(!c.fp32_tuned) return set_err(GALIC_ERR_STATE, "galic_fp32_autotune must run first");
+  // (orbit kernel workgroups per CU beside the judging, merit kernel workgroups per CU (0: all))
+  const int base = c.fp32_bpc;
+  const int cand[3][2] = {{std::max(1, base / 2), 0}, {base, 0}, {std::max(1, base / 2), 1}};
+  const int ncand = 3;
+  // two runs per candidate, interleaved, the better of the two counted (the first run of all
+  // also pays the call's first allocations); the last run takes the reps left over
+  const int nrep = (ntrial + rep_size - 1) / rep_size;
+  const int per = std::max(1, nrep / (2 * ncand));
+  double rate[3] = {0, 0, 0};
+  for(int i = 0, r = 0; r < nrep; i++) {
+    const int r1 = i < 2 * ncand - 1 ? std::min(nrep, r + per) : nrep;
+    const int a = r * rep_size, b = std::min(ntrial, r1 * rep_size);
+    const int ci = i % ncand;
+    c.fp32_bpc_overlap = cand[ci][0];
+    c.merit_wg_per_cu = cand[ci][1];
+    const auto t0 = std::chrono::steady_clock::now();
+    CHK(fp32_run(ctx, b - a, idx + a, K, rep_size, nullptr, sigma_frac, seed, accept ? accept + a : nullptr));
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double rt = (double)(b - a) / std::max(sec, 1e-9);  // trials per second
+    if(getenv("GALIC_VERBOSE"))
+      fprintf(stderr, "[galic] schedule tuning: orbit %d WG/CU beside the judging, merit %d WG/CU (0: all): "
+                      "%d reps, %.4g trials/s\n", cand[ci][0], cand[ci][1], r1 - r, rt);
+    rate[ci] = std::max(rate[ci], rt);
+    r = r1;
+  }
+  int bi = 1;  // (the autotuned launch when no candidate was measured)
+  for(int i = 0; i < ncand; i++)
+    if(rate[i] > rate[bi]) bi = i;
+  const int bo = cand[bi][0], bm = cand[bi][1];
+  c.fp32_bpc_overlap = bo;
+  c.merit_wg_per_cu = bm;
+  if(orbit_wg_per_cu) *orbit_wg_per_cu = bo;
+  if(merit_wg_per_cu) *merit_wg_per_cu = bm;
+  return GALIC_OK;
 }
 
 int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk) {

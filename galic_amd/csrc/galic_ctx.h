@@ -219,6 +219,7 @@ struct Ctx {
   int set_parity = 0;                 // 1 while the sets are swapped
   int overlap = -1;                   // FP32 reps: integrate the next group's orbits while judging a group
                                       // (-1 when a call needs more than one launch, 0 never, 1 always)
+  int merit_wg_per_cu = 0;            // FP32 merit kernel: workgroups per CU at most (0: as many as fit)
   int max_group_trials = 0;           // FP32 reps: at most this many trials per orbit launch (0: the arena's)
 
   // timing of the last launches (ms), measured with HIP events on ctx stream
@@ -230,6 +231,7 @@ struct Ctx {
   long long acc_orbits = 0, acc_steps = 0;
   int fp32_variant = 0;   // orbit kernel variant (one so far)
   int fp32_bpc = 4;       // persistent workgroups per CU for k_orbit32 (autotuned)
+  int fp32_bpc_overlap = 0;  // the same for the launches beside a group's judging (0: fp32_bpc)
   int fp32_nfresh = 0;    // device trials per particle drawn fresh (optimize()'s distribution)
   bool fp32_tuned = false;
   double fp32_scale[5];  // fixed-point scales per field (orbit-averaged)

@@ -275,6 +275,9 @@ struct WalkArgs {
   const float4 *quadf;  // [nnodes][2] FP32 copy (FP32 walk)
   int nnodes;
   const float *len;
+  const int *child;     // [node][8] (the group walk)
+  int blk_nb;           // group walk: targets are the (section, z, R) table nodes of nb x nb grids,
+                        // a group an 8 x 8 block of one section; 0: 64 consecutive targets
   double theta, h, G;
   int plummer;          // Plummer softening of length eps instead of the spline of length h
   double eps;
@@ -774,11 +777,170 @@ done:
   return rc;
 }
 
+// The group walk (GALIC_TREE_GROUP, FP32 interactions): the 64 targets of a wave walk the tree
+// together.  A node is accepted for the group only if the opening criterion holds at the point
+// of the targets' bounding box nearest to its centre of mass, so every target interacts with
+// nodes that its own criterion would accept, or with their descendants: more interactions than
+// the per-target walk (forcetree_walk.c:12-222), none less accurate.  The wave takes up to 64
+// pending entries of an LDS stack at a time (one per lane): accepted nodes, particles and
+// opened leaf buckets go to an LDS interaction list, opened nodes push their children.  A full
+// list is flushed: every lane adds the list's entries (read at wave-uniform addresses) to its
+// own target.  With less stack room than 64 entries could need, one entry is taken at a time
+// (a depth-first descent needs at most 7 entries per level).
+constexpr int kGStack = 1024, kGList = 1024, kGDeep = 7 * (kMaxDepth + 3);
+
+__device__ __forceinline__ void group_interact(const WalkArgs &A, int e, double px, double py, double pz, float h,
+                                               float eps, float &ax, float &ay, float &az, float &pot,
+                                               unsigned int &nint) {
+  float fac, wp;
+  if(e >= 0) {
+    float dx, dy, dz, mass;
+    int q = -1;
+    if(e < A.nsrc) {
+      dx = (float)(A.x[e] - px);
+      dy = (float)(A.y[e] - py);
+      dz = (float)(A.z[e] - pz);
+      mass = (float)A.m[e];
+    } else {
+      q = e - A.nsrc;
+      dx = (float)(A.sx[q] - px);
+      dy = (float)(A.sy[q] - py);
+      dz = (float)(A.sz[q] - pz);
+      mass = (float)A.mass[q];
+    }
+    if(mass == 0) return;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if(q >= 0 && A.quadf) {
+      const float4 qa = A.quadf[2 * q], qb = A.quadf[2 * q + 1];
+      const float qq[6] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y};
+      const float s2 = A.plummer ? r2 + eps * eps : r2;
+      if(A.plummer || s2 >= h * h) quad_term(qq, dx, dy, dz, s2, __builtin_amdgcn_rsqf(s2), ax, ay, az, pot);
+    }
+    if(A.plummer)
+      plummer32(mass, r2, eps, fac, wp);
+    else
+      interact32(mass, r2, h, fac, wp);
+    ax += dx * fac;
+    ay += dy * fac;
+    az += dz * fac;
+    pot += wp;
+    nint++;
+  } else {
+    // an opened leaf bucket: its particles one by one
+    const int q = ~e, p0 = A.first[q], p1 = p0 + A.count[q];
+    for(int p = p0; p < p1; p++) {
+      if(A.m[p] == 0) continue;
+      const float ex = (float)(A.x[p] - px), ey = (float)(A.y[p] - py), ez = (float)(A.z[p] - pz);
+      const float e2 = ex * ex + ey * ey + ez * ez;
+      if(A.plummer)
+        plummer32((float)A.m[p], e2, eps, fac, wp);
+      else
+        interact32((float)A.m[p], e2, h, fac, wp);
+      ax += ex * fac;
+      ay += ey * fac;
+      az += ez * fac;
+      pot += wp;
+      nint++;
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+  for(int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for(int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__global__ void __launch_bounds__(64) k_walk32g(WalkArgs A) {
+  __shared__ int stk[kGStack];
+  __shared__ int lst[kGList];
+  const int lane = threadIdx.x, g = blockIdx.x;
+  long long t;
+  if(A.blk_nb > 0) {
+    const int nbb = A.blk_nb / 8, s = g / (nbb * nbb), b = g % (nbb * nbb);
+    t = ((long long)s * A.blk_nb + (b / nbb) * 8 + lane / 8) * A.blk_nb + (b % nbb) * 8 + lane % 8;
+  } else {
+    t = (long long)g * 64 + lane;
+  }
+  const bool live = t < A.ntarget;
+  const long long tl = live ? t : (long long)g * 64;  // (dead lanes: the group's first target)
+  const double px = A.tx[tl], py = A.ty[tl], pz = A.tz[tl];
+  const double bx0 = wave_min(px), by0 = wave_min(py), bz0 = wave_min(pz);
+  const double bx1 = wave_max(px), by1 = wave_max(py), bz1 = wave_max(pz);
+  const float h = (float)A.h, th2 = (float)(A.theta * A.theta), eps = (float)A.eps;
+  float ax = 0, ay = 0, az = 0, pot = 0;
+  unsigned int nint = 0;
+  if(lane == 0) stk[0] = A.nsrc;  // the root node
+  int top = A.nnodes > 0 ? 1 : 0, nl = 0;
+  __syncthreads();
+  while(top > 0) {
+    const int k = (kGStack - top >= 7 * 64 + kGDeep) ? min(64, top) : 1;
+    if(nl + k > kGList) {
+      for(int i = 0; i < nl; i++)
+        group_interact(A, __builtin_amdgcn_readfirstlane(lst[i]), px, py, pz, h, eps, ax, ay, az, pot, nint);
+      nl = 0;
+    }
+    const int e = lane < k ? stk[top - 1 - lane] : -1;
+    top -= k;
+    __syncthreads();
+    int item = 0, open = -1;  // item: the list entry (0: none)
+    bool listed = false;
+    if(lane < k) {
+      if(e < A.nsrc) {
+        listed = true;
+        item = e;
+      } else {
+        const int q = e - A.nsrc;
+        const double cx = A.sx[q], cy = A.sy[q], cz = A.sz[q];
+        const float dx = (float)fmax(fmax(bx0 - cx, cx - bx1), 0.0);
+        const float dy = (float)fmax(fmax(by0 - cy, cy - by1), 0.0);
+        const float dz = (float)fmax(fmax(bz0 - cz, cz - bz1), 0.0);
+        const float l = A.len[q];
+        if(l * l > (dx * dx + dy * dy + dz * dz) * th2) {
+          if(A.bucket[q]) {
+            listed = true;
+            item = ~q;
+          } else {
+            open = q;
+          }
+        } else {
+          listed = true;
+          item = e;
+        }
+      }
+    }
+    const unsigned long long lm = __ballot(listed);
+    if(listed) lst[nl + __builtin_amdgcn_mbcnt_hi((unsigned)(lm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)lm, 0))] = item;
+    nl += __popcll(lm);
+    for(int c = 0; c < 8; c++) {
+      const int ch = open >= 0 ? A.child[(size_t)open * 8 + c] : -1;
+      const unsigned long long cm = __ballot(ch >= 0);
+      if(ch >= 0) stk[top + __builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)cm, 0))] = ch;
+      top += __popcll(cm);
+    }
+    __syncthreads();
+  }
+  for(int i = 0; i < nl; i++)
+    group_interact(A, __builtin_amdgcn_readfirstlane(lst[i]), px, py, pz, h, eps, ax, ay, az, pot, nint);
+  if(live) {
+    A.ax[t] = (double)ax * A.G;
+    A.ay[t] = (double)ay * A.G;
+    A.az[t] = (double)az * A.G;
+    A.pot[t] = (double)pot * A.G;
+    atomicAdd(A.nint, (unsigned long long)nint);
+  }
+}
+
 // Walk of ntarget targets (device SoA) through T; the results are scaled by G (gravity_tree
 // multiplies the walk's sums by G afterwards, gravtree.c:455-463).
+// blk_nb > 0: the targets are table nodes (section, z, R) of blk_nb x blk_nb grids, which the group
+// walk takes in 8 x 8 blocks.
 int tree_walk_dev(const galic_params *p, hipStream_t st, const Tree &T, int ntarget, const double *dtx,
                   const double *dty, const double *dtz, double *dax, double *day, double *daz, double *dpot,
-                  int precision, double G, long long *ninteractions, float *walk_ms) {
+                  int precision, double G, long long *ninteractions, float *walk_ms, int blk_nb = 0) {
   int rc = GALIC_OK;
   DevBufs B;
   unsigned long long *nint = nullptr;
@@ -806,6 +968,8 @@ int tree_walk_dev(const galic_params *p, hipStream_t st, const Tree &T, int ntar
   W.sy = T.sy;
   W.sz = T.sz;
   W.len = T.len;
+  W.child = T.child;
+  W.blk_nb = 0;
   W.quad = (precision & GALIC_TREE_QUADRUPOLE) ? T.quad : nullptr;
   W.quadf = (W.quad && (precision & GALIC_TREE_FP32)) ? T.quadf : nullptr;
   if(W.quad && (precision & GALIC_TREE_FP32) && !W.quadf) {
@@ -832,10 +996,15 @@ int tree_walk_dev(const galic_params *p, hipStream_t st, const Tree &T, int ntar
   TCHK(hipEventCreate(&e1));
   TCHK(hipEventRecord(e0, st));
   if(ntarget > 0) {
-    if(!(precision & GALIC_TREE_FP32))
+    if(precision & GALIC_TREE_GROUP) {
+      const bool blk = blk_nb > 0 && blk_nb % 8 == 0 && (long long)ntarget % ((long long)blk_nb * blk_nb) == 0;
+      W.blk_nb = blk ? blk_nb : 0;
+      hipLaunchKernelGGL(k_walk32g, dim3((unsigned int)(((long long)ntarget + 63) / 64)), dim3(64), 0, st, W);
+    } else if(!(precision & GALIC_TREE_FP32)) {
       hipLaunchKernelGGL(k_walk64, dim3(nb256(ntarget)), dim3(256), 0, st, W);
-    else
+    } else {
       hipLaunchKernelGGL(k_walk32, dim3(nb256(ntarget)), dim3(256), 0, st, W);
+    }
     TCHK(hipGetLastError());
   }
   TCHK(hipEventRecord(e1, st));
@@ -856,14 +1025,14 @@ done:
 int tree_forces_dev(const galic_params *p, hipStream_t st, int nsrc, const double *dpx, const double *dpy,
                     const double *dpz, const double *dpm, int ntarget, const double *dtx, const double *dty,
                     const double *dtz, double corner[3], double dlen, double *dax, double *day, double *daz,
-                    double *dpot, int precision, long long *ninteractions, float *walk_ms) {
+                    double *dpot, int precision, long long *ninteractions, float *walk_ms, int blk_nb = 0) {
   DevBufs B;
   Tree T;
   float build_ms = 0, wms = 0;
   long long ni = 0;
   int rc = tree_build_dev(p, st, nsrc, dpx, dpy, dpz, dpm, corner, dlen, precision, B, T, nullptr, &build_ms);
   if(rc != GALIC_OK) return rc;
-  rc = tree_walk_dev(p, st, T, ntarget, dtx, dty, dtz, dax, day, daz, dpot, precision, p->G, &ni, &wms);
+  rc = tree_walk_dev(p, st, T, ntarget, dtx, dty, dtz, dax, day, daz, dpot, precision, p->G, &ni, &wms, blk_nb);
   if(rc != GALIC_OK) return rc;
   if(ninteractions) *ninteractions = ni;
   if(walk_ms) *walk_ms = wms;
@@ -1139,7 +1308,9 @@ done:
 
 int galic_retabulate(galic_ctx *ctx, int precision, long long *ninteractions) {
   if(!ctx) return terr(GALIC_ERR_ARG, "null context");
-  if(precision & ~7) return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER");
+  if((precision & ~15) || ((precision & GALIC_TREE_GROUP) && !(precision & GALIC_TREE_FP32)))
+    return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER | "
+                               "GALIC_TREE_GROUP (with GALIC_TREE_FP32)");
   galic::Ctx &c = galic::ctx_state(ctx);
   if(!c.tsrc || c.tsrc_n < 1) return terr(GALIC_ERR_STATE, "no tree sources (galic_set_tree_sources)");
   const galic_params *p = &c.p;
@@ -1188,7 +1359,7 @@ int galic_retabulate(galic_ctx *ctx, int precision, long long *ninteractions) {
   if((rc = tree_build_dev(p, c.stream, nsrc, c.tsrc, c.tsrc + nsrc, c.tsrc + 2 * (size_t)nsrc,
                           c.tsrc + 3 * (size_t)nsrc, corner, len, precision, B, T, nullptr, &bms)) != GALIC_OK)
     goto done;
-  if((rc = tree_walk_dev(p, c.stream, T, (int)ns, tx, ty, tz, ax, ay, az, pt, precision, p->G, &ni, &wms)) !=
+  if((rc = tree_walk_dev(p, c.stream, T, (int)ns, tx, ty, tz, ax, ay, az, pt, precision, p->G, &ni, &wms, nb)) !=
      GALIC_OK)
     goto done;
   // installed as the context's tables: FG_DPotDR, FG_DPotDz, FG_Pot and the FP32 copy
@@ -1212,7 +1383,9 @@ int galic_tree_forces(const galic_params *p, int device, int nsrc, const double 
                       long long *ninteractions) {
   if(!p || nsrc < 0 || ntarget < 0 || (nsrc > 0 && (!spos || !smass)) || (ntarget > 0 && (!tpos || !acc || !pot)))
     return terr(GALIC_ERR_ARG, "bad tree arguments");
-  if(precision & ~7) return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER");
+  if((precision & ~15) || ((precision & GALIC_TREE_GROUP) && !(precision & GALIC_TREE_FP32)))
+    return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER | "
+                               "GALIC_TREE_GROUP (with GALIC_TREE_FP32)");
   int rc = GALIC_OK;
   hipStream_t st = nullptr;
   DevBufs B;
@@ -1250,7 +1423,9 @@ int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *s
                     double *pot, double *dpotdr, double *dpotdz, int precision, long long *ninteractions) {
   if(!p || nsrc < 0 || (nsrc > 0 && (!spos || !smass)) || !pot || !dpotdr || !dpotdz || p->FG_Nbin < 2)
     return terr(GALIC_ERR_ARG, "bad table-fill arguments");
-  if(precision & ~7) return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER");
+  if((precision & ~15) || ((precision & GALIC_TREE_GROUP) && !(precision & GALIC_TREE_FP32)))
+    return terr(GALIC_ERR_ARG, "mode: GALIC_TREE_FP32 | GALIC_TREE_QUADRUPOLE | GALIC_TREE_PLUMMER | "
+                               "GALIC_TREE_GROUP (with GALIC_TREE_FP32)");
   const int nb = p->FG_Nbin;
   const size_t ns = (size_t)GALIC_FG_SECTIONS * nb * nb;
   if(ns > 0x7fffffff) return terr(GALIC_ERR_ARG, "FG_Nbin too large");
@@ -1291,7 +1466,7 @@ int galic_tree_fill(const galic_params *p, int device, int nsrc, const double *s
   TCHK(hipMemcpy(dcs, cs.data(), sizeof(double) * GALIC_FG_SECTIONS, hipMemcpyHostToDevice));
   TCHK(hipMemcpy(dsn, sn.data(), sizeof(double) * GALIC_FG_SECTIONS, hipMemcpyHostToDevice));
   rc = tree_forces_dev(p, st, nsrc, px, py, pz, pm, (int)ns, tx, ty, tz, corner, len, ax, ay, az, pt, precision,
-                       ninteractions, &ms);
+                       ninteractions, &ms, nb);
   if(rc != GALIC_OK) goto done;
   hipLaunchKernelGGL(k_project, dim3(nb256((size_t)nb * nb)), dim3(256), 0, st, nb, (const double *)ax,
                      (const double *)ay, (const double *)az, (const double *)pt, (const double *)dcs,

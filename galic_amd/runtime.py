@@ -130,6 +130,18 @@ class GalicGPU:
                                           iptr(acc)))
         return acc
 
+    def fp32_autotune_schedule(self, idx, k=1, rep_size=100000, sigma_frac=0.1, seed=1):
+        """fp32_iterate (device-drawn trials, the same results) in six runs of reps, two for each
+        of three sharings of the CUs between orbit launches and judging; keeps the fastest.
+        Returns (accept, orbit workgroups per CU beside the judging, merit workgroups per CU)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        acc = np.zeros(len(idx), dtype=np.int32)
+        o = np.zeros(1, dtype=np.int32)
+        m = np.zeros(1, dtype=np.int32)
+        check(self.lib.galic_fp32_autotune_schedule(self.h, len(idx), iptr(idx), k, int(rep_size), sigma_frac, seed,
+                                                    iptr(acc), iptr(o), iptr(m)))
+        return acc, int(o[0]), int(m[0])
+
     def fp32_trial_merits(self, ntrial, k):
         """F_try - F_base of every trial of the last fp32_rep / fp32_iterate call, shape (ntrial, k)."""
         out = np.zeros((ntrial, k))
@@ -149,6 +161,10 @@ class GalicGPU:
         """0 automatic (4 where it applies), 1 dense 32x32 LDS tiles at every level, 2 never tiles,
         3 sparse 32x32 LDS tiles at every level, 4 16x16 tiles in each particle's int32 fixed point."""
         check(self.lib.galic_fp32_set_merit_form(self.h, int(form)))
+
+    def fp32_set_merit_launch(self, wg_per_cu=0):
+        """At most wg_per_cu workgroups per CU for the tile merit kernels (0: as many as fit)."""
+        check(self.lib.galic_fp32_set_merit_launch(self.h, int(wg_per_cu)))
 
     def fp32_set_schedule(self, max_group_trials=0, overlap=-1, graph=1):
         """Group the FP32 reps' orbit launches at most max_group_trials trials each (0: as many as
@@ -267,7 +283,7 @@ def tree_fill(params, spos, smass, precision=0, device=0):
 
 
 # walk modes of the tree calls (include/galic_hip.h)
-TREE_FP32, TREE_QUADRUPOLE, TREE_PLUMMER = 1, 2, 4
+TREE_FP32, TREE_QUADRUPOLE, TREE_PLUMMER, TREE_GROUP = 1, 2, 4, 8
 
 
 def direct_forces(params, spos, smass, tpos, device=0, plummer=False):

@@ -222,6 +222,22 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
  * an orbit's total per record), its deltas are the same int64 sums. */
 int galic_fp32_set_merit_form(galic_ctx *ctx, int form);
 
+/* galic_fp32_iterate over trials idx[0, ntrial) (device-drawn trials, the same results bit for
+ * bit), its reps split into six runs, two for each of three sharings of the CUs between the
+ * orbit launches and the judging they overlap (orbit kernel workgroups per CU beside the
+ * judging, merit kernel workgroups per CU): (autotuned / 2, all), (autotuned, all),
+ * (autotuned / 2, 1).  The fastest (the better run, trials per second) is kept for later calls
+ * and returned.  No counterpart in the
+ * reference (an autotuner of this build); galic_fp32_autotune must run first. */
+int galic_fp32_autotune_schedule(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size,
+                                 double sigma_frac, unsigned long long seed, int *accept, int *orbit_wg_per_cu,
+                                 int *merit_wg_per_cu);
+
+/* Workgroups per CU of the FP32 path's tile merit kernels (its persistent grid): at most
+ * wg_per_cu, 0 (default) as many as the CU holds.  No counterpart in the reference; set by
+ * galic_fp32_autotune.  The results do not depend on it. */
+int galic_fp32_set_merit_launch(galic_ctx *ctx, int wg_per_cu);
+
 /* Schedule of the FP32 reps (no counterpart in the reference, whose reps are one loop).
  * The trials of a call are integrated in groups of whole reps, one orbit launch per group;
  * max_group_trials > 0 caps a group at that many trials (0: as many as the log arena holds).
@@ -250,6 +266,9 @@ int galic_fp32_set_launch(galic_ctx *ctx, int variant, int blocks_per_cu);
 #define GALIC_TREE_FP32 1        /* FP32 interactions and sums per target, FP64 moments */
 #define GALIC_TREE_QUADRUPOLE 2  /* traceless quadrupole moments (FP64 sums) at accepted nodes; the reference has none */
 #define GALIC_TREE_PLUMMER 4     /* Plummer softening of length ForceSoftening / 2.8 (the Softening parameter) */
+#define GALIC_TREE_GROUP 8       /* with GALIC_TREE_FP32: 64 targets walk together, a node accepted only where the
+                                    criterion holds for their whole bounding box (more interactions, none less
+                                    accurate); table fills take the targets in 8 x 8 blocks of a section */
 
 /* Accelerations and potentials (times G) at ntarget points from nsrc sources
  * (forcetree.c:41-722, forcetree_walk.c:12-222, gravtree.c:455-463).

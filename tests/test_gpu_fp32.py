@@ -143,6 +143,29 @@ def test_fp32_schedule_changes_nothing(w, group, overlap, graph):
         np.testing.assert_array_equal(a, b)
 
 
+def test_fp32_schedule_tuning_changes_nothing(w):
+    """The schedule tuner runs the iteration's reps in three runs, each with
+    another CU sharing, and gives the bits of one plain iteration."""
+    print("[gpu] fp32 schedule tuning", flush=True)
+    perm = model.permutation(w, 2)
+    out = []
+    for tune in (False, True):
+        g = GalicGPU(w)
+        g.fp32_calc_all_response_fields()
+        if tune:
+            g.fp32_autotune(2000)
+            g.fp32_set_schedule(1000, 1, 1)
+            acc, ob, mb = g.fp32_autotune_schedule(perm, k=3, rep_size=700, sigma_frac=0.1, seed=5)
+            assert ob >= 1 and mb in (0, 1)
+        else:
+            acc = g.fp32_iterate(perm, k=3, rep_size=700, sigma_frac=0.1, seed=5)
+        out.append([acc] + [g.stack(t, wh) for t in w.types for wh in range(10)] + [g.particles()[0]]
+                   + [g.fit(t) for t in w.types])
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_fp32_history_tracks_fp64(w):
     """Same trials (K=1) on both paths for several reps."""
     print("[gpu] fp32 vs fp64 history", flush=True)

@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from galic_amd import model
-from galic_amd.runtime import (TREE_FP32, TREE_PLUMMER, TREE_QUADRUPOLE, direct_forces, tree_fill, tree_forces,
+from galic_amd.runtime import (TREE_FP32, TREE_GROUP, TREE_PLUMMER, TREE_QUADRUPOLE, direct_forces, tree_fill,
+                               tree_forces,
                                 tree_last_timings)
 
 pytestmark = pytest.mark.gpu
@@ -187,6 +188,34 @@ def test_quadrupole_walks_on_the_composite():
         assert med[soft | TREE_QUADRUPOLE | TREE_FP32] < 2 * med[soft | TREE_QUADRUPOLE] + 1e-5
 
 
+@pytest.mark.parametrize("mode", [TREE_FP32, TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER])
+def test_group_walk_no_less_accurate(mode):
+    """The group walk (64 targets share one interaction list, a node accepted only where the
+    criterion holds for the group's bounding box) against FP64 direct summation: every target's
+    median and 99th-percentile errors no larger than the per-target FP32 walk's, up to FP32
+    rounding, with at least as many interactions; a ragged last group of widely spread targets
+    is among them."""
+    print(f"[gpu] group walk mode {mode}", flush=True)
+    p, spos, smass = sources(200000)
+    p.ErrTolTheta = 0.4
+    rng = np.random.default_rng(12)
+    centres = rng.normal(size=(40, 3)) * 6
+    tpos = (centres[:, None, :] + rng.normal(size=(40, 64, 3)) * 0.05).reshape(-1, 3)
+    tpos = np.concatenate([tpos, rng.normal(size=(100, 3)) * 6])  # a last, ragged and spread group
+    acc_d, pot_d = direct_forces(p, spos, smass, tpos, plummer=bool(mode & TREE_PLUMMER))
+    acc_w, pot_w, n_w = tree_forces(p, spos, smass, tpos, precision=mode)
+    acc_g, pot_g, n_g = tree_forces(p, spos, smass, tpos, precision=mode | TREE_GROUP)
+    e_w = np.linalg.norm(acc_w - acc_d, axis=1) / np.linalg.norm(acc_d, axis=1)
+    e_g = np.linalg.norm(acc_g - acc_d, axis=1) / np.linalg.norm(acc_d, axis=1)
+    print("median errors: per target", np.median(e_w), "group", np.median(e_g), "interactions", n_w, n_g)
+    assert n_g >= n_w
+    assert np.median(e_g) <= np.median(e_w) * 1.05 + 1e-6
+    assert np.quantile(e_g, 0.99) <= np.quantile(e_w, 0.99) * 1.05 + 1e-6
+    p_w, p_g = np.abs(pot_w / pot_d - 1), np.abs(pot_g / pot_d - 1)
+    assert np.median(p_g) <= np.median(p_w) * 1.05 + 1e-7
+    assert p_g.max() <= p_w.max() * 1.5 + 1e-6
+
+
 # ----------------------------------------------- re-tabulation and drop-ins --
 
 def table_model(nbin=32, **kw):
@@ -194,7 +223,8 @@ def table_model(nbin=32, **kw):
     return model.make_workload(cfg, n=1200, target_samples=50000)
 
 
-@pytest.mark.parametrize("mode", [0, TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER])
+@pytest.mark.parametrize("mode", [0, TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER,
+                                  TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER | TREE_GROUP])
 def test_retabulate_equals_tree_fill(mode):
     """galic_retabulate (the table fill on the device from the context's sources, installed in
     place) builds the same targets and domain as galic_tree_fill: identical tables; and the
