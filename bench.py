@@ -340,6 +340,7 @@ def main():
                     help="particles of the FP64 perturbation control on the precision-form model (0: skip)")
     ap.add_argument("--control-reps", type=int, default=100)
     ap.add_argument("--only-control", action="store_true", help="run the control leg alone and print its JSON")
+    ap.add_argument("--only-tree", action="store_true", help="run the table-fill leg alone and print its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -347,6 +348,10 @@ def main():
               "(the path shards by particles, see the design notes, but no multi-GPU run is made).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.only_tree:
+        print(json.dumps(tree_leg(args.c4_tree_n or 10000000, 512, 4096)), flush=True)
+        return
 
     if args.only_control:
         print(json.dumps(control_leg(args.control_n or 1000000, args.rep, args.control_reps)), flush=True)
