@@ -457,6 +457,7 @@ galic::FastModel fast_model(galic::Ctx &c) {
   F.max_steps = c.M.max_steps;
   F.acc = c.FG_Acc_f;
   F.vstream = c.FG_Vstream_f;
+  F.vstream_nz = c.vstream_nz;
   F.cellsize = c.DG_CellSize_f;
   for(int f = 0; f < 5; f++) F.qscale[f] = c.fp32_scale[f];
   return F;
@@ -977,6 +978,7 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.FG_Vstream, (size_t)GALIC_NTYPES * c.FG_Ngrid));
   CHK(dalloc(&c.FG_Acc_f, c.FG_Ngrid));
   CHK(dalloc(&c.FG_Vstream_f, (size_t)GALIC_NTYPES * c.FG_Ngrid));
+  c.vstream_nz = 0;
   CHK(dalloc(&c.DG_CellSize, c.Ngrid));
   CHK(dalloc(&c.DG_CellSize_f, c.Ngrid));
   CHK(dalloc(&c.stacks, (size_t)GALIC_NTYPES * galic::ST_COUNT * c.Nstack));
@@ -1224,8 +1226,13 @@ int galic_set_vstream(galic_ctx *ctx, int type, const double *vstream) {
   if(vstream) memcpy(v.data(), vstream, n * sizeof(double));
   HIPCHK(hipMemcpy(c.FG_Vstream + (size_t)type * n, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
   std::vector<float> f(n);
-  for(size_t i = 0; i < n; i++) f[i] = (float)v[i];
+  bool nz = false;
+  for(size_t i = 0; i < n; i++) {
+    f[i] = (float)v[i];
+    nz = nz || f[i] != 0.f;
+  }
   HIPCHK(hipMemcpy(c.FG_Vstream_f + (size_t)type * n, f.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  c.vstream_nz = nz ? (c.vstream_nz | (1u << type)) : (c.vstream_nz & ~(1u << type));
   return GALIC_OK;
 }
 

@@ -102,6 +102,7 @@ struct Ctx {
   double *DG_CellSize = nullptr;
   float2 *FG_Acc_f = nullptr;
   float *FG_Vstream_f = nullptr;
+  unsigned int vstream_nz = 0;     // bit t: type t's FP32 streaming table has a nonzero node
   float *DG_CellSize_f = nullptr;
 
   // grids

@@ -19,6 +19,7 @@ struct FastModel {
   long long max_steps;
   const float2 *acc;  // (DPotDR, DPotDz) per node
   const float *vstream;  // [type][node] streaming velocity
+  unsigned int vstream_nz;  // bit t: type t's streaming table has a nonzero node (else it is not read)
   const float *cellsize;
   double qscale[kFields];  // fixed-point scale per orbit-averaged field
 };

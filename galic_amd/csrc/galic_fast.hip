@@ -72,10 +72,15 @@ __device__ __forceinline__ Taps32 load_taps32(const FastModel &F, const Fg32 &g,
   t.a01 = T[1];
   t.a10 = T[nb];
   t.a11 = T[nb + 1];
-  t.v00 = V[0];
-  t.v01 = V[1];
-  t.v10 = V[nb];
-  t.v11 = V[nb + 1];
+  // (a table of zeros, as the isotropic components have, gives vstr = +0 without its loads)
+  if((F.vstream_nz >> type) & 1u) {
+    t.v00 = V[0];
+    t.v01 = V[1];
+    t.v10 = V[nb];
+    t.v11 = V[nb + 1];
+  } else {
+    t.v00 = t.v01 = t.v10 = t.v11 = 0.f;
+  }
   t.cell = F.cellsize[icell];
   return t;
 }
