@@ -557,6 +557,7 @@ def main():
                              f"bytes per launch ({HBM_BYTES_PER_STEP} B/step from calibrated FETCH_SIZE+WRITE_SIZE "
                              "of the profile: the response-log writes and the table lookups that miss L2)"},
         "steps_per_s": steps_total / wall,
+        "timed_orbit_launches": int(tm["orbit_launches"]), "timed_leapfrog_steps": int(steps_total),
         "ms_per_step_breakdown": breakdown,
         "fp32_initial_pass_s": init_s,
         "accept_frac": accepted / max(1, nparticles),
