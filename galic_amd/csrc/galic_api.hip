@@ -431,6 +431,23 @@ int commit(galic_ctx *ctx) {
 }
 
 
+// The FP32 cell sizes of the density grid, and their per-index table when every cell's size is
+// the size of cell (m, m), m = max(iR, iz) (the geometry of grid.c:329-487 gives this: a cell's
+// size is the larger of its R and z widths, both the same function of the index).
+int upload_cell_sizes_f(galic::Ctx &c, const double *cs) {
+  const int nb = c.Nbin;
+  std::vector<float> f(c.Ngrid), d(nb);
+  for(int i = 0; i < c.Ngrid; i++) f[i] = (float)cs[i];
+  for(int m = 0; m < nb; m++) d[m] = f[(size_t)m * nb + m];
+  bool ok = nb <= galic::kCell1Max;
+  for(int k = 0; k < nb && ok; k++)
+    for(int j = 0; j < nb && ok; j++) ok = f[(size_t)k * nb + j] == d[std::max(j, k)];
+  HIPCHK(hipMemcpy(c.DG_CellSize_f, f.data(), sizeof(float) * c.Ngrid, hipMemcpyHostToDevice));
+  if(ok) HIPCHK(hipMemcpy(c.DG_Cell1_f, d.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+  c.cell1_ok = ok;
+  return GALIC_OK;
+}
+
 galic::FastModel fast_model(galic::Ctx &c) {
   galic::FastModel F;
   memset(&F, 0, sizeof(F));
@@ -459,6 +476,7 @@ galic::FastModel fast_model(galic::Ctx &c) {
   F.vstream = c.FG_Vstream_f;
   F.vstream_nz = c.vstream_nz;
   F.cellsize = c.DG_CellSize_f;
+  F.cell1 = c.cell1_ok ? c.DG_Cell1_f : nullptr;
   for(int f = 0; f < 5; f++) F.qscale[f] = c.fp32_scale[f];
   return F;
 }
@@ -981,6 +999,8 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   c.vstream_nz = 0;
   CHK(dalloc(&c.DG_CellSize, c.Ngrid));
   CHK(dalloc(&c.DG_CellSize_f, c.Ngrid));
+  CHK(dalloc(&c.DG_Cell1_f, (size_t)c.Nbin));
+  c.cell1_ok = false;
   CHK(dalloc(&c.stacks, (size_t)GALIC_NTYPES * galic::ST_COUNT * c.Nstack));
   CHK(dalloc(&c.weights, (size_t)GALIC_NTYPES * galic::W_COUNT * c.Nstack));
   CHK(dalloc(&c.merit_bw, (size_t)GALIC_NTYPES * galic::kFields * c.Nstack));
@@ -1008,11 +1028,7 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   ctx->cellsize.assign(c.Ngrid, 0.0);
   CHK(galic_density_geometry(p, ctx->cellsize.data(), nullptr, ctx->distance.data()));
   HIPCHK(hipMemcpy(c.DG_CellSize, ctx->cellsize.data(), sizeof(double) * c.Ngrid, hipMemcpyHostToDevice));
-  {
-    std::vector<float> f(c.Ngrid);
-    for(int i = 0; i < c.Ngrid; i++) f[i] = (float)ctx->cellsize[i];
-    HIPCHK(hipMemcpy(c.DG_CellSize_f, f.data(), sizeof(float) * c.Ngrid, hipMemcpyHostToDevice));
-  }
+  CHK(upload_cell_sizes_f(c, ctx->cellsize.data()));
   for(int t = 1; t < GALIC_NTYPES; t++)
     for(int k = 0; k < 6; k++) ctx->targets[t][k].assign(c.Nstack, 0.0);
 
@@ -1067,7 +1083,7 @@ int galic_destroy(galic_ctx *ctx) {
   (void)hipSetDevice(c.device);
   (void)hipStreamSynchronize(c.stream);
   dfree(c.FG_Pot); dfree(c.FG_DPotDR); dfree(c.FG_DPotDz); dfree(c.FG_Vstream);
-  dfree(c.FG_Acc_f); dfree(c.FG_Vstream_f); dfree(c.DG_CellSize); dfree(c.DG_CellSize_f);
+  dfree(c.FG_Acc_f); dfree(c.FG_Vstream_f); dfree(c.DG_CellSize); dfree(c.DG_CellSize_f); dfree(c.DG_Cell1_f);
   dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw); dfree(c.tcut);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
@@ -1189,9 +1205,7 @@ void forcedensitygrid_create(void) {
   bool ok = galic_density_geometry(&c.p, cs.data(), nullptr, dist.data()) == GALIC_OK &&
             hipSetDevice(c.device) == hipSuccess &&
             hipMemcpy(c.DG_CellSize, cs.data(), sizeof(double) * c.Ngrid, hipMemcpyHostToDevice) == hipSuccess;
-  std::vector<float> f(c.Ngrid);
-  for(int i = 0; i < c.Ngrid; i++) f[i] = (float)cs[i];
-  ok = ok && hipMemcpy(c.DG_CellSize_f, f.data(), sizeof(float) * c.Ngrid, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && upload_cell_sizes_f(c, cs.data()) == GALIC_OK;
   for(int t = 0; t < GALIC_NTYPES && ok; t++)
     ok = hipMemcpy(c.stacks + ((size_t)t * galic::ST_COUNT + galic::ST_DIST) * c.Nstack, dist.data(),
                    sizeof(double) * c.Nstack, hipMemcpyHostToDevice) == hipSuccess;

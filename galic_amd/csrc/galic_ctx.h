@@ -103,6 +103,8 @@ struct Ctx {
   float2 *FG_Acc_f = nullptr;
   float *FG_Vstream_f = nullptr;
   unsigned int vstream_nz = 0;     // bit t: type t's FP32 streaming table has a nonzero node
+  float *DG_Cell1_f = nullptr;     // [Nbin] cell size by max(iR, iz), where the table has that form
+  bool cell1_ok = false;
   float *DG_CellSize_f = nullptr;
 
   // grids

@@ -21,8 +21,11 @@ struct FastModel {
   const float *vstream;  // [type][node] streaming velocity
   unsigned int vstream_nz;  // bit t: type t's streaming table has a nonzero node (else it is not read)
   const float *cellsize;
+  const float *cell1;      // or null: cellsize[iz * nb + iR] = cell1[max(iR, iz)] (staged in LDS)
   double qscale[kFields];  // fixed-point scale per orbit-averaged field
 };
+
+constexpr int kCell1Max = 1024;  // density-grid cells per side the LDS cell-size table holds
 
 struct OrbitArgs32 {
   int n;

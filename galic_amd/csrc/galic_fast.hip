@@ -52,7 +52,9 @@ __device__ __forceinline__ Fg32 fg_index32(const FastModel &F, float x, float y,
 // waits on memory once: the four force-table nodes and the four streaming
 // velocity nodes around the point (both on the force grid, indices clamped
 // into the table; the off-grid force below does not use the force taps), and
-// the density-grid cell size that sets the next step's dt.
+// the density-grid cell size that sets the next step's dt (from the LDS copy of
+// its per-index table where the grid has one: the size of cell (iR, iz) is the
+// larger of the R and z widths, the same function of either index).
 struct Taps32 {
   float2 a00, a01, a10, a11;  // (DPotDR, DPotDz) at (z row, R column) +(0,0), (0,1), (1,0), (1,1)
   float v00, v01, v10, v11;   // streaming velocity at the same nodes
@@ -60,7 +62,8 @@ struct Taps32 {
   int jb, ib;                 // force-grid node left of / below the point (unclamped)
 };
 
-__device__ __forceinline__ Taps32 load_taps32(const FastModel &F, const Fg32 &g, int type, int icell) {
+__device__ __forceinline__ Taps32 load_taps32(const FastModel &F, const Fg32 &g, int type, int iR, int iz,
+                                              const float *s_cell) {
   Taps32 t;
   const int nb = F.fg_nbin;
   t.jb = max((int)g.jd, 0);
@@ -81,7 +84,7 @@ __device__ __forceinline__ Taps32 load_taps32(const FastModel &F, const Fg32 &g,
   } else {
     t.v00 = t.v01 = t.v10 = t.v11 = 0.f;
   }
-  t.cell = F.cellsize[icell];
+  t.cell = F.cell1 ? s_cell[max(iR, iz)] : F.cellsize[iz * F.dg_nbin + iR];
   return t;
 }
 
@@ -381,7 +384,7 @@ __device__ __forceinline__ bool take_orbit32(const OrbitArgs32 &A, int nb, Lane3
 // pass of the loop on the lookups at its start point (the pre-loop lookups of
 // orbit_response.c:199-210); every later pass is one step of :211-458.
 template <bool LOG>
-__device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs32 &A) {
+__device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs32 &A, const float *s_cell) {
   const int nb = F.dg_nbin, ngrid = nb * nb;
   Lane32 L;
   Log32 lg;
@@ -443,7 +446,7 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
     }
     Fg32 g = fg_index32(F, x, y, z);
     cell32(F, g.R, z, iR, iz, fR, fz);
-    const Taps32 taps = load_taps32(F, g, L.type, iz * nb + iR);
+    const Taps32 taps = load_taps32(F, g, L.type, iR, iz, s_cell);
     float nax, nay, naz;
     accel32(F, x, y, z, g, taps, nax, nay, naz);
     ax = nax;
@@ -515,9 +518,22 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   }
 }
 
-__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) { orbit32_body<false>(F, A); }
+// (the cell-size table into LDS before any lane can leave)
+__device__ __forceinline__ void stage_cell1(const FastModel &F, float *s_cell) {
+  if(F.cell1)
+    for(int i = threadIdx.x; i < F.dg_nbin; i += blockDim.x) s_cell[i] = F.cell1[i];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_orbit32(FastModel F, OrbitArgs32 A) {
+  __shared__ float s_cell[kCell1Max];
+  stage_cell1(F, s_cell);
+  orbit32_body<false>(F, A, s_cell);
+}
 __global__ void __launch_bounds__(256) k_orbit32_log(FastModel F, OrbitArgs32 A) {
-  orbit32_body<true>(F, A);
+  __shared__ float s_cell[kCell1Max];
+  stage_cell1(F, s_cell);
+  orbit32_body<true>(F, A, s_cell);
 }
 
 // ------------------------------------------------------------ trial draws --
@@ -2493,14 +2509,18 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     int *stk = &s_stk[f][0][0];  // [l][k], read and written by lane k of wave f only
     if(lane < KR)
       for(int l = 0; l < NSTK; l++) stk[l * KR + lane] = 0;
-    double2 pc[4], p1, up = make_double2(0.0, 0.0);
+    double2 pc[4] = {}, p1 = make_double2(0.0, 0.0), up = make_double2(0.0, 0.0);
+    // (only the levels with cut-A nodes in the tile: the others' terms are not formed)
     auto load_pairs = [&](int m) {
       const int gz = t16_z(m) * T16 + 2 * jy, gr = t16_r(m) * T16 + 2 * jx;
-      pc[0] = bwf[GALIC_STACKOFFSET(L, gz, gr)];
-      pc[1] = bwf[GALIC_STACKOFFSET(L, gz, gr + 1)];
-      pc[2] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr)];
-      pc[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
-      p1 = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
+      const unsigned int cmn = A.tcut[(size_t)t * ntile + m];
+      if(cmn & 1u) {
+        pc[0] = bwf[GALIC_STACKOFFSET(L, gz, gr)];
+        pc[1] = bwf[GALIC_STACKOFFSET(L, gz, gr + 1)];
+        pc[2] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr)];
+        pc[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
+      }
+      if(cmn & 2u) p1 = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
     };
     // closes trial k's open ancestors of the tile prev (all of them, or those m does not
     // share) from level lt - 1 upwards; up holds their pairs (lane l: level l)
@@ -2583,21 +2603,23 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         up = bwf[GALIC_STACKOFFSET(lane, t16_z(a), t16_r(a))];
       }
       {
-        // the 4x4, 8x8 and tile nodes' pairs, used after the trials (their loads overlap them)
-        const int gz = tz * T16 + 2 * jy, gr = tr * T16 + 2 * jx;
-        const double2 p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
-        const double2 p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
-        const double2 p4 = bwf[GALIC_STACKOFFSET(L - 4, tz, tr)];
-        // residuals in the particle's units
-#pragma unroll
-        for(int c = 0; c < 4; c++) pc[c].x = ldexp(pc[c].x, kp);
-        p1.x = ldexp(p1.x, kp);
-        const unsigned int tm = t16_mask_get<KR>(tmask, m);
         // the levels that hold cut-A nodes in this tile: only their terms can be nonzero (the
         // weights vanish off the cut), and the node sums are formed only if a level above the
         // cells needs them
         const unsigned int cm = A.tcut[(size_t)t * ntile + m];
         const bool cells = cm & 1u, node1 = cm & 2u, sums = cm & 0x3eu;
+        // the 4x4, 8x8 and tile nodes' pairs of the levels on the cut, used after the trials
+        // (their loads overlap them)
+        const int gz = tz * T16 + 2 * jy, gr = tr * T16 + 2 * jx;
+        double2 p2 = make_double2(0.0, 0.0), p3 = p2, p4 = p2;
+        if(cm & 4u) p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
+        if(cm & 8u) p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
+        if(cm & 16u) p4 = bwf[GALIC_STACKOFFSET(L - 4, tz, tr)];
+        // residuals in the particle's units
+#pragma unroll
+        for(int c = 0; c < 4; c++) pc[c].x = ldexp(pc[c].x, kp);
+        p1.x = ldexp(p1.x, kp);
+        const unsigned int tm = t16_mask_get<KR>(tmask, m);
         // The coarser nodes' sums of trial k are handed to one lane each and their terms formed
         // after the loop: the 4x4 node's (in all four lanes of its quad after the quad sum) to
         // lane 4q + k % 4, register k / 4; the 8x8 node's (in all 16 lanes of its row) to lane
