@@ -2276,8 +2276,8 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       return E;
     };
     // the 16x16-tile merit of trials [a, b) of the group and the write-back of each choice; the
-    // launch order by decreasing record count made by hipcub, or in one workgroup (device_order,
-    // which a graph can hold: hipcub's sort sets memory with memset nodes)
+    // launch order by decreasing record count made by rocPRIM's radix sort, or in one workgroup
+    // (device_order, which a graph can hold: the radix sort sets memory with memset nodes)
     auto t16_rep = [&](galic::EvalArgs32 &E, int a, int b, bool device_order) -> int {
       E.nblk = b - a;
       E.tlist2 = (uint2 *)c.tile_list;

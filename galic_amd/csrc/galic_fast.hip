@@ -15,7 +15,7 @@
 #include <math.h>
 #include <stdint.h>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "galic_ctx.h"
 #include "galic_fast.h"
@@ -740,7 +740,7 @@ int sort_trials_by_records(int n, int K, const int *slot_pos, const int *log_n, 
 int sort_desc(int ntrial, unsigned int *key, unsigned int *key2, int *ord, int *ord2, void **tmp, size_t *tmp_bytes,
               hipStream_t stream) {
   size_t need = 0;
-  if(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, need, key, key2, ord, ord2, ntrial, 0, 32, stream) !=
+  if(rocprim::radix_sort_pairs_desc(nullptr, need, key, key2, ord, ord2, (unsigned int)ntrial, 0, 32, stream) !=
      hipSuccess)
     return -1;
   if(need > *tmp_bytes) {
@@ -750,7 +750,7 @@ int sort_desc(int ntrial, unsigned int *key, unsigned int *key2, int *ord, int *
     if(hipMalloc(tmp, need) != hipSuccess) return -1;
     *tmp_bytes = need;
   }
-  if(hipcub::DeviceRadixSort::SortPairsDescending(*tmp, need, key, key2, ord, ord2, ntrial, 0, 32, stream) !=
+  if(rocprim::radix_sort_pairs_desc(*tmp, need, key, key2, ord, ord2, (unsigned int)ntrial, 0, 32, stream) !=
      hipSuccess)
     return -1;
   return 0;

@@ -22,7 +22,7 @@
 // would keep splitting; this changes results only for targets within that
 // distance of such a pair.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -701,9 +701,9 @@ int tree_build_dev(const galic_params *p, hipStream_t st, int nsrc, const double
     hipLaunchKernelGGL(k_keys, dim3(nb256(nsrc)), dim3(256), 0, st, nsrc, dpx, dpy, dpz, corner[0], corner[1],
                        corner[2], 1.0 / dlen, keys_in, idx_in);
     TCHK(hipGetLastError());
-    TCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, T.key, idx_in, idx, nsrc, 0, 63, st));
+    TCHK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, T.key, idx_in, idx, (unsigned int)nsrc, 0, 63, st));
     TCHK(B.alloc((char **)&tmp, tmp_bytes));
-    TCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, T.key, idx_in, idx, nsrc, 0, 63, st));
+    TCHK(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, T.key, idx_in, idx, (unsigned int)nsrc, 0, 63, st));
     hipLaunchKernelGGL(k_gather, dim3(nb256(nsrc)), dim3(256), 0, st, nsrc, (const int *)idx, dpx, dpy, dpz, dpm, T.x,
                        T.y, T.z, T.m);
     TCHK(hipGetLastError());
