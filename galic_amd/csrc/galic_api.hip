@@ -1004,6 +1004,7 @@ int galic_create(const galic_params *p, int device, galic_ctx **out) {
   CHK(dalloc(&c.stacks, (size_t)GALIC_NTYPES * galic::ST_COUNT * c.Nstack));
   CHK(dalloc(&c.weights, (size_t)GALIC_NTYPES * galic::W_COUNT * c.Nstack));
   CHK(dalloc(&c.merit_bw, (size_t)GALIC_NTYPES * galic::kFields * c.Nstack));
+  CHK(dalloc(&c.merit_bwf, (size_t)GALIC_NTYPES * galic::kFields * c.Nstack));
   CHK(dalloc(&c.cutB_of, (size_t)GALIC_NTYPES * c.Ngrid));
   CHK(dalloc(&c.locA, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
   CHK(dalloc(&c.locA_d, (size_t)GALIC_NTYPES * galic::kFields * c.Ngrid));
@@ -1084,7 +1085,7 @@ int galic_destroy(galic_ctx *ctx) {
   (void)hipStreamSynchronize(c.stream);
   dfree(c.FG_Pot); dfree(c.FG_DPotDR); dfree(c.FG_DPotDz); dfree(c.FG_Vstream);
   dfree(c.FG_Acc_f); dfree(c.FG_Vstream_f); dfree(c.DG_CellSize); dfree(c.DG_CellSize_f); dfree(c.DG_Cell1_f);
-  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw); dfree(c.tcut);
+  dfree(c.stacks); dfree(c.weights); dfree(c.cutB_of); dfree(c.merit_bw); dfree(c.merit_bwf); dfree(c.tcut);
   dfree(c.locA); dfree(c.locA_d); dfree(c.locI); dfree(c.locI_d); dfree(c.locM);
   dfree(c.locA_q); dfree(c.locA_dq); dfree(c.locI_q); dfree(c.locI_dq);
   dfree(c.fit); dfree(c.locM_q); dfree(c.qinv); dfree(c.tovs_dev); dfree(c.t_vk); dfree(c.t_dFk); dfree(c.t_vinst);
@@ -2240,6 +2241,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       E.tvz = vtz;
       E.stacks = c.stacks;
       E.merit_bw = c.merit_bw;
+      E.merit_bwf = c.merit_bwf;
       E.fit = c.fit;
       E.cutB_of = c.cutB_of;
       E.log_blocks = c.log_blocks;
@@ -2377,7 +2379,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
           const int a = std::max(ra, s0), b = std::min(rb, s1);
           if(!packed) {
             hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream,
-                               (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.Nstack);
+                               (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.merit_bwf, c.Nstack);
             packed = true;
           }
           galic::EvalArgs32 E = eval_args(a);
@@ -2427,7 +2429,7 @@ int fp32_run(galic_ctx *ctx, int ntrial, const int *idx, int K, int rep_size, co
       const int a = std::max(ra, s0), b = std::min(rb, s1);
       if(!packed) {
         hipLaunchKernelGGL(galic::k_pack_merit, dim3(nblocks(nbw, 256)), dim3(256), 0, c.stream,
-                           (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.Nstack);
+                           (const double *)c.stacks, (const double *)c.weights, c.merit_bw, c.merit_bwf, c.Nstack);
         HIPCHK(hipGetLastError());
         packed = true;
       }

@@ -167,6 +167,7 @@ struct Ctx {
   size_t sort_tmp_bytes = 0;
   int *psteps = nullptr;  // per particle: steps of its current orbit (FP32 path's length prediction)
   double2 *merit_bw = nullptr;  // FP32 merit: (response - target, node weight) [type][5][Nstack]
+  float2 *merit_bwf = nullptr;  // the same in FP32 (the 16x16-tile merit)
   // response logs of the batched FP32 path (arena carved from `slots`)
   int *log_blocks = nullptr;     // [orbit][log_M]
   size_t log_blocks_len = 0;

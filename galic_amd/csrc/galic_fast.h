@@ -95,6 +95,7 @@ struct EvalArgs32 {
   const double *tvx, *tvy, *tvz;  // trial velocities [i*K + k]
   const double *stacks, *fit;
   const double2 *merit_bw;  // (response - target, weight) per node [type][5][Nstack]
+  const float2 *merit_bwf;  // the same in FP32 (the 16x16-tile form)
   const int *cutB_of;
   const int *log_blocks, *log_n;  // the chunk's response logs (OrbitArgs32)
   int log_M;
@@ -133,7 +134,7 @@ int sort_trials_by_steps(int ntrial, const int *tidx, const int *psteps, unsigne
                          int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
 int sort_trials_by_records(int n, int K, const int *slot_pos, const int *log_n, unsigned int *key, unsigned int *key2,
                            int *ord, int *ord2, void **tmp, size_t *tmp_bytes, hipStream_t stream);
-__global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack);
+__global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, float2 *bwf, int Nstack);
 template <bool USE_O, bool GMEM, int LF>
 __global__ void k_eval_log32(DevModel M, EvalArgs32 A);
 // Tile form of the FP32 merit: the response grid is cut into tiles of TILE x TILE cells,

@@ -809,7 +809,7 @@ __device__ double inst_delta_q(const DevModel &M, const double *S, const int *cu
 // response - target and w = the node's weight, 1/min(dist, R200/20) for the
 // mass term and (Tm/E_x)/min(dist, R200/20) for an S term on the cut nodes, 0
 // elsewhere (calc_stack_difference_mod flags 0 and 2, grid.c:1115-1151).
-__global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, int Nstack) {
+__global__ void k_pack_merit(const double *stacks, const double *weights, double2 *bw, float2 *bwf, int Nstack) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if(q >= GALIC_NTYPES * kFields * Nstack) return;
   const int node = q % Nstack, f = (q / Nstack) % kFields, t = q / (Nstack * kFields);
@@ -823,6 +823,7 @@ __global__ void k_pack_merit(const double *stacks, const double *weights, double
   v.x = S[stR[f] * Nstack + node] - S[stT[f] * Nstack + node];
   v.y = (den != 0 && ratio != 0) ? ratio / den : 0.0;
   bw[q] = v;
+  if(bwf) bwf[q] = make_float2((float)v.x, (float)v.y);
 }
 
 __device__ __forceinline__ double merit_term(const double2 *bw, int node, double d) {
@@ -2496,7 +2497,9 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
 
     // 2. the tiles in Morton order
     const int f = wave;
-    const double2 *bwf = A.merit_bw + ((size_t)t * kFields + f) * Nstack;
+    // (the pairs in FP32: the residual's rounding moves a term |b + D| - |b| only where b + D
+    // changes sign, by at most 2^-23 of |D|)
+    const float2 *bwf = A.merit_bwf + ((size_t)t * kFields + f) * Nstack;
     const int kp = s_kp[f];
     double acc[KR];
 #pragma unroll
@@ -2509,18 +2512,18 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     int *stk = &s_stk[f][0][0];  // [l][k], read and written by lane k of wave f only
     if(lane < KR)
       for(int l = 0; l < NSTK; l++) stk[l * KR + lane] = 0;
-    double2 pc[4] = {}, p1 = make_double2(0.0, 0.0), up = make_double2(0.0, 0.0);
+    float2 pcf[4] = {}, p1f = make_float2(0.f, 0.f), up = make_float2(0.f, 0.f);
     // (only the levels with cut-A nodes in the tile: the others' terms are not formed)
     auto load_pairs = [&](int m) {
       const int gz = t16_z(m) * T16 + 2 * jy, gr = t16_r(m) * T16 + 2 * jx;
       const unsigned int cmn = A.tcut[(size_t)t * ntile + m];
       if(cmn & 1u) {
-        pc[0] = bwf[GALIC_STACKOFFSET(L, gz, gr)];
-        pc[1] = bwf[GALIC_STACKOFFSET(L, gz, gr + 1)];
-        pc[2] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr)];
-        pc[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
+        pcf[0] = bwf[GALIC_STACKOFFSET(L, gz, gr)];
+        pcf[1] = bwf[GALIC_STACKOFFSET(L, gz, gr + 1)];
+        pcf[2] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr)];
+        pcf[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
       }
-      if(cmn & 2u) p1 = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
+      if(cmn & 2u) p1f = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
     };
     // closes trial k's open ancestors of the tile prev (all of them, or those m does not
     // share) from level lt - 1 upwards; up holds their pairs (lane l: level l)
@@ -2533,7 +2536,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
           if(!all && (prev >> sh) == (m >> sh)) {
             open = false;
           } else {
-            const double b = ldexp(__shfl(up.x, l), kp), w = __shfl(up.y, l);
+            const double b = ldexp((double)__shfl(up.x, l), kp), w = (double)__shfl(up.y, l);
             const int sl = lane < KR ? stk[l * KR + lane] : 0;
             accs = t16_term(b, w, (double)sl, accs);
             if(lane < KR) {
@@ -2611,14 +2614,15 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         // the 4x4, 8x8 and tile nodes' pairs of the levels on the cut, used after the trials
         // (their loads overlap them)
         const int gz = tz * T16 + 2 * jy, gr = tr * T16 + 2 * jx;
-        double2 p2 = make_double2(0.0, 0.0), p3 = p2, p4 = p2;
+        float2 p2 = make_float2(0.f, 0.f), p3 = p2, p4 = p2;
         if(cm & 4u) p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
         if(cm & 8u) p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
         if(cm & 16u) p4 = bwf[GALIC_STACKOFFSET(L - 4, tz, tr)];
         // residuals in the particle's units
+        double2 pc[4];
 #pragma unroll
-        for(int c = 0; c < 4; c++) pc[c].x = ldexp(pc[c].x, kp);
-        p1.x = ldexp(p1.x, kp);
+        for(int c = 0; c < 4; c++) pc[c] = make_double2(ldexp((double)pcf[c].x, kp), (double)pcf[c].y);
+        const double2 p1 = make_double2(ldexp((double)p1f.x, kp), (double)p1f.y);
         const unsigned int tm = t16_mask_get<KR>(tmask, m);
         // The coarser nodes' sums of trial k are handed to one lane each and their terms formed
         // after the loop: the 4x4 node's (in all four lanes of its quad after the quad sum) to
@@ -2670,9 +2674,10 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         // the coarser nodes' terms of the lane's trials (0 for trials beyond K or not evaluated)
         if(cm & 4u)
 #pragma unroll
-          for(int u = 0; u < (KR + 3) / 4; u++) acc2[u] = t16_term(ldexp(p2.x, kp), p2.y, (double)s2x[u], acc2[u]);
-        if(cm & 8u) acc3 = t16_term(ldexp(p3.x, kp), p3.y, (double)s3x, acc3);
-        if(cm & 16u) accs = t16_term(ldexp(p4.x, kp), p4.y, (double)tsk, accs);
+          for(int u = 0; u < (KR + 3) / 4; u++)
+            acc2[u] = t16_term(ldexp((double)p2.x, kp), (double)p2.y, (double)s2x[u], acc2[u]);
+        if(cm & 8u) acc3 = t16_term(ldexp((double)p3.x, kp), (double)p3.y, (double)s3x, acc3);
+        if(cm & 16u) accs = t16_term(ldexp((double)p4.x, kp), (double)p4.y, (double)tsk, accs);
         if((cm & 32u) && lt > 0 && lane < KR) stk[(lt - 1) * KR + lane] += tsk;
       }
       prev = m;
