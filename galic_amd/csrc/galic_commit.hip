@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_commit_regions(CommitArgs A, const int4
     const int lz = (quad >> A.L) - gz0, lr = (quad & (nb - 1)) - gr0;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      const float4 x = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+      const float4 x = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f];
       const double q = en.y ? -A.q[f] : A.q[f];
 #pragma unroll
       for(int c = 0; c < 4; c++) {

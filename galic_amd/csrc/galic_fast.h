@@ -49,7 +49,7 @@ struct OrbitArgs32 {
   float *log_tot;        // [orbit][5] sum of the orbit's record values per field
   int log_M;             // block-table entries per orbit
   int *log_q;            // arena: [block][LOG_B] quad index
-  float4 *log_v;         // arena: [block][5][LOG_B] values
+  float4 *log_v;         // arena: [block][LOG_B][5] values (a record's five fields side by side)
   unsigned int *log_next;  // next free arena block
   unsigned int log_cap;    // arena blocks
 };

@@ -279,11 +279,11 @@ __device__ __forceinline__ void log32_flush(const Run32 &r, Log32 &lg, int o, co
   if(!lg.full) {
     const int j = lg.n & (LOG_B - 1);
     *(int2 *)(A.log_q + (size_t)lg.blk * LOG_B + j) = make_int2(lg.rec.quad, r.quad);
-    float4 *v = A.log_v + (size_t)lg.blk * kFields * LOG_B + j;
+    float4 *v = A.log_v + ((size_t)lg.blk * LOG_B + j) * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
-      v[f * LOG_B] = make_float4(lg.rec.w[f][0], lg.rec.w[f][1], lg.rec.w[f][2], lg.rec.w[f][3]);
-      v[f * LOG_B + 1] = make_float4(r.w[f][0], r.w[f][1], r.w[f][2], r.w[f][3]);
+      v[f] = make_float4(lg.rec.w[f][0], lg.rec.w[f][1], lg.rec.w[f][2], lg.rec.w[f][3]);
+      v[kFields + f] = make_float4(r.w[f][0], r.w[f][1], r.w[f][2], r.w[f][3]);
     }
   }
   lg.n += 2;
@@ -297,10 +297,10 @@ __device__ __forceinline__ void log32_finish(Log32 &lg, int o, const OrbitArgs32
   if(!lg.full) {
     const int j = lg.n & (LOG_B - 1);
     A.log_q[(size_t)lg.blk * LOG_B + j] = lg.rec.quad;
-    float4 *v = A.log_v + (size_t)lg.blk * kFields * LOG_B + j;
+    float4 *v = A.log_v + ((size_t)lg.blk * LOG_B + j) * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++)
-      v[f * LOG_B] = make_float4(lg.rec.w[f][0], lg.rec.w[f][1], lg.rec.w[f][2], lg.rec.w[f][3]);
+      v[f] = make_float4(lg.rec.w[f][0], lg.rec.w[f][1], lg.rec.w[f][2], lg.rec.w[f][3]);
   }
   lg.n++;
   lg.held = false;
@@ -848,7 +848,7 @@ __device__ __forceinline__ void log_load(const EvalArgs32 &A, const int *tbl, in
     if(r < n) {
       const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
       R.quad[u] = A.log_q[(size_t)blk * LOG_B + jj];
-      R.x[u] = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+      R.x[u] = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f];
     }
   }
 }
@@ -892,7 +892,7 @@ __device__ __forceinline__ void log_add(const EvalArgs32 &A, const int *tbl, int
   for(int r = threadIdx.x + LOG_RPT * blockDim.x; r < n; r += blockDim.x) {
     const int blk = tbl[r / LOG_B], jj = r & (LOG_B - 1);
     const int quad = A.log_q[(size_t)blk * LOG_B + jj];
-    add_record(G, quad, A.log_v[((size_t)blk * kFields + f) * LOG_B + jj], sc, nb, L, rs);
+    add_record(G, quad, A.log_v[((size_t)blk * LOG_B + jj) * kFields + f], sc, nb, L, rs);
     if(mask) mark_nodes(mask, quad, nb, L);
   }
 }
@@ -1590,7 +1590,7 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
           const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
           pf_o[u] = o;
           pf_quad[u] = A.log_q[(size_t)blk * LOG_B + jj];
-          pf_x[u] = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
+          pf_x[u] = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f2];
         }
       }
       if(BWL) {
@@ -1632,7 +1632,7 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
             const int o = (int)(en >> 24), r = (int)(en & 0xffffffu);
             const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
             const int quad = A.log_q[(size_t)blk * LOG_B + jj];
-            const float4 x = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+            const float4 x = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f];
             long long *g = grids + (size_t)(o == 0 ? cur : o + 1) * TILE_CELLS;
             tile_add((unsigned long long *)g, (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x, sc);
           }
@@ -1716,7 +1716,7 @@ __global__ void __launch_bounds__(1024) k_eval_tile32(DevModel M, EvalArgs32 A) 
             const int r = (int)(en & 0xffffffu);
             const int blk = tbl[o * A.log_M + r / LOG_B], jj = r & (LOG_B - 1);
             const int quad = A.log_q[(size_t)blk * LOG_B + jj];
-            const float4 x = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+            const float4 x = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f];
             tile_add((unsigned long long *)G, (quad >> L) - tz * TILE, (quad & (nb - 1)) - tr * TILE, x,
                      o == 0 ? -sc : sc);
           }
@@ -1951,7 +1951,7 @@ __global__ void __launch_bounds__(1024) k_eval_sparse32(DevModel M, EvalArgs32 A
         const int quad = A.log_q[(size_t)blk * LOG_B + jj];
         float4 x[kFields];
 #pragma unroll
-        for(int f = 0; f < kFields; f++) x[f] = A.log_v[((size_t)blk * kFields + f) * LOG_B + jj];
+        for(int f = 0; f < kFields; f++) x[f] = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f];
         // the (one to four) tiles of the record's cells, as tile_visit takes them
         const int iz = quad >> L, iR = quad & (nb - 1);
         const int z0 = iz / TILE, z1 = (iz + 1) / TILE, r0 = iR / TILE, r1 = (iR + 1) / TILE;
@@ -2578,7 +2578,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
       const int blk = (int)(rec / LOG_B), jj = (int)(rec % LOG_B);
       float4 x[kFields];
 #pragma unroll
-      for(int f2 = 0; f2 < kFields; f2++) x[f2] = A.log_v[((size_t)blk * kFields + f2) * LOG_B + jj];
+      for(int f2 = 0; f2 < kFields; f2++) x[f2] = A.log_v[((size_t)blk * LOG_B + jj) * kFields + f2];
       add_values((int)en.y, A.log_q[rec], x, tz, tr);
     };
     uint2 pe_cur = fetch_entry(0);
