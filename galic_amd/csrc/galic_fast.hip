@@ -2513,6 +2513,7 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
     if(lane < KR)
       for(int l = 0; l < NSTK; l++) stk[l * KR + lane] = 0;
     float2 pcf[4] = {}, p1f = make_float2(0.f, 0.f), up = make_float2(0.f, 0.f);
+    float2 p2f = p1f, p3f = p1f, p4f = p1f;  // the 4x4, 8x8 and tile nodes' pairs
     // (only the levels with cut-A nodes in the tile: the others' terms are not formed)
     auto load_pairs = [&](int m) {
       const int gz = t16_z(m) * T16 + 2 * jy, gr = t16_r(m) * T16 + 2 * jx;
@@ -2524,6 +2525,9 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         pcf[3] = bwf[GALIC_STACKOFFSET(L, gz + 1, gr + 1)];
       }
       if(cmn & 2u) p1f = bwf[GALIC_STACKOFFSET(L - 1, gz >> 1, gr >> 1)];
+      if(cmn & 4u) p2f = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
+      if(cmn & 8u) p3f = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
+      if(cmn & 16u) p4f = bwf[GALIC_STACKOFFSET(L - 4, t16_z(m), t16_r(m))];
     };
     // closes trial k's open ancestors of the tile prev (all of them, or those m does not
     // share) from level lt - 1 upwards; up holds their pairs (lane l: level l)
@@ -2611,13 +2615,9 @@ __global__ void __launch_bounds__(64 * kFields, 4) k_eval_t16(DevModel M, EvalAr
         // cells needs them
         const unsigned int cm = A.tcut[(size_t)t * ntile + m];
         const bool cells = cm & 1u, node1 = cm & 2u, sums = cm & 0x3eu;
-        // the 4x4, 8x8 and tile nodes' pairs of the levels on the cut, used after the trials
-        // (their loads overlap them)
-        const int gz = tz * T16 + 2 * jy, gr = tr * T16 + 2 * jx;
-        float2 p2 = make_float2(0.f, 0.f), p3 = p2, p4 = p2;
-        if(cm & 4u) p2 = bwf[GALIC_STACKOFFSET(L - 2, gz >> 2, gr >> 2)];
-        if(cm & 8u) p3 = bwf[GALIC_STACKOFFSET(L - 3, gz >> 3, gr >> 3)];
-        if(cm & 16u) p4 = bwf[GALIC_STACKOFFSET(L - 4, tz, tr)];
+        // the 4x4, 8x8 and tile nodes' pairs of the levels on the cut (loaded a tile ahead with
+        // the cells' and 2x2 nodes'), used after the trials
+        const float2 p2 = p2f, p3 = p3f, p4 = p4f;
         // residuals in the particle's units
         double2 pc[4];
 #pragma unroll
