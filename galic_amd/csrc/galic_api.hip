@@ -1366,12 +1366,18 @@ int galic_sample_targets(galic_ctx *ctx, int type, long long nsamples, unsigned 
     A.pos_out = dpts;
   }
   const int grid = orbit_grid((int)std::min<long long>(nsamples, 1 << 30), 256, 8);
-  // The maps are summed per workgroup in LDS where they fit in kTargetLds bytes (64^2
+  // The maps are summed per workgroup in LDS where they fit in target_lds bytes (64^2
   // grids: the density map, then the five energy maps two at a time), so the
   // atomics on the small maps in memory are few; each launch draws the same points
   // again (cheap beside the atomics), and only the first counts them and writes them out.
-  constexpr size_t kTargetLds = 65536;
-  const size_t per_map = ng * sizeof(long long), fit = kTargetLds / per_map;
+  // (64 KiB keeps two workgroups per CU; less where the device gives less)
+  size_t target_lds = 65536;
+  {
+    int v = 0;
+    if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess && v > 0)
+      target_lds = std::min(target_lds, (size_t)v);
+  }
+  const size_t per_map = ng * sizeof(long long), fit = target_lds / per_map;
   struct Pass {
     bool dg;
     int f0, f1;

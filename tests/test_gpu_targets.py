@@ -41,9 +41,13 @@ def test_device_points_equal_host_draws(w, oracle_mod):
         g.close()
 
 
-@pytest.mark.parametrize("name,kw", [("composite-4e6", {}), ("halo-disk-1e6", {"disk_tovs": 2})])
+@pytest.mark.parametrize("name,kw", [("composite-4e6", {}), ("halo-disk-1e6", {"disk_tovs": 2}),
+                                     ("composite-4e6", {"level": 5}), ("composite-4e6", {"level": 7})])
 def test_device_targets_match_oracle_binning(name, kw, oracle_mod):
-    """Velocity structures 1 and 0 (halo, bulge), 3 and 2 (disk)."""
+    """Velocity structures 1 and 0 (halo, bulge), 3 and 2 (disk); at levels 5, 6 and 7, whose maps
+    the sampler sums in three layouts (in LDS the density map, then all five energy maps in one
+    pass, at 32^2; the density map, then the energy maps in groups, at 64^2; atomics straight to
+    memory at 128^2)."""
     w = model.make_workload(model.config(name, **kw), n=4000, target_samples=20000)
     p = w.params
     g = GalicGPU(w)
