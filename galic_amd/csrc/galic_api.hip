@@ -1874,12 +1874,16 @@ int galic_fp32_autotune(galic_ctx *ctx, int nsample, int *variant, int *blocks_p
                      (const int *)c.t_ord2, c.t_idx, (const double *)nullptr, (const double *)nullptr,
                      (const double *)nullptr, c.P, orbit_in(c));
   HIPCHK(hipGetLastError());
-  // grid sizes: half and all of the workgroups a CU holds at once, and twice that
-  // (the results are bit-identical; only the schedule differs)
+  // grid sizes in workgroups per CU (the results are bit-identical; only the schedule differs)
   int occ = 4;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(galic::k_orbit32_log), 256, 0));
   occ = std::max(1, std::min(occ, 16));
-  const int variants[1] = {0}, bpcs[3] = {std::max(1, occ / 2), occ, 2 * occ};
+  // every count from 1 to the resident limit, and twice that (a persistent grid larger than
+  // the CUs hold only queues)
+  const int variants[1] = {0};
+  std::vector<int> bpcs;
+  for(int b = 1; b <= occ; b++) bpcs.push_back(b);
+  bpcs.push_back(2 * occ);
   double best = 1e300;
   int bv = 0, bb = 4;
   const long long steps0 = c.acc_steps, orbits0 = c.acc_orbits;

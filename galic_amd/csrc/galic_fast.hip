@@ -240,6 +240,8 @@ struct Log32 {
   int blk;            // current arena block
   bool full;          // out of block-table entries or arena blocks
   bool held;          // rec holds a finished record not yet written
+  bool qheld;         // q0, q1: the quads of the written pair at n - 2, not yet stored (n % 4 == 2)
+  int q0, q1;
   unsigned int next;  // the lane's claimed blocks [next, end)
   unsigned int end;
   float tot[kFields];  // sums of the orbit's record values per field (all values are >= 0)
@@ -277,8 +279,16 @@ __device__ __forceinline__ void log32_flush(const Run32 &r, Log32 &lg, int o, co
   }
   log32_claim(lg, o, A);  // n is even: the pair shares a block
   if(!lg.full) {
+    // (the quads of two pairs go out as one 16-byte store: a pair at j % 4 == 0 holds its own)
     const int j = lg.n & (LOG_B - 1);
-    *(int2 *)(A.log_q + (size_t)lg.blk * LOG_B + j) = make_int2(lg.rec.quad, r.quad);
+    if((j & 2) == 0) {
+      lg.q0 = lg.rec.quad;
+      lg.q1 = r.quad;
+      lg.qheld = true;
+    } else {
+      *(int4 *)(A.log_q + (size_t)lg.blk * LOG_B + j - 2) = make_int4(lg.q0, lg.q1, lg.rec.quad, r.quad);
+      lg.qheld = false;
+    }
     float4 *v = A.log_v + ((size_t)lg.blk * LOG_B + j) * kFields;
 #pragma unroll
     for(int f = 0; f < kFields; f++) {
@@ -292,6 +302,11 @@ __device__ __forceinline__ void log32_flush(const Run32 &r, Log32 &lg, int o, co
 
 // At the orbit's end: the record still held, alone.
 __device__ __forceinline__ void log32_finish(Log32 &lg, int o, const OrbitArgs32 &A) {
+  // the quads of a pair still held (in the current block: n % 4 == 2)
+  if(lg.qheld) {
+    if(!lg.full) *(int2 *)(A.log_q + (size_t)lg.blk * LOG_B + ((lg.n - 2) & (LOG_B - 1))) = make_int2(lg.q0, lg.q1);
+    lg.qheld = false;
+  }
   if(!lg.held) return;
   log32_claim(lg, o, A);
   if(!lg.full) {
@@ -392,6 +407,8 @@ __device__ __forceinline__ void orbit32_body(const FastModel &F, const OrbitArgs
   lg.blk = 0;
   lg.full = false;
   lg.held = false;
+  lg.qheld = false;
+  lg.q0 = lg.q1 = 0;
   lg.next = lg.end = 0;
 #pragma unroll
   for(int f = 0; f < kFields; f++) lg.tot[f] = 0.f;

@@ -204,8 +204,8 @@ int galic_fp32_iterate(galic_ctx *ctx, int ntrial, const int *idx, int k, int re
  * and k must be that call's. */
 int galic_fp32_trial_merits(galic_ctx *ctx, int ntrial, int k, double *dFk);
 
-/* Times the FP32 orbit kernel's persistent grid sizes (half, all and twice the
- * workgroups a CU holds at once) on nsample orbits of the model and keeps the
+/* Times the FP32 orbit kernel's persistent grid sizes (1 to all of the workgroups a
+ * CU holds at once, and twice that) on nsample orbits of the model and keeps the
  * fastest for later reps; reports the choice (variant: the kernel variant, 0).
  * The launches give bit-identical results.  nsample should exceed the lanes the
  * GPU holds at once. */
