@@ -48,8 +48,8 @@ TREE_MODE_TARGET = TREE_FP32 | TREE_QUADRUPOLE | TREE_PLUMMER   # the force-fiel
 TREE_MODE = TREE_MODE_TARGET | TREE_GROUP   # ... with the targets' groups walking together
 # HBM bytes per leapfrog step of k_orbit32_log in the full model's timed launches, from the FETCH_SIZE
 # and WRITE_SIZE counter runs calibrated against a plain copy (profiles/k_orbit32_log_composite-2e7.txt:
-# 3.5 B fetched + 41.4 B written per step)
-HBM_BYTES_PER_STEP = 44.9
+# 3.0 B fetched + 38.1 B written per step)
+HBM_BYTES_PER_STEP = 41.1
 
 
 def log(*a):
